@@ -1,0 +1,118 @@
+// Internal definitions of libgiraffe_amd (host side + kernel launch prototypes).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include <vector>
+#include "../../include/giraffe_amd.h"
+
+// ---- error handling --------------------------------------------------------
+void gf_set_error(const std::string &msg);
+#define GF_HIP(call)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess) {                                                       \
+            gf_set_error(std::string(#call) + ": " + hipGetErrorString(e_));           \
+            return GF_E_DEVICE;                                                       \
+        }                                                                             \
+    } while (0)
+
+// ---- per-element constant record of Beam_1 (structure of arrays, [BC_N][nb]) ----
+enum {
+    BC_T3 = 0,        // 9: transform3 rows E1,E2,E3
+    BC_D = 9,         // 7: GA1 GA2 EA EI1 EI2 EI12 GJ
+    BC_E3R = 16,      // 3: reference axis in the local frame
+    BC_JAC = 19,      // 1: L/2
+    BC_RHOA = 20,     // 1: rho*A
+    BC_N = 21,        // 6: N[gp][3]
+    BC_DN = 27,       // 6: dN[gp][3]
+    BC_COUNT = 33
+};
+// Gauss-point state of Beam_1, [BG_COUNT][nb]: per gp Q_i(9) kappa_i(3) z'_i(3)
+enum { BG_PER_GP = 15, BG_COUNT = 30 };
+
+// Nodal loads grouped by node (one device thread per loaded node).
+enum { NL_KIND = 0, NL_CS = 1, NL_MULT = 10, NL_TAB0 = 16, NL_TABN = 17, NL_REC = 18 };
+
+// Block-tridiagonal factor workspace (cyclic reduction), precision T.
+struct gf_btd {
+    int m = 0, nblk = 0, levels = 0;
+    size_t bsz = 0;                 // m*m
+    void *A = nullptr, *B = nullptr, *C = nullptr;   // [nblk][m][m]
+    void *Lsv = nullptr, *Rsv = nullptr;             // saved coupling blocks per (level, kept row)
+    int *piv = nullptr;             // [nblk][m]
+    std::vector<int> lvl_n;         // rows active at each level
+    std::vector<size_t> lvl_save;   // offsets (in blocks) into Lsv/Rsv per level
+    int elem_size = 8;
+};
+
+struct gf_model {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int precision = GF_PREC_FP64;
+    int refine_max = 8;
+    double refine_tol = 1e-14;
+
+    int nn = 0, nb = 0, ns = 0, nl = 0;
+    int n_free = 0;
+
+    // host mirrors
+    std::vector<int> dof;           // [nn*6] band row or -1
+    std::vector<int> row_nodecomp;  // [n_free] node*6+comp
+    std::vector<unsigned char> active, fix;
+    std::vector<int> beam_order;    // colour-sorted position -> element id
+    std::vector<int> beam_color_off;
+    std::vector<double> h_nodes_ref;
+    double sol_start = 0, sol_end = 1;
+    double t_last = 0, dt = 0;
+    double gravity[3] = {0, 0, 0};
+    int gravity_on = 0;
+    int half_band = 0;
+
+    // device: node state
+    double *d_disp = nullptr;       // [nn][6] incremental displacements
+    double *d_copy = nullptr;       // [nn][6] converged X Y Z, alpha
+    double *d_euler = nullptr;      // [nn][3]
+    int *d_dof = nullptr;           // [nn][6]
+    int *d_row_nc = nullptr;        // [n_free]
+
+    // device: beams (colour-sorted)
+    int *d_bconn = nullptr;         // [3][nb]
+    double *d_bconst = nullptr;     // [BC_COUNT][nb]
+    double *d_bgp = nullptr;        // [BG_COUNT][nb]
+
+    // device: nodal loads
+    int n_load_nodes = 0;
+    int *d_ln_node = nullptr, *d_ln_off = nullptr, *d_ln_ids = nullptr;
+    double *d_lrec = nullptr;       // [nl][NL_REC]
+    double *d_tab_t = nullptr, *d_tab_v = nullptr;
+
+    // device: global system
+    double *d_r = nullptr;          // residual r = f_ext - f_int [n_free]
+    double *d_x = nullptr;          // increment [n_free]
+    gf_btd btd;
+    double *d_scal = nullptr;       // small reduction results
+    double *h_scal = nullptr;       // pinned mirror
+    int *d_flag = nullptr;          // solver failure flag
+    bool factored = false;
+};
+
+// ---- kernel launchers (defined in the .hip files) ----------------------------
+int gf_launch_fill(hipStream_t s, double *p, size_t n, double v);
+int gf_launch_fill_bytes(hipStream_t s, void *p, size_t bytes);
+int gf_beam_assemble(gf_model *m, double lf);
+int gf_beam_commit(gf_model *m);
+int gf_nodal_loads(gf_model *m, double t);
+int gf_scan_residual(gf_model *m);
+int gf_update_disp(gf_model *m);
+int gf_node_commit(gf_model *m);
+
+int gf_btd_alloc(gf_btd &b, int m, int nblk, int elem_size);
+void gf_btd_free(gf_btd &b);
+int gf_btd_factor(gf_btd &b, hipStream_t s, int *d_flag);
+int gf_btd_solve(gf_btd &b, hipStream_t s, const double *rhs, double *x, int n);
+
+int gf_beam_eval_device(int count, const double *x, const double *cs, const double *prop,
+                        const double *d, const double *gp, double *K_out, double *f_out);
+void gf_beam_host_const(const double *x9, const double *cs, const double *prop, double *rec /*[BC_COUNT]*/,
+                        double *gp /*[BG_COUNT]*/);

@@ -1,0 +1,509 @@
+// Host side of libgiraffe_amd: model set-up and the per-phase C ABI.
+//
+// Reference: Database::PreCalc src/Database.cpp:704-757; Solution::DOFsActive /
+// SetGlobalDOFs src/Solution.cpp:121-224, 40-118; NodalLoad::UpdateforSolutionStep
+// src/NodalLoad.cpp:236-266; Solution::Zeros / Clear / MountLocal /
+// MountElementLoads / MountLoads / MountGlobal / UpdateDisps / SaveConfiguration /
+// RestoreConfiguration src/Solution.cpp:227-574, 798-863; sparsesystem
+// src/SparseMatrix.cpp:74-166.
+//
+// Device layout (HBM):
+//   node state    disp[nn][6], copy[nn][6] (X Y Z alpha), euler[nn][3]   FP64
+//   DOF map       dof[nn][6] -> band row or -1; row_nc[n_free]           int32
+//   Beam_1        conn[3][nb], const[33][nb], gp state[30][nb]           SoA, colour-sorted
+//   system        r, x [nblk*m] FP64; A,B,C [nblk][m][m] block-tridiagonal
+#include "gf_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+static thread_local std::string g_err;
+void gf_set_error(const std::string &msg) { g_err = msg; }
+
+extern "C" const char *gf_last_error(void) { return g_err.c_str(); }
+extern "C" int gf_version(void) { return 100; }
+
+int gf_pad_diag(gf_model *m);
+
+extern "C" int gf_default_options(gf_options *o)
+{
+    if (!o) return GF_E_ARG;
+    o->precision = GF_PREC_FP64;
+    o->device = 0;
+    o->stream = nullptr;
+    o->refine_max = 8;
+    o->refine_tol = 1e-14;
+    return GF_OK;
+}
+
+template <typename T> static int dalloc(T **p, size_t n)
+{
+    hipError_t e = hipMalloc((void **)p, sizeof(T) * (n ? n : 1));
+    if (e != hipSuccess) {
+        gf_set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+        return GF_E_NOMEM;
+    }
+    return GF_OK;
+}
+template <typename T> static int upload(gf_model *m, T *dst, const T *src, size_t n)
+{
+    if (n == 0) return GF_OK;
+    GF_HIP(hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyHostToDevice, m->stream));
+    return GF_OK;
+}
+
+#define GF_TRY(x)                 \
+    do {                          \
+        int s_ = (x);             \
+        if (s_ != GF_OK) return s_; \
+    } while (0)
+
+static void release(gf_model *m)
+{
+    if (!m) return;
+    hipFree(m->d_disp); hipFree(m->d_copy); hipFree(m->d_euler); hipFree(m->d_dof); hipFree(m->d_row_nc);
+    hipFree(m->d_bconn); hipFree(m->d_bconst); hipFree(m->d_bgp);
+    hipFree(m->d_ln_node); hipFree(m->d_ln_off); hipFree(m->d_ln_ids); hipFree(m->d_lrec);
+    hipFree(m->d_tab_t); hipFree(m->d_tab_v);
+    hipFree(m->d_r); hipFree(m->d_x); hipFree(m->d_scal); hipFree(m->d_flag);
+    if (m->h_scal) hipHostFree(m->h_scal);
+    gf_btd_free(m->btd);
+    if (m->own_stream && m->stream) hipStreamDestroy(m->stream);
+    delete m;
+}
+
+static int create(const gf_model_desc *d, const gf_options *opt, gf_model *m)
+{
+    if (d->n_nodes <= 0 || !d->node_xyz) { gf_set_error("model has no nodes"); return GF_E_ARG; }
+    if (d->n_shells > 0) { gf_set_error("Shell_1 is not built in this library version"); return GF_E_UNSUPPORTED; }
+    m->nn = d->n_nodes;
+    m->nb = d->n_beams;
+    m->ns = d->n_shells;
+    m->nl = d->n_loads;
+    const int nn = m->nn, nb = m->nb;
+    for (int e = 0; e < nb; e++)
+        for (int a = 0; a < 3; a++) {
+            int n = d->beam_conn[3 * e + a];
+            if (n < 0 || n >= nn) { gf_set_error("beam connectivity out of range"); return GF_E_ARG; }
+        }
+    // DOFsActive + NodalConstraint + SetGlobalDOFs (node-major numbering)
+    m->active.assign(6 * (size_t)nn, 0);
+    m->fix.assign(6 * (size_t)nn, 0);
+    for (int e = 0; e < nb; e++)
+        for (int a = 0; a < 3; a++)
+            for (int k = 0; k < 6; k++) m->active[6 * (size_t)d->beam_conn[3 * e + a] + k] = 1;
+    if (d->node_fix)
+        for (size_t i = 0; i < 6 * (size_t)nn; i++) m->fix[i] = d->node_fix[i] ? 1 : 0;
+    m->dof.assign(6 * (size_t)nn, -1);
+    m->row_nodecomp.clear();
+    int nf = 0;
+    for (int i = 0; i < nn; i++)
+        for (int k = 0; k < 6; k++) {
+            size_t id = 6 * (size_t)i + k;
+            if (m->active[id] && !m->fix[id]) {
+                m->dof[id] = nf++;
+                m->row_nodecomp.push_back((int)id);
+            }
+        }
+    m->n_free = nf;
+    if (nf == 0) { gf_set_error("model has no free DOFs"); return GF_E_ARG; }
+    // half-bandwidth and the block size of the block-tridiagonal form
+    std::vector<int> lo(nb), hi(nb);
+    int hb = 0;
+    for (int e = 0; e < nb; e++) {
+        int l = 1 << 30, h = -1;
+        for (int a = 0; a < 3; a++)
+            for (int k = 0; k < 6; k++) {
+                int r = m->dof[6 * (size_t)d->beam_conn[3 * e + a] + k];
+                if (r >= 0) { l = std::min(l, r); h = std::max(h, r); }
+            }
+        lo[e] = l; hi[e] = h;
+        if (h >= 0) hb = std::max(hb, h - l);
+    }
+    m->half_band = hb;
+    int bm = -1;
+    for (int cand = std::max(1, (hb + 2) / 2); cand <= 32 && bm < 0; cand++) {
+        bool ok = true;
+        for (int e = 0; e < nb && ok; e++)
+            if (hi[e] >= 0 && hi[e] / cand - lo[e] / cand > 1) ok = false;
+        if (ok) bm = cand;
+    }
+    if (bm < 0) {
+        gf_set_error("half-bandwidth too large for the block-tridiagonal solver");
+        return GF_E_UNSUPPORTED;
+    }
+    int nblk = (nf + bm - 1) / bm;
+    // colouring: no two beams of one colour share a node
+    std::vector<unsigned long long> used(nn, 0ull);
+    std::vector<int> color(nb);
+    int ncol = 0;
+    for (int e = 0; e < nb; e++) {
+        unsigned long long u = 0;
+        for (int a = 0; a < 3; a++) u |= used[d->beam_conn[3 * e + a]];
+        int c = 0;
+        while (c < 64 && (u >> c & 1ull)) c++;
+        if (c == 64) { gf_set_error("colouring needs more than 64 colours"); return GF_E_UNSUPPORTED; }
+        color[e] = c;
+        ncol = std::max(ncol, c + 1);
+        for (int a = 0; a < 3; a++) used[d->beam_conn[3 * e + a]] |= 1ull << c;
+    }
+    m->beam_order.resize(nb);
+    m->beam_color_off.assign(ncol + 1, 0);
+    for (int e = 0; e < nb; e++) m->beam_color_off[color[e] + 1]++;
+    for (int c = 0; c < ncol; c++) m->beam_color_off[c + 1] += m->beam_color_off[c];
+    {
+        std::vector<int> fill(m->beam_color_off.begin(), m->beam_color_off.end() - 1);
+        for (int e = 0; e < nb; e++) m->beam_order[fill[color[e]]++] = e;
+    }
+    // device + stream
+    m->device = opt ? opt->device : 0;
+    GF_HIP(hipSetDevice(m->device));
+    if (opt && opt->stream) {
+        m->stream = (hipStream_t)opt->stream;
+    } else {
+        GF_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+        m->own_stream = true;
+    }
+    m->precision = opt ? opt->precision : GF_PREC_FP64;
+    if (m->precision != GF_PREC_FP64) {
+        gf_set_error("only GF_PREC_FP64 is built in this library version");
+        return GF_E_UNSUPPORTED;
+    }
+    // node state
+    std::vector<double> copy(6 * (size_t)nn, 0.0);
+    for (int i = 0; i < nn; i++)
+        for (int k = 0; k < 3; k++) copy[6 * (size_t)i + k] = d->node_xyz[3 * (size_t)i + k];
+    m->h_nodes_ref.assign(d->node_xyz, d->node_xyz + 3 * (size_t)nn);
+    GF_TRY(dalloc(&m->d_disp, 6 * (size_t)nn));
+    GF_TRY(dalloc(&m->d_copy, 6 * (size_t)nn));
+    GF_TRY(dalloc(&m->d_euler, 3 * (size_t)nn));
+    GF_TRY(dalloc(&m->d_dof, 6 * (size_t)nn));
+    GF_TRY(dalloc(&m->d_row_nc, (size_t)nf));
+    GF_TRY(upload(m, m->d_copy, copy.data(), copy.size()));
+    GF_TRY(upload(m, m->d_dof, m->dof.data(), m->dof.size()));
+    GF_TRY(upload(m, m->d_row_nc, m->row_nodecomp.data(), (size_t)nf));
+    GF_TRY(gf_launch_fill(m->stream, m->d_disp, 6 * (size_t)nn, 0.0));
+    GF_TRY(gf_launch_fill(m->stream, m->d_euler, 3 * (size_t)nn, 0.0));
+    // beams, colour-sorted SoA
+    if (nb > 0) {
+        size_t ld = (size_t)nb;
+        std::vector<int> conn(3 * ld);
+        std::vector<double> cst(BC_COUNT * ld), gps(BG_COUNT * ld), rec(BC_COUNT), g0(BG_COUNT);
+        for (int s = 0; s < nb; s++) {
+            int e = m->beam_order[s];
+            double x[9];
+            for (int a = 0; a < 3; a++)
+                for (int k = 0; k < 3; k++) x[3 * a + k] = d->node_xyz[3 * (size_t)d->beam_conn[3 * e + a] + k];
+            gf_beam_host_const(x, d->beam_cs + 9 * (size_t)e, d->beam_prop + GF_BEAM_NPROP * (size_t)e, rec.data(),
+                               g0.data());
+            for (int k = 0; k < BC_COUNT; k++) cst[k * ld + s] = rec[k];
+            for (int k = 0; k < BG_COUNT; k++) gps[k * ld + s] = g0[k];
+            for (int a = 0; a < 3; a++) conn[a * ld + s] = d->beam_conn[3 * e + a];
+        }
+        GF_TRY(dalloc(&m->d_bconn, conn.size()));
+        GF_TRY(dalloc(&m->d_bconst, cst.size()));
+        GF_TRY(dalloc(&m->d_bgp, gps.size()));
+        GF_TRY(upload(m, m->d_bconn, conn.data(), conn.size()));
+        GF_TRY(upload(m, m->d_bconst, cst.data(), cst.size()));
+        GF_TRY(upload(m, m->d_bgp, gps.data(), gps.size()));
+    }
+    // nodal loads grouped by node; multipliers as UpdateforSolutionStep
+    {
+        std::vector<std::pair<int, int>> pairs;  // (node, load)
+        std::vector<double> lrec((size_t)std::max(1, m->nl) * NL_REC, 0.0);
+        for (int l = 0; l < m->nl; l++) {
+            int kind = d->load_kind[l];
+            if (kind == GF_LOAD_SHELL_PRESSURE) continue;
+            if (kind != GF_LOAD_NODAL && kind != GF_LOAD_NODAL_FOLLOWER) { gf_set_error("unknown load kind"); return GF_E_ARG; }
+            int cf[3] = {0, 0, 0}, cm[3] = {0, 0, 0};
+            for (int s = d->load_set_off[l]; s < d->load_set_off[l + 1]; s++) {
+                int node = d->load_set_items[s];
+                if (node < 0 || node >= nn) { gf_set_error("load node out of range"); return GF_E_ARG; }
+                for (int j = 0; j < 3; j++) {
+                    if (m->active[6 * (size_t)node + j]) cf[j]++;
+                    if (m->active[6 * (size_t)node + 3 + j]) cm[j]++;
+                }
+                pairs.emplace_back(node, l);
+            }
+            double *R = lrec.data() + (size_t)l * NL_REC;
+            R[NL_KIND] = kind;
+            for (int i = 0; i < 9; i++) R[NL_CS + i] = d->load_cs[9 * (size_t)l + i];
+            for (int j = 0; j < 3; j++) {
+                R[NL_MULT + j] = 1.0 / cf[j];
+                R[NL_MULT + 3 + j] = 1.0 / cm[j];
+            }
+            R[NL_TAB0] = d->load_tab_off[l];
+            R[NL_TABN] = d->load_tab_off[l + 1] - d->load_tab_off[l];
+        }
+        std::stable_sort(pairs.begin(), pairs.end(),
+                         [](const std::pair<int, int> &a, const std::pair<int, int> &b) { return a.first < b.first; });
+        std::vector<int> ln_node, ln_off(1, 0), ln_ids;
+        for (size_t i = 0; i < pairs.size(); i++) {
+            if (i == 0 || pairs[i].first != pairs[i - 1].first) {
+                if (i) ln_off.push_back((int)ln_ids.size());
+                ln_node.push_back(pairs[i].first);
+            }
+            ln_ids.push_back(pairs[i].second);
+        }
+        if (!pairs.empty()) ln_off.push_back((int)ln_ids.size());
+        m->n_load_nodes = (int)ln_node.size();
+        int rows = m->nl ? d->load_tab_off[m->nl] : 0;
+        GF_TRY(dalloc(&m->d_ln_node, ln_node.size()));
+        GF_TRY(dalloc(&m->d_ln_off, ln_off.size()));
+        GF_TRY(dalloc(&m->d_ln_ids, ln_ids.size()));
+        GF_TRY(dalloc(&m->d_lrec, lrec.size()));
+        GF_TRY(dalloc(&m->d_tab_t, (size_t)rows));
+        GF_TRY(dalloc(&m->d_tab_v, 6 * (size_t)rows));
+        GF_TRY(upload(m, m->d_ln_node, ln_node.data(), ln_node.size()));
+        GF_TRY(upload(m, m->d_ln_off, ln_off.data(), ln_off.size()));
+        GF_TRY(upload(m, m->d_ln_ids, ln_ids.data(), ln_ids.size()));
+        GF_TRY(upload(m, m->d_lrec, lrec.data(), lrec.size()));
+        GF_TRY(upload(m, m->d_tab_t, d->tab_time, (size_t)rows));
+        GF_TRY(upload(m, m->d_tab_v, d->tab_val, 6 * (size_t)rows));
+    }
+    for (int k = 0; k < 3; k++) m->gravity[k] = d->gravity[k];
+    m->gravity_on = d->gravity_on;
+    // global system
+    size_t npad = (size_t)nblk * bm;
+    GF_TRY(dalloc(&m->d_r, npad));
+    GF_TRY(dalloc(&m->d_x, npad));
+    GF_TRY(gf_btd_alloc(m->btd, bm, nblk, 8));
+    GF_TRY(dalloc(&m->d_scal, 16));
+    GF_TRY(dalloc(&m->d_flag, 4));
+    GF_HIP(hipHostMalloc((void **)&m->h_scal, 16 * sizeof(double), hipHostMallocDefault));
+    GF_HIP(hipStreamSynchronize(m->stream));
+    return GF_OK;
+}
+
+extern "C" int gf_model_create(const gf_model_desc *desc, const gf_options *opt, gf_model **out)
+{
+    if (!desc || !out) { gf_set_error("null argument"); return GF_E_ARG; }
+    *out = nullptr;
+    gf_model *m = new gf_model();
+    if (opt) {
+        m->refine_max = opt->refine_max;
+        m->refine_tol = opt->refine_tol;
+    }
+    int st = create(desc, opt, m);
+    if (st != GF_OK) {
+        release(m);
+        return st;
+    }
+    *out = m;
+    return GF_OK;
+}
+
+extern "C" int gf_model_destroy(gf_model *m)
+{
+    if (m) {
+        hipSetDevice(m->device);
+        if (m->stream) hipStreamSynchronize(m->stream);
+        release(m);
+    }
+    return GF_OK;
+}
+
+extern "C" int gf_model_info(const gf_model *m, long long *o)
+{
+    if (!m || !o) return GF_E_ARG;
+    o[0] = m->n_free;
+    o[1] = m->half_band;
+    o[2] = m->btd.m;
+    o[3] = m->btd.nblk;
+    o[4] = (long long)m->beam_color_off.size() - 1;
+    o[5] = (long long)(3 * (size_t)m->btd.nblk * m->btd.bsz * m->btd.elem_size);
+    o[6] = m->btd.levels;
+    o[7] = m->precision;
+    return GF_OK;
+}
+
+extern "C" int gf_begin_step(gf_model *m, double t_last, double dt)
+{
+    if (!m) return GF_E_ARG;
+    GF_HIP(hipSetDevice(m->device));
+    m->t_last = t_last;
+    m->dt = dt;
+    m->factored = false;
+    return gf_launch_fill(m->stream, m->d_disp, 6 * (size_t)m->nn, 0.0);
+}
+
+// Sets the solution-time window used by the gravity ramp (BoolTable factor).
+extern "C" int gf_set_solution_window(gf_model *m, double start, double end)
+{
+    if (!m) return GF_E_ARG;
+    m->sol_start = start;
+    m->sol_end = end;
+    return GF_OK;
+}
+
+extern "C" int gf_assemble(gf_model *m, double *res_out)
+{
+    if (!m) return GF_E_ARG;
+    GF_HIP(hipSetDevice(m->device));
+    size_t npad = (size_t)m->btd.nblk * m->btd.m;
+    size_t nbl = (size_t)m->btd.nblk * m->btd.bsz;
+    GF_TRY(gf_launch_fill(m->stream, m->d_r, npad, 0.0));
+    GF_TRY(gf_launch_fill(m->stream, (double *)m->btd.A, nbl, 0.0));
+    GF_TRY(gf_launch_fill(m->stream, (double *)m->btd.B, nbl, 0.0));
+    GF_TRY(gf_launch_fill(m->stream, (double *)m->btd.C, nbl, 0.0));
+    GF_TRY(gf_pad_diag(m));
+    double t = m->t_last + m->dt;
+    GF_TRY(gf_nodal_loads(m, t));
+    double lf = m->gravity_on ? (t - m->sol_start) / (m->sol_end - m->sol_start) : 0.0;
+    GF_TRY(gf_beam_assemble(m, lf));
+    m->factored = false;
+    if (res_out) {
+        GF_TRY(gf_scan_residual(m));
+        GF_HIP(hipMemcpyAsync(m->h_scal, m->d_scal, 3 * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+        GF_HIP(hipStreamSynchronize(m->stream));
+        unsigned long long *u = (unsigned long long *)m->h_scal;
+        double v0, v1;
+        memcpy(&v0, &u[0], 8);
+        memcpy(&v1, &u[1], 8);
+        res_out[0] = v0;
+        res_out[1] = v1;
+        res_out[2] = u[2] ? 1.0 : 0.0;
+    }
+    return GF_OK;
+}
+
+extern "C" int gf_solve(gf_model *m, int *info_fail)
+{
+    if (!m || !info_fail) return GF_E_ARG;
+    GF_HIP(hipSetDevice(m->device));
+    *info_fail = 0;
+    GF_TRY(gf_launch_fill_bytes(m->stream, m->d_flag, 4));
+    GF_TRY(gf_btd_factor(m->btd, m->stream, m->d_flag));
+    m->factored = true;
+    size_t npad = (size_t)m->btd.nblk * m->btd.m;
+    GF_TRY(gf_btd_solve(m->btd, m->stream, m->d_r, m->d_x, (int)npad));
+    int flag = 0;
+    GF_HIP(hipMemcpyAsync(&m->h_scal[8], m->d_flag, 4, hipMemcpyDeviceToHost, m->stream));
+    GF_HIP(hipStreamSynchronize(m->stream));
+    memcpy(&flag, &m->h_scal[8], 4);
+    if (flag) *info_fail = 1;
+    return GF_OK;
+}
+
+extern "C" int gf_update(gf_model *m, double *inc_out)
+{
+    if (!m) return GF_E_ARG;
+    GF_HIP(hipSetDevice(m->device));
+    GF_TRY(gf_update_disp(m));
+    if (inc_out) {
+        GF_HIP(hipMemcpyAsync(m->h_scal, m->d_scal, 5 * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+        GF_HIP(hipStreamSynchronize(m->stream));
+        unsigned long long *u = (unsigned long long *)m->h_scal;
+        for (int i = 0; i < 4; i++) memcpy(&inc_out[i], &u[i], 8);
+        inc_out[4] = u[4] ? 1.0 : 0.0;
+    }
+    return GF_OK;
+}
+
+extern "C" int gf_commit(gf_model *m)
+{
+    if (!m) return GF_E_ARG;
+    GF_HIP(hipSetDevice(m->device));
+    GF_TRY(gf_node_commit(m));
+    GF_TRY(gf_beam_commit(m));
+    GF_HIP(hipStreamSynchronize(m->stream));
+    return GF_OK;
+}
+
+extern "C" int gf_rollback(gf_model *m)
+{
+    if (!m) return GF_E_ARG;
+    GF_HIP(hipSetDevice(m->device));
+    GF_TRY(gf_launch_fill(m->stream, m->d_disp, 6 * (size_t)m->nn, 0.0));
+    return GF_OK;
+}
+
+extern "C" int gf_read_nodes(gf_model *m, double *out)
+{
+    if (!m || !out) return GF_E_ARG;
+    GF_HIP(hipSetDevice(m->device));
+    std::vector<double> c(6 * (size_t)m->nn), eu(3 * (size_t)m->nn);
+    GF_HIP(hipMemcpyAsync(c.data(), m->d_copy, sizeof(double) * c.size(), hipMemcpyDeviceToHost, m->stream));
+    GF_HIP(hipMemcpyAsync(eu.data(), m->d_euler, sizeof(double) * eu.size(), hipMemcpyDeviceToHost, m->stream));
+    GF_HIP(hipStreamSynchronize(m->stream));
+    for (int i = 0; i < m->nn; i++) {
+        for (int k = 0; k < 3; k++) out[6 * (size_t)i + k] = c[6 * (size_t)i + k];
+        for (int k = 0; k < 3; k++) out[6 * (size_t)i + 3 + k] = eu[3 * (size_t)i + k];
+    }
+    return GF_OK;
+}
+
+extern "C" int gf_read_disp(gf_model *m, double *out)
+{
+    if (!m || !out) return GF_E_ARG;
+    GF_HIP(hipSetDevice(m->device));
+    GF_HIP(hipMemcpyAsync(out, m->d_disp, sizeof(double) * 6 * (size_t)m->nn, hipMemcpyDeviceToHost, m->stream));
+    GF_HIP(hipStreamSynchronize(m->stream));
+    return GF_OK;
+}
+
+extern "C" int gf_write_disp(gf_model *m, const double *in)
+{
+    if (!m || !in) return GF_E_ARG;
+    GF_HIP(hipSetDevice(m->device));
+    GF_HIP(hipMemcpyAsync(m->d_disp, in, sizeof(double) * 6 * (size_t)m->nn, hipMemcpyHostToDevice, m->stream));
+    GF_HIP(hipStreamSynchronize(m->stream));
+    return GF_OK;
+}
+
+static int read_rows(gf_model *m, const double *dev, double *out)
+{
+    GF_HIP(hipSetDevice(m->device));
+    std::vector<double> v((size_t)m->n_free);
+    GF_HIP(hipMemcpyAsync(v.data(), dev, sizeof(double) * v.size(), hipMemcpyDeviceToHost, m->stream));
+    GF_HIP(hipStreamSynchronize(m->stream));
+    std::fill(out, out + 6 * (size_t)m->nn, 0.0);
+    for (int r = 0; r < m->n_free; r++) out[m->row_nodecomp[r]] = v[r];
+    return GF_OK;
+}
+
+extern "C" int gf_read_residual(gf_model *m, double *out)
+{
+    if (!m || !out) return GF_E_ARG;
+    return read_rows(m, m->d_r, out);
+}
+
+extern "C" int gf_read_increment(gf_model *m, double *out)
+{
+    if (!m || !out) return GF_E_ARG;
+    return read_rows(m, m->d_x, out);
+}
+
+extern "C" int gf_read_tangent_dense(gf_model *m, double *out, long long cap)
+{
+    if (!m || !out) return GF_E_ARG;
+    long long n6 = 6LL * m->nn;
+    if (cap < n6 * n6) { gf_set_error("output buffer too small"); return GF_E_CAPACITY; }
+    if (m->factored) { gf_set_error("tangent already factored"); return GF_E_STATE; }
+    GF_HIP(hipSetDevice(m->device));
+    size_t nbl = (size_t)m->btd.nblk * m->btd.bsz;
+    std::vector<double> A(nbl), B(nbl), C(nbl);
+    GF_HIP(hipMemcpyAsync(A.data(), m->btd.A, nbl * 8, hipMemcpyDeviceToHost, m->stream));
+    GF_HIP(hipMemcpyAsync(B.data(), m->btd.B, nbl * 8, hipMemcpyDeviceToHost, m->stream));
+    GF_HIP(hipMemcpyAsync(C.data(), m->btd.C, nbl * 8, hipMemcpyDeviceToHost, m->stream));
+    GF_HIP(hipStreamSynchronize(m->stream));
+    std::fill(out, out + n6 * n6, 0.0);
+    int bm = m->btd.m;
+    for (int R = 0; R < m->n_free; R++)
+        for (int Cc = std::max(0, (R / bm - 1) * bm); Cc < std::min(m->n_free, (R / bm + 2) * bm); Cc++) {
+            int bi = R / bm, bj = Cc / bm;
+            size_t off = ((size_t)bi * bm + (R - bi * bm)) * bm + (Cc - bj * bm);
+            double v = bj == bi ? B[off] : (bj == bi - 1 ? A[off] : C[off]);
+            out[(size_t)m->row_nodecomp[R] * n6 + m->row_nodecomp[Cc]] = v;
+        }
+    return GF_OK;
+}
+
+extern "C" int gf_beam_eval(int count, const double *x, const double *cs, const double *prop, const double *d,
+                            const double *gp, double *K_out, double *f_out)
+{
+    if (count < 0 || (count > 0 && (!x || !cs || !prop || !d || !gp || !K_out || !f_out))) return GF_E_ARG;
+    return gf_beam_eval_device(count, x, cs, prop, d, gp, K_out, f_out);
+}

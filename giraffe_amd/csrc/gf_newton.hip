@@ -1,0 +1,299 @@
+// Newton-step support kernels: nodal loads, FP64 max-norm scans, displacement
+// update and the Rodrigues commit of the nodes.
+//
+// Reference: NodalLoad::Mount src/NodalLoad.cpp:322-401,
+// NodalFollowerLoad::Mount src/NodalFollowerLoad.cpp:243-326,
+// Table::GetValueAt src/Table.cpp:73-101,
+// ConvergenceCriteria scans src/ConvergenceCriteria.cpp:187-288, 289-459, 460-598,
+// Solution::UpdateDisps src/Solution.cpp:390-423,
+// Node::SaveConfiguration src/Node.cpp:325-384.
+#include "gf_internal.h"
+#include "gf_device.h"
+
+namespace {
+
+__global__ void fill_kernel(double *p, size_t n, double v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+__global__ void fill_u32_kernel(unsigned *p, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = 0u;
+}
+
+__device__ __forceinline__ double table_value(const double *tt, const double *tv, int r0, int nt, double t, int col)
+{
+    if (nt == 0) return 0.0;
+    const double *T = tt + r0;
+    const double *Vv = tv + 6 * (size_t)r0;
+    if (t >= T[nt - 1]) return Vv[6 * (nt - 1) + col];
+    if (t <= T[0]) return Vv[col];
+    int idx = 0;
+    while (t >= T[idx]) idx++;
+    double fac = (t - T[idx - 1]) / (T[idx] - T[idx - 1]);
+    return Vv[6 * (idx - 1) + col] + fac * (Vv[6 * idx + col] - Vv[6 * (idx - 1) + col]);
+}
+
+__device__ __forceinline__ size_t btd_off(int m, int R, int Cc, int &which)
+{
+    int bi = R / m, bj = Cc / m;
+    which = bj - bi + 1;
+    return ((size_t)bi * m + (R - bi * m)) * m + (Cc - bj * m);
+}
+
+__device__ __forceinline__ void kadd(double *bA, double *bB, double *bC, int m, int R, int Cc, double v)
+{
+    if (R < 0 || Cc < 0) return;
+    int w;
+    size_t off = btd_off(m, R, Cc, w);
+    double *base = w == 1 ? bB : (w == 0 ? bA : bC);
+    base[off] += v;
+}
+
+// One thread per loaded node: all loads on that node, in load order.
+__global__ void nodal_loads_kernel(int n_ln, const int *__restrict__ ln_node, const int *__restrict__ ln_off,
+                                   const int *__restrict__ ln_ids, const double *__restrict__ rec,
+                                   const double *__restrict__ tab_t, const double *__restrict__ tab_v,
+                                   const double *__restrict__ disp, const double *__restrict__ copy,
+                                   const int *__restrict__ dof, double *bA, double *bB, double *bC, int m,
+                                   double *r, double t)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_ln) return;
+    int node = ln_node[k];
+    int R[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) R[i] = dof[6 * node + i];
+    v3<double> alpha = mk3(disp[6 * node + 3], disp[6 * node + 4], disp[6 * node + 5]);
+    for (int q = ln_off[k]; q < ln_off[k + 1]; q++) {
+        const double *L = rec + (size_t)ln_ids[q] * NL_REC;
+        int kind = (int)L[NL_KIND];
+        m3<double> CS;
+#pragma unroll
+        for (int i = 0; i < 9; i++) CS.a[i] = L[NL_CS + i];
+        int r0 = (int)L[NL_TAB0], nt = (int)L[NL_TABN];
+        v3<double> f = mk3(L[NL_MULT + 0] * table_value(tab_t, tab_v, r0, nt, t, 0),
+                           L[NL_MULT + 1] * table_value(tab_t, tab_v, r0, nt, t, 1),
+                           L[NL_MULT + 2] * table_value(tab_t, tab_v, r0, nt, t, 2));
+        v3<double> mo = mk3(L[NL_MULT + 3] * table_value(tab_t, tab_v, r0, nt, t, 3),
+                            L[NL_MULT + 4] * table_value(tab_t, tab_v, r0, nt, t, 4),
+                            L[NL_MULT + 5] * table_value(tab_t, tab_v, r0, nt, t, 5));
+        if (kind == GF_LOAD_NODAL) {
+            // dead force + pseudo-moment Xi^T m with stiffness -V(alpha, Xi^T m)
+            f = mtv(CS, f);
+            mo = mtv(CS, mo);
+            m3<double> Q, Xi;
+            double g;
+            rodrigues(alpha, Q, Xi, g);
+            mo = mtv(Xi, mo);
+            m3<double> S = opV(alpha, mo);
+            double q6[6] = {f.x, f.y, f.z, mo.x, mo.y, mo.z};
+#pragma unroll
+            for (int i = 0; i < 6; i++)
+                if (R[i] >= 0) r[R[i]] += q6[i];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) kadd(bA, bB, bC, m, R[3 + i], R[3 + j], -S.a[3 * i + j]);
+        } else if (kind == GF_LOAD_NODAL_FOLLOWER) {
+            v3<double> ai = mk3(copy[6 * node + 3], copy[6 * node + 4], copy[6 * node + 5]);
+            m3<double> Qi, Xii, Q, Xi;
+            double gi, g;
+            rodrigues(ai, Qi, Xii, gi);
+            Qi = mm3(Qi, tr3(CS));
+            rodrigues(alpha, Q, Xi, g);
+            v3<double> fip = mv(mm3(Q, Qi), f);
+            v3<double> Qim = mv(Qi, mo);
+            v3<double> mip = mv(Xi, Qim);
+            m3<double> K12 = mscl(-1.0, mm3(skew3(fip), Xi));
+            m3<double> dy;
+#pragma unroll
+            for (int i = 0; i < 9; i++) dy.a[i] = 0.0;
+            add_outer(dy, 1.0, Qim, alpha);
+            m3<double> K22 = mscl(-0.5 * g, madd(skew3(Qim), mm3(Xi, dy)));
+            double q6[6] = {fip.x, fip.y, fip.z, mip.x, mip.y, mip.z};
+#pragma unroll
+            for (int i = 0; i < 6; i++)
+                if (R[i] >= 0) r[R[i]] += q6[i];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    kadd(bA, bB, bC, m, R[i], R[3 + j], -K12.a[3 * i + j]);
+                    kadd(bA, bB, bC, m, R[3 + i], R[3 + j], -K22.a[3 * i + j]);
+                }
+        }
+    }
+}
+
+// Max-norms of a vector over band rows split by DOF kind (translation /
+// rotation) plus a non-finite flag.  out (as unsigned long long bits of
+// non-negative doubles): [0] trans, [1] rot, [2] flag.
+__device__ __forceinline__ void amax(unsigned long long *p, double v)
+{
+    atomicMax(p, (unsigned long long)__double_as_longlong(v));
+}
+
+__global__ void scan_kernel(int n, const double *__restrict__ v, const int *__restrict__ row_nc,
+                            unsigned long long *out)
+{
+    double mt = 0.0, mr = 0.0;
+    int badv = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double a = fabs(v[i]);
+        if (!(a == a) || a >= 1e300) badv = 1;
+        else if (row_nc[i] % 6 < 3) mt = fmax(mt, a);
+        else mr = fmax(mr, a);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        mt = fmax(mt, __shfl_xor(mt, off));
+        mr = fmax(mr, __shfl_xor(mr, off));
+        badv |= __shfl_xor(badv, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        amax(out + 0, mt);
+        amax(out + 1, mr);
+        if (badv) atomicMax(out + 2, 1ull);
+    }
+}
+
+// d += x on free DOFs; scans |x| and the updated |d| (CheckGLConvergence).
+__global__ void update_kernel(int n, const double *__restrict__ x, const int *__restrict__ row_nc,
+                              double *__restrict__ disp, unsigned long long *out)
+{
+    double xt = 0.0, xr = 0.0, dt = 0.0, dr = 0.0;
+    int badv = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int nc = row_nc[i];
+        double xi = x[i];
+        double dn = disp[nc] + xi;
+        disp[nc] = dn;
+        double ax = fabs(xi), ad = fabs(dn);
+        if (!(ax == ax) || ax >= 1e300) badv = 1;
+        if (nc % 6 < 3) { xt = fmax(xt, ax); dt = fmax(dt, ad); }
+        else { xr = fmax(xr, ax); dr = fmax(dr, ad); }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        xt = fmax(xt, __shfl_xor(xt, off));
+        xr = fmax(xr, __shfl_xor(xr, off));
+        dt = fmax(dt, __shfl_xor(dt, off));
+        dr = fmax(dr, __shfl_xor(dr, off));
+        badv |= __shfl_xor(badv, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        amax(out + 0, xt);
+        amax(out + 1, xr);
+        amax(out + 2, dt);
+        amax(out + 3, dr);
+        if (badv) atomicMax(out + 4, 1ull);
+    }
+}
+
+// Node::SaveConfiguration: x += u, alpha <- compose(alpha_delta, alpha), Euler vector.
+__global__ void node_commit_kernel(int nn, double *__restrict__ copy, const double *__restrict__ disp,
+                                   double *__restrict__ euler)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    double *c = copy + 6 * (size_t)i;
+    const double *d = disp + 6 * (size_t)i;
+    c[0] += d[0]; c[1] += d[1]; c[2] += d[2];
+    v3<double> a3 = compose(mk3(d[3], d[4], d[5]), mk3(c[3], c[4], c[5]));
+    c[3] = a3.x; c[4] = a3.y; c[5] = a3.z;
+    double na = sqrt(dot3(a3, a3));
+    double th = 2.0 * atan(na / 2.0);
+    double s = na != 0.0 ? th / na : 0.0;
+    euler[3 * (size_t)i + 0] = s * a3.x;
+    euler[3 * (size_t)i + 1] = s * a3.y;
+    euler[3 * (size_t)i + 2] = s * a3.z;
+}
+
+// Unit diagonal for the padding rows past n_free in the last block.
+__global__ void pad_diag_kernel(double *bB, int m, int nblk, int n_free)
+{
+    int i = n_free + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nblk * m) return;
+    int bi = i / m, lr = i - bi * m;
+    bB[((size_t)bi * m + lr) * m + lr] = 1.0;
+}
+
+}  // namespace
+
+static inline int grid_for(size_t n, int block, int cap = 4096)
+{
+    size_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > (size_t)cap) g = cap;
+    return (int)g;
+}
+
+int gf_launch_fill(hipStream_t s, double *p, size_t n, double v)
+{
+    if (n == 0) return GF_OK;
+    hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, v);
+    GF_HIP(hipGetLastError());
+    return GF_OK;
+}
+
+int gf_launch_fill_bytes(hipStream_t s, void *p, size_t bytes)
+{
+    size_t n = bytes / 4;
+    if (n == 0) return GF_OK;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, (unsigned *)p, n);
+    GF_HIP(hipGetLastError());
+    return GF_OK;
+}
+
+int gf_nodal_loads(gf_model *m, double t)
+{
+    if (m->n_load_nodes == 0) return GF_OK;
+    hipLaunchKernelGGL(nodal_loads_kernel, dim3((m->n_load_nodes + 127) / 128), dim3(128), 0, m->stream,
+                       m->n_load_nodes, m->d_ln_node, m->d_ln_off, m->d_ln_ids, m->d_lrec, m->d_tab_t, m->d_tab_v,
+                       m->d_disp, m->d_copy, m->d_dof, (double *)m->btd.A, (double *)m->btd.B, (double *)m->btd.C,
+                       m->btd.m, m->d_r, t);
+    GF_HIP(hipGetLastError());
+    return GF_OK;
+}
+
+int gf_scan_residual(gf_model *m)
+{
+    int st = gf_launch_fill_bytes(m->stream, m->d_scal, 8 * sizeof(double));
+    if (st) return st;
+    hipLaunchKernelGGL(scan_kernel, dim3(grid_for(m->n_free, 256, 1024)), dim3(256), 0, m->stream, m->n_free, m->d_r,
+                       m->d_row_nc, (unsigned long long *)m->d_scal);
+    GF_HIP(hipGetLastError());
+    return GF_OK;
+}
+
+int gf_update_disp(gf_model *m)
+{
+    int st = gf_launch_fill_bytes(m->stream, m->d_scal, 8 * sizeof(double));
+    if (st) return st;
+    hipLaunchKernelGGL(update_kernel, dim3(grid_for(m->n_free, 256, 1024)), dim3(256), 0, m->stream, m->n_free,
+                       m->d_x, m->d_row_nc, m->d_disp, (unsigned long long *)m->d_scal);
+    GF_HIP(hipGetLastError());
+    return GF_OK;
+}
+
+int gf_node_commit(gf_model *m)
+{
+    hipLaunchKernelGGL(node_commit_kernel, dim3((m->nn + 255) / 256), dim3(256), 0, m->stream, m->nn, m->d_copy,
+                       m->d_disp, m->d_euler);
+    GF_HIP(hipGetLastError());
+    return GF_OK;
+}
+
+int gf_pad_diag(gf_model *m)
+{
+    int n = m->btd.nblk * m->btd.m - m->n_free;
+    if (n <= 0) return GF_OK;
+    hipLaunchKernelGGL(pad_diag_kernel, dim3((n + 63) / 64), dim3(64), 0, m->stream, (double *)m->btd.B, m->btd.m,
+                       m->btd.nblk, m->n_free);
+    GF_HIP(hipGetLastError());
+    return GF_OK;
+}

@@ -1,0 +1,313 @@
+"""Model descriptions for the static Newton-step path and seeded generators for
+the benchmark workloads.
+
+A `Model` is the resolved hot-path subset of a GIRAFFE deck (node sets
+expanded, sections and materials folded into per-element records); `desc()`
+returns the `gf_model_desc` the C ABI takes.  Generators:
+
+* `cantilever_rollup`  -- straight Beam_1 cantilever under a follower tip
+  moment ramped to 2*pi*EI/L (rolls into a full circle; Simo & Vu-Quoc 1986).
+* `catenary`           -- Beam_1 riser catenary between two pinned ends under
+  self-weight and seeded N(0, sigma) lateral nodal loads.
+* `shell_plate`        -- Shell_1 clamped square plate under follower pressure.
+
+Deck grammar these correspond to: Node/Beam_1/Shell_1/Hooke/Tube/Homogeneous/CS/
+NodalLoad/NodalFollowerLoad/ShellLoad/NodalConstraint/GravityData
+(src/IO.cpp:290-660 dispatch, readers cited in giraffe_amd/deck.py).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field, asdict
+
+import numpy as np
+
+from . import abi
+
+PI = 3.1415926535897932384626433832795  # src/Geometry.h:4
+
+
+def tube_section(De: float, Di: float):
+    """SecTube::PreCalc src/SecTube.cpp:86-106: A, I11, I22, I12, It."""
+    A = (PI / 4.0) * (De * De - Di * Di)
+    I = (PI / 64.0) * (De ** 4 - Di ** 4)
+    It = (PI / 32.0) * (De ** 4 - Di ** 4)
+    return A, I, I, 0.0, It
+
+
+def cs_rows(E1, E3):
+    """CoordinateSystem::Read src/CoordinateSystem.cpp:23-92: rows E1, E2=E3xE1, E3 (normalised)."""
+    E1 = np.asarray(E1, float)
+    E3 = np.asarray(E3, float)
+    E2 = np.cross(E3, E1)
+    n1, n2, n3 = np.linalg.norm(E1), np.linalg.norm(E2), np.linalg.norm(E3)
+    if n1 != 1.0:
+        E1 = E1 / n1
+    if n2 != 1.0:
+        E2 = E2 / n2
+    if n3 != 1.0:
+        E3 = E3 / n3
+    return np.concatenate([E1, E2, E3])
+
+
+GLOBAL_CS = cs_rows([1, 0, 0], [0, 0, 1])
+
+
+@dataclass
+class StaticParams:
+    """Static (src/Static.cpp:43-130) + ConvergenceCriteria (src/ConvergenceCriteria.cpp:64-165)."""
+    start_time: float = 0.0
+    end_time: float = 1.0
+    time_step: float = 0.1
+    max_time_step: float = 0.1
+    min_time_step: float = 1e-4
+    max_it: int = 30
+    min_it: int = 2
+    conv_increase: int = 4
+    inc_factor: float = 1.0
+    sample: int = 1
+    force_tol: float = 1e-4
+    moment_tol: float = 1e-4
+    force_min: float = 1e-5
+    moment_min: float = 1e-5
+    constraint_min: float = 1e-7
+    disp_tol: float = 1e-4
+    rot_tol: float = 1e-4
+    lag_tol: float = 1e-4
+    disp_min: float = 1e-6
+    rot_min: float = 1e-6
+    lag_min: float = 1e-6
+    divergence_ref: float = 1e15
+
+    def c(self) -> abi.GfStaticParams:
+        return abi.GfStaticParams(**asdict(self))
+
+
+class Model:
+    """Resolved model; indices are 0-based."""
+
+    def __init__(self, name: str = "model"):
+        self.name = name
+        self.xyz = []
+        self.fix = []
+        self.beam_conn, self.beam_cs, self.beam_prop = [], [], []
+        self.shell_conn, self.shell_prop = [], []
+        self.loads = []  # (kind, items, cs9, flags, times, vals[n][6])
+        self.gravity = (0.0, 0.0, 0.0)
+        self.gravity_on = 0
+        self._frozen = None
+
+    # -- building ---------------------------------------------------------
+    def add_nodes(self, xyz):
+        xyz = np.asarray(xyz, float).reshape(-1, 3)
+        i0 = len(self.xyz)
+        self.xyz.extend(xyz.tolist())
+        self.fix.extend([[0] * 6 for _ in range(len(xyz))])
+        self._frozen = None
+        return i0
+
+    def fix_dofs(self, node: int, dofs):
+        for k in dofs:
+            self.fix[node][k] = 1
+        self._frozen = None
+
+    def add_beam(self, n1, n2, n3, cs9, E, nu, rho, A, I11, I22, I12, It, T0=0.0):
+        self.beam_conn.append([n1, n2, n3])
+        self.beam_cs.append(list(cs9))
+        self.beam_prop.append([E, nu, rho, A, I11, I22, I12, It, T0])
+        self._frozen = None
+
+    def add_shell(self, nodes6, E, nu, rho, t):
+        self.shell_conn.append(list(nodes6))
+        self.shell_prop.append([E, nu, rho, t])
+        self._frozen = None
+
+    def add_load(self, kind, items, times, vals, cs9=GLOBAL_CS, flags=0):
+        vals = np.asarray(vals, float).reshape(len(times), 6)
+        self.loads.append((kind, list(items), np.asarray(cs9, float), int(flags),
+                           np.asarray(times, float), vals))
+        self._frozen = None
+
+    # -- sizes ------------------------------------------------------------
+    @property
+    def n_nodes(self):
+        return len(self.xyz)
+
+    @property
+    def n_beams(self):
+        return len(self.beam_conn)
+
+    @property
+    def n_shells(self):
+        return len(self.shell_conn)
+
+    def freeze(self):
+        if self._frozen is not None:
+            return self._frozen
+        f = {}
+        f["xyz"] = np.ascontiguousarray(np.asarray(self.xyz, float).reshape(-1, 3))
+        f["fix"] = np.ascontiguousarray(np.asarray(self.fix, np.uint8).reshape(-1, 6))
+        f["beam_conn"] = np.ascontiguousarray(np.asarray(self.beam_conn, np.int32).reshape(-1, 3))
+        f["beam_cs"] = np.ascontiguousarray(np.asarray(self.beam_cs, float).reshape(-1, 9))
+        f["beam_prop"] = np.ascontiguousarray(np.asarray(self.beam_prop, float).reshape(-1, abi.GF_BEAM_NPROP))
+        f["shell_conn"] = np.ascontiguousarray(np.asarray(self.shell_conn, np.int32).reshape(-1, 6))
+        f["shell_prop"] = np.ascontiguousarray(np.asarray(self.shell_prop, float).reshape(-1, abi.GF_SHELL_NPROP))
+        nl = len(self.loads)
+        f["load_kind"] = np.asarray([l[0] for l in self.loads], np.int32)
+        off = [0]
+        items = []
+        toff = [0]
+        tt, tv = [], []
+        for l in self.loads:
+            items.extend(l[1])
+            off.append(len(items))
+            tt.extend(l[4].tolist())
+            tv.append(l[5])
+            toff.append(len(tt))
+        f["load_set_off"] = np.asarray(off, np.int32)
+        f["load_set_items"] = np.asarray(items if items else [0], np.int32)
+        f["load_cs"] = np.ascontiguousarray(np.asarray([l[2] for l in self.loads], float).reshape(-1, 9)) if nl else np.zeros((1, 9))
+        f["load_flags"] = np.asarray([l[3] for l in self.loads] if nl else [0], np.int32)
+        f["load_tab_off"] = np.asarray(toff, np.int32)
+        f["tab_time"] = np.asarray(tt if tt else [0.0], float)
+        f["tab_val"] = np.ascontiguousarray(np.concatenate(tv, 0)) if tv else np.zeros((1, 6))
+        self._frozen = f
+        return f
+
+    def desc(self) -> abi.GfModelDesc:
+        f = self.freeze()
+        d = abi.GfModelDesc()
+        d.n_nodes = self.n_nodes
+        d.node_xyz = abi.ptr(f["xyz"], C.c_double)
+        d.node_fix = abi.ptr(f["fix"], C.c_ubyte)
+        d.n_beams = self.n_beams
+        d.beam_conn = abi.ptr(f["beam_conn"], C.c_int)
+        d.beam_cs = abi.ptr(f["beam_cs"], C.c_double)
+        d.beam_prop = abi.ptr(f["beam_prop"], C.c_double)
+        d.n_shells = self.n_shells
+        d.shell_conn = abi.ptr(f["shell_conn"], C.c_int)
+        d.shell_prop = abi.ptr(f["shell_prop"], C.c_double)
+        d.n_loads = len(self.loads)
+        d.load_kind = abi.ptr(f["load_kind"], C.c_int)
+        d.load_set_off = abi.ptr(f["load_set_off"], C.c_int)
+        d.load_set_items = abi.ptr(f["load_set_items"], C.c_int)
+        d.load_cs = abi.ptr(f["load_cs"], C.c_double)
+        d.load_flags = abi.ptr(f["load_flags"], C.c_int)
+        d.load_tab_off = abi.ptr(f["load_tab_off"], C.c_int)
+        d.tab_time = abi.ptr(f["tab_time"], C.c_double)
+        d.tab_val = abi.ptr(f["tab_val"], C.c_double)
+        d.gravity = (C.c_double * 3)(*self.gravity)
+        d.gravity_on = int(self.gravity_on)
+        d._keep = f  # keep arrays alive with the struct
+        return d
+
+
+# ---------------------------------------------------------------------------
+# Generators
+# ---------------------------------------------------------------------------
+
+def cantilever_rollup(n_elem=100, L=10.0, De=0.2, Di=0.18, E=210e9, nu=0.3, rho=7850.0,
+                      n_steps=20, turns=1.0, tol=1e-10):
+    """Cantilever along x, clamped at node 0, follower moment about z at the tip
+    ramped linearly to turns*2*pi*EI/L over n_steps load steps."""
+    m = Model("cantilever_rollup")
+    nn = 2 * n_elem + 1
+    xs = np.linspace(0.0, L, nn)
+    m.add_nodes(np.stack([xs, np.zeros(nn), np.zeros(nn)], 1))
+    A, I11, I22, I12, It = tube_section(De, Di)
+    cs = cs_rows([0, 1, 0], [1, 0, 0])
+    for e in range(n_elem):
+        m.add_beam(2 * e, 2 * e + 1, 2 * e + 2, cs, E, nu, rho, A, I11, I22, I12, It)
+    m.fix_dofs(0, range(6))
+    EI = E * I22
+    M = turns * 2.0 * PI * EI / L
+    m.add_load(abi.GF_LOAD_NODAL_FOLLOWER, [nn - 1], [0.0, 1.0],
+               [[0, 0, 0, 0, 0, 0], [0, 0, 0, 0, 0, M]], GLOBAL_CS)
+    dt = 1.0 / n_steps
+    p = StaticParams(end_time=1.0, time_step=dt, max_time_step=dt, min_time_step=dt / 1024.0,
+                     max_it=30, min_it=2, conv_increase=4, inc_factor=1.0,
+                     force_tol=tol, moment_tol=tol, disp_tol=tol, rot_tol=tol,
+                     force_min=M / L, moment_min=M, disp_min=1e-6, rot_min=1e-6)
+    info = {"EI": EI, "M": M, "L": L, "tip": nn - 1}
+    return m, p, info
+
+
+def catenary_shape(span_h, span_v, arc_ratio):
+    """Catenary z = a cosh((x - x0)/a) + c through (0,0) and (span_h, span_v) with
+    arc length arc_ratio * chord.  Returns (a, x0, c, S)."""
+    chord = math.hypot(span_h, span_v)
+    S = arc_ratio * chord
+    target = math.sqrt(S * S - span_v * span_v)
+    lo, hi = 1e-3 * span_h, 1e6 * span_h
+    for _ in range(200):  # 2 a sinh(span_h / 2a) = target, decreasing in a
+        mid = math.sqrt(lo * hi)
+        val = 2 * mid * math.sinh(min(span_h / (2 * mid), 700.0))
+        if val > target:
+            lo = mid
+        else:
+            hi = mid
+    a = math.sqrt(lo * hi)
+    # z(span_h) - z(0) = span_v fixes x0
+    # a[cosh((D-x0)/a) - cosh(-x0/a)] = span_v -> 2a sinh(D/2a) sinh((D/2 - x0)/a) = span_v
+    s = span_v / (2 * a * math.sinh(span_h / (2 * a)))
+    x0 = span_h / 2 - a * math.asinh(s)
+    c = -a * math.cosh(-x0 / a)
+    return a, x0, c, S
+
+
+def catenary(n_elem=10000, span_h=1500.0, span_v=1000.0, arc_ratio=1.15, De=0.3, Di=0.26,
+             E=210e9, nu=0.3, w=1500.0, g=9.81, sigma=500.0, seed=1002, n_steps=10,
+             lateral_start=0.3, tol=1e-10):
+    """Beam_1 riser catenary hung between pins at (0,0,0) and (span_h,0,span_v).
+
+    Geometry: nodes equally spaced in arc length on the catenary, each element
+    straight (its middle node at the midpoint of nodes 1 and 3).  Self-weight w
+    [N/m] through rho = w/(g A), ramped 0->1 over the solution (GravityData
+    BoolTable).  Lateral (global y) nodal loads N(0, sigma) per node, seeded,
+    ramped from t = lateral_start to 1 (the stress-free first steps carry
+    gravity only).  Pins: UX UY UZ at both ends, plus ROTX (rotation about the
+    horizontal axis of the catenary plane) at both ends, which removes the
+    zero-stiffness rigid swing about the chord and the free end twist.
+    """
+    m = Model("catenary")
+    a, x0, c, S = catenary_shape(span_h, span_v, arc_ratio)
+    # arc length s(x) = a [sinh((x-x0)/a) - sinh(-x0/a)] -> x(s)
+    sh0 = math.sinh(-x0 / a)
+    ne = n_elem
+    s_end = np.linspace(0.0, S, ne + 1)
+    xe = x0 + a * np.arcsinh(s_end / a + sh0)
+    ze = a * np.cosh((xe - x0) / a) + c
+    ze[0], ze[-1] = 0.0, span_v
+    xe[0], xe[-1] = 0.0, span_h
+    nn = 2 * ne + 1
+    xyz = np.zeros((nn, 3))
+    xyz[0::2, 0] = xe
+    xyz[0::2, 2] = ze
+    xyz[1::2, 0] = 0.5 * (xe[:-1] + xe[1:])
+    xyz[1::2, 2] = 0.5 * (ze[:-1] + ze[1:])
+    m.add_nodes(xyz)
+    A, I11, I22, I12, It = tube_section(De, Di)
+    rho = w / (g * A)
+    for e in range(ne):
+        p1, p3 = xyz[2 * e], xyz[2 * e + 2]
+        e3 = p3 - p1
+        cs = cs_rows([0, 1, 0], e3 / np.linalg.norm(e3))
+        m.add_beam(2 * e, 2 * e + 1, 2 * e + 2, cs, E, nu, rho, A, I11, I22, I12, It)
+    m.fix_dofs(0, [0, 1, 2, 3])
+    m.fix_dofs(nn - 1, [0, 1, 2, 3])
+    m.gravity = (0.0, 0.0, -g)
+    m.gravity_on = 1
+    rng = np.random.default_rng(seed)
+    fy = rng.normal(0.0, sigma, nn)
+    for i in range(nn):
+        m.add_load(abi.GF_LOAD_NODAL, [i], [0.0, lateral_start, 1.0],
+                   [[0] * 6, [0] * 6, [0, fy[i], 0, 0, 0, 0]], GLOBAL_CS)
+    dt = 1.0 / n_steps
+    Wtot = w * S
+    p = StaticParams(end_time=1.0, time_step=dt, max_time_step=dt, min_time_step=dt / 1024.0,
+                     max_it=40, min_it=2, conv_increase=4, inc_factor=1.0,
+                     force_tol=tol, moment_tol=tol, disp_tol=tol, rot_tol=tol,
+                     force_min=Wtot / 10.0, moment_min=Wtot, disp_min=1e-6, rot_min=1e-6)
+    info = {"a": a, "S": S, "rho": rho, "A": A, "n_free_expected": 6 * nn - 8}
+    return m, p, info

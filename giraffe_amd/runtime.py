@@ -1,0 +1,193 @@
+"""Loader and thin Python binding of libgiraffe_amd.so (the HIP compute path).
+
+There is no CPU fallback: if the library is missing, or no GPU is visible,
+every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgiraffe_amd.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "giraffe_amd.h")
+_lib = None
+
+
+class GiraffeError(RuntimeError):
+    pass
+
+
+def header_symbols():
+    """Function names declared in include/giraffe_amd.h."""
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*|void)\s*\*?\s*(gf_\w+)\s*\(", txt, flags=re.M)))
+
+
+def load():
+    """Load libgiraffe_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GiraffeError(f"{LIB_PATH} is missing: run __graft_entry__.build() (make -C giraffe_amd/csrc)")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    dp = C.POINTER(C.c_double)
+    L.gf_last_error.restype = C.c_char_p
+    L.gf_version.restype = C.c_int
+    L.gf_default_options.argtypes = [C.POINTER(abi.GfOptions)]
+    L.gf_model_create.argtypes = [C.POINTER(abi.GfModelDesc), C.POINTER(abi.GfOptions), C.POINTER(vp)]
+    L.gf_model_destroy.argtypes = [vp]
+    L.gf_model_info.argtypes = [vp, C.POINTER(C.c_longlong)]
+    L.gf_set_solution_window.argtypes = [vp, C.c_double, C.c_double]
+    L.gf_begin_step.argtypes = [vp, C.c_double, C.c_double]
+    L.gf_assemble.argtypes = [vp, dp]
+    L.gf_solve.argtypes = [vp, C.POINTER(C.c_int)]
+    L.gf_update.argtypes = [vp, dp]
+    L.gf_commit.argtypes = [vp]
+    L.gf_rollback.argtypes = [vp]
+    L.gf_static_solve.argtypes = [vp, C.POINTER(abi.GfStaticParams), C.POINTER(abi.GfStaticReport)]
+    for f in ("gf_read_nodes", "gf_read_disp", "gf_read_residual", "gf_read_increment", "gf_write_disp"):
+        getattr(L, f).argtypes = [vp, dp]
+    L.gf_read_tangent_dense.argtypes = [vp, dp, C.c_longlong]
+    L.gf_beam_eval.argtypes = [C.c_int, dp, dp, dp, dp, dp, dp, dp]
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def check(status, what=""):
+    if status != abi.GF_OK:
+        msg = load().gf_last_error().decode(errors="replace")
+        raise GiraffeError(f"{what} failed with status {status}: {msg}")
+
+
+def options(precision=abi.GF_PREC_FP64, device=0, refine_max=8, refine_tol=1e-14):
+    o = abi.GfOptions()
+    load().gf_default_options(C.byref(o))
+    o.precision = precision
+    o.device = device
+    o.refine_max = refine_max
+    o.refine_tol = refine_tol
+    return o
+
+
+class DeviceModel:
+    """One model resident on the GPU; mirrors the gf_* entry points."""
+
+    def __init__(self, model, precision=abi.GF_PREC_FP64, device=0):
+        L = load()
+        self.model = model
+        self._desc = model.desc()
+        self._opt = options(precision, device)
+        h = C.c_void_p()
+        check(L.gf_model_create(C.byref(self._desc), C.byref(self._opt), C.byref(h)), "gf_model_create")
+        self.h = h
+        self.nn = model.n_nodes
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().gf_model_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        o = (C.c_longlong * 8)()
+        check(load().gf_model_info(self.h, o), "gf_model_info")
+        keys = ["n_free", "half_band", "block", "n_blocks", "colors", "factor_bytes", "levels", "precision"]
+        return dict(zip(keys, list(o)))
+
+    def set_window(self, start, end):
+        check(load().gf_set_solution_window(self.h, start, end), "gf_set_solution_window")
+
+    def begin_step(self, t_last, dt):
+        check(load().gf_begin_step(self.h, t_last, dt), "gf_begin_step")
+
+    def assemble(self):
+        r = np.zeros(3)
+        check(load().gf_assemble(self.h, _dp(r)), "gf_assemble")
+        return r
+
+    def solve(self):
+        f = C.c_int(0)
+        check(load().gf_solve(self.h, C.byref(f)), "gf_solve")
+        return f.value
+
+    def update(self):
+        r = np.zeros(5)
+        check(load().gf_update(self.h, _dp(r)), "gf_update")
+        return r
+
+    def commit(self):
+        check(load().gf_commit(self.h), "gf_commit")
+
+    def rollback(self):
+        check(load().gf_rollback(self.h), "gf_rollback")
+
+    def _read(self, fn, shape):
+        out = np.zeros(shape)
+        check(getattr(load(), fn)(self.h, _dp(out)), fn)
+        return out
+
+    def nodes(self):
+        return self._read("gf_read_nodes", (self.nn, 6))
+
+    def disp(self):
+        return self._read("gf_read_disp", (self.nn, 6))
+
+    def residual(self):
+        return self._read("gf_read_residual", (self.nn, 6))
+
+    def increment(self):
+        return self._read("gf_read_increment", (self.nn, 6))
+
+    def write_disp(self, d):
+        d = np.ascontiguousarray(d, float).reshape(self.nn, 6)
+        check(load().gf_write_disp(self.h, _dp(d)), "gf_write_disp")
+
+    def tangent_dense(self):
+        n6 = 6 * self.nn
+        out = np.zeros((n6, n6))
+        check(load().gf_read_tangent_dense(self.h, _dp(out), out.size), "gf_read_tangent_dense")
+        return out
+
+    def static_solve(self, params, cap=4096):
+        iters = np.zeros(cap, np.int32)
+        rep = abi.GfStaticReport()
+        rep.step_iters = iters.ctypes.data_as(C.POINTER(C.c_int))
+        rep.step_iters_cap = cap
+        pc = params.c()
+        check(load().gf_static_solve(self.h, C.byref(pc), C.byref(rep)), "gf_static_solve")
+        n = min(rep.n_step_iters, cap)
+        return rep, iters[:n].copy()
+
+
+def beam_eval(x, cs, prop, d, gp):
+    """Element-level Beam_1::Mount on the device for independent elements.
+
+    x [n,9], cs [n,9], prop [n,9], d [n,18], gp [n,2,15] -> K [n,18,18], f [n,18]."""
+    x = np.ascontiguousarray(x, float)
+    n = x.shape[0]
+    cs = np.ascontiguousarray(cs, float)
+    prop = np.ascontiguousarray(prop, float)
+    d = np.ascontiguousarray(d, float)
+    gp = np.ascontiguousarray(gp, float)
+    K = np.zeros((n, 18, 18))
+    f = np.zeros((n, 18))
+    check(load().gf_beam_eval(n, _dp(x), _dp(cs), _dp(prop), _dp(d), _dp(gp), _dp(K), _dp(f)), "gf_beam_eval")
+    return K, f

@@ -1,0 +1,194 @@
+/*
+ * giraffe_amd.h -- C ABI of the MI355X-native Newton-step hot path of GIRAFFE's
+ * static analysis (Beam_1 / Shell_1 element evaluation, global scatter, tangent
+ * solve, Newton update / commit / rollback and the Static load-step loop).
+ *
+ * Plain C types only: pointers, sizes, doubles, ints.  Every entry point returns
+ * an int status (GF_OK == 0, negative on error) and never throws.  Host arrays
+ * passed in are borrowed for the duration of the call only.
+ *
+ * Reference interfaces each group replaces (GIRAFFE, alfredogneto/GIRAFFE):
+ *   model set-up      Database::PreCalc src/Database.cpp:704-757,
+ *                     Solution::DOFsActive / SetGlobalDOFs src/Solution.cpp:121-224, 40-118
+ *   gf_begin_step     Solution::Zeros src/Solution.cpp:798-814 and the load-time
+ *                     bookkeeping of Static::Solve src/Static.cpp:178-191
+ *   gf_assemble       Solution::Clear / MountLocal / MountElementLoads / MountLoads /
+ *                     MountGlobal / MountSparse src/Solution.cpp:227-387, 833-863,
+ *                     Element::Mount / MountGlobal src/Element.h:53-76,
+ *                     the sign flip src/Static.cpp:210 and the max-norm scans of
+ *                     ConvergenceCriteria::EstablishResidualCriteria /
+ *                     CheckResidualConvergence src/ConvergenceCriteria.cpp:187-288, 460-598
+ *   gf_solve          Matrix sparsesystem(SparseMatrix&, Matrix&, int* info_fail, int, int)
+ *                     src/SparseMatrix.h:28, src/SparseMatrix.cpp:74-166
+ *   gf_update         Solution::UpdateDisps src/Solution.cpp:390-423 and the scans of
+ *                     ConvergenceCriteria::CheckGLConvergence src/ConvergenceCriteria.cpp:289-459
+ *   gf_commit         Solution::SaveConfiguration src/Solution.cpp:426-529
+ *                     (Node::SaveConfiguration src/Node.cpp:325-384,
+ *                      Beam_1::SaveLagrange src/Beam_1.cpp:1494-1506,
+ *                      Shell_1::SaveLagrange src/Shell_1.cpp:1650-1664)
+ *   gf_rollback       Solution::RestoreConfiguration src/Solution.cpp:532-574
+ *   gf_static_solve   bool Static::Solve() src/Static.cpp:140-311
+ */
+#ifndef GIRAFFE_AMD_H
+#define GIRAFFE_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define GF_OK            0
+#define GF_E_ARG        -1   /* bad argument / inconsistent model description   */
+#define GF_E_DEVICE     -2   /* HIP runtime error                               */
+#define GF_E_NOMEM      -3   /* device or host allocation failed                */
+#define GF_E_SINGULAR   -4   /* zero or non-finite pivot (sparsesystem info_fail) */
+#define GF_E_STATE      -5   /* call out of sequence                            */
+#define GF_E_UNSUPPORTED -6  /* feature not built                               */
+#define GF_E_CAPACITY   -7   /* request exceeds a fixed capacity                */
+
+/* ---- precision of the tangent path -------------------------------------- */
+#define GF_PREC_FP64   0     /* tangent assembled and factored in FP64          */
+#define GF_PREC_MIXED  1     /* FP32 tangent + FP32 factor, FP64 refinement     */
+
+/* ---- load kinds (src/NodalLoad.cpp, src/NodalFollowerLoad.cpp, src/ShellLoad.cpp) */
+#define GF_LOAD_NODAL           0  /* dead force + pseudo-moment (NodalLoad)      */
+#define GF_LOAD_NODAL_FOLLOWER  1  /* follower force / moment (NodalFollowerLoad) */
+#define GF_LOAD_SHELL_PRESSURE  2  /* follower pressure on Shell_1 (ShellLoad)    */
+
+/* Per-element property records. */
+#define GF_BEAM_NPROP   9   /* E, nu, rho, A, I11, I22, I12, It, T0 (pre-tension) */
+#define GF_SHELL_NPROP  4   /* E, nu, rho, thickness                              */
+
+/* Model description: the hot-path subset of a GIRAFFE deck, already resolved
+ * (node sets expanded, sections and materials folded into per-element records).
+ * Node and element indices are 0-based. */
+typedef struct gf_model_desc {
+    int n_nodes;
+    const double *node_xyz;          /* [n_nodes][3] reference coordinates          */
+    const unsigned char *node_fix;   /* [n_nodes][6] 1 = DOF prescribed to zero     */
+
+    int n_beams;
+    const int *beam_conn;            /* [n_beams][3] nodes 1,2,3 of Beam_1            */
+    const double *beam_cs;           /* [n_beams][9] element CS rows E1,E2,E3 (unit)  */
+    const double *beam_prop;         /* [n_beams][GF_BEAM_NPROP]                      */
+
+    int n_shells;
+    const int *shell_conn;           /* [n_shells][6] corners 1-3, mid-sides 4-6      */
+    const double *shell_prop;        /* [n_shells][GF_SHELL_NPROP]                    */
+
+    int n_loads;
+    const int *load_kind;            /* [n_loads] GF_LOAD_*                           */
+    const int *load_set_off;         /* [n_loads+1] offsets into load_set_items       */
+    const int *load_set_items;       /* node ids (nodal) or shell element ids (pressure) */
+    const double *load_cs;           /* [n_loads][9] CS rows E1,E2,E3 of the load     */
+    const int *load_flags;           /* [n_loads] ShellLoad AreaUpdate (0/1)          */
+    const int *load_tab_off;         /* [n_loads+1] row offsets into tab_time/tab_val */
+    const double *tab_time;          /* [rows] table instants                         */
+    const double *tab_val;           /* [rows][6] FX FY FZ MX MY MZ (pressure: col 0) */
+
+    double gravity[3];               /* GravityData G                                 */
+    int gravity_on;                  /* 1: BoolTable ramp 0->1 over the solution     */
+} gf_model_desc;
+
+/* Static solution-step and ConvergenceCriteria parameters, in the order of
+ * Static::Read src/Static.cpp:43-130 and ConvergenceCriteria::Read
+ * src/ConvergenceCriteria.cpp:64-165. */
+typedef struct gf_static_params {
+    double start_time, end_time, time_step, max_time_step, min_time_step;
+    int max_it, min_it, conv_increase;
+    double inc_factor;
+    int sample;
+    double force_tol, moment_tol, force_min, moment_min, constraint_min;
+    double disp_tol, rot_tol, lag_tol, disp_min, rot_min, lag_min;
+    double divergence_ref;
+} gf_static_params;
+
+/* Outcome of one Static::Solve.  step_iters (optional, caller-owned) receives
+ * the Newton iteration count of every attempted load step, negated for steps
+ * that diverged and were bisected. */
+typedef struct gf_static_report {
+    int aborted;                 /* Static::Solve returned false                  */
+    int converged_steps;
+    int diverged_steps;
+    int total_iterations;        /* Newton iterations (assemblies)                */
+    int total_solves;            /* tangent solves                                */
+    double final_time;
+    double max_rel_residual;     /* max over converged steps of the final
+                                    max(|r_f|/ref_f, |r_m|/ref_m)                 */
+    int *step_iters;             /* [step_iters_cap] or NULL                      */
+    int step_iters_cap;
+    int n_step_iters;            /* entries written                               */
+    double t_assemble_ms, t_solve_ms, t_total_ms;
+} gf_static_report;
+
+/* Device options for a model. */
+typedef struct gf_options {
+    int precision;               /* GF_PREC_FP64 or GF_PREC_MIXED                 */
+    int device;                  /* HIP device ordinal                            */
+    void *stream;                /* hipStream_t to run on, or NULL for a new one  */
+    int refine_max;              /* max FP64 refinement sweeps per solve (mixed)  */
+    double refine_tol;           /* target ||r - K x||_inf / ||r||_inf            */
+} gf_options;
+
+typedef struct gf_model gf_model;
+
+/* ---- library ------------------------------------------------------------- */
+int gf_version(void);                                    /* 100*major + minor  */
+const char *gf_last_error(void);                          /* thread-local text */
+int gf_default_options(gf_options *opt);
+
+/* ---- model lifetime: Database::PreCalc + DOFsActive + SetGlobalDOFs ------- */
+int gf_model_create(const gf_model_desc *desc, const gf_options *opt, gf_model **out);
+int gf_model_destroy(gf_model *m);
+/* n_free, half-bandwidth, solver block size, colours, bytes of the factor. */
+int gf_model_info(const gf_model *m, long long *info_out /* [8] */);
+
+/* ---- Newton step pieces --------------------------------------------------- */
+/* Time window of the current solution step; the gravity BoolTable ramp
+ * (BoolTable::GetLinearFactorAtCurrentTime src/BoolTable.cpp:84-109) reads it.
+ * gf_static_solve sets it from its parameters. */
+int gf_set_solution_window(gf_model *m, double start_time, double end_time);
+/* New load step: incremental displacements := 0, load time = t_last + dt. */
+int gf_begin_step(gf_model *m, double t_last, double dt);
+/* Assemble K and r = f_ext - f_int at the current state; res_out[0..2] =
+ * max|r| over free translational DOFs, over free rotational DOFs, and 1.0 if
+ * any entry is non-finite (>=1e300 or NaN) else 0.0. */
+int gf_assemble(gf_model *m, double *res_out /* [3] */);
+/* x = K^-1 r.  *info_fail = 1 when the factorisation meets a zero or non-finite
+ * pivot (the caller then bisects), as sparsesystem reports PARDISO errors. */
+int gf_solve(gf_model *m, int *info_fail);
+/* d += x; inc_out = max|x_u|, max|x_rot|, max|d_u|, max|d_rot|, non-finite flag. */
+int gf_update(gf_model *m, double *inc_out /* [5] */);
+int gf_commit(gf_model *m);
+int gf_rollback(gf_model *m);
+
+/* Whole Static::Solve loop (adaptive steps, bisection, convergence criteria). */
+int gf_static_solve(gf_model *m, const gf_static_params *p, gf_static_report *rep);
+
+/* ---- results --------------------------------------------------------------- */
+/* Per node: X Y Z of the last converged configuration and the Euler rotation
+ * vector (Node::WriteResults src/Node.cpp:208-219). */
+int gf_read_nodes(gf_model *m, double *out /* [n_nodes][6] */);
+/* Per node and DOF: incremental displacement d, or residual r (0 where not free). */
+int gf_read_disp(gf_model *m, double *out /* [n_nodes][6] */);
+int gf_read_residual(gf_model *m, double *out /* [n_nodes][6] */);
+/* Last solve's increment x per node and DOF (0 where not free). */
+int gf_read_increment(gf_model *m, double *out /* [n_nodes][6] */);
+/* Overwrite the incremental displacements (testing of assembly at chosen states). */
+int gf_write_disp(gf_model *m, const double *in /* [n_nodes][6] */);
+/* Dense copy of the free-free tangent K_AA in node-DOF space for small models:
+ * out[(6*i+a)*(6*n_nodes) + 6*j+b]; entries of non-free DOFs are 0. */
+int gf_read_tangent_dense(gf_model *m, double *out, long long cap);
+
+/* ---- element-level parity entry (Beam_1::Mount on the device) -------------
+ * For `count` independent elements: x[9] node coordinates of nodes 1,2,3,
+ * cs[9], prop[GF_BEAM_NPROP], d[18] incremental node displacements (global),
+ * gp[2][15] Gauss-point state (Q_i row-major 9, kappa_i 3, z'_i 3).
+ * Outputs the global-frame 18x18 tangent (row-major) and internal force 18. */
+int gf_beam_eval(int count, const double *x, const double *cs, const double *prop,
+                 const double *d, const double *gp, double *K_out, double *f_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GIRAFFE_AMD_H */

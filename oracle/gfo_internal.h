@@ -1,0 +1,96 @@
+/*
+ * ORACLE -- test infrastructure only.  A plain-C FP64 restatement of GIRAFFE's
+ * static Newton-step path, used by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py as the checker.  Nothing in the product
+ * (giraffe_amd/) links, loads or calls it.
+ *
+ * Each routine cites the reference file:line it restates.  See the per-file
+ * headers for which functions are pinned by which independent check.
+ */
+#ifndef GFO_INTERNAL_H
+#define GFO_INTERNAL_H
+
+#include <stddef.h>
+#include "../include/giraffe_amd.h"
+
+/* ---- 3x3 helpers (row-major double[9]) ---------------------------------- */
+static inline void m3_zero(double *a) { for (int i = 0; i < 9; i++) a[i] = 0.0; }
+static inline void m3_eye(double *a) { m3_zero(a); a[0] = a[4] = a[8] = 1.0; }
+static inline void m3_mul(const double *a, const double *b, double *c) {
+    double t[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            t[3 * i + j] = a[3 * i] * b[j] + a[3 * i + 1] * b[3 + j] + a[3 * i + 2] * b[6 + j];
+    for (int i = 0; i < 9; i++) c[i] = t[i];
+}
+static inline void m3_tr(const double *a, double *c) {
+    double t[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) t[3 * i + j] = a[3 * j + i];
+    for (int i = 0; i < 9; i++) c[i] = t[i];
+}
+static inline void m3_vec(const double *a, const double *x, double *y) {
+    double t0 = a[0] * x[0] + a[1] * x[1] + a[2] * x[2];
+    double t1 = a[3] * x[0] + a[4] * x[1] + a[5] * x[2];
+    double t2 = a[6] * x[0] + a[7] * x[1] + a[8] * x[2];
+    y[0] = t0; y[1] = t1; y[2] = t2;
+}
+static inline void m3_trvec(const double *a, const double *x, double *y) {
+    double t0 = a[0] * x[0] + a[3] * x[1] + a[6] * x[2];
+    double t1 = a[1] * x[0] + a[4] * x[1] + a[7] * x[2];
+    double t2 = a[2] * x[0] + a[5] * x[1] + a[8] * x[2];
+    y[0] = t0; y[1] = t1; y[2] = t2;
+}
+static inline void m3_skew(const double *v, double *a) {
+    /* skew src/Matrix.cpp:1812-1833 */
+    a[0] = 0.0;   a[1] = -v[2]; a[2] = v[1];
+    a[3] = v[2];  a[4] = 0.0;   a[5] = -v[0];
+    a[6] = -v[1]; a[7] = v[0];  a[8] = 0.0;
+}
+static inline double v3_dot(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static inline void v3_cross(const double *a, const double *b, double *c) {
+    double t0 = a[1] * b[2] - a[2] * b[1];
+    double t1 = a[2] * b[0] - a[0] * b[2];
+    double t2 = a[0] * b[1] - a[1] * b[0];
+    c[0] = t0; c[1] = t1; c[2] = t2;
+}
+
+/* Rodrigues rotation Q(a) = I + g (A + A^2/2), g = 4/(4+|a|^2), and
+ * Xi(a) = g (I + A/2) (src/Beam_1.cpp:748-753). */
+void gfo_rodrigues(const double *a, double *Q, double *Xi, double *g_out);
+/* V and d_V operators src/Matrix.cpp:1999-2033 (h3 = h5 = h7 = 0). */
+void gfo_V(const double *x, const double *t, double *out);
+void gfo_dV(const double *x, const double *dx, const double *t, double *out);
+/* Rodrigues composition src/Node.cpp:342: parameters of Q(a2) Q(a1). */
+void gfo_compose(const double *a2, const double *a1, double *a3);
+
+/* ---- Beam_1 ----------------------------------------------------------------- */
+#define GFO_BEAM_GPS 21   /* per Gauss point: Q_i 9, kappa_i 3, z'_i 3, u_i 3, alpha_i 3 */
+
+typedef struct gfo_beam_const {
+    double T3[9];          /* transform3 rows E1,E2,E3                      */
+    double D[36];          /* 6x6 constitutive matrix (row-major)           */
+    double e3r[3];         /* reference axis in the local frame             */
+    double jac;            /* jacobian = L/2                                */
+    double N[2][3], dN[2][3];
+    double rhoA;           /* rho * A for self-weight                       */
+} gfo_beam_const;
+
+void gfo_beam_precalc(const double *x /*[9]*/, const double *cs, const double *prop,
+                      gfo_beam_const *c, double *gp /*[2][GFO_BEAM_GPS]*/);
+/* Beam_1::Mount: K (18x18 row-major, global) and f_int (18, global). */
+void gfo_beam_mount(const gfo_beam_const *c, const double *gp, const double *d /*[18]*/,
+                    double *K, double *f);
+/* Beam_1::SaveLagrange with the kinematics of the current d. */
+void gfo_beam_save(const gfo_beam_const *c, double *gp, const double *d);
+
+/* ---- Shell_1 ---------------------------------------------------------------- */
+#define GFO_SHELL_GPS 27  /* per Gauss point: Q_i 9, alpha_i 3, u_i 3, kr1 3, kr2 3, zx1 3, zx2 3 */
+typedef struct gfo_shell_const gfo_shell_const;
+
+/* ---- band LU with partial pivoting (LAPACK gbtrf/gbtrs restatement) -------- */
+/* ab: column-major band storage, ldab = 2*kl+ku+1, A(i,j) = ab[kl+ku+i-j + j*ldab]. */
+int gfo_gbtrf(int n, int kl, int ku, double *ab, int ldab, int *ipiv);
+void gfo_gbtrs(int n, int kl, int ku, const double *ab, int ldab, const int *ipiv, double *b);
+
+#endif

@@ -1,0 +1,273 @@
+"""ORACLE -- test infrastructure only.
+
+ctypes wrapper of liboracle.so, the plain-C FP64 restatement of GIRAFFE's static
+Newton-step path.  Imported only by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker; the product never loads it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from giraffe_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+SOLVER_BAND = 0
+SOLVER_PARDISO = 1
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB):
+        subprocess.check_call(["make", "-s", "-C", HERE, "-B" if force else "liboracle.so"])
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        vp = C.c_void_p
+        dp = C.POINTER(C.c_double)
+        L.gfo_create.restype = vp
+        L.gfo_create.argtypes = [C.POINTER(abi.GfModelDesc), C.c_int]
+        L.gfo_destroy.argtypes = [vp]
+        L.gfo_begin_solution.argtypes = [vp, C.c_double, C.c_double]
+        L.gfo_begin_step.argtypes = [vp, C.c_double, C.c_double]
+        L.gfo_assemble.argtypes = [vp, dp]
+        L.gfo_solve.argtypes = [vp, C.POINTER(C.c_int)]
+        L.gfo_update.argtypes = [vp, dp]
+        L.gfo_commit.argtypes = [vp]
+        L.gfo_rollback.argtypes = [vp]
+        for f in ("gfo_read_nodes", "gfo_read_copy", "gfo_read_disp", "gfo_read_increment",
+                  "gfo_read_residual", "gfo_read_gp_beam", "gfo_read_gp_shell"):
+            getattr(L, f).argtypes = [vp, dp]
+            getattr(L, f).restype = None
+        L.gfo_write_disp.argtypes = [vp, dp]
+        L.gfo_read_tangent_dense.argtypes = [vp, dp]
+        L.gfo_read_element_beam.argtypes = [vp, C.c_int, dp, dp]
+        L.gfo_read_element_shell.argtypes = [vp, C.c_int, dp, dp]
+        L.gfo_static_solve.argtypes = [vp, C.POINTER(abi.GfStaticParams), C.POINTER(abi.GfStaticReport)]
+        L.gfo_n_free.argtypes = [vp]
+        L.gfo_bandwidth.argtypes = [vp]
+        L.gfo_dof_map.argtypes = [vp, C.POINTER(C.c_int)]
+        L.gfo_pardiso_available.restype = C.c_int
+        L.gfo_gbtrf.argtypes = [C.c_int, C.c_int, C.c_int, dp, C.c_int, C.POINTER(C.c_int)]
+        L.gfo_gbtrs.argtypes = [C.c_int, C.c_int, C.c_int, dp, C.c_int, C.POINTER(C.c_int), dp]
+        L.gfo_rodrigues.argtypes = [dp, dp, dp, dp]
+        L.gfo_V.argtypes = [dp, dp, dp]
+        L.gfo_dV.argtypes = [dp, dp, dp, dp]
+        L.gfo_compose.argtypes = [dp, dp, dp]
+        L.gfo_beam_precalc.argtypes = [dp, dp, dp, C.c_void_p, dp]
+        L.gfo_beam_mount.argtypes = [C.c_void_p, dp, dp, dp, dp]
+        L.gfo_beam_save.argtypes = [C.c_void_p, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def pardiso_available() -> bool:
+    return bool(lib().gfo_pardiso_available())
+
+
+class OracleModel:
+    """Phase-by-phase access to the oracle (mirrors the gf_* entry points)."""
+
+    def __init__(self, model, solver=SOLVER_BAND):
+        self.model = model
+        self._desc = model.desc()
+        self.h = lib().gfo_create(C.byref(self._desc), solver)
+        self.nn = model.n_nodes
+
+    def close(self):
+        if self.h:
+            lib().gfo_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def begin_solution(self, start=0.0, end=1.0):
+        st = lib().gfo_begin_solution(self.h, start, end)
+        if st != 0:
+            raise RuntimeError(f"gfo_begin_solution: {st}")
+
+    def begin_step(self, t_last, dt):
+        lib().gfo_begin_step(self.h, t_last, dt)
+
+    def assemble(self):
+        r = np.zeros(3)
+        lib().gfo_assemble(self.h, _dp(r))
+        return r
+
+    def solve(self):
+        f = C.c_int(0)
+        lib().gfo_solve(self.h, C.byref(f))
+        return f.value
+
+    def update(self):
+        r = np.zeros(5)
+        lib().gfo_update(self.h, _dp(r))
+        return r
+
+    def commit(self):
+        lib().gfo_commit(self.h)
+
+    def rollback(self):
+        lib().gfo_rollback(self.h)
+
+    def _read(self, fn, shape):
+        out = np.zeros(shape)
+        getattr(lib(), fn)(self.h, _dp(out))
+        return out
+
+    def nodes(self):
+        return self._read("gfo_read_nodes", (self.nn, 6))
+
+    def copy(self):
+        return self._read("gfo_read_copy", (self.nn, 6))
+
+    def disp(self):
+        return self._read("gfo_read_disp", (self.nn, 6))
+
+    def increment(self):
+        return self._read("gfo_read_increment", (self.nn, 6))
+
+    def residual(self):
+        return self._read("gfo_read_residual", (self.nn, 6))
+
+    def write_disp(self, d):
+        d = np.ascontiguousarray(d, float).reshape(self.nn, 6)
+        lib().gfo_write_disp(self.h, _dp(d))
+
+    def tangent_dense(self):
+        n6 = 6 * self.nn
+        out = np.zeros((n6, n6))
+        st = lib().gfo_read_tangent_dense(self.h, _dp(out))
+        if st != 0:
+            raise RuntimeError("tangent already factored")
+        return out
+
+    def element_beam(self, e):
+        K = np.zeros((18, 18))
+        P = np.zeros(18)
+        lib().gfo_read_element_beam(self.h, e, _dp(K), _dp(P))
+        return K, P
+
+    def gp_beam(self):
+        return self._read("gfo_read_gp_beam", (self.model.n_beams, 2, 21))
+
+    def n_free(self):
+        return lib().gfo_n_free(self.h)
+
+    def bandwidth(self):
+        return lib().gfo_bandwidth(self.h)
+
+    def dof_map(self):
+        out = np.zeros((self.nn, 6), np.int32)
+        lib().gfo_dof_map(self.h, out.ctypes.data_as(C.POINTER(C.c_int)))
+        return out
+
+    def static_solve(self, params, cap=4096):
+        iters = np.zeros(cap, np.int32)
+        rep = abi.GfStaticReport()
+        rep.step_iters = iters.ctypes.data_as(C.POINTER(C.c_int))
+        rep.step_iters_cap = cap
+        pc = params.c()
+        st = lib().gfo_static_solve(self.h, C.byref(pc), C.byref(rep))
+        if st != 0:
+            raise RuntimeError(f"gfo_static_solve: {st}")
+        n = min(rep.n_step_iters, cap)
+        return rep, iters[:n].copy()
+
+
+# ---- element-level helpers ----------------------------------------------------
+class _BeamConst(C.Structure):
+    _fields_ = [("T3", C.c_double * 9), ("D", C.c_double * 36), ("e3r", C.c_double * 3),
+                ("jac", C.c_double), ("N", C.c_double * 6), ("dN", C.c_double * 6), ("rhoA", C.c_double)]
+
+
+def beam_element(x9, cs9, prop9):
+    """Returns (const struct, gp state [2][21]) for one Beam_1."""
+    c = _BeamConst()
+    gp = np.zeros((2, 21))
+    x9 = np.ascontiguousarray(x9, float)
+    cs9 = np.ascontiguousarray(cs9, float)
+    prop9 = np.ascontiguousarray(prop9, float)
+    lib().gfo_beam_precalc(_dp(x9), _dp(cs9), _dp(prop9), C.byref(c), _dp(gp))
+    return c, gp
+
+
+def beam_mount(c, gp, d18):
+    K = np.zeros((18, 18))
+    f = np.zeros(18)
+    d18 = np.ascontiguousarray(d18, float)
+    gp = np.ascontiguousarray(gp, float)
+    lib().gfo_beam_mount(C.byref(c), _dp(gp), _dp(d18), _dp(K), _dp(f))
+    return K, f
+
+
+def beam_save(c, gp, d18):
+    gp = np.ascontiguousarray(gp, float).copy()
+    d18 = np.ascontiguousarray(d18, float)
+    lib().gfo_beam_save(C.byref(c), _dp(gp), _dp(d18))
+    return gp
+
+
+def rodrigues(a):
+    a = np.ascontiguousarray(a, float)
+    Q = np.zeros(9)
+    Xi = np.zeros(9)
+    g = np.zeros(1)
+    lib().gfo_rodrigues(_dp(a), _dp(Q), _dp(Xi), _dp(g))
+    return Q.reshape(3, 3), Xi.reshape(3, 3), g[0]
+
+
+def compose(a2, a1):
+    out = np.zeros(3)
+    lib().gfo_compose(_dp(np.ascontiguousarray(a2, float)), _dp(np.ascontiguousarray(a1, float)), _dp(out))
+    return out
+
+
+def V(x, t):
+    out = np.zeros(9)
+    lib().gfo_V(_dp(np.ascontiguousarray(x, float)), _dp(np.ascontiguousarray(t, float)), _dp(out))
+    return out.reshape(3, 3)
+
+
+def dV(x, dx, t):
+    out = np.zeros(9)
+    lib().gfo_dV(_dp(np.ascontiguousarray(x, float)), _dp(np.ascontiguousarray(dx, float)),
+                 _dp(np.ascontiguousarray(t, float)), _dp(out))
+    return out.reshape(3, 3)
+
+
+def gbsolve(A, kl, ku, b):
+    """Solve A x = b with the oracle's band LU (A dense, within the band)."""
+    n = A.shape[0]
+    ldab = 2 * kl + ku + 1
+    ab = np.zeros((n, ldab))  # column j at ab[j, :]
+    kv = kl + ku
+    for j in range(n):
+        for i in range(max(0, j - ku), min(n, j + kl + 1)):
+            ab[j, kv + i - j] = A[i, j]
+    ipiv = np.zeros(n, np.int32)
+    ip = ipiv.ctypes.data_as(C.POINTER(C.c_int))
+    info = lib().gfo_gbtrf(n, kl, ku, _dp(ab), ldab, ip)
+    x = np.ascontiguousarray(b, float).copy()
+    if info == 0:
+        lib().gfo_gbtrs(n, kl, ku, _dp(ab), ldab, ip, _dp(x))
+    return info, x
