@@ -357,9 +357,11 @@ int gf_beam_assemble(gf_model *m, double lf)
 {
     const size_t ld = (size_t)m->nb;
     int ncol = (int)m->beam_color_off.size() - 1;
+    int launches = 0;
     for (int c = 0; c < ncol; c++) {
         int e0 = m->beam_color_off[c], e1 = m->beam_color_off[c + 1];
         if (e1 <= e0) continue;
+        launches++;
         int grid = (e1 - e0 + EPB - 1) / EPB;
         hipLaunchKernelGGL(beam_assemble_kernel<0>, dim3(grid), dim3(WG), 0, m->stream, e0, e1, ld, m->d_bconn,
                            m->d_bconst, m->d_bgp, m->d_disp, m->d_dof, (double *)m->btd.A, (double *)m->btd.B,
@@ -367,7 +369,7 @@ int gf_beam_assemble(gf_model *m, double lf)
                            (double *)nullptr, (double *)nullptr);
     }
     GF_HIP(hipGetLastError());
-    return GF_OK;
+    return launches;
 }
 
 int gf_beam_commit(gf_model *m)
@@ -418,7 +420,8 @@ int gf_beam_eval_device(int count, const double *x, const double *cs, const doub
     GF_HIP(hipMemcpyAsync(K_out, d_K, sizeof(double) * 324 * ld, hipMemcpyDeviceToHost, s));
     GF_HIP(hipMemcpyAsync(f_out, d_f, sizeof(double) * 18 * ld, hipMemcpyDeviceToHost, s));
     GF_HIP(hipStreamSynchronize(s));
-    hipFree(d_conn); hipFree(d_cst); hipFree(d_gps); hipFree(d_disp); hipFree(d_K); hipFree(d_f);
-    hipStreamDestroy(s);
+    (void)hipFree(d_conn); (void)hipFree(d_cst); (void)hipFree(d_gps);
+    (void)hipFree(d_disp); (void)hipFree(d_K); (void)hipFree(d_f);
+    (void)hipStreamDestroy(s);
     return GF_OK;
 }

@@ -271,7 +271,8 @@ int gf_btd_alloc(gf_btd &b, int m, int nblk, int elem_size)
 
 void gf_btd_free(gf_btd &b)
 {
-    hipFree(b.A); hipFree(b.B); hipFree(b.C); hipFree(b.Lsv); hipFree(b.Rsv); hipFree(b.piv);
+    (void)hipFree(b.A); (void)hipFree(b.B); (void)hipFree(b.C);
+    (void)hipFree(b.Lsv); (void)hipFree(b.Rsv); (void)hipFree(b.piv);
     b.A = b.B = b.C = b.Lsv = b.Rsv = nullptr;
     b.piv = nullptr;
 }
@@ -280,16 +281,21 @@ template <typename T>
 static int btd_factor_t(gf_btd &b, hipStream_t s, int *d_flag)
 {
     T *A = (T *)b.A, *B = (T *)b.B, *C = (T *)b.C;
+    int launches = 0;
     for (int L = 0; L < b.levels; L++) {
         int nL = b.lvl_n[L];
         T *Lsv = (T *)b.Lsv + b.lvl_save[L] * b.bsz, *Rsv = (T *)b.Rsv + b.lvl_save[L] * b.bsz;
         hipLaunchKernelGGL(btd_elim_kernel<T>, dim3((nL + 1) / 2), dim3(64), 0, s, L, nL, b.m, A, B, C, b.piv, d_flag);
-        if (nL / 2 > 0)
+        launches++;
+        if (nL / 2 > 0) {
             hipLaunchKernelGGL(btd_kept_kernel<T>, dim3(nL / 2), dim3(64), 0, s, L, nL, b.m, A, B, C, Lsv, Rsv);
+            launches++;
+        }
     }
     hipLaunchKernelGGL(btd_top_kernel<T>, dim3(1), dim3(64), 0, s, b.levels, b.m, B, b.piv, d_flag);
+    launches++;
     GF_HIP(hipGetLastError());
-    return GF_OK;
+    return launches;
 }
 
 int gf_btd_factor(gf_btd &b, hipStream_t s, int *d_flag)
@@ -302,20 +308,26 @@ template <typename T>
 static int btd_solve_t(gf_btd &b, hipStream_t s, double *x)
 {
     const T *A = (const T *)b.A, *B = (const T *)b.B, *C = (const T *)b.C;
+    int launches = 0;
     for (int L = 0; L < b.levels; L++) {
         int nL = b.lvl_n[L];
         const T *Lsv = (const T *)b.Lsv + b.lvl_save[L] * b.bsz, *Rsv = (const T *)b.Rsv + b.lvl_save[L] * b.bsz;
         hipLaunchKernelGGL((btd_fwd_elim_kernel<T, double>), dim3((nL + 1) / 2), dim3(64), 0, s, L, nL, b.m, B, b.piv, x, 2);
-        if (nL / 2 > 0)
+        launches++;
+        if (nL / 2 > 0) {
             hipLaunchKernelGGL((btd_fwd_kept_kernel<T, double>), dim3(nL / 2), dim3(64), 0, s, L, nL, b.m, Lsv, Rsv, x);
+            launches++;
+        }
     }
     hipLaunchKernelGGL((btd_fwd_elim_kernel<T, double>), dim3(1), dim3(64), 0, s, b.levels, 1, b.m, B, b.piv, x, 2);
+    launches++;
     for (int L = b.levels - 1; L >= 0; L--) {
         int nL = b.lvl_n[L];
         hipLaunchKernelGGL((btd_bwd_kernel<T, double>), dim3((nL + 1) / 2), dim3(64), 0, s, L, nL, b.m, A, C, x);
+        launches++;
     }
     GF_HIP(hipGetLastError());
-    return GF_OK;
+    return launches;
 }
 
 // Solves in place: x holds the right-hand side (nblk*m entries) on entry.

@@ -95,11 +95,21 @@ struct gf_model {
     double *h_scal = nullptr;       // pinned mirror
     int *d_flag = nullptr;          // solver failure flag
     bool factored = false;
+
+    // reset support: initial node copy and Gauss-point state, kept on the device
+    double *d_copy0 = nullptr, *d_bgp0 = nullptr;
+
+    // phase timing (HIP events on `stream`)
+    bool timing = false;
+    hipEvent_t ev[2 * GF_NPHASE] = {};
+    double t_ms[GF_NPHASE] = {0, 0, 0};
+    long long launches[GF_NPHASE] = {0, 0, 0};
 };
 
 // ---- kernel launchers (defined in the .hip files) ----------------------------
 int gf_launch_fill(hipStream_t s, double *p, size_t n, double v);
 int gf_launch_fill_bytes(hipStream_t s, void *p, size_t bytes);
+// Returns the number of kernel launches issued (>= 0) or a negative status.
 int gf_beam_assemble(gf_model *m, double lf);
 int gf_beam_commit(gf_model *m);
 int gf_nodal_loads(gf_model *m, double t);
@@ -109,6 +119,7 @@ int gf_node_commit(gf_model *m);
 
 int gf_btd_alloc(gf_btd &b, int m, int nblk, int elem_size);
 void gf_btd_free(gf_btd &b);
+// Both return the number of kernel launches issued (>= 0) or a negative status.
 int gf_btd_factor(gf_btd &b, hipStream_t s, int *d_flag);
 int gf_btd_solve(gf_btd &b, hipStream_t s, const double *rhs, double *x, int n);
 

@@ -60,17 +60,42 @@ template <typename T> static int upload(gf_model *m, T *dst, const T *src, size_
         if (s_ != GF_OK) return s_; \
     } while (0)
 
+// Launchers that return a launch count (>= 0) or a negative status.
+#define GF_TRYN(var, x)              \
+    int var = (x);                   \
+    do {                             \
+        if (var < 0) return var;     \
+    } while (0)
+
+static void tick(gf_model *m, int ph, bool start)
+{
+    if (m->timing) (void)hipEventRecord(m->ev[2 * ph + (start ? 0 : 1)], m->stream);
+}
+// Call after the stream work of the phase was enqueued; waits for its end event.
+static int tock(gf_model *m, int ph, int launches)
+{
+    if (!m->timing) return GF_OK;
+    GF_HIP(hipEventSynchronize(m->ev[2 * ph + 1]));
+    float ms = 0.f;
+    GF_HIP(hipEventElapsedTime(&ms, m->ev[2 * ph], m->ev[2 * ph + 1]));
+    m->t_ms[ph] += ms;
+    m->launches[ph] += launches;
+    return GF_OK;
+}
+
 static void release(gf_model *m)
 {
     if (!m) return;
-    hipFree(m->d_disp); hipFree(m->d_copy); hipFree(m->d_euler); hipFree(m->d_dof); hipFree(m->d_row_nc);
-    hipFree(m->d_bconn); hipFree(m->d_bconst); hipFree(m->d_bgp);
-    hipFree(m->d_ln_node); hipFree(m->d_ln_off); hipFree(m->d_ln_ids); hipFree(m->d_lrec);
-    hipFree(m->d_tab_t); hipFree(m->d_tab_v);
-    hipFree(m->d_r); hipFree(m->d_x); hipFree(m->d_scal); hipFree(m->d_flag);
-    if (m->h_scal) hipHostFree(m->h_scal);
+    for (int i = 0; i < 2 * GF_NPHASE; i++)
+        if (m->ev[i]) (void)hipEventDestroy(m->ev[i]);
+    void *bufs[] = {m->d_disp, m->d_copy, m->d_euler, m->d_dof, m->d_row_nc, m->d_bconn, m->d_bconst, m->d_bgp,
+                    m->d_ln_node, m->d_ln_off, m->d_ln_ids, m->d_lrec, m->d_tab_t, m->d_tab_v, m->d_r, m->d_x,
+                    m->d_scal, m->d_flag, m->d_copy0, m->d_bgp0};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    if (m->h_scal) (void)hipHostFree(m->h_scal);
     gf_btd_free(m->btd);
-    if (m->own_stream && m->stream) hipStreamDestroy(m->stream);
+    if (m->own_stream && m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
 }
 
@@ -182,6 +207,8 @@ static int create(const gf_model_desc *d, const gf_options *opt, gf_model *m)
     GF_TRY(dalloc(&m->d_dof, 6 * (size_t)nn));
     GF_TRY(dalloc(&m->d_row_nc, (size_t)nf));
     GF_TRY(upload(m, m->d_copy, copy.data(), copy.size()));
+    GF_TRY(dalloc(&m->d_copy0, copy.size()));
+    GF_TRY(upload(m, m->d_copy0, copy.data(), copy.size()));
     GF_TRY(upload(m, m->d_dof, m->dof.data(), m->dof.size()));
     GF_TRY(upload(m, m->d_row_nc, m->row_nodecomp.data(), (size_t)nf));
     GF_TRY(gf_launch_fill(m->stream, m->d_disp, 6 * (size_t)nn, 0.0));
@@ -208,6 +235,8 @@ static int create(const gf_model_desc *d, const gf_options *opt, gf_model *m)
         GF_TRY(upload(m, m->d_bconn, conn.data(), conn.size()));
         GF_TRY(upload(m, m->d_bconst, cst.data(), cst.size()));
         GF_TRY(upload(m, m->d_bgp, gps.data(), gps.size()));
+        GF_TRY(dalloc(&m->d_bgp0, gps.size()));
+        GF_TRY(upload(m, m->d_bgp0, gps.data(), gps.size()));
     }
     // nodal loads grouped by node; multipliers as UpdateforSolutionStep
     {
@@ -273,7 +302,42 @@ static int create(const gf_model_desc *d, const gf_options *opt, gf_model *m)
     GF_TRY(dalloc(&m->d_scal, 16));
     GF_TRY(dalloc(&m->d_flag, 4));
     GF_HIP(hipHostMalloc((void **)&m->h_scal, 16 * sizeof(double), hipHostMallocDefault));
+    for (int i = 0; i < 2 * GF_NPHASE; i++) GF_HIP(hipEventCreate(&m->ev[i]));
     GF_HIP(hipStreamSynchronize(m->stream));
+    return GF_OK;
+}
+
+extern "C" int gf_reset_state(gf_model *m)
+{
+    if (!m) return GF_E_ARG;
+    GF_HIP(hipSetDevice(m->device));
+    GF_HIP(hipMemcpyAsync(m->d_copy, m->d_copy0, sizeof(double) * 6 * (size_t)m->nn, hipMemcpyDeviceToDevice,
+                          m->stream));
+    if (m->nb)
+        GF_HIP(hipMemcpyAsync(m->d_bgp, m->d_bgp0, sizeof(double) * BG_COUNT * (size_t)m->nb,
+                              hipMemcpyDeviceToDevice, m->stream));
+    GF_TRY(gf_launch_fill(m->stream, m->d_disp, 6 * (size_t)m->nn, 0.0));
+    GF_TRY(gf_launch_fill(m->stream, m->d_euler, 3 * (size_t)m->nn, 0.0));
+    GF_HIP(hipStreamSynchronize(m->stream));
+    m->factored = false;
+    return GF_OK;
+}
+
+extern "C" int gf_timing_enable(gf_model *m, int on)
+{
+    if (!m) return GF_E_ARG;
+    m->timing = on != 0;
+    return GF_OK;
+}
+
+extern "C" int gf_timing_read(gf_model *m, double *ms_out, long long *launches_out, int reset)
+{
+    if (!m) return GF_E_ARG;
+    for (int i = 0; i < GF_NPHASE; i++) {
+        if (ms_out) ms_out[i] = m->t_ms[i];
+        if (launches_out) launches_out[i] = m->launches[i];
+        if (reset) { m->t_ms[i] = 0.0; m->launches[i] = 0; }
+    }
     return GF_OK;
 }
 
@@ -298,8 +362,8 @@ extern "C" int gf_model_create(const gf_model_desc *desc, const gf_options *opt,
 extern "C" int gf_model_destroy(gf_model *m)
 {
     if (m) {
-        hipSetDevice(m->device);
-        if (m->stream) hipStreamSynchronize(m->stream);
+        (void)hipSetDevice(m->device);
+        if (m->stream) (void)hipStreamSynchronize(m->stream);
         release(m);
     }
     return GF_OK;
@@ -352,7 +416,10 @@ extern "C" int gf_assemble(gf_model *m, double *res_out)
     double t = m->t_last + m->dt;
     GF_TRY(gf_nodal_loads(m, t));
     double lf = m->gravity_on ? (t - m->sol_start) / (m->sol_end - m->sol_start) : 0.0;
-    GF_TRY(gf_beam_assemble(m, lf));
+    tick(m, GF_PH_ASSEMBLE, true);
+    GF_TRYN(nl_beam, gf_beam_assemble(m, lf));
+    tick(m, GF_PH_ASSEMBLE, false);
+    GF_TRY(tock(m, GF_PH_ASSEMBLE, nl_beam));
     m->factored = false;
     if (res_out) {
         GF_TRY(gf_scan_residual(m));
@@ -375,10 +442,16 @@ extern "C" int gf_solve(gf_model *m, int *info_fail)
     GF_HIP(hipSetDevice(m->device));
     *info_fail = 0;
     GF_TRY(gf_launch_fill_bytes(m->stream, m->d_flag, 4));
-    GF_TRY(gf_btd_factor(m->btd, m->stream, m->d_flag));
+    tick(m, GF_PH_FACTOR, true);
+    GF_TRYN(nl_fac, gf_btd_factor(m->btd, m->stream, m->d_flag));
+    tick(m, GF_PH_FACTOR, false);
     m->factored = true;
     size_t npad = (size_t)m->btd.nblk * m->btd.m;
-    GF_TRY(gf_btd_solve(m->btd, m->stream, m->d_r, m->d_x, (int)npad));
+    tick(m, GF_PH_SOLVE, true);
+    GF_TRYN(nl_sol, gf_btd_solve(m->btd, m->stream, m->d_r, m->d_x, (int)npad));
+    tick(m, GF_PH_SOLVE, false);
+    GF_TRY(tock(m, GF_PH_FACTOR, nl_fac));
+    GF_TRY(tock(m, GF_PH_SOLVE, nl_sol));
     int flag = 0;
     GF_HIP(hipMemcpyAsync(&m->h_scal[8], m->d_flag, 4, hipMemcpyDeviceToHost, m->stream));
     GF_HIP(hipStreamSynchronize(m->stream));

@@ -311,3 +311,69 @@ def catenary(n_elem=10000, span_h=1500.0, span_v=1000.0, arc_ratio=1.15, De=0.3,
                      force_min=Wtot / 10.0, moment_min=Wtot, disp_min=1e-6, rot_min=1e-6)
     info = {"a": a, "S": S, "rho": rho, "A": A, "n_free_expected": 6 * nn - 8}
     return m, p, info
+
+
+def shell_plate(nq=64, L=1.0, t=0.005, E=70e9, nu=0.33, rho=2700.0, p_max=1.7e7, n_steps=15,
+                imperf=1e-4, seed=1003, area_update=1, tol=1e-10, ramp="cubic"):
+    """Clamped square Shell_1 plate: nq x nq quads, each split into two 6-node
+    triangles ((2nq+1)^2 nodes on a grid of pitch L/(2nq)).  Corner nodes of the
+    triangles get seeded N(0, imperf) out-of-plane offsets (interior only);
+    mid-side nodes sit at the midpoints of their edges, so every element stays
+    planar.  All boundary nodes are clamped.  Follower pressure (ShellLoad,
+    AreaUpdate area_update) ramped linearly to p_max over t in [0, 1]."""
+    m = Model("shell_plate")
+    ng = 2 * nq + 1
+    h = L / (2 * nq)
+    ii, jj = np.meshgrid(np.arange(ng), np.arange(ng), indexing="xy")
+    xyz = np.zeros((ng * ng, 3))
+    xyz[:, 0] = (ii * h).ravel()
+    xyz[:, 1] = (jj * h).ravel()
+    rng = np.random.default_rng(seed)
+    z = np.zeros((ng, ng))  # z[j, i]
+    zc = rng.normal(0.0, imperf, (nq + 1, nq + 1)) if imperf > 0 else np.zeros((nq + 1, nq + 1))
+    zc[0, :] = zc[-1, :] = zc[:, 0] = zc[:, -1] = 0.0
+    z[0::2, 0::2] = zc
+    z[1::2, 0::2] = 0.5 * (zc[:-1, :] + zc[1:, :])
+    z[0::2, 1::2] = 0.5 * (zc[:, :-1] + zc[:, 1:])
+    z[1::2, 1::2] = 0.5 * (zc[:-1, :-1] + zc[1:, 1:])  # on the diagonal (2p,2q)-(2p+2,2q+2)
+    xyz[:, 2] = z.ravel()
+    m.add_nodes(xyz)
+
+    def nid(i, j):
+        return j * ng + i
+
+    elems = []
+    for q in range(nq):
+        for p in range(nq):
+            i0, j0 = 2 * p, 2 * q
+            a = [nid(i0, j0), nid(i0 + 2, j0), nid(i0 + 2, j0 + 2), nid(i0 + 1, j0), nid(i0 + 2, j0 + 1),
+                 nid(i0 + 1, j0 + 1)]
+            b = [nid(i0, j0), nid(i0 + 2, j0 + 2), nid(i0, j0 + 2), nid(i0 + 1, j0 + 1), nid(i0 + 1, j0 + 2),
+                 nid(i0, j0 + 1)]
+            for c6 in (a, b):
+                m.add_shell(c6, E, nu, rho, t)
+                elems.append(len(elems))
+    for j in range(ng):
+        for i in range(ng):
+            if i == 0 or j == 0 or i == ng - 1 or j == ng - 1:
+                m.fix_dofs(nid(i, j), range(6))
+    if ramp == "cubic":
+        ts = np.linspace(0.0, 1.0, n_steps + 1)
+        vals = np.zeros((n_steps + 1, 6))
+        vals[:, 0] = p_max * ts ** 3
+    else:
+        ts = np.array([0.0, 1.0])
+        vals = np.array([[0.0] * 6, [p_max, 0, 0, 0, 0, 0]])
+    m.add_load(abi.GF_LOAD_SHELL_PRESSURE, elems, ts, vals, flags=area_update)
+    dt = 1.0 / n_steps
+    Dflex = E * t ** 3 / (12 * (1 - nu * nu))
+    # residuals judged against 1% of the total pressure resultant (above the FP64
+    # noise floor of the membrane forces)
+    fref = 0.01 * abs(p_max) * L * L
+    p = StaticParams(end_time=1.0, time_step=dt, max_time_step=dt, min_time_step=dt / 1024.0,
+                     max_it=40, min_it=2, conv_increase=4, inc_factor=1.0,
+                     force_tol=tol, moment_tol=tol, disp_tol=tol, rot_tol=tol,
+                     force_min=max(fref, 1e-6), moment_min=max(fref * L, 1e-6), disp_min=1e-6, rot_min=1e-6)
+    center = nid(nq, nq)
+    info = {"center": center, "D": Dflex, "ng": ng, "n_free_expected": None}
+    return m, p, info

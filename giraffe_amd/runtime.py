@@ -58,6 +58,9 @@ def load():
         getattr(L, f).argtypes = [vp, dp]
     L.gf_read_tangent_dense.argtypes = [vp, dp, C.c_longlong]
     L.gf_beam_eval.argtypes = [C.c_int, dp, dp, dp, dp, dp, dp, dp]
+    L.gf_reset_state.argtypes = [vp]
+    L.gf_timing_enable.argtypes = [vp, C.c_int]
+    L.gf_timing_read.argtypes = [vp, dp, C.POINTER(C.c_longlong), C.c_int]
     _lib = L
     return L
 
@@ -165,6 +168,19 @@ class DeviceModel:
         out = np.zeros((n6, n6))
         check(load().gf_read_tangent_dense(self.h, _dp(out), out.size), "gf_read_tangent_dense")
         return out
+
+    def reset(self):
+        check(load().gf_reset_state(self.h), "gf_reset_state")
+
+    def timing(self, on=True):
+        check(load().gf_timing_enable(self.h, 1 if on else 0), "gf_timing_enable")
+
+    def timing_read(self, reset=True):
+        ms = np.zeros(3)
+        n = (C.c_longlong * 3)()
+        check(load().gf_timing_read(self.h, _dp(ms), n, 1 if reset else 0), "gf_timing_read")
+        names = ["assemble", "factor", "solve"]
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(names)}
 
     def static_solve(self, params, cap=4096):
         iters = np.zeros(cap, np.int32)

@@ -165,6 +165,21 @@ int gf_rollback(gf_model *m);
 /* Whole Static::Solve loop (adaptive steps, bisection, convergence criteria). */
 int gf_static_solve(gf_model *m, const gf_static_params *p, gf_static_report *rep);
 
+/* Return the model to the state right after gf_model_create (reference
+ * configuration, identity Gauss-point rotations), keeping the device buffers,
+ * so the same resident model can be solved again. */
+int gf_reset_state(gf_model *m);
+
+/* ---- device timing ------------------------------------------------------------
+ * HIP events on the model's stream around each phase; accumulated until read. */
+#define GF_PH_ASSEMBLE 0   /* element kernels (all colours)            */
+#define GF_PH_FACTOR   1   /* tangent factorisation                    */
+#define GF_PH_SOLVE    2   /* forward / back substitution, refinement  */
+#define GF_NPHASE      3
+int gf_timing_enable(gf_model *m, int on);
+/* ms_out[GF_NPHASE] accumulated milliseconds, launches_out[GF_NPHASE] kernel launches. */
+int gf_timing_read(gf_model *m, double *ms_out, long long *launches_out, int reset);
+
 /* ---- results --------------------------------------------------------------- */
 /* Per node: X Y Z of the last converged configuration and the Euler rotation
  * vector (Node::WriteResults src/Node.cpp:208-219). */

@@ -477,6 +477,7 @@ int gfo_assemble(gfo_model *m, double *res)
         gather_d(m, m->sconn + 6 * e, 6, d);
         gfo_shell_mount(gfo_shell_at(m->sc, e), m->sgp + (size_t)e * 3 * GFO_SHELL_GPS, d,
                         m->sK + 729 * (size_t)e, m->sP + 27 * (size_t)e);
+        if (m->grav_on) gfo_shell_gravity(gfo_shell_at(m->sc, e), lf, m->grav, m->sP + 27 * (size_t)e);
     }
     /* MountLoads: nodal loads write the global system; ShellLoad edits element buffers */
     mount_nodal_loads(m);
@@ -490,7 +491,10 @@ int gfo_assemble(gfo_model *m, double *res)
             gather_d(m, m->sconn + 6 * e, 6, d);
             double xc[18];
             for (int a = 0; a < 6; a++)
-                for (int k = 0; k < 3; k++) xc[3 * a + k] = m->copy[6 * m->sconn[6 * e + a] + k];
+                for (int k = 0; k < 3; k++) {
+                    int nd = m->sconn[6 * e + a];
+                    xc[3 * a + k] = m->copy[6 * nd + k] - m->xyz_ref[3 * nd + k];
+                }
             gfo_shell_pressure(gfo_shell_at(m->sc, e), xc, d, p, m->lflags[l],
                                m->sK + 729 * (size_t)e, m->sP + 27 * (size_t)e);
         }
@@ -805,6 +809,7 @@ int gfo_static_solve(gfo_model *m, const gf_static_params *p, gf_static_report *
     if (st != GF_OK) return st;
     conv_state cs = {0.0, 0.0, 0};
     int aborted = 0, convergence_counter = 0, converged_number = 0, diverged = 0;
+    int verbose = getenv("GFO_VERBOSE") != NULL;
     double time_step = p->time_step;
     double time = p->start_time;
     double last_converged = p->start_time;
@@ -880,6 +885,11 @@ int gfo_static_solve(gfo_model *m, const gf_static_params *p, gf_static_report *
             r.n_step_iters++;
             r.converged_steps++;
             if (last_rel > r.max_rel_residual) r.max_rel_residual = last_rel;
+            if (verbose) {
+                fprintf(stderr, "oracle: time %.6f step %.6f converged in %d iterations (%.1f s)\n", time, time_step,
+                        it - 1, (now_ms() - t0) * 1e-3);
+                fflush(stderr);
+            }
             if (it <= p->min_it || convergence_counter >= p->conv_increase) {
                 time_step = time_step * p->inc_factor;
                 if (time_step > p->max_time_step) time_step = p->max_time_step;

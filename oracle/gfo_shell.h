@@ -9,7 +9,9 @@ gfo_shell_const *gfo_shell_alloc(int n);
 gfo_shell_const *gfo_shell_at(gfo_shell_const *base, int e);
 void gfo_shell_precalc(const double *x /*[18]*/, const double *prop, gfo_shell_const *c, double *gp);
 void gfo_shell_mount(const gfo_shell_const *c, const double *gp, const double *d /*[36]*/, double *K, double *P);
-void gfo_shell_pressure(const gfo_shell_const *c, const double *xcopy /*[18]*/, const double *d /*[36]*/,
+/* utot: total translation (converged - reference) of the 6 nodes [18] */
+void gfo_shell_pressure(const gfo_shell_const *c, const double *utot, const double *d /*[36]*/,
                         double p, int area_update, double *K, double *P);
+void gfo_shell_gravity(const gfo_shell_const *c, double lf, const double *G, double *P);
 void gfo_shell_save(const gfo_shell_const *c, double *gp, const double *d);
 #endif

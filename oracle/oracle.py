@@ -271,3 +271,56 @@ def gbsolve(A, kl, ku, b):
     if info == 0:
         lib().gfo_gbtrs(n, kl, ku, _dp(ab), ldab, ip, _dp(x))
     return info, x
+
+
+# ---- Shell_1 element-level helpers -------------------------------------------
+def _shell_sigs():
+    L = lib()
+    if getattr(L, "_shell_sigs", False):
+        return L
+    dp = C.POINTER(C.c_double)
+    L.gfo_shell_new.restype = C.c_void_p
+    L.gfo_shell_new.argtypes = [dp, dp, dp]
+    L.gfo_shell_free.argtypes = [C.c_void_p]
+    L.gfo_shell_mount.argtypes = [C.c_void_p, dp, dp, dp, dp]
+    L.gfo_shell_pressure.argtypes = [C.c_void_p, dp, dp, C.c_double, C.c_int, dp, dp]
+    L.gfo_shell_gravity.argtypes = [C.c_void_p, C.c_double, dp, dp]
+    L.gfo_shell_save.argtypes = [C.c_void_p, dp, dp]
+    L._shell_sigs = True
+    return L
+
+
+class ShellElement:
+    """One Shell_1 (x [6][3] reference nodes, prop = E, nu, rho, t)."""
+
+    # element DOF i -> index into the 36-vector of 6 nodes x 6 DOFs
+    DOF = np.array([6 * (i // 3) + i % 3 for i in range(18)] + [6 * (3 + (i // 3)) + 3 + i % 3 for i in range(9)])
+
+    def __init__(self, x18, prop):
+        L = _shell_sigs()
+        self.gp = np.zeros((3, 27))
+        self.h = L.gfo_shell_new(_dp(np.ascontiguousarray(x18, float)), _dp(np.ascontiguousarray(prop, float)),
+                                 _dp(self.gp))
+
+    def __del__(self):
+        try:
+            _shell_sigs().gfo_shell_free(self.h)
+        except Exception:
+            pass
+
+    def mount(self, d36, gp=None):
+        gp = self.gp if gp is None else np.ascontiguousarray(gp, float)
+        K = np.zeros((27, 27))
+        f = np.zeros(27)
+        _shell_sigs().gfo_shell_mount(self.h, _dp(gp), _dp(np.ascontiguousarray(d36, float)), _dp(K), _dp(f))
+        return K, f
+
+    def pressure(self, utot18, d36, p, area_update, K=None, P=None):
+        K = np.zeros((27, 27)) if K is None else np.ascontiguousarray(K, float).copy()
+        P = np.zeros(27) if P is None else np.ascontiguousarray(P, float).copy()
+        _shell_sigs().gfo_shell_pressure(self.h, _dp(np.ascontiguousarray(utot18, float)),
+                                         _dp(np.ascontiguousarray(d36, float)), p, int(area_update), _dp(K), _dp(P))
+        return K, P
+
+    def save(self, d36):
+        _shell_sigs().gfo_shell_save(self.h, _dp(self.gp), _dp(np.ascontiguousarray(d36, float)))

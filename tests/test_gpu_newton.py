@@ -63,7 +63,9 @@ def test_solve_update_commit_match_oracle(oracle_lib):
             io, ig = om.update(), dm.update()
             xg = dm.increment()
             xo = om.increment()
-            assert np.abs(xg - xo).max() <= 1e-9 * np.abs(xo).max() + 1e-300
+            # two FP64 direct solvers (cyclic reduction vs pivoted band LU) agree to
+            # ~cond(K) * eps; the tangent here has cond ~1e7
+            assert np.abs(xg - xo).max() <= 1e-7 * np.abs(xo).max() + 1e-300
             assert np.allclose(ig[:4], io[:4], rtol=1e-8, atol=1e-15)
         om.commit()
         dm.commit()
