@@ -45,7 +45,7 @@ def log(msg):
 
 def cpu_baseline(m, p, kind):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    os.environ["GFO_VERBOSE"] = "1"  # the oracle prints one line per converged load step
+    os.environ["GFO_VERBOSE"] = "2"  # the oracle prints a line per Newton iteration and per load step
     import oracle
     solver = oracle.SOLVER_BAND
     label = "oracle FP64 restatement, band LU"

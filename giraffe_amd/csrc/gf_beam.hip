@@ -84,22 +84,12 @@ __device__ __forceinline__ void gather_disp(const double *__restrict__ disp, con
     }
 }
 
-// Address of global entry (R, Cc) in the block-tridiagonal storage.
-__device__ __forceinline__ size_t btd_off(int m, int R, int Cc, int &which)
-{
-    int bi = R / m, bj = Cc / m;
-    int lr = R - bi * m, lc = Cc - bj * m;
-    which = bj - bi + 1;  // 0: A (sub), 1: B (diag), 2: C (super)
-    return ((size_t)bi * m + lr) * m + lc;
-}
-
 // MODE 0: scatter into (A,B,C) and r.  MODE 1: write dense element outputs.
 template <int MODE>
 __global__ __launch_bounds__(WG) void beam_assemble_kernel(
     int e0, int e1, size_t ld, const int *__restrict__ conn, const double *__restrict__ cst,
     const double *__restrict__ gps, const double *__restrict__ disp, const int *__restrict__ dof,
-    double *__restrict__ bA, double *__restrict__ bB, double *__restrict__ bC, int m,
-    double *__restrict__ r, double lf, double gx, double gy, double gz,
+    gf_gmat gm, double *__restrict__ r, double lf, double gx, double gy, double gz,
     double *__restrict__ Kout, double *__restrict__ fout)
 {
     __shared__ double sm[EPB * 2 * LDS_GP];
@@ -222,13 +212,7 @@ __global__ __launch_bounds__(WG) void beam_assemble_kernel(
                     for (int i = 0; i < 3; i++)
 #pragma unroll
                         for (int j = 0; j < 3; j++) {
-                            int R = rr[ro + i], Cc = cc[co + j];
-                            if (R >= 0 && Cc >= 0) {
-                                int which;
-                                size_t off = btd_off(m, R, Cc, which);
-                                double *base = which == 1 ? bB : (which == 0 ? bA : bC);
-                                base[off] += blk[q].a[3 * i + j];
-                            }
+                            gm_add(gm, rr[ro + i], cc[co + j], blk[q].a[3 * i + j]);
                         }
                 }
             } else {
@@ -364,9 +348,8 @@ int gf_beam_assemble(gf_model *m, double lf)
         launches++;
         int grid = (e1 - e0 + EPB - 1) / EPB;
         hipLaunchKernelGGL(beam_assemble_kernel<0>, dim3(grid), dim3(WG), 0, m->stream, e0, e1, ld, m->d_bconn,
-                           m->d_bconst, m->d_bgp, m->d_disp, m->d_dof, (double *)m->btd.A, (double *)m->btd.B,
-                           (double *)m->btd.C, m->btd.m, m->d_r, lf, m->gravity[0], m->gravity[1], m->gravity[2],
-                           (double *)nullptr, (double *)nullptr);
+                           m->d_bconst, m->d_bgp, m->d_disp, m->d_dof, gf_model_gmat(m), m->d_r, lf, m->gravity[0],
+                           m->gravity[1], m->gravity[2], (double *)nullptr, (double *)nullptr);
     }
     GF_HIP(hipGetLastError());
     return launches;
@@ -413,9 +396,9 @@ int gf_beam_eval_device(int count, const double *x, const double *cs, const doub
     GF_HIP(hipMemcpyAsync(d_gps, gps.data(), sizeof(double) * gps.size(), hipMemcpyHostToDevice, s));
     GF_HIP(hipMemcpyAsync(d_disp, disp.data(), sizeof(double) * disp.size(), hipMemcpyHostToDevice, s));
     int grid = (count + EPB - 1) / EPB;
+    gf_gmat none = {};
     hipLaunchKernelGGL(beam_assemble_kernel<1>, dim3(grid), dim3(WG), 0, s, 0, count, ld, d_conn, d_cst, d_gps,
-                       d_disp, (const int *)nullptr, (double *)nullptr, (double *)nullptr, (double *)nullptr, 1,
-                       (double *)nullptr, 0.0, 0.0, 0.0, 0.0, d_K, d_f);
+                       d_disp, (const int *)nullptr, none, (double *)nullptr, 0.0, 0.0, 0.0, 0.0, d_K, d_f);
     GF_HIP(hipGetLastError());
     GF_HIP(hipMemcpyAsync(K_out, d_K, sizeof(double) * 324 * ld, hipMemcpyDeviceToHost, s));
     GF_HIP(hipMemcpyAsync(f_out, d_f, sizeof(double) * 18 * ld, hipMemcpyDeviceToHost, s));

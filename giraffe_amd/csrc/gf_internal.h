@@ -30,6 +30,64 @@ enum {
 // Gauss-point state of Beam_1, [BG_COUNT][nb]: per gp Q_i(9) kappa_i(3) z'_i(3)
 enum { BG_PER_GP = 15, BG_COUNT = 30 };
 
+// ---- global tangent storage -------------------------------------------------
+// GM_BTD : block-tridiagonal blocks A (sub), B (diag), C (super), [nblk][m][m]
+//          row-major; used when the half-bandwidth fits a block of m <= 32.
+// GM_BAND: skewed dense band, entry (i, c) with |i - c| <= hb at
+//          ab[i * 2hb + c + hb] (each row is a contiguous window of 2hb+1);
+//          sub-matrices inside the band are plain strided matrices with
+//          leading dimension 2hb, which is what the band LU tiles use.
+enum { GM_BTD = 0, GM_BAND = 1 };
+struct gf_gmat {
+    int layout;
+    int m;
+    double *A, *B, *C;
+    double *ab;
+    int hb;
+};
+
+__device__ __forceinline__ void gm_add(const gf_gmat &g, int R, int Cc, double v)
+{
+    if (R < 0 || Cc < 0) return;
+    if (g.layout == GM_BTD) {
+        int bi = R / g.m, bj = Cc / g.m;
+        size_t off = ((size_t)bi * g.m + (R - bi * g.m)) * g.m + (Cc - bj * g.m);
+        double *base = bj == bi ? g.B : (bj < bi ? g.A : g.C);
+        base[off] += v;
+    } else {
+        g.ab[(size_t)R * (size_t)(2 * g.hb) + (size_t)(Cc + g.hb)] += v;
+    }
+}
+
+// Table::GetValueAt (src/Table.cpp:73-101): piecewise-linear, clamped at both ends.
+__device__ __forceinline__ double gf_table_value(const double *tt, const double *tv, int r0, int nt, double t, int col)
+{
+    if (nt == 0) return 0.0;
+    const double *T = tt + r0;
+    const double *Vv = tv + 6 * (size_t)r0;
+    if (t >= T[nt - 1]) return Vv[6 * (nt - 1) + col];
+    if (t <= T[0]) return Vv[col];
+    int idx = 0;
+    while (t >= T[idx]) idx++;
+    double fac = (t - T[idx - 1]) / (T[idx] - T[idx - 1]);
+    return Vv[6 * (idx - 1) + col] + fac * (Vv[6 * idx + col] - Vv[6 * (idx - 1) + col]);
+}
+
+// ---- Shell_1 constant record ([SC_COUNT][ns]) and Gauss-point state ([SG_COUNT][ns]) ----
+enum {
+    SC_T3 = 0,        // 9: rows e1r, e2r, e3r
+    SC_ALPHA1 = 9,    // area / 3
+    SC_MU = 10, SC_LAMBDA = 11, SC_THICK = 12, SC_DRILL = 13, SC_RHO = 14,
+    SC_NU = 15,       // [3][6] quadratic u shapes at the mid-side points
+    SC_NU1 = 33, SC_NU2 = 51,
+    SC_NA = 69,       // [3][3] linear alpha shapes
+    SC_NA1 = 78, SC_NA2 = 87,
+    SC_GW = 96,       // [6] sum_g w4_g N4u_g[a] (self-weight)
+    SC_COUNT = 102
+};
+enum { SP_N4U = 0, SP_N4U1 = 36, SP_N4U2 = 72, SP_W4 = 108, SP_COUNT = 114 };  // Cowper points [6][6]
+enum { SG_PER_GP = 21, SG_COUNT = 63 };  // per gp: Q_i 9, kr1 3, kr2 3, zx1 3, zx2 3
+
 // Nodal loads grouped by node (one device thread per loaded node).
 enum { NL_KIND = 0, NL_CS = 1, NL_MULT = 10, NL_TAB0 = 16, NL_TABN = 17, NL_REC = 18 };
 
@@ -81,6 +139,18 @@ struct gf_model {
     double *d_bconst = nullptr;     // [BC_COUNT][nb]
     double *d_bgp = nullptr;        // [BG_COUNT][nb]
 
+    // device: shells (colour-sorted)
+    int *d_sconn = nullptr;         // [6][ns]
+    double *d_sconst = nullptr;     // [SC_COUNT][ns]
+    double *d_spconst = nullptr;    // [SP_COUNT][ns]
+    double *d_sgp = nullptr;        // [SG_COUNT][ns]
+    double *d_sgp0 = nullptr;
+    std::vector<int> shell_order, shell_color_off;
+    // shell pressure loads: per shell (colour-sorted) the load record index or -1
+    int *d_spload = nullptr;        // [ns]
+    int *d_spflag = nullptr;        // [ns] AreaUpdate
+    bool any_pressure = false;
+
     // device: nodal loads
     int n_load_nodes = 0;
     int *d_ln_node = nullptr, *d_ln_off = nullptr, *d_ln_ids = nullptr;
@@ -90,7 +160,11 @@ struct gf_model {
     // device: global system
     double *d_r = nullptr;          // residual r = f_ext - f_int [n_free]
     double *d_x = nullptr;          // increment [n_free]
+    int layout = GM_BTD;
     gf_btd btd;
+    double *d_band = nullptr;       // GM_BAND storage, FP64
+    size_t band_elems = 0;
+    int npad = 0;                   // padded system size
     double *d_scal = nullptr;       // small reduction results
     double *h_scal = nullptr;       // pinned mirror
     int *d_flag = nullptr;          // solver failure flag
@@ -112,6 +186,14 @@ int gf_launch_fill_bytes(hipStream_t s, void *p, size_t bytes);
 // Returns the number of kernel launches issued (>= 0) or a negative status.
 int gf_beam_assemble(gf_model *m, double lf);
 int gf_beam_commit(gf_model *m);
+int gf_shell_assemble(gf_model *m, double lf);
+int gf_shell_pressure(gf_model *m, double t);
+int gf_shell_eval_device(int count, const double *x, const double *prop, const double *d, const double *gp,
+                         double *K_out, double *f_out);
+int gf_shell_commit(gf_model *m);
+void gf_shell_host_const(const double *x18, const double *prop, double *rec /*[SC_COUNT]*/,
+                         double *prec /*[SP_COUNT]*/, double *gp /*[SG_COUNT]*/);
+gf_gmat gf_model_gmat(const gf_model *m);
 int gf_nodal_loads(gf_model *m, double t);
 int gf_scan_residual(gf_model *m);
 int gf_update_disp(gf_model *m);
@@ -122,6 +204,10 @@ void gf_btd_free(gf_btd &b);
 // Both return the number of kernel launches issued (>= 0) or a negative status.
 int gf_btd_factor(gf_btd &b, hipStream_t s, int *d_flag);
 int gf_btd_solve(gf_btd &b, hipStream_t s, const double *rhs, double *x, int n);
+
+int gf_band_pad(int n);
+int gf_band_factor(void *ab, int elem_size, int n, int hb, hipStream_t s, int *flag);
+int gf_band_solve(const void *ab, int elem_size, int n, int hb, hipStream_t s, double *x);
 
 int gf_beam_eval_device(int count, const double *x, const double *cs, const double *prop,
                         const double *d, const double *gp, double *K_out, double *f_out);

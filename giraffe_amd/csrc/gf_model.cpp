@@ -83,6 +83,19 @@ static int tock(gf_model *m, int ph, int launches)
     return GF_OK;
 }
 
+gf_gmat gf_model_gmat(const gf_model *m)
+{
+    gf_gmat g = {};
+    g.layout = m->layout;
+    g.m = m->btd.m;
+    g.A = (double *)m->btd.A;
+    g.B = (double *)m->btd.B;
+    g.C = (double *)m->btd.C;
+    g.ab = m->d_band;
+    g.hb = m->half_band;
+    return g;
+}
+
 static void release(gf_model *m)
 {
     if (!m) return;
@@ -90,7 +103,8 @@ static void release(gf_model *m)
         if (m->ev[i]) (void)hipEventDestroy(m->ev[i]);
     void *bufs[] = {m->d_disp, m->d_copy, m->d_euler, m->d_dof, m->d_row_nc, m->d_bconn, m->d_bconst, m->d_bgp,
                     m->d_ln_node, m->d_ln_off, m->d_ln_ids, m->d_lrec, m->d_tab_t, m->d_tab_v, m->d_r, m->d_x,
-                    m->d_scal, m->d_flag, m->d_copy0, m->d_bgp0};
+                    m->d_scal, m->d_flag, m->d_copy0, m->d_bgp0, m->d_sconn, m->d_sconst, m->d_spconst, m->d_sgp,
+                    m->d_sgp0, m->d_spload, m->d_spflag, m->d_band};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (m->h_scal) (void)hipHostFree(m->h_scal);
@@ -99,26 +113,63 @@ static void release(gf_model *m)
     delete m;
 }
 
+// Greedy colouring: no two elements of a colour share a node.  Returns the
+// colour-sorted element order and the colour offsets.
+static int colour_elements(const int *conn, int ne, int nper, int nn, std::vector<int> &order, std::vector<int> &off)
+{
+    std::vector<unsigned long long> used(nn, 0ull);
+    std::vector<int> color(ne);
+    int ncol = 0;
+    for (int e = 0; e < ne; e++) {
+        unsigned long long u = 0;
+        for (int a = 0; a < nper; a++) u |= used[conn[nper * (size_t)e + a]];
+        int c = 0;
+        while (c < 64 && (u >> c & 1ull)) c++;
+        if (c == 64) {
+            gf_set_error("colouring needs more than 64 colours");
+            return GF_E_UNSUPPORTED;
+        }
+        color[e] = c;
+        ncol = std::max(ncol, c + 1);
+        for (int a = 0; a < nper; a++) used[conn[nper * (size_t)e + a]] |= 1ull << c;
+    }
+    order.resize(ne);
+    off.assign(ncol + 1, 0);
+    for (int e = 0; e < ne; e++) off[color[e] + 1]++;
+    for (int c = 0; c < ncol; c++) off[c + 1] += off[c];
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    for (int e = 0; e < ne; e++) order[fill[color[e]]++] = e;
+    return GF_OK;
+}
+
 static int create(const gf_model_desc *d, const gf_options *opt, gf_model *m)
 {
     if (d->n_nodes <= 0 || !d->node_xyz) { gf_set_error("model has no nodes"); return GF_E_ARG; }
-    if (d->n_shells > 0) { gf_set_error("Shell_1 is not built in this library version"); return GF_E_UNSUPPORTED; }
     m->nn = d->n_nodes;
     m->nb = d->n_beams;
     m->ns = d->n_shells;
     m->nl = d->n_loads;
-    const int nn = m->nn, nb = m->nb;
+    const int nn = m->nn, nb = m->nb, ns = m->ns;
     for (int e = 0; e < nb; e++)
         for (int a = 0; a < 3; a++) {
             int n = d->beam_conn[3 * e + a];
             if (n < 0 || n >= nn) { gf_set_error("beam connectivity out of range"); return GF_E_ARG; }
         }
-    // DOFsActive + NodalConstraint + SetGlobalDOFs (node-major numbering)
+    for (int e = 0; e < ns; e++)
+        for (int a = 0; a < 6; a++) {
+            int n = d->shell_conn[6 * e + a];
+            if (n < 0 || n >= nn) { gf_set_error("shell connectivity out of range"); return GF_E_ARG; }
+        }
+    // DOFsActive (Beam_1: 6 per node; Shell_1: u at corners, u + alpha at mid-sides,
+    // src/Shell_1.cpp:51-68) + NodalConstraint + SetGlobalDOFs (node-major numbering)
     m->active.assign(6 * (size_t)nn, 0);
     m->fix.assign(6 * (size_t)nn, 0);
     for (int e = 0; e < nb; e++)
         for (int a = 0; a < 3; a++)
             for (int k = 0; k < 6; k++) m->active[6 * (size_t)d->beam_conn[3 * e + a] + k] = 1;
+    for (int e = 0; e < ns; e++)
+        for (int a = 0; a < 6; a++)
+            for (int k = 0; k < (a < 3 ? 3 : 6); k++) m->active[6 * (size_t)d->shell_conn[6 * e + a] + k] = 1;
     if (d->node_fix)
         for (size_t i = 0; i < 6 * (size_t)nn; i++) m->fix[i] = d->node_fix[i] ? 1 : 0;
     m->dof.assign(6 * (size_t)nn, -1);
@@ -134,54 +185,40 @@ static int create(const gf_model_desc *d, const gf_options *opt, gf_model *m)
         }
     m->n_free = nf;
     if (nf == 0) { gf_set_error("model has no free DOFs"); return GF_E_ARG; }
-    // half-bandwidth and the block size of the block-tridiagonal form
-    std::vector<int> lo(nb), hi(nb);
+    // half-bandwidth; row span of every element
+    std::vector<int> lo(nb + ns), hi(nb + ns);
     int hb = 0;
-    for (int e = 0; e < nb; e++) {
+    for (int e = 0; e < nb + ns; e++) {
+        const int *cn = e < nb ? d->beam_conn + 3 * (size_t)e : d->shell_conn + 6 * (size_t)(e - nb);
+        int nper = e < nb ? 3 : 6;
         int l = 1 << 30, h = -1;
-        for (int a = 0; a < 3; a++)
+        for (int a = 0; a < nper; a++)
             for (int k = 0; k < 6; k++) {
-                int r = m->dof[6 * (size_t)d->beam_conn[3 * e + a] + k];
+                int r = m->dof[6 * (size_t)cn[a] + k];
                 if (r >= 0) { l = std::min(l, r); h = std::max(h, r); }
             }
         lo[e] = l; hi[e] = h;
         if (h >= 0) hb = std::max(hb, h - l);
     }
     m->half_band = hb;
+    // narrow bands: block-tridiagonal form with the smallest block size m <= 32
     int bm = -1;
     for (int cand = std::max(1, (hb + 2) / 2); cand <= 32 && bm < 0; cand++) {
         bool ok = true;
-        for (int e = 0; e < nb && ok; e++)
+        for (int e = 0; e < nb + ns && ok; e++)
             if (hi[e] >= 0 && hi[e] / cand - lo[e] / cand > 1) ok = false;
         if (ok) bm = cand;
     }
-    if (bm < 0) {
-        gf_set_error("half-bandwidth too large for the block-tridiagonal solver");
-        return GF_E_UNSUPPORTED;
+    m->layout = bm > 0 ? GM_BTD : GM_BAND;
+    int nblk = 0;
+    if (m->layout == GM_BTD) {
+        nblk = (nf + bm - 1) / bm;
+        m->npad = nblk * bm;
+    } else {
+        m->npad = gf_band_pad(nf);
     }
-    int nblk = (nf + bm - 1) / bm;
-    // colouring: no two beams of one colour share a node
-    std::vector<unsigned long long> used(nn, 0ull);
-    std::vector<int> color(nb);
-    int ncol = 0;
-    for (int e = 0; e < nb; e++) {
-        unsigned long long u = 0;
-        for (int a = 0; a < 3; a++) u |= used[d->beam_conn[3 * e + a]];
-        int c = 0;
-        while (c < 64 && (u >> c & 1ull)) c++;
-        if (c == 64) { gf_set_error("colouring needs more than 64 colours"); return GF_E_UNSUPPORTED; }
-        color[e] = c;
-        ncol = std::max(ncol, c + 1);
-        for (int a = 0; a < 3; a++) used[d->beam_conn[3 * e + a]] |= 1ull << c;
-    }
-    m->beam_order.resize(nb);
-    m->beam_color_off.assign(ncol + 1, 0);
-    for (int e = 0; e < nb; e++) m->beam_color_off[color[e] + 1]++;
-    for (int c = 0; c < ncol; c++) m->beam_color_off[c + 1] += m->beam_color_off[c];
-    {
-        std::vector<int> fill(m->beam_color_off.begin(), m->beam_color_off.end() - 1);
-        for (int e = 0; e < nb; e++) m->beam_order[fill[color[e]]++] = e;
-    }
+    GF_TRY(colour_elements(d->beam_conn, nb, 3, nn, m->beam_order, m->beam_color_off));
+    GF_TRY(colour_elements(d->shell_conn, ns, 6, nn, m->shell_order, m->shell_color_off));
     // device + stream
     m->device = opt ? opt->device : 0;
     GF_HIP(hipSetDevice(m->device));
@@ -238,13 +275,66 @@ static int create(const gf_model_desc *d, const gf_options *opt, gf_model *m)
         GF_TRY(dalloc(&m->d_bgp0, gps.size()));
         GF_TRY(upload(m, m->d_bgp0, gps.data(), gps.size()));
     }
+    // shells, colour-sorted SoA
+    if (ns > 0) {
+        size_t ld = (size_t)ns;
+        std::vector<int> conn(6 * ld);
+        std::vector<double> cst(SC_COUNT * ld), pcs(SP_COUNT * ld), gps(SG_COUNT * ld);
+        std::vector<double> rec(SC_COUNT), prec(SP_COUNT), g0(SG_COUNT);
+        for (int s2 = 0; s2 < ns; s2++) {
+            int e = m->shell_order[s2];
+            double x[18];
+            for (int a = 0; a < 6; a++)
+                for (int k = 0; k < 3; k++) x[3 * a + k] = d->node_xyz[3 * (size_t)d->shell_conn[6 * e + a] + k];
+            gf_shell_host_const(x, d->shell_prop + GF_SHELL_NPROP * (size_t)e, rec.data(), prec.data(), g0.data());
+            for (int k = 0; k < SC_COUNT; k++) cst[k * ld + s2] = rec[k];
+            for (int k = 0; k < SP_COUNT; k++) pcs[k * ld + s2] = prec[k];
+            for (int k = 0; k < SG_COUNT; k++) gps[k * ld + s2] = g0[k];
+            for (int a = 0; a < 6; a++) conn[a * ld + s2] = d->shell_conn[6 * e + a];
+        }
+        GF_TRY(dalloc(&m->d_sconn, conn.size()));
+        GF_TRY(dalloc(&m->d_sconst, cst.size()));
+        GF_TRY(dalloc(&m->d_spconst, pcs.size()));
+        GF_TRY(dalloc(&m->d_sgp, gps.size()));
+        GF_TRY(dalloc(&m->d_sgp0, gps.size()));
+        GF_TRY(upload(m, m->d_sconn, conn.data(), conn.size()));
+        GF_TRY(upload(m, m->d_sconst, cst.data(), cst.size()));
+        GF_TRY(upload(m, m->d_spconst, pcs.data(), pcs.size()));
+        GF_TRY(upload(m, m->d_sgp, gps.data(), gps.size()));
+        GF_TRY(upload(m, m->d_sgp0, gps.data(), gps.size()));
+        // ShellLoad -> per element (colour-sorted position) load index and AreaUpdate flag
+        std::vector<int> pos(ns);
+        for (int s2 = 0; s2 < ns; s2++) pos[m->shell_order[s2]] = s2;
+        std::vector<int> pl(ns, -1), pf(ns, 0);
+        for (int l = 0; l < m->nl; l++) {
+            if (d->load_kind[l] != GF_LOAD_SHELL_PRESSURE) continue;
+            for (int q = d->load_set_off[l]; q < d->load_set_off[l + 1]; q++) {
+                int e = d->load_set_items[q];
+                if (e < 0 || e >= ns) { gf_set_error("pressure load element out of range"); return GF_E_ARG; }
+                if (pl[pos[e]] >= 0) { gf_set_error("more than one pressure load on a shell element"); return GF_E_UNSUPPORTED; }
+                pl[pos[e]] = l;
+                pf[pos[e]] = d->load_flags ? d->load_flags[l] : 0;
+                m->any_pressure = true;
+            }
+        }
+        GF_TRY(dalloc(&m->d_spload, pl.size()));
+        GF_TRY(dalloc(&m->d_spflag, pf.size()));
+        GF_TRY(upload(m, m->d_spload, pl.data(), pl.size()));
+        GF_TRY(upload(m, m->d_spflag, pf.data(), pf.size()));
+    }
     // nodal loads grouped by node; multipliers as UpdateforSolutionStep
     {
         std::vector<std::pair<int, int>> pairs;  // (node, load)
         std::vector<double> lrec((size_t)std::max(1, m->nl) * NL_REC, 0.0);
         for (int l = 0; l < m->nl; l++) {
             int kind = d->load_kind[l];
-            if (kind == GF_LOAD_SHELL_PRESSURE) continue;
+            if (kind == GF_LOAD_SHELL_PRESSURE) {
+                double *R = lrec.data() + (size_t)l * NL_REC;
+                R[NL_KIND] = kind;
+                R[NL_TAB0] = d->load_tab_off[l];
+                R[NL_TABN] = d->load_tab_off[l + 1] - d->load_tab_off[l];
+                continue;
+            }
             if (kind != GF_LOAD_NODAL && kind != GF_LOAD_NODAL_FOLLOWER) { gf_set_error("unknown load kind"); return GF_E_ARG; }
             int cf[3] = {0, 0, 0}, cm[3] = {0, 0, 0};
             for (int s = d->load_set_off[l]; s < d->load_set_off[l + 1]; s++) {
@@ -295,10 +385,15 @@ static int create(const gf_model_desc *d, const gf_options *opt, gf_model *m)
     for (int k = 0; k < 3; k++) m->gravity[k] = d->gravity[k];
     m->gravity_on = d->gravity_on;
     // global system
-    size_t npad = (size_t)nblk * bm;
+    size_t npad = (size_t)m->npad;
     GF_TRY(dalloc(&m->d_r, npad));
     GF_TRY(dalloc(&m->d_x, npad));
-    GF_TRY(gf_btd_alloc(m->btd, bm, nblk, 8));
+    if (m->layout == GM_BTD) {
+        GF_TRY(gf_btd_alloc(m->btd, bm, nblk, 8));
+    } else {
+        m->band_elems = npad * (size_t)(2 * hb + 1);
+        GF_TRY(dalloc(&m->d_band, m->band_elems));
+    }
     GF_TRY(dalloc(&m->d_scal, 16));
     GF_TRY(dalloc(&m->d_flag, 4));
     GF_HIP(hipHostMalloc((void **)&m->h_scal, 16 * sizeof(double), hipHostMallocDefault));
@@ -315,6 +410,9 @@ extern "C" int gf_reset_state(gf_model *m)
                           m->stream));
     if (m->nb)
         GF_HIP(hipMemcpyAsync(m->d_bgp, m->d_bgp0, sizeof(double) * BG_COUNT * (size_t)m->nb,
+                              hipMemcpyDeviceToDevice, m->stream));
+    if (m->ns)
+        GF_HIP(hipMemcpyAsync(m->d_sgp, m->d_sgp0, sizeof(double) * SG_COUNT * (size_t)m->ns,
                               hipMemcpyDeviceToDevice, m->stream));
     GF_TRY(gf_launch_fill(m->stream, m->d_disp, 6 * (size_t)m->nn, 0.0));
     GF_TRY(gf_launch_fill(m->stream, m->d_euler, 3 * (size_t)m->nn, 0.0));
@@ -374,11 +472,12 @@ extern "C" int gf_model_info(const gf_model *m, long long *o)
     if (!m || !o) return GF_E_ARG;
     o[0] = m->n_free;
     o[1] = m->half_band;
-    o[2] = m->btd.m;
-    o[3] = m->btd.nblk;
-    o[4] = (long long)m->beam_color_off.size() - 1;
-    o[5] = (long long)(3 * (size_t)m->btd.nblk * m->btd.bsz * m->btd.elem_size);
-    o[6] = m->btd.levels;
+    o[2] = m->layout == GM_BTD ? m->btd.m : 0;
+    o[3] = m->layout == GM_BTD ? m->btd.nblk : m->npad;
+    o[4] = (long long)std::max(m->beam_color_off.size(), m->shell_color_off.size()) - 1;
+    o[5] = m->layout == GM_BTD ? (long long)(3 * (size_t)m->btd.nblk * m->btd.bsz * m->btd.elem_size)
+                               : (long long)(m->band_elems * 8);
+    o[6] = m->layout;
     o[7] = m->precision;
     return GF_OK;
 }
@@ -406,20 +505,26 @@ extern "C" int gf_assemble(gf_model *m, double *res_out)
 {
     if (!m) return GF_E_ARG;
     GF_HIP(hipSetDevice(m->device));
-    size_t npad = (size_t)m->btd.nblk * m->btd.m;
-    size_t nbl = (size_t)m->btd.nblk * m->btd.bsz;
+    size_t npad = (size_t)m->npad;
     GF_TRY(gf_launch_fill(m->stream, m->d_r, npad, 0.0));
-    GF_TRY(gf_launch_fill(m->stream, (double *)m->btd.A, nbl, 0.0));
-    GF_TRY(gf_launch_fill(m->stream, (double *)m->btd.B, nbl, 0.0));
-    GF_TRY(gf_launch_fill(m->stream, (double *)m->btd.C, nbl, 0.0));
+    if (m->layout == GM_BTD) {
+        size_t nbl = (size_t)m->btd.nblk * m->btd.bsz;
+        GF_TRY(gf_launch_fill(m->stream, (double *)m->btd.A, nbl, 0.0));
+        GF_TRY(gf_launch_fill(m->stream, (double *)m->btd.B, nbl, 0.0));
+        GF_TRY(gf_launch_fill(m->stream, (double *)m->btd.C, nbl, 0.0));
+    } else {
+        GF_TRY(gf_launch_fill(m->stream, m->d_band, m->band_elems, 0.0));
+    }
     GF_TRY(gf_pad_diag(m));
     double t = m->t_last + m->dt;
     GF_TRY(gf_nodal_loads(m, t));
     double lf = m->gravity_on ? (t - m->sol_start) / (m->sol_end - m->sol_start) : 0.0;
     tick(m, GF_PH_ASSEMBLE, true);
     GF_TRYN(nl_beam, gf_beam_assemble(m, lf));
+    GF_TRYN(nl_shell, gf_shell_assemble(m, lf));
     tick(m, GF_PH_ASSEMBLE, false);
-    GF_TRY(tock(m, GF_PH_ASSEMBLE, nl_beam));
+    GF_TRY(tock(m, GF_PH_ASSEMBLE, nl_beam + nl_shell));
+    GF_TRY(gf_shell_pressure(m, t));
     m->factored = false;
     if (res_out) {
         GF_TRY(gf_scan_residual(m));
@@ -442,13 +547,22 @@ extern "C" int gf_solve(gf_model *m, int *info_fail)
     GF_HIP(hipSetDevice(m->device));
     *info_fail = 0;
     GF_TRY(gf_launch_fill_bytes(m->stream, m->d_flag, 4));
+    size_t npad = (size_t)m->npad;
     tick(m, GF_PH_FACTOR, true);
-    GF_TRYN(nl_fac, gf_btd_factor(m->btd, m->stream, m->d_flag));
+    int nl_fac, nl_sol;
+    if (m->layout == GM_BTD) nl_fac = gf_btd_factor(m->btd, m->stream, m->d_flag);
+    else nl_fac = gf_band_factor(m->d_band, 8, m->npad, m->half_band, m->stream, m->d_flag);
+    if (nl_fac < 0) return nl_fac;
     tick(m, GF_PH_FACTOR, false);
     m->factored = true;
-    size_t npad = (size_t)m->btd.nblk * m->btd.m;
     tick(m, GF_PH_SOLVE, true);
-    GF_TRYN(nl_sol, gf_btd_solve(m->btd, m->stream, m->d_r, m->d_x, (int)npad));
+    if (m->layout == GM_BTD) {
+        nl_sol = gf_btd_solve(m->btd, m->stream, m->d_r, m->d_x, (int)npad);
+    } else {
+        GF_HIP(hipMemcpyAsync(m->d_x, m->d_r, sizeof(double) * npad, hipMemcpyDeviceToDevice, m->stream));
+        nl_sol = gf_band_solve(m->d_band, 8, m->npad, m->half_band, m->stream, m->d_x);
+    }
+    if (nl_sol < 0) return nl_sol;
     tick(m, GF_PH_SOLVE, false);
     GF_TRY(tock(m, GF_PH_FACTOR, nl_fac));
     GF_TRY(tock(m, GF_PH_SOLVE, nl_sol));
@@ -481,6 +595,7 @@ extern "C" int gf_commit(gf_model *m)
     GF_HIP(hipSetDevice(m->device));
     GF_TRY(gf_node_commit(m));
     GF_TRY(gf_beam_commit(m));
+    GF_TRY(gf_shell_commit(m));
     GF_HIP(hipStreamSynchronize(m->stream));
     return GF_OK;
 }
@@ -556,6 +671,18 @@ extern "C" int gf_read_tangent_dense(gf_model *m, double *out, long long cap)
     if (cap < n6 * n6) { gf_set_error("output buffer too small"); return GF_E_CAPACITY; }
     if (m->factored) { gf_set_error("tangent already factored"); return GF_E_STATE; }
     GF_HIP(hipSetDevice(m->device));
+    if (m->layout == GM_BAND) {
+        std::vector<double> ab(m->band_elems);
+        GF_HIP(hipMemcpyAsync(ab.data(), m->d_band, m->band_elems * 8, hipMemcpyDeviceToHost, m->stream));
+        GF_HIP(hipStreamSynchronize(m->stream));
+        std::fill(out, out + n6 * n6, 0.0);
+        int hb = m->half_band;
+        for (int R = 0; R < m->n_free; R++)
+            for (int Cc = std::max(0, R - hb); Cc <= std::min(m->n_free - 1, R + hb); Cc++)
+                out[(size_t)m->row_nodecomp[R] * n6 + m->row_nodecomp[Cc]] =
+                    ab[(size_t)R * (size_t)(2 * hb) + (size_t)(Cc + hb)];
+        return GF_OK;
+    }
     size_t nbl = (size_t)m->btd.nblk * m->btd.bsz;
     std::vector<double> A(nbl), B(nbl), C(nbl);
     GF_HIP(hipMemcpyAsync(A.data(), m->btd.A, nbl * 8, hipMemcpyDeviceToHost, m->stream));
@@ -572,6 +699,57 @@ extern "C" int gf_read_tangent_dense(gf_model *m, double *out, long long cap)
             out[(size_t)m->row_nodecomp[R] * n6 + m->row_nodecomp[Cc]] = v;
         }
     return GF_OK;
+}
+
+extern "C" int gf_bandsystem(int n, int hb, const double *ab_rows, const double *b, double *x, int *info_fail)
+{
+    if (n <= 0 || hb < 0 || !ab_rows || !b || !x || !info_fail) return GF_E_ARG;
+    *info_fail = 0;
+    int npad = gf_band_pad(n);
+    size_t w = (size_t)(2 * hb + 1);
+    size_t elems = (size_t)npad * w;
+    double *d_ab = nullptr, *d_x = nullptr;
+    int *d_flag = nullptr;
+    hipStream_t s;
+    GF_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    GF_HIP(hipMalloc(&d_ab, elems * 8));
+    GF_HIP(hipMalloc(&d_x, (size_t)npad * 8));
+    GF_HIP(hipMalloc(&d_flag, 4));
+    GF_TRY(gf_launch_fill(s, d_ab, elems, 0.0));
+    GF_TRY(gf_launch_fill(s, d_x, (size_t)npad, 0.0));
+    GF_TRY(gf_launch_fill_bytes(s, d_flag, 4));
+    // rows of the band: row i holds columns i-hb .. i+hb; columns outside [0, n) must be zero
+    std::vector<double> rows((size_t)n * w);
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < (int)w; k++) {
+            int c = i - hb + k;
+            rows[(size_t)i * w + k] = (c >= 0 && c < n) ? ab_rows[(size_t)i * w + k] : 0.0;
+        }
+    std::vector<double> diag((size_t)(npad - n) * w, 0.0);
+    for (int i = n; i < npad; i++) diag[(size_t)(i - n) * w + hb] = 1.0;
+    GF_HIP(hipMemcpyAsync(d_ab, rows.data(), rows.size() * 8, hipMemcpyHostToDevice, s));
+    if (npad > n) GF_HIP(hipMemcpyAsync(d_ab + (size_t)n * w, diag.data(), diag.size() * 8, hipMemcpyHostToDevice, s));
+    GF_HIP(hipMemcpyAsync(d_x, b, (size_t)n * 8, hipMemcpyHostToDevice, s));
+    int st = gf_band_factor(d_ab, 8, npad, hb, s, d_flag);
+    if (st >= 0) st = gf_band_solve(d_ab, 8, npad, hb, s, d_x);
+    int flag = 0;
+    if (st >= 0) {
+        GF_HIP(hipMemcpyAsync(x, d_x, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+        GF_HIP(hipMemcpyAsync(&flag, d_flag, 4, hipMemcpyDeviceToHost, s));
+        GF_HIP(hipStreamSynchronize(s));
+        st = GF_OK;
+    }
+    (void)hipFree(d_ab); (void)hipFree(d_x); (void)hipFree(d_flag);
+    (void)hipStreamDestroy(s);
+    if (flag) *info_fail = 1;
+    return st;
+}
+
+extern "C" int gf_shell_eval(int count, const double *x, const double *prop, const double *d, const double *gp,
+                             double *K_out, double *f_out)
+{
+    if (count < 0 || (count > 0 && (!x || !prop || !d || !gp || !K_out || !f_out))) return GF_E_ARG;
+    return gf_shell_eval_device(count, x, prop, d, gp, K_out, f_out);
 }
 
 extern "C" int gf_beam_eval(int count, const double *x, const double *cs, const double *prop, const double *d,

@@ -26,42 +26,14 @@ __global__ void fill_u32_kernel(unsigned *p, size_t n)
     for (; i < n; i += stride) p[i] = 0u;
 }
 
-__device__ __forceinline__ double table_value(const double *tt, const double *tv, int r0, int nt, double t, int col)
-{
-    if (nt == 0) return 0.0;
-    const double *T = tt + r0;
-    const double *Vv = tv + 6 * (size_t)r0;
-    if (t >= T[nt - 1]) return Vv[6 * (nt - 1) + col];
-    if (t <= T[0]) return Vv[col];
-    int idx = 0;
-    while (t >= T[idx]) idx++;
-    double fac = (t - T[idx - 1]) / (T[idx] - T[idx - 1]);
-    return Vv[6 * (idx - 1) + col] + fac * (Vv[6 * idx + col] - Vv[6 * (idx - 1) + col]);
-}
-
-__device__ __forceinline__ size_t btd_off(int m, int R, int Cc, int &which)
-{
-    int bi = R / m, bj = Cc / m;
-    which = bj - bi + 1;
-    return ((size_t)bi * m + (R - bi * m)) * m + (Cc - bj * m);
-}
-
-__device__ __forceinline__ void kadd(double *bA, double *bB, double *bC, int m, int R, int Cc, double v)
-{
-    if (R < 0 || Cc < 0) return;
-    int w;
-    size_t off = btd_off(m, R, Cc, w);
-    double *base = w == 1 ? bB : (w == 0 ? bA : bC);
-    base[off] += v;
-}
+#define table_value gf_table_value
 
 // One thread per loaded node: all loads on that node, in load order.
 __global__ void nodal_loads_kernel(int n_ln, const int *__restrict__ ln_node, const int *__restrict__ ln_off,
                                    const int *__restrict__ ln_ids, const double *__restrict__ rec,
                                    const double *__restrict__ tab_t, const double *__restrict__ tab_v,
                                    const double *__restrict__ disp, const double *__restrict__ copy,
-                                   const int *__restrict__ dof, double *bA, double *bB, double *bC, int m,
-                                   double *r, double t)
+                                   const int *__restrict__ dof, gf_gmat gm, double *r, double t)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n_ln) return;
@@ -99,7 +71,7 @@ __global__ void nodal_loads_kernel(int n_ln, const int *__restrict__ ln_node, co
 #pragma unroll
             for (int i = 0; i < 3; i++)
 #pragma unroll
-                for (int j = 0; j < 3; j++) kadd(bA, bB, bC, m, R[3 + i], R[3 + j], -S.a[3 * i + j]);
+                for (int j = 0; j < 3; j++) gm_add(gm, R[3 + i], R[3 + j], -S.a[3 * i + j]);
         } else if (kind == GF_LOAD_NODAL_FOLLOWER) {
             v3<double> ai = mk3(copy[6 * node + 3], copy[6 * node + 4], copy[6 * node + 5]);
             m3<double> Qi, Xii, Q, Xi;
@@ -124,8 +96,8 @@ __global__ void nodal_loads_kernel(int n_ln, const int *__restrict__ ln_node, co
             for (int i = 0; i < 3; i++)
 #pragma unroll
                 for (int j = 0; j < 3; j++) {
-                    kadd(bA, bB, bC, m, R[i], R[3 + j], -K12.a[3 * i + j]);
-                    kadd(bA, bB, bC, m, R[3 + i], R[3 + j], -K22.a[3 * i + j]);
+                    gm_add(gm, R[i], R[3 + j], -K12.a[3 * i + j]);
+                    gm_add(gm, R[3 + i], R[3 + j], -K22.a[3 * i + j]);
                 }
         }
     }
@@ -213,13 +185,12 @@ __global__ void node_commit_kernel(int nn, double *__restrict__ copy, const doub
     euler[3 * (size_t)i + 2] = s * a3.z;
 }
 
-// Unit diagonal for the padding rows past n_free in the last block.
-__global__ void pad_diag_kernel(double *bB, int m, int nblk, int n_free)
+// Unit diagonal for the padding rows past n_free.
+__global__ void pad_diag_kernel(gf_gmat gm, int npad, int n_free)
 {
     int i = n_free + blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nblk * m) return;
-    int bi = i / m, lr = i - bi * m;
-    bB[((size_t)bi * m + lr) * m + lr] = 1.0;
+    if (i >= npad) return;
+    gm_add(gm, i, i, 1.0);
 }
 
 }  // namespace
@@ -254,8 +225,7 @@ int gf_nodal_loads(gf_model *m, double t)
     if (m->n_load_nodes == 0) return GF_OK;
     hipLaunchKernelGGL(nodal_loads_kernel, dim3((m->n_load_nodes + 127) / 128), dim3(128), 0, m->stream,
                        m->n_load_nodes, m->d_ln_node, m->d_ln_off, m->d_ln_ids, m->d_lrec, m->d_tab_t, m->d_tab_v,
-                       m->d_disp, m->d_copy, m->d_dof, (double *)m->btd.A, (double *)m->btd.B, (double *)m->btd.C,
-                       m->btd.m, m->d_r, t);
+                       m->d_disp, m->d_copy, m->d_dof, gf_model_gmat(m), m->d_r, t);
     GF_HIP(hipGetLastError());
     return GF_OK;
 }
@@ -290,10 +260,10 @@ int gf_node_commit(gf_model *m)
 
 int gf_pad_diag(gf_model *m)
 {
-    int n = m->btd.nblk * m->btd.m - m->n_free;
+    int n = m->npad - m->n_free;
     if (n <= 0) return GF_OK;
-    hipLaunchKernelGGL(pad_diag_kernel, dim3((n + 63) / 64), dim3(64), 0, m->stream, (double *)m->btd.B, m->btd.m,
-                       m->btd.nblk, m->n_free);
+    hipLaunchKernelGGL(pad_diag_kernel, dim3((n + 63) / 64), dim3(64), 0, m->stream, gf_model_gmat(m), m->npad,
+                       m->n_free);
     GF_HIP(hipGetLastError());
     return GF_OK;
 }

@@ -58,6 +58,8 @@ def load():
         getattr(L, f).argtypes = [vp, dp]
     L.gf_read_tangent_dense.argtypes = [vp, dp, C.c_longlong]
     L.gf_beam_eval.argtypes = [C.c_int, dp, dp, dp, dp, dp, dp, dp]
+    L.gf_shell_eval.argtypes = [C.c_int, dp, dp, dp, dp, dp, dp]
+    L.gf_bandsystem.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.POINTER(C.c_int)]
     L.gf_reset_state.argtypes = [vp]
     L.gf_timing_enable.argtypes = [vp, C.c_int]
     L.gf_timing_read.argtypes = [vp, dp, C.POINTER(C.c_longlong), C.c_int]
@@ -207,3 +209,27 @@ def beam_eval(x, cs, prop, d, gp):
     f = np.zeros((n, 18))
     check(load().gf_beam_eval(n, _dp(x), _dp(cs), _dp(prop), _dp(d), _dp(gp), _dp(K), _dp(f)), "gf_beam_eval")
     return K, f
+
+
+def shell_eval(x, prop, d, gp):
+    """Element-level Shell_1::Mount on the device for independent elements.
+
+    x [n,18], prop [n,4], d [n,36], gp [n,3,21] -> K [n,27,27], f [n,27]."""
+    x = np.ascontiguousarray(x, float)
+    n = x.shape[0]
+    K = np.zeros((n, 27, 27))
+    f = np.zeros((n, 27))
+    check(load().gf_shell_eval(n, _dp(x), _dp(np.ascontiguousarray(prop, float)), _dp(np.ascontiguousarray(d, float)),
+                               _dp(np.ascontiguousarray(gp, float)), _dp(K), _dp(f)), "gf_shell_eval")
+    return K, f
+
+
+def bandsystem(A_rows, hb, b):
+    """Solve with the device band LU; A_rows [n, 2hb+1] (row i: columns i-hb..i+hb)."""
+    A_rows = np.ascontiguousarray(A_rows, float)
+    n = A_rows.shape[0]
+    x = np.zeros(n)
+    fail = C.c_int(0)
+    check(load().gf_bandsystem(n, hb, _dp(A_rows), _dp(np.ascontiguousarray(b, float)), _dp(x), C.byref(fail)),
+          "gf_bandsystem")
+    return x, fail.value

@@ -140,7 +140,8 @@ int gf_default_options(gf_options *opt);
 /* ---- model lifetime: Database::PreCalc + DOFsActive + SetGlobalDOFs ------- */
 int gf_model_create(const gf_model_desc *desc, const gf_options *opt, gf_model **out);
 int gf_model_destroy(gf_model *m);
-/* n_free, half-bandwidth, solver block size, colours, bytes of the factor. */
+/* n_free, half-bandwidth, BTD block size (0 for the band layout), block count or
+ * padded order, colours, bytes of the tangent storage, layout (0 BTD, 1 band), precision. */
 int gf_model_info(const gf_model *m, long long *info_out /* [8] */);
 
 /* ---- Newton step pieces --------------------------------------------------- */
@@ -202,6 +203,23 @@ int gf_read_tangent_dense(gf_model *m, double *out, long long cap);
  * Outputs the global-frame 18x18 tangent (row-major) and internal force 18. */
 int gf_beam_eval(int count, const double *x, const double *cs, const double *prop,
                  const double *d, const double *gp, double *K_out, double *f_out);
+
+/* ---- standalone band solve -------------------------------------------------
+ * Replaces bandsystem (src/BandMatrix.cpp:257-287, LAPACK dgbtrf/dgbtrs) with
+ * the device band LU (no row interchanges) for one right-hand side.
+ * ab_rows[i*(2hb+1) + (c - i + hb)] = A(i, c) for |i - c| <= hb (entries with c
+ * outside [0, n) are ignored).  *info_fail = 1 on a zero or non-finite pivot. */
+int gf_bandsystem(int n, int hb, const double *ab_rows, const double *b, double *x, int *info_fail);
+
+/* ---- element-level parity entry (Shell_1::Mount on the device) ------------
+ * x[18] reference coordinates of nodes 1..6, prop[GF_SHELL_NPROP], d[36]
+ * incremental displacements of the 6 nodes (6 per node; corner rotations are
+ * ignored), gp[3][21] Gauss-point state (Q_i row-major 9, kappa_r1_i 3,
+ * kappa_r2_i 3, z_x1_i 3, z_x2_i 3, local frame).  Outputs the global-frame
+ * 27x27 tangent (row-major, DOF order of Shell_1::MountGlobal: u of nodes
+ * 1..6, then alpha of nodes 4..6) and internal force 27. */
+int gf_shell_eval(int count, const double *x, const double *prop, const double *d,
+                  const double *gp, double *K_out, double *f_out);
 
 #ifdef __cplusplus
 }

@@ -524,6 +524,9 @@ static int mkl_state = 0; /* 0 unknown, 1 ok, -1 absent */
 int gfo_pardiso_available(void)
 {
     if (mkl_state) return mkl_state > 0;
+    /* one OpenMP runtime in the process: MKL threads through libgomp like the
+       element loop (two runtimes spinning on the same cores stall each other) */
+    setenv("MKL_THREADING_LAYER", "GNU", 0);
     const char *names[] = {"libmkl_rt.so.1", "libmkl_rt.so", "/opt/conda/lib/libmkl_rt.so.1", NULL};
     void *h = NULL;
     for (int i = 0; names[i] && !h; i++) h = dlopen(names[i], RTLD_NOW | RTLD_GLOBAL);
@@ -809,7 +812,7 @@ int gfo_static_solve(gfo_model *m, const gf_static_params *p, gf_static_report *
     if (st != GF_OK) return st;
     conv_state cs = {0.0, 0.0, 0};
     int aborted = 0, convergence_counter = 0, converged_number = 0, diverged = 0;
-    int verbose = getenv("GFO_VERBOSE") != NULL;
+    int verbose = getenv("GFO_VERBOSE") ? atoi(getenv("GFO_VERBOSE")) : 0;
     double time_step = p->time_step;
     double time = p->start_time;
     double last_converged = p->start_time;
@@ -858,6 +861,10 @@ int gfo_static_solve(gfo_model *m, const gf_static_params *p, gf_static_report *
                 double ts = now_ms();
                 gfo_solve(m, &info_fail);
                 r.t_solve_ms += now_ms() - ts;
+                if (verbose > 1) {
+                    fprintf(stderr, "oracle: time %.6f iteration %d solved (%.1f s)\n", time, it, (now_ms() - t0) * 1e-3);
+                    fflush(stderr);
+                }
                 r.total_solves++;
                 res_conv = 0;
                 double inc[5];
