@@ -27,12 +27,16 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 template <typename T> struct mfma;
+// Operands of both: A(i, k) in lane i + 16 k, B(k, j) in lane j + 16 k.  The
+// 16 x 16 result differs: item v of lane l holds row 4 v + l / 16 for f64 and
+// row 4 (l / 16) + v for f32, column l % 16 in both (checked on gfx950).
 template <> struct mfma<double> {
     typedef v4d acc_t;
     static __device__ __forceinline__ acc_t run(double a, double b, acc_t c)
     {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * v + (lane >> 4); }
 };
 template <> struct mfma<float> {
     typedef v4f acc_t;
@@ -40,6 +44,7 @@ template <> struct mfma<float> {
     {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
 __device__ __forceinline__ size_t bidx(int i, int c, int hb) { return (size_t)i * (size_t)(2 * hb) + (size_t)(c + hb); }
@@ -198,7 +203,7 @@ template <typename T> __global__ __launch_bounds__(256) void band_update_kernel(
         for (int b = 0; b < 2; b++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                int i = wr + 16 * a + 4 * (lane >> 4) + v, j = wc + 16 * b + (lane & 15);
+                int i = wr + 16 * a + mfma<T>::row(lane, v), j = wc + 16 * b + (lane & 15);
                 int gi = R0 + i, gj = C0 + j;
                 // rows/columns past the window carry a zero update (masked operands)
                 if (gi < n && gj < n && inband(gi, gj, hb)) {
@@ -221,11 +226,14 @@ template <typename T> __global__ __launch_bounds__(256) void band_update_kernel(
 
 // ---- triangular solves ------------------------------------------------------------
 // Forward, panel k0: every workgroup stages L11 in LDS and wave 0 solves
-// L11 y_p = y_p (unit lower) with register shuffles; workgroup 0 stores y_p;
-// then each wave takes rows below (one row per wave, coalesced along the row)
-// and subtracts L21(row, :) y_p.  Launches follow panel order.
+// L11 y_p = b_p (unit lower) with register shuffles; workgroup 0 stores y_p in
+// z; then each wave takes rows below (one row per wave, coalesced along the
+// row) and subtracts L21(row, :) y_p from b.  b_p is only read and the rows
+// below are only written, so the workgroups of one launch never race.
+// Launches follow panel order; z holds the forward solution at the end.
 template <typename T, typename V>
-__global__ __launch_bounds__(256) void band_fwd_kernel(const T *ab, int hb, int k0, int n, V *y, int rows_per_wg)
+__global__ __launch_bounds__(256) void band_fwd_kernel(const T *ab, int hb, int k0, int n, V *b, V *z,
+                                                      int rows_per_wg)
 {
     __shared__ T sl[NB * (NB + 1)];
     __shared__ V sy[NB];
@@ -236,13 +244,13 @@ __global__ __launch_bounds__(256) void band_fwd_kernel(const T *ab, int hb, int 
     }
     __syncthreads();
     if (wave == 0) {
-        V yi = y[k0 + lane];
+        V yi = b[k0 + lane];
         for (int c = 0; c < NB - 1; c++) {
             V yc = __shfl(yi, c);
             if (lane > c) yi -= (V)sl[lane * (NB + 1) + c] * yc;
         }
         sy[lane] = yi;
-        if (blockIdx.x == 0) y[k0 + lane] = yi;
+        if (blockIdx.x == 0) z[k0 + lane] = yi;
     }
     __syncthreads();
     const int lim = min(n, k0 + NB + hb);
@@ -254,14 +262,15 @@ __global__ __launch_bounds__(256) void band_fwd_kernel(const T *ab, int hb, int 
         int c = k0 + lane;
         if (r - c <= hb) s = (V)ab[bidx(r, c, hb)] * yl;
         for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-        if (lane == 0) y[r] -= s;
+        if (lane == 0) b[r] -= s;
     }
 }
 
-// Backward, panel k0 (from the last): wave 0 solves U11 x_p = y_p; rows above
-// that couple to the panel subtract U(row, panel) x_p.
+// Backward, panel k0 (from the last): wave 0 solves U11 x_p = z_p and
+// workgroup 0 stores x_p in x; rows above that couple to the panel subtract
+// U(row, panel) x_p from z.  Same read/write split as the forward sweep.
 template <typename T, typename V>
-__global__ __launch_bounds__(256) void band_bwd_kernel(const T *ab, int hb, int k0, V *y, int rows_per_wg)
+__global__ __launch_bounds__(256) void band_bwd_kernel(const T *ab, int hb, int k0, V *z, V *x, int rows_per_wg)
 {
     __shared__ T su[NB * (NB + 1)];
     __shared__ V sx[NB];
@@ -272,14 +281,14 @@ __global__ __launch_bounds__(256) void band_bwd_kernel(const T *ab, int hb, int 
     }
     __syncthreads();
     if (wave == 0) {
-        V yi = y[k0 + lane];
+        V yi = z[k0 + lane];
         for (int c = NB - 1; c >= 0; c--) {
             V xc = __shfl(yi, c) / (V)su[c * (NB + 1) + c];
             if (lane == c) yi = xc;
             else if (lane < c) yi -= (V)su[lane * (NB + 1) + c] * xc;
         }
         sx[lane] = yi;
-        if (blockIdx.x == 0) y[k0 + lane] = yi;
+        if (blockIdx.x == 0) x[k0 + lane] = yi;
     }
     __syncthreads();
     const int top = max(0, k0 - hb);
@@ -291,7 +300,7 @@ __global__ __launch_bounds__(256) void band_bwd_kernel(const T *ab, int hb, int 
         int c = k0 + lane;
         if (c - r <= hb) s = (V)ab[bidx(r, c, hb)] * xl;
         for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-        if (lane == 0) y[r] -= s;
+        if (lane == 0) z[r] -= s;
     }
 }
 
@@ -316,7 +325,8 @@ template <typename T> static int band_factor_t(T *ab, int n, int hb, hipStream_t
     return launches;
 }
 
-template <typename T, typename V> static int band_solve_t(const T *ab, int n, int hb, hipStream_t s, V *x)
+// x: right-hand side in, solution out; z: n entries of work space.
+template <typename T, typename V> static int band_solve_t(const T *ab, int n, int hb, hipStream_t s, V *x, V *z)
 {
     const int RPW = 64;  // coupled rows per workgroup
     int launches = 0;
@@ -324,13 +334,13 @@ template <typename T, typename V> static int band_solve_t(const T *ab, int n, in
         int lim = std::min(n, k0 + NB + hb);
         int rows = std::max(0, lim - (k0 + NB));
         int grid = std::max(1, (rows + RPW - 1) / RPW);
-        hipLaunchKernelGGL((band_fwd_kernel<T, V>), dim3(grid), dim3(256), 0, s, ab, hb, k0, n, x, RPW);
+        hipLaunchKernelGGL((band_fwd_kernel<T, V>), dim3(grid), dim3(256), 0, s, ab, hb, k0, n, x, z, RPW);
         launches++;
     }
     for (int k0 = n - NB; k0 >= 0; k0 -= NB) {
         int rows = k0 - std::max(0, k0 - hb);
         int grid = std::max(1, (rows + RPW - 1) / RPW);
-        hipLaunchKernelGGL((band_bwd_kernel<T, V>), dim3(grid), dim3(256), 0, s, ab, hb, k0, x, RPW);
+        hipLaunchKernelGGL((band_bwd_kernel<T, V>), dim3(grid), dim3(256), 0, s, ab, hb, k0, z, x, RPW);
         launches++;
     }
     GF_HIP(hipGetLastError());
@@ -347,10 +357,10 @@ int gf_band_factor(void *ab, int elem_size, int n, int hb, hipStream_t s, int *f
     return band_factor_t<float>((float *)ab, n, hb, s, flag);
 }
 
-int gf_band_solve(const void *ab, int elem_size, int n, int hb, hipStream_t s, double *x)
+int gf_band_solve(const void *ab, int elem_size, int n, int hb, hipStream_t s, double *x, double *work)
 {
-    if (elem_size == 8) return band_solve_t<double, double>((const double *)ab, n, hb, s, x);
-    return band_solve_t<float, double>((const float *)ab, n, hb, s, x);
+    if (elem_size == 8) return band_solve_t<double, double>((const double *)ab, n, hb, s, x, work);
+    return band_solve_t<float, double>((const float *)ab, n, hb, s, x, work);
 }
 
 int gf_band_pad(int n) { return (n + NB - 1) / NB * NB; }

@@ -5,335 +5,638 @@
 // is block-tridiagonal: rows of block i couple only to blocks i-1 (A_i), i (B_i)
 // and i+1 (C_i).  Cyclic reduction removes the even positions of the active
 // block rows at each level (all independent), keeping the odd ones:
-//   elimination row e:  B_e = P L U,  X_e = B_e^-1 A_e,  Y_e = B_e^-1 C_e
+//   eliminated row e:  Binv_e = B_e^-1,  X_e = Binv_e A_e,  Y_e = Binv_e C_e
 //   kept row k (neighbours l = p-1, r = p+1):
 //       B_k -= A_k Y_l + C_k X_r,   A_k <- -A_k X_l,   C_k <- -C_k Y_r
-// until one block row is left.  The right-hand side follows the same levels
-// (z_e = B_e^-1 f_e, f_k -= A_k z_l + C_k z_r with the pre-update A_k, C_k)
-// and back-substitution runs the levels in reverse (x_e = z_e - X_e x_l - Y_e x_r).
-// Block LU uses partial pivoting inside each m x m diagonal block.
-// Active row p at level L is global block row (p+1)*2^L - 1.
+// until one block row (the top) is left.  The right-hand side follows the same
+// levels (z_e = Binv_e f_e, f_k -= A_k z_l + C_k z_r with the pre-update A_k,
+// C_k) and back-substitution runs the levels in reverse
+// (x_e = z_e - X_e x_l - Y_e x_r).  Active row p at level L is global block
+// row (p+1)*2^L - 1.
 //
-// One 64-lane wavefront (one workgroup) per block row, blocks staged in LDS,
-// m <= 32.
+// gfx950 mapping:
+//  * one workgroup of two wavefronts per kept row and level: each wavefront
+//    eliminates one neighbour with an in-register Gauss-Jordan sweep over the
+//    augmented [B | I | A | C] (lane = column, 4 MT columns over 64 lanes,
+//    threshold partial pivoting with wave-uniform pivot choice), then the two
+//    waves form the kept-row products with 16x16x4 MFMA from LDS.  Both kept
+//    neighbours of an eliminated row recompute it (one writes it), so a level
+//    is one launch and no workgroup waits on another;
+//  * the eliminated-row results go to separate arrays (Bi, X, Y), so the
+//    redundant readers never see a half-written block and the assembled
+//    blocks of eliminated rows stay untouched;
+//  * the last levels (few rows) run in one workgroup with barriers between
+//    levels, together with the top and their back-substitution;
+//  * the right-hand side can ride along with the factorisation (factor_solve),
+//    so a Newton step pays one launch per level plus one per back level.
+// Blocks are m x m row-major with m <= 32; kernels work on MT = 16 or 32
+// (padding rows/columns get identity on B and zeros elsewhere).
 #include "gf_internal.h"
-#include <cmath>
+
+#include <algorithm>
 
 namespace {
 
-constexpr int MMAX = 32;
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// 16x16x4 MFMA.  A(i, k) in lane i + 16 k, B(k, j) in lane j + 16 k; result item v
+// of lane l is row 4 v + l / 16 (f64) or 4 (l / 16) + v (f32), column l % 16.
+template <typename T> struct mma;
+template <> struct mma<double> {
+    typedef v4d acc_t;
+    static __device__ __forceinline__ acc_t run(double a, double b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * v + (lane >> 4); }
+};
+template <> struct mma<float> {
+    typedef v4f acc_t;
+    static __device__ __forceinline__ acc_t run(float a, float b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+__device__ __forceinline__ double rdl(double v, int l)
+{
+    int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float rdl(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
 
 __device__ __forceinline__ int lvl_row(int p, int L) { return ((p + 1) << L) - 1; }
 
-template <typename T> __device__ __forceinline__ bool bad(T v) { return !(v == v) || fabs((double)v) >= 1e300 || v == T(0); }
-
-// LU with partial pivoting of sB (m x m, row-major) in LDS; piv -> global.
-template <typename T>
-__device__ void wave_lu(T *sB, int m, int *piv_out, int *flag, int *spiv)
+template <typename T> __device__ __forceinline__ bool bad(T v)
 {
-    const int lane = threadIdx.x;
-    for (int k = 0; k < m; k++) {
-        // pivot search in column k (m <= 32 <= 64 lanes)
-        T v = (lane >= k && lane < m) ? (T)fabs((double)sB[lane * m + k]) : T(-1);
-        int idx = lane;
-        for (int off = 32; off > 0; off >>= 1) {
-            T ov = __shfl_xor(v, off);
-            int oi = __shfl_xor(idx, off);
-            if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
-        }
-        int p = idx;
-        if (lane == 0) spiv[k] = p;
-        if (p != k && lane < m) {
-            T t = sB[k * m + lane];
-            sB[k * m + lane] = sB[p * m + lane];
-            sB[p * m + lane] = t;
-        }
-        __syncthreads();
-        T pivot = sB[k * m + k];
-        if (bad(pivot)) {
-            if (lane == 0) atomicExch(flag, 1);
-            pivot = T(1);
-        }
-        if (lane > k && lane < m) sB[lane * m + k] /= pivot;
-        __syncthreads();
-        int w = m - k - 1;
-        for (int t = lane; t < w * w; t += 64) {
-            int i = k + 1 + t / w, j = k + 1 + t % w;
-            sB[i * m + j] -= sB[i * m + k] * sB[k * m + j];
-        }
-        __syncthreads();
-    }
-    if (lane < m) piv_out[lane] = spiv[lane];
+    double a = fabs((double)v);
+    return !(a > 0.0) || a >= 1e300 || !(v == v) || (sizeof(T) == 4 && a >= 3.0e38);
 }
 
-// Solve (P L U) y = rhs for the column `col` held by this lane (strided by ld in LDS).
-template <typename T>
-__device__ __forceinline__ void lane_lu_solve(const T *sB, const int *spiv, int m, T *c, int ld)
+template <int MT> struct cfg {
+    static constexpr int S = MT / 16;        // augmented columns per lane (4 MT / 64)
+    static constexpr int LD = MT + 1;        // LDS leading dimension (odd: no bank conflicts)
+    static constexpr int BLK = MT * LD;      // one padded block in LDS
+    static constexpr int PAIRS = MT == 16 ? 4 : 1;  // wave pairs of the tail workgroup
+};
+
+// Per wave pair LDS: eliminated neighbours [l, r] x [Binv, X, Y], kept row A_k, C_k, z_l, z_r.
+template <typename T, int MT> struct PairSmem {
+    T e[2][3][cfg<MT>::BLK];
+    T k[2][cfg<MT>::BLK];
+    T z[2][MT];
+};
+
+// Loads the augmented [B | I | A | C] of block row e into the wave's registers
+// (column c = lane + 64 s).  hasA / hasC: couplings that exist at this level.
+template <typename T, int MT>
+__device__ __forceinline__ void load_aug(T (&col)[cfg<MT>::S][MT], const T *A, const T *B, const T *C, int e, int m,
+                                         bool hasA, bool hasC)
 {
-    for (int k = 0; k < m; k++) {
-        int p = spiv[k];
-        if (p != k) { T t = c[k * ld]; c[k * ld] = c[p * ld]; c[p * ld] = t; }
-    }
-    for (int i = 1; i < m; i++) {
-        T s = c[i * ld];
-        for (int k = 0; k < i; k++) s -= sB[i * m + k] * c[k * ld];
-        c[i * ld] = s;
-    }
-    for (int i = m - 1; i >= 0; i--) {
-        T s = c[i * ld];
-        for (int k = i + 1; k < m; k++) s -= sB[i * m + k] * c[k * ld];
-        c[i * ld] = s / sB[i * m + i];
+    const int lane = threadIdx.x & 63;
+    const size_t bs = (size_t)m * m, o = (size_t)e * bs;
+#pragma unroll
+    for (int s = 0; s < cfg<MT>::S; s++) {
+        const int c = lane + 64 * s, part = c / MT, cc = c % MT;
+        const T *src = part == 0 ? B : part == 2 ? A : C;
+        const bool live = part == 0 || (part == 2 && hasA) || (part == 3 && hasC);
+#pragma unroll
+        for (int i = 0; i < MT; i++) {
+            T v = T(0);
+            if (part == 1) v = i == cc ? T(1) : T(0);
+            else if (i < m && cc < m) { if (live) v = src[o + (size_t)i * m + cc]; }
+            else if (part == 0 && i == cc) v = T(1);
+            col[s][i] = v;
+        }
     }
 }
 
-template <typename T>
-__global__ __launch_bounds__(64) void btd_elim_kernel(int L, int nL, int m, T *A, T *B, T *C, int *piv, int *flag)
+// Gauss-Jordan on the wave-held augmented matrix: [B | I | A | C] -> [I | Binv | X | Y].
+// Pivot: the largest |entry| of column k in rows >= k unless |B_kk| >= half of it.
+template <typename T, int MT> __device__ void wave_gj(T (&col)[cfg<MT>::S][MT], int *flag)
 {
-    __shared__ T sB[MMAX * MMAX];
-    __shared__ T sR[MMAX * 2 * MMAX];  // [m][2m]: X | Y right-hand sides
-    __shared__ int spiv[MMAX];
-    const int p = 2 * blockIdx.x;      // even positions are eliminated
-    if (p >= nL) return;
-    const int e = lvl_row(p, L);
+    constexpr int S = cfg<MT>::S;
+#pragma unroll
+    for (int k = 0; k < MT; k++) {
+        T mk[MT];
+#pragma unroll
+        for (int i = 0; i < MT; i++) mk[i] = rdl(col[0][i], k);  // column k sits in lane k, slot 0
+        T amax = T(0);
+        int p = k;
+#pragma unroll
+        for (int i = k; i < MT; i++) {
+            T a = fabs(mk[i]);
+            if (a > amax) { amax = a; p = i; }
+        }
+        if (fabs(mk[k]) < T(0.5) * amax) {
+#pragma unroll
+            for (int s = 0; s < S; s++) {
+                T ck = col[s][k], cp = ck;
+#pragma unroll
+                for (int i = k + 1; i < MT; i++) cp = i == p ? col[s][i] : cp;
+#pragma unroll
+                for (int i = k + 1; i < MT; i++) col[s][i] = i == p ? ck : col[s][i];
+                col[s][k] = cp;
+            }
+            T tk = mk[k], tp = tk;
+#pragma unroll
+            for (int i = k + 1; i < MT; i++) tp = i == p ? mk[i] : tp;
+#pragma unroll
+            for (int i = k + 1; i < MT; i++) mk[i] = i == p ? tk : mk[i];
+            mk[k] = tp;
+        }
+        T piv = mk[k];
+        if (bad(piv)) {
+            if ((threadIdx.x & 63) == 0) atomicExch(flag, 1);
+            piv = T(1);
+        }
+        const T inv = T(1) / piv;
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            const T rk = col[s][k] * inv;
+#pragma unroll
+            for (int i = 0; i < MT; i++)
+                if (i != k) col[s][i] -= mk[i] * rk;
+            col[s][k] = rk;
+        }
+    }
+}
+
+// Stores Binv, X, Y of the wave's augmented matrix into LDS (rows padded to LD)
+// and, when `to_global`, into Bi/X/Y of block row e.
+template <typename T, int MT>
+__device__ __forceinline__ void store_elim(const T (&col)[cfg<MT>::S][MT], T *sE, T *Bi, T *X, T *Y, int e, int m,
+                                           bool to_global)
+{
+    const int lane = threadIdx.x & 63;
+    const size_t bs = (size_t)m * m, o = (size_t)e * bs;
+#pragma unroll
+    for (int s = 0; s < cfg<MT>::S; s++) {
+        const int c = lane + 64 * s, part = c / MT, cc = c % MT;
+        if (part == 0) continue;
+        T *dst = sE + (part - 1) * cfg<MT>::BLK;
+        T *g = part == 1 ? Bi : part == 2 ? X : Y;
+#pragma unroll
+        for (int i = 0; i < MT; i++) {
+            dst[i * cfg<MT>::LD + cc] = col[s][i];
+            if (to_global && i < m && cc < m) g[o + (size_t)i * m + cc] = col[s][i];
+        }
+    }
+}
+
+// acc (MT/16 x MT/16 tiles) += P (LDS, LD) * Q (LDS, LD), MT x MT x MT, one wavefront.
+template <typename T, int MT>
+__device__ __forceinline__ void mma_acc(typename mma<T>::acc_t (&acc)[MT / 16][MT / 16], const T *P, const T *Q)
+{
+    const int lane = threadIdx.x & 63;
+    constexpr int LD = cfg<MT>::LD;
+#pragma unroll
+    for (int k0 = 0; k0 < MT; k0 += 4) {
+        T a[MT / 16], b[MT / 16];
+#pragma unroll
+        for (int t = 0; t < MT / 16; t++) {
+            a[t] = P[(16 * t + (lane & 15)) * LD + k0 + (lane >> 4)];
+            b[t] = Q[(k0 + (lane >> 4)) * LD + 16 * t + (lane & 15)];
+        }
+#pragma unroll
+        for (int ti = 0; ti < MT / 16; ti++)
+#pragma unroll
+            for (int tj = 0; tj < MT / 16; tj++) acc[ti][tj] = mma<T>::run(a[ti], b[tj], acc[ti][tj]);
+    }
+}
+
+// dst (global m x m block) = base - acc (base may be null: -acc).
+template <typename T, int MT>
+__device__ __forceinline__ void store_acc(const typename mma<T>::acc_t (&acc)[MT / 16][MT / 16], T *dst,
+                                          const T *base, int m)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int ti = 0; ti < MT / 16; ti++)
+#pragma unroll
+        for (int tj = 0; tj < MT / 16; tj++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                int i = 16 * ti + mma<T>::row(lane, v), j = 16 * tj + (lane & 15);
+                if (i < m && j < m) {
+                    size_t o = (size_t)i * m + j;
+                    dst[o] = (base ? base[o] : T(0)) - acc[ti][tj][v];
+                }
+            }
+}
+
+struct Fac {
+    void *A, *B, *C, *Bi, *X, *Y;
+    int m;
+    int *flag;
+};
+
+// One kept row q of level L (active rows nL), processed by the wave pair `wp`
+// (threads 128 wp .. 128 wp + 127 of the workgroup).  Every thread of the
+// workgroup must call it the same number of times (it holds barriers); pairs
+// with nothing to do pass active = false.
+// f: the right-hand side being reduced (null: factor only); zout: z of the
+// eliminated rows.
+template <typename T, int MT>
+__device__ void fac_step(const Fac &F, PairSmem<T, MT> &sm, bool active, int L, int nL, int q, T *Lsv, T *Rsv,
+                         double *f, double *zout)
+{
+    T *A = (T *)F.A, *B = (T *)F.B, *C = (T *)F.C;
+    const int m = F.m;
     const size_t bs = (size_t)m * m;
-    const int lane = threadIdx.x;
-    const bool hasL = p >= 1, hasR = p + 1 < nL;
-    for (int t = lane; t < m * m; t += 64) {
-        int i = t / m, j = t % m;
-        sB[t] = B[e * bs + t];
-        sR[i * 2 * m + j] = hasL ? A[e * bs + t] : T(0);
-        sR[i * 2 * m + m + j] = hasR ? C[e * bs + t] : T(0);
-    }
-    __syncthreads();
-    wave_lu(sB, m, piv + (size_t)e * m, flag, spiv);
-    if (lane < 2 * m) lane_lu_solve(sB, spiv, m, sR + lane, 2 * m);
-    __syncthreads();
-    for (int t = lane; t < m * m; t += 64) {
-        int i = t / m, j = t % m;
-        B[e * bs + t] = sB[t];
-        A[e * bs + t] = sR[i * 2 * m + j];        // X_e
-        C[e * bs + t] = sR[i * 2 * m + m + j];    // Y_e
-    }
-}
-
-template <typename T>
-__global__ __launch_bounds__(64) void btd_kept_kernel(int L, int nL, int m, T *A, T *B, T *C, T *Lsv, T *Rsv)
-{
-    __shared__ T sAk[MMAX * MMAX], sCk[MMAX * MMAX], sXl[MMAX * MMAX], sYl[MMAX * MMAX], sXr[MMAX * MMAX],
-        sYr[MMAX * MMAX];
-    const int q = blockIdx.x;
+    const int lane = threadIdx.x & 63, wv = (threadIdx.x >> 6) & 1;
     const int p = 2 * q + 1;
-    if (p >= nL) return;
-    const int k = lvl_row(p, L), l = lvl_row(p - 1, L);
-    const bool hasR = p + 1 < nL;
-    const int r = hasR ? lvl_row(p + 1, L) : 0;
-    const size_t bs = (size_t)m * m;
-    const int lane = threadIdx.x;
-    for (int t = lane; t < m * m; t += 64) {
-        T ak = A[k * bs + t], ck = C[k * bs + t];
-        sAk[t] = ak;
-        sCk[t] = ck;
-        Lsv[(size_t)q * bs + t] = ak;
-        Rsv[(size_t)q * bs + t] = ck;
-        sXl[t] = A[l * bs + t];
-        sYl[t] = C[l * bs + t];
-        sXr[t] = hasR ? A[r * bs + t] : T(0);
-        sYr[t] = hasR ? C[r * bs + t] : T(0);
+    const bool hasR = active && p + 1 < nL;
+    const int k = lvl_row(p, L), l = lvl_row(p - 1, L), r = hasR ? lvl_row(p + 1, L) : 0;
+    const bool doE = active && (wv == 0 || hasR);
+    const int e = wv == 0 ? l : r, pe = wv == 0 ? p - 1 : p + 1;
+    const bool own = wv == 0 || p + 2 == nL;  // the right neighbour is written here only if it is the last row
+    if (doE) {
+        T col[cfg<MT>::S][MT];
+        load_aug<T, MT>(col, A, B, C, e, m, pe >= 1, pe + 1 < nL);
+        wave_gj<T, MT>(col, F.flag);
+        store_elim<T, MT>(col, &sm.e[wv][0][0], (T *)F.Bi, (T *)F.X, (T *)F.Y, e, m, own);
+    } else if (active) {
+        // no right neighbour: zero X_r, Y_r so the products below need no special case
+        for (int t = lane; t < 3 * cfg<MT>::BLK; t += 64) (&sm.e[1][0][0])[t] = T(0);
+    }
+    if (active) {
+        // kept-row couplings (wave 0: A_k, wave 1: C_k), saved for later solves
+        const T *src = wv == 0 ? A : C;
+        T *sv = wv == 0 ? Lsv : Rsv;
+        T *dst = sm.k[wv];
+        for (int t = lane; t < MT * MT; t += 64) {
+            int i = t / MT, j = t % MT;
+            T v = T(0);
+            if (i < m && j < m) {
+                v = src[(size_t)k * bs + (size_t)i * m + j];
+                sv[(size_t)q * bs + (size_t)i * m + j] = v;
+            }
+            dst[i * cfg<MT>::LD + j] = v;
+        }
     }
     __syncthreads();
-    for (int t = lane; t < m * m; t += 64) {
-        int i = t / m, j = t % m;
-        T b = T(0), a = T(0), c = T(0);
-        for (int s = 0; s < m; s++) {
-            T aks = sAk[i * m + s], cks = sCk[i * m + s];
-            b += aks * sYl[s * m + j] + cks * sXr[s * m + j];
-            a += aks * sXl[s * m + j];
-            c += cks * sYr[s * m + j];
+    if (active && f) {
+        // z of the eliminated neighbour: Binv_e f_e
+        if (doE && lane < MT) {
+            T s = T(0);
+            if (lane < m)
+                for (int j = 0; j < m; j++) s += sm.e[wv][0][lane * cfg<MT>::LD + j] * (T)f[(size_t)e * m + j];
+            sm.z[wv][lane] = s;
+            if (own && lane < m) zout[(size_t)e * m + lane] = (double)s;
+        } else if (!doE && lane < MT) {
+            sm.z[wv][lane] = T(0);
         }
-        B[k * bs + t] -= b;
-        A[k * bs + t] = -a;
-        C[k * bs + t] = -c;
+    }
+    if (active) {
+        typedef typename mma<T>::acc_t acc_t;
+        if (wv == 0) {
+            acc_t acc[MT / 16][MT / 16];
+#pragma unroll
+            for (int a = 0; a < MT / 16; a++)
+#pragma unroll
+                for (int b = 0; b < MT / 16; b++) acc[a][b] = acc_t{};
+            mma_acc<T, MT>(acc, sm.k[0], sm.e[0][2]);  // A_k Y_l
+            mma_acc<T, MT>(acc, sm.k[1], sm.e[1][1]);  // C_k X_r
+            store_acc<T, MT>(acc, B + (size_t)k * bs, B + (size_t)k * bs, m);
+        } else {
+            acc_t acc[MT / 16][MT / 16];
+#pragma unroll
+            for (int a = 0; a < MT / 16; a++)
+#pragma unroll
+                for (int b = 0; b < MT / 16; b++) acc[a][b] = acc_t{};
+            mma_acc<T, MT>(acc, sm.k[0], sm.e[0][1]);  // A_k X_l
+            store_acc<T, MT>(acc, A + (size_t)k * bs, (const T *)nullptr, m);
+#pragma unroll
+            for (int a = 0; a < MT / 16; a++)
+#pragma unroll
+                for (int b = 0; b < MT / 16; b++) acc[a][b] = acc_t{};
+            mma_acc<T, MT>(acc, sm.k[1], sm.e[1][2]);  // C_k Y_r
+            store_acc<T, MT>(acc, C + (size_t)k * bs, (const T *)nullptr, m);
+        }
+    }
+    __syncthreads();
+    if (active && f && wv == 0 && lane < m) {
+        T s = T(0);
+        for (int j = 0; j < m; j++)
+            s += sm.k[0][lane * cfg<MT>::LD + j] * sm.z[0][j] + sm.k[1][lane * cfg<MT>::LD + j] * sm.z[1][j];
+        f[(size_t)k * m + lane] -= (double)s;
+    }
+    __syncthreads();
+}
+
+// Top row (the single active row after the last level): Binv_top, and
+// zout_top = Binv_top f_top when f is given.  One wavefront (wave 0 of pair 0).
+template <typename T, int MT>
+__device__ void top_step(const Fac &F, PairSmem<T, MT> &sm, int levels, const double *f, double *zout)
+{
+    const int m = F.m, lane = threadIdx.x & 63;
+    const int e = lvl_row(0, levels);
+    if (threadIdx.x < 64) {
+        T col[cfg<MT>::S][MT];
+        load_aug<T, MT>(col, (const T *)F.A, (const T *)F.B, (const T *)F.C, e, m, false, false);
+        wave_gj<T, MT>(col, F.flag);
+        store_elim<T, MT>(col, &sm.e[0][0][0], (T *)F.Bi, (T *)F.X, (T *)F.Y, e, m, true);
+        if (f && lane < m) {
+            T s = T(0);
+            for (int j = 0; j < m; j++) s += sm.e[0][0][lane * cfg<MT>::LD + j] * (T)f[(size_t)e * m + j];
+            zout[(size_t)e * m + lane] = (double)s;
+        }
+    }
+    __syncthreads();
+}
+
+// Back-substitution of level L for eliminated rows [tid0 .. ) strided by nthr:
+// x_e = z_e - X_e x_l - Y_e x_r (x holds z_e on entry for eliminated rows).
+template <typename T>
+__device__ __forceinline__ void back_rows(const T *X, const T *Y, int m, int L, int nL, double *x, int tid0, int nthr)
+{
+    const int ne = (nL + 1) / 2;
+    const size_t bs = (size_t)m * m;
+    for (int t = tid0; t < ne * m; t += nthr) {
+        const int pe = 2 * (t / m), i = t % m;
+        const int e = lvl_row(pe, L);
+        double s = x[(size_t)e * m + i];
+        if (pe >= 1) {
+            const int l = lvl_row(pe - 1, L);
+            const T *Xr = X + (size_t)e * bs + (size_t)i * m;
+            for (int j = 0; j < m; j++) s -= (double)Xr[j] * x[(size_t)l * m + j];
+        }
+        if (pe + 1 < nL) {
+            const int r = lvl_row(pe + 1, L);
+            const T *Yr = Y + (size_t)e * bs + (size_t)i * m;
+            for (int j = 0; j < m; j++) s -= (double)Yr[j] * x[(size_t)r * m + j];
+        }
+        x[(size_t)e * m + i] = s;
+    }
+}
+
+// ---- factor kernels -----------------------------------------------------------
+template <typename T, int MT>
+__global__ __launch_bounds__(128) void btd_level_kernel(Fac F, int L, int nL, T *Lsv, T *Rsv, double *f,
+                                                        double *zout)
+{
+    __shared__ PairSmem<T, MT> sm;
+    fac_step<T, MT>(F, sm, true, L, nL, blockIdx.x, Lsv, Rsv, f, zout);
+}
+
+// Levels L0 .. levels-1, the top and (with f) the back-substitution of those
+// levels, in one workgroup of PAIRS wave pairs.
+template <typename T, int MT>
+__global__ __launch_bounds__(128 * cfg<MT>::PAIRS) void btd_tail_kernel(Fac F, int L0, int levels, int nL0,
+                                                                        T *Lsv, T *Rsv, const long long *save,
+                                                                        double *f, double *x)
+{
+    constexpr int PAIRS = cfg<MT>::PAIRS;
+    __shared__ PairSmem<T, MT> sm[PAIRS];
+    const int wp = threadIdx.x >> 7;
+    int nL = nL0;
+    for (int L = L0; L < levels; L++) {
+        const int nk = nL / 2;
+        for (int base = 0; base < nk; base += PAIRS) {
+            const int q = base + wp;
+            fac_step<T, MT>(F, sm[wp], q < nk, L, nL, q < nk ? q : 0, Lsv + save[L] * (size_t)F.m * F.m,
+                            Rsv + save[L] * (size_t)F.m * F.m, f, x);
+        }
+        nL /= 2;
+    }
+    top_step<T, MT>(F, sm[0], levels, f, x);
+    if (!f) return;
+    // back-substitution of the tail levels (nL of level L recomputed downwards)
+    for (int L = levels - 1; L >= L0; L--) {
+        int n = nL0;
+        for (int j = L0; j < L; j++) n /= 2;
+        back_rows<T>((const T *)F.X, (const T *)F.Y, F.m, L, n, x, threadIdx.x, blockDim.x);
+        __syncthreads();
+    }
+}
+
+// ---- solve-only kernels (factors already in place) ------------------------------
+// Forward of level L: one wavefront per kept row q; lanes 0..31 handle z_l,
+// lanes 32..63 z_r, then f_k -= Lsv z_l + Rsv z_r.
+template <typename T>
+__device__ __forceinline__ void fwd_row(const T *Bi, const T *Lsv, const T *Rsv, int m, int L, int nL, int q,
+                                        double *f, double *zout, double *sz)
+{
+    const int lane = threadIdx.x & 63;
+    const int p = 2 * q + 1;
+    const bool hasR = p + 1 < nL;
+    const int k = lvl_row(p, L), l = lvl_row(p - 1, L), r = hasR ? lvl_row(p + 1, L) : 0;
+    const size_t bs = (size_t)m * m;
+    const int side = lane >> 5, i = lane & 31;
+    double z = 0.0;
+    if (i < m && (side == 0 || hasR)) {
+        const int e = side == 0 ? l : r;
+        const T *Br = Bi + (size_t)e * bs + (size_t)i * m;
+        for (int j = 0; j < m; j++) z += (double)Br[j] * f[(size_t)e * m + j];
+        if (side == 0 || p + 2 == nL) zout[(size_t)e * m + i] = z;
+    }
+    sz[lane] = z;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (lane < m) {
+        double s = 0.0;
+        const T *Lr = Lsv + (size_t)q * bs + (size_t)lane * m, *Rr = Rsv + (size_t)q * bs + (size_t)lane * m;
+        for (int j = 0; j < m; j++) s += (double)Lr[j] * sz[j] + (hasR ? (double)Rr[j] * sz[32 + j] : 0.0);
+        f[(size_t)k * m + lane] -= s;
     }
 }
 
 template <typename T>
-__global__ __launch_bounds__(64) void btd_top_kernel(int L, int m, T *B, int *piv, int *flag)
+__global__ __launch_bounds__(256) void btd_fwd_kernel(const T *Bi, const T *Lsv, const T *Rsv, int m, int L, int nL,
+                                                      double *f, double *zout)
 {
-    __shared__ T sB[MMAX * MMAX];
-    __shared__ int spiv[MMAX];
-    const int e = lvl_row(0, L);
-    const size_t bs = (size_t)m * m;
-    for (int t = threadIdx.x; t < m * m; t += 64) sB[t] = B[e * bs + t];
-    __syncthreads();
-    wave_lu(sB, m, piv + (size_t)e * m, flag, spiv);
-    for (int t = threadIdx.x; t < m * m; t += 64) B[e * bs + t] = sB[t];
+    __shared__ double sz[4][64];
+    const int wave = threadIdx.x >> 6;
+    const int q = blockIdx.x * 4 + wave;
+    if (q >= nL / 2) return;
+    fwd_row<T>(Bi, Lsv, Rsv, m, L, nL, q, f, zout, sz[wave]);
 }
 
-// ---- solve ------------------------------------------------------------------
-// z_e = B_e^-1 f_e for the eliminated rows of level L (in place in x).
-template <typename T, typename V>
-__global__ __launch_bounds__(64) void btd_fwd_elim_kernel(int L, int nL, int m, const T *B, const int *piv, V *x, int stride2)
+template <typename T>
+__global__ __launch_bounds__(1024) void btd_fwd_tail_kernel(const T *Bi, const T *X, const T *Y, const T *Lsv,
+                                                            const T *Rsv, const long long *save, int m, int L0,
+                                                            int levels, int nL0, double *f, double *x)
 {
-    __shared__ T sB[MMAX * MMAX];
-    __shared__ T sc[MMAX];
-    __shared__ int spiv[MMAX];
-    const int p = stride2 * blockIdx.x;
-    if (p >= nL) return;
-    const int e = lvl_row(p, L);
+    __shared__ double sz[16][64];
+    const int wave = threadIdx.x >> 6;
     const size_t bs = (size_t)m * m;
-    const int lane = threadIdx.x;
-    for (int t = lane; t < m * m; t += 64) sB[t] = B[e * bs + t];
-    if (lane < m) { sc[lane] = (T)x[(size_t)e * m + lane]; spiv[lane] = piv[(size_t)e * m + lane]; }
-    __syncthreads();
-    if (lane == 0) lane_lu_solve(sB, spiv, m, sc, 1);
-    __syncthreads();
-    if (lane < m) x[(size_t)e * m + lane] = (V)sc[lane];
-}
-
-// f_k -= Lsv z_l + Rsv z_r for the kept rows of level L.
-template <typename T, typename V>
-__global__ __launch_bounds__(64) void btd_fwd_kept_kernel(int L, int nL, int m, const T *Lsv, const T *Rsv, V *x)
-{
-    const int q = blockIdx.x;
-    const int p = 2 * q + 1;
-    if (p >= nL) return;
-    const int k = lvl_row(p, L), l = lvl_row(p - 1, L);
-    const bool hasR = p + 1 < nL;
-    const int r = hasR ? lvl_row(p + 1, L) : 0;
-    const size_t bs = (size_t)m * m;
-    const int i = threadIdx.x;
-    if (i >= m) return;
-    V s = V(0);
-    for (int j = 0; j < m; j++) {
-        s += (V)Lsv[(size_t)q * bs + i * m + j] * x[(size_t)l * m + j];
-        if (hasR) s += (V)Rsv[(size_t)q * bs + i * m + j] * x[(size_t)r * m + j];
+    int nL = nL0;
+    for (int L = L0; L < levels; L++) {
+        for (int q = wave; q < nL / 2; q += 16)
+            fwd_row<T>(Bi, Lsv + save[L] * bs, Rsv + save[L] * bs, m, L, nL, q, f, x, sz[wave]);
+        __syncthreads();
+        nL /= 2;
     }
-    x[(size_t)k * m + i] -= s;
-}
-
-// x_e = z_e - X_e x_l - Y_e x_r for the eliminated rows of level L.
-template <typename T, typename V>
-__global__ __launch_bounds__(64) void btd_bwd_kernel(int L, int nL, int m, const T *A, const T *C, V *x)
-{
-    __shared__ V sl[MMAX], sr[MMAX];
-    const int p = 2 * blockIdx.x;
-    if (p >= nL) return;
-    const int e = lvl_row(p, L);
-    const bool hasL = p >= 1, hasR = p + 1 < nL;
-    const int l = hasL ? lvl_row(p - 1, L) : 0, r = hasR ? lvl_row(p + 1, L) : 0;
-    const size_t bs = (size_t)m * m;
-    const int i = threadIdx.x;
-    if (i < m) {
-        sl[i] = hasL ? x[(size_t)l * m + i] : V(0);
-        sr[i] = hasR ? x[(size_t)r * m + i] : V(0);
+    const int top = lvl_row(0, levels);
+    if (threadIdx.x < (unsigned)m) {
+        double s = 0.0;
+        const T *Br = Bi + (size_t)top * bs + (size_t)threadIdx.x * m;
+        for (int j = 0; j < m; j++) s += (double)Br[j] * f[(size_t)top * m + j];
+        x[(size_t)top * m + threadIdx.x] = s;
     }
     __syncthreads();
-    if (i >= m) return;
-    V s = x[(size_t)e * m + i];
-    for (int j = 0; j < m; j++) s -= (V)A[e * bs + i * m + j] * sl[j] + (V)C[e * bs + i * m + j] * sr[j];
-    x[(size_t)e * m + i] = s;
+    for (int L = levels - 1; L >= L0; L--) {
+        int n = nL0;
+        for (int j = L0; j < L; j++) n /= 2;
+        back_rows<T>(X, Y, m, L, n, x, threadIdx.x, blockDim.x);
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void btd_back_kernel(const T *X, const T *Y, int m, int L, int nL, double *x)
+{
+    back_rows<T>(X, Y, m, L, nL, x, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
 }  // namespace
 
 int gf_btd_alloc(gf_btd &b, int m, int nblk, int elem_size)
 {
-    if (m < 1 || m > MMAX) {
+    if (m < 1 || m > 32) {
         gf_set_error("block-tridiagonal solver: block size out of range");
         return GF_E_UNSUPPORTED;
     }
     b.m = m;
+    b.mt = m <= 16 ? 16 : 32;
     b.nblk = nblk;
     b.bsz = (size_t)m * m;
     b.elem_size = elem_size;
     b.lvl_n.clear();
     b.lvl_save.clear();
+    const int tail_rows = b.mt == 16 ? 16 : 4;  // 4 x the tail workgroup's wave pairs
     int n = nblk, L = 0;
     size_t save = 0;
+    b.tail_level = -1;
     while (n > 1) {
+        if (b.tail_level < 0 && n <= tail_rows) b.tail_level = L;
         b.lvl_n.push_back(n);
-        b.lvl_save.push_back(save);
+        b.lvl_save.push_back((long long)save);
         save += n / 2;
         n /= 2;
         L++;
     }
     b.levels = L;
+    if (b.tail_level < 0) b.tail_level = L;
     size_t blocks = (size_t)nblk * b.bsz * elem_size;
     size_t sv = (save ? save : 1) * b.bsz * elem_size;
-    GF_HIP(hipMalloc(&b.A, blocks));
-    GF_HIP(hipMalloc(&b.B, blocks));
-    GF_HIP(hipMalloc(&b.C, blocks));
+    void **arrs[] = {&b.A, &b.B, &b.C, &b.Bi, &b.X, &b.Y};
+    for (void **a : arrs) GF_HIP(hipMalloc(a, blocks));
     GF_HIP(hipMalloc(&b.Lsv, sv));
     GF_HIP(hipMalloc(&b.Rsv, sv));
-    GF_HIP(hipMalloc(&b.piv, sizeof(int) * (size_t)nblk * m));
+    GF_HIP(hipMalloc(&b.d_save, sizeof(long long) * (b.lvl_save.size() + 1)));
+    std::vector<long long> sv_h(b.lvl_save);
+    sv_h.push_back((long long)save);
+    GF_HIP(hipMemcpy(b.d_save, sv_h.data(), sizeof(long long) * sv_h.size(), hipMemcpyHostToDevice));
+    // X and Y of rows that are never eliminated with a coupling stay zero
+    GF_HIP(hipMemset(b.X, 0, blocks));
+    GF_HIP(hipMemset(b.Y, 0, blocks));
     return GF_OK;
 }
 
 void gf_btd_free(gf_btd &b)
 {
-    (void)hipFree(b.A); (void)hipFree(b.B); (void)hipFree(b.C);
-    (void)hipFree(b.Lsv); (void)hipFree(b.Rsv); (void)hipFree(b.piv);
-    b.A = b.B = b.C = b.Lsv = b.Rsv = nullptr;
-    b.piv = nullptr;
+    void *arrs[] = {b.A, b.B, b.C, b.Bi, b.X, b.Y, b.Lsv, b.Rsv, b.d_save};
+    for (void *a : arrs)
+        if (a) (void)hipFree(a);
+    b.A = b.B = b.C = b.Bi = b.X = b.Y = b.Lsv = b.Rsv = nullptr;
+    b.d_save = nullptr;
 }
 
-template <typename T>
-static int btd_factor_t(gf_btd &b, hipStream_t s, int *d_flag)
+template <typename T, int MT>
+static int btd_factor_t(gf_btd &b, hipStream_t s, int *d_flag, double *f, double *x)
 {
-    T *A = (T *)b.A, *B = (T *)b.B, *C = (T *)b.C;
+    Fac F{b.A, b.B, b.C, b.Bi, b.X, b.Y, b.m, d_flag};
+    T *Lsv = (T *)b.Lsv, *Rsv = (T *)b.Rsv;
+    const size_t bs = b.bsz;
     int launches = 0;
-    for (int L = 0; L < b.levels; L++) {
-        int nL = b.lvl_n[L];
-        T *Lsv = (T *)b.Lsv + b.lvl_save[L] * b.bsz, *Rsv = (T *)b.Rsv + b.lvl_save[L] * b.bsz;
-        hipLaunchKernelGGL(btd_elim_kernel<T>, dim3((nL + 1) / 2), dim3(64), 0, s, L, nL, b.m, A, B, C, b.piv, d_flag);
+    for (int L = 0; L < b.tail_level; L++) {
+        const int nL = b.lvl_n[L];
+        hipLaunchKernelGGL((btd_level_kernel<T, MT>), dim3(nL / 2), dim3(128), 0, s, F, L, nL, Lsv + b.lvl_save[L] * bs,
+                           Rsv + b.lvl_save[L] * bs, f, x);
         launches++;
-        if (nL / 2 > 0) {
-            hipLaunchKernelGGL(btd_kept_kernel<T>, dim3(nL / 2), dim3(64), 0, s, L, nL, b.m, A, B, C, Lsv, Rsv);
+    }
+    const int nL0 = b.tail_level < b.levels ? b.lvl_n[b.tail_level] : 1;
+    hipLaunchKernelGGL((btd_tail_kernel<T, MT>), dim3(1), dim3(128 * cfg<MT>::PAIRS), 0, s, F, b.tail_level, b.levels,
+                       nL0, Lsv, Rsv, (const long long *)b.d_save, f, x);
+    launches++;
+    if (f) {
+        for (int L = b.tail_level - 1; L >= 0; L--) {
+            const int nL = b.lvl_n[L];
+            const int work = (nL + 1) / 2 * b.m;
+            hipLaunchKernelGGL((btd_back_kernel<T>), dim3((work + 255) / 256), dim3(256), 0, s, (const T *)b.X,
+                               (const T *)b.Y, b.m, L, nL, x);
             launches++;
         }
     }
-    hipLaunchKernelGGL(btd_top_kernel<T>, dim3(1), dim3(64), 0, s, b.levels, b.m, B, b.piv, d_flag);
-    launches++;
     GF_HIP(hipGetLastError());
     return launches;
+}
+
+template <typename T> static int btd_factor_dispatch(gf_btd &b, hipStream_t s, int *d_flag, double *f, double *x)
+{
+    return b.mt == 16 ? btd_factor_t<T, 16>(b, s, d_flag, f, x) : btd_factor_t<T, 32>(b, s, d_flag, f, x);
 }
 
 int gf_btd_factor(gf_btd &b, hipStream_t s, int *d_flag)
 {
-    if (b.elem_size == 8) return btd_factor_t<double>(b, s, d_flag);
-    return btd_factor_t<float>(b, s, d_flag);
+    if (b.elem_size == 8) return btd_factor_dispatch<double>(b, s, d_flag, nullptr, nullptr);
+    return btd_factor_dispatch<float>(b, s, d_flag, nullptr, nullptr);
 }
 
-template <typename T>
-static int btd_solve_t(gf_btd &b, hipStream_t s, double *x)
+int gf_btd_factor_solve(gf_btd &b, hipStream_t s, int *d_flag, const double *rhs, double *x, int n, double *work)
 {
-    const T *A = (const T *)b.A, *B = (const T *)b.B, *C = (const T *)b.C;
-    int launches = 0;
-    for (int L = 0; L < b.levels; L++) {
-        int nL = b.lvl_n[L];
-        const T *Lsv = (const T *)b.Lsv + b.lvl_save[L] * b.bsz, *Rsv = (const T *)b.Rsv + b.lvl_save[L] * b.bsz;
-        hipLaunchKernelGGL((btd_fwd_elim_kernel<T, double>), dim3((nL + 1) / 2), dim3(64), 0, s, L, nL, b.m, B, b.piv, x, 2);
-        launches++;
-        if (nL / 2 > 0) {
-            hipLaunchKernelGGL((btd_fwd_kept_kernel<T, double>), dim3(nL / 2), dim3(64), 0, s, L, nL, b.m, Lsv, Rsv, x);
-            launches++;
-        }
+    if (rhs == x || work == x) {
+        gf_set_error("block-tridiagonal factor_solve: x must differ from rhs and work");
+        return GF_E_ARG;
     }
-    hipLaunchKernelGGL((btd_fwd_elim_kernel<T, double>), dim3(1), dim3(64), 0, s, b.levels, 1, b.m, B, b.piv, x, 2);
+    if (rhs != work) GF_HIP(hipMemcpyAsync(work, rhs, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
+    if (b.elem_size == 8) return btd_factor_dispatch<double>(b, s, d_flag, work, x);
+    return btd_factor_dispatch<float>(b, s, d_flag, work, x);
+}
+
+template <typename T> static int btd_solve_t(gf_btd &b, hipStream_t s, double *f, double *x)
+{
+    const T *Bi = (const T *)b.Bi, *X = (const T *)b.X, *Y = (const T *)b.Y;
+    const T *Lsv = (const T *)b.Lsv, *Rsv = (const T *)b.Rsv;
+    const size_t bs = b.bsz;
+    int launches = 0;
+    for (int L = 0; L < b.tail_level; L++) {
+        const int nL = b.lvl_n[L];
+        hipLaunchKernelGGL((btd_fwd_kernel<T>), dim3((nL / 2 + 3) / 4), dim3(256), 0, s, Bi, Lsv + b.lvl_save[L] * bs,
+                           Rsv + b.lvl_save[L] * bs, b.m, L, nL, f, x);
+        launches++;
+    }
+    const int nL0 = b.tail_level < b.levels ? b.lvl_n[b.tail_level] : 1;
+    hipLaunchKernelGGL((btd_fwd_tail_kernel<T>), dim3(1), dim3(1024), 0, s, Bi, X, Y, Lsv, Rsv,
+                       (const long long *)b.d_save, b.m, b.tail_level, b.levels, nL0, f, x);
     launches++;
-    for (int L = b.levels - 1; L >= 0; L--) {
-        int nL = b.lvl_n[L];
-        hipLaunchKernelGGL((btd_bwd_kernel<T, double>), dim3((nL + 1) / 2), dim3(64), 0, s, L, nL, b.m, A, C, x);
+    for (int L = b.tail_level - 1; L >= 0; L--) {
+        const int nL = b.lvl_n[L];
+        const int work = (nL + 1) / 2 * b.m;
+        hipLaunchKernelGGL((btd_back_kernel<T>), dim3((work + 255) / 256), dim3(256), 0, s, X, Y, b.m, L, nL, x);
         launches++;
     }
     GF_HIP(hipGetLastError());
     return launches;
 }
 
-// Solves in place: x holds the right-hand side (nblk*m entries) on entry.
-int gf_btd_solve(gf_btd &b, hipStream_t s, const double *rhs, double *x, int n)
+// x = K^-1 rhs with the factors in place; work (n entries) holds the reduced
+// right-hand side; rhs may alias work but not x.
+int gf_btd_solve(gf_btd &b, hipStream_t s, const double *rhs, double *x, int n, double *work)
 {
-    if (rhs != x) GF_HIP(hipMemcpyAsync(x, rhs, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
-    if (b.elem_size == 8) return btd_solve_t<double>(b, s, x);
-    return btd_solve_t<float>(b, s, x);
+    if (rhs == x || work == x) {
+        gf_set_error("block-tridiagonal solve: x must differ from rhs and work");
+        return GF_E_ARG;
+    }
+    if (rhs != work) GF_HIP(hipMemcpyAsync(work, rhs, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
+    if (b.elem_size == 8) return btd_solve_t<double>(b, s, work, x);
+    return btd_solve_t<float>(b, s, work, x);
 }

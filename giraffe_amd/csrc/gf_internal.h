@@ -93,13 +93,14 @@ enum { NL_KIND = 0, NL_CS = 1, NL_MULT = 10, NL_TAB0 = 16, NL_TABN = 17, NL_REC 
 
 // Block-tridiagonal factor workspace (cyclic reduction), precision T.
 struct gf_btd {
-    int m = 0, nblk = 0, levels = 0;
+    int m = 0, mt = 16, nblk = 0, levels = 0, tail_level = 0;
     size_t bsz = 0;                 // m*m
-    void *A = nullptr, *B = nullptr, *C = nullptr;   // [nblk][m][m]
-    void *Lsv = nullptr, *Rsv = nullptr;             // saved coupling blocks per (level, kept row)
-    int *piv = nullptr;             // [nblk][m]
+    void *A = nullptr, *B = nullptr, *C = nullptr;     // [nblk][m][m] assembled blocks (kept rows reduced in place)
+    void *Bi = nullptr, *X = nullptr, *Y = nullptr;    // eliminated rows: B^-1, B^-1 A, B^-1 C
+    void *Lsv = nullptr, *Rsv = nullptr;               // pre-update A_k, C_k per (level, kept row)
+    long long *d_save = nullptr;    // lvl_save on the device
     std::vector<int> lvl_n;         // rows active at each level
-    std::vector<size_t> lvl_save;   // offsets (in blocks) into Lsv/Rsv per level
+    std::vector<long long> lvl_save; // offsets (in blocks) into Lsv/Rsv per level
     int elem_size = 8;
 };
 
@@ -160,10 +161,16 @@ struct gf_model {
     // device: global system
     double *d_r = nullptr;          // residual r = f_ext - f_int [n_free]
     double *d_x = nullptr;          // increment [n_free]
+    double *d_swork = nullptr;      // band triangular-solve work vector [npad]
     int layout = GM_BTD;
     gf_btd btd;
     double *d_band = nullptr;       // GM_BAND storage, FP64
     size_t band_elems = 0;
+    // mixed precision: equilibrated FP32 factor + FP64 refinement buffers
+    gf_btd btd32;
+    float *d_band32 = nullptr;
+    double *d_scale = nullptr, *d_res = nullptr, *d_w = nullptr;
+    long long n_solves = 0, n_sweeps = 0, n_fallbacks = 0;
     int npad = 0;                   // padded system size
     double *d_scal = nullptr;       // small reduction results
     double *h_scal = nullptr;       // pinned mirror
@@ -203,11 +210,19 @@ int gf_btd_alloc(gf_btd &b, int m, int nblk, int elem_size);
 void gf_btd_free(gf_btd &b);
 // Both return the number of kernel launches issued (>= 0) or a negative status.
 int gf_btd_factor(gf_btd &b, hipStream_t s, int *d_flag);
-int gf_btd_solve(gf_btd &b, hipStream_t s, const double *rhs, double *x, int n);
+// x = K^-1 rhs (factors in place); work: n entries, may alias rhs, not x
+int gf_btd_solve(gf_btd &b, hipStream_t s, const double *rhs, double *x, int n, double *work);
+// factorisation with the right-hand side reduced alongside: x = K^-1 rhs
+int gf_btd_factor_solve(gf_btd &b, hipStream_t s, int *d_flag, const double *rhs, double *x, int n, double *work);
+
+int gf_mixed_alloc(gf_model *m);
+void gf_mixed_free(gf_model *m);
+int gf_mixed_factor(gf_model *m);
+int gf_mixed_solve(gf_model *m, int *fallback, int *sweeps);
 
 int gf_band_pad(int n);
 int gf_band_factor(void *ab, int elem_size, int n, int hb, hipStream_t s, int *flag);
-int gf_band_solve(const void *ab, int elem_size, int n, int hb, hipStream_t s, double *x);
+int gf_band_solve(const void *ab, int elem_size, int n, int hb, hipStream_t s, double *x, double *work);
 
 int gf_beam_eval_device(int count, const double *x, const double *cs, const double *prop,
                         const double *d, const double *gp, double *K_out, double *f_out);

@@ -1,0 +1,238 @@
+// Mixed-precision tangent solve: FP32 factor of the equilibrated tangent,
+// FP64 iterative refinement against the FP64 tangent.
+//
+//   D = diag(|K_ii|)^-1/2,  K32 = fl32(D K D)  (factored in FP32)
+//   x = 0, s = r
+//   repeat: w = K32^-1 (D s) ;  x += D w ;  s = r - K x   (FP64)
+// until the correction |D w| is below tol |x| or stops shrinking; a stagnating
+// refinement falls back to the FP64 factorisation, so the Newton increment is
+// always the FP64 solution of the FP64 tangent.  The residual r, the tangent K
+// and every update stay FP64; only the factor is FP32.
+#include "gf_internal.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace {
+
+__global__ void diag_scale_kernel(gf_gmat g, int n, int npad, double *D)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npad) return;
+    double v;
+    if (g.layout == GM_BTD) {
+        int bi = i / g.m, lr = i - bi * g.m;
+        v = g.B[((size_t)bi * g.m + lr) * g.m + lr];
+    } else {
+        v = g.ab[(size_t)i * (size_t)(2 * g.hb) + (size_t)(i + g.hb)];
+    }
+    double a = fabs(v);
+    D[i] = (i < n && a > 0.0 && a < 1e300) ? 1.0 / sqrt(a) : 1.0;
+}
+
+// BTD blocks (A, B, C) -> scaled FP32 copies.
+__global__ void convert_btd_kernel(const double *A, const double *B, const double *C, float *A32, float *B32,
+                                   float *C32, int m, int nblk, const double *D)
+{
+    size_t bs = (size_t)m * m;
+    size_t total = (size_t)nblk * bs;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        size_t blk = t / bs;
+        int lr = (int)((t % bs) / m), lc = (int)(t % m);
+        int R = (int)blk * m + lr;
+        double dr = D[R];
+        double db = D[blk * m + lc];
+        B32[t] = (float)(dr * B[t] * db);
+        A32[t] = blk > 0 ? (float)(dr * A[t] * D[(blk - 1) * m + lc]) : 0.0f;
+        C32[t] = blk + 1 < (size_t)nblk ? (float)(dr * C[t] * D[(blk + 1) * m + lc]) : 0.0f;
+    }
+}
+
+// Band storage -> scaled FP32 copy (same skewed layout).
+__global__ void convert_band_kernel(const double *ab, float *ab32, int npad, int hb, const double *D)
+{
+    size_t w = (size_t)(2 * hb + 1);
+    size_t total = (size_t)npad * w;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        int i = (int)(t / w);
+        int c = i - hb + (int)(t % w);
+        double v = 0.0;
+        if (c >= 0 && c < npad) v = D[i] * ab[(size_t)i * (size_t)(2 * hb) + (size_t)(c + hb)] * D[c];
+        ab32[(size_t)i * (size_t)(2 * hb) + (size_t)(c + hb)] = (float)v;
+    }
+}
+
+// s = r - K x, one wavefront per row.
+__global__ void residual_kernel(gf_gmat g, int npad, const double *r, const double *x, double *s)
+{
+    const int lane = threadIdx.x & 63;
+    const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (row >= npad) return;
+    double acc = 0.0;
+    if (g.layout == GM_BTD) {
+        int m = g.m, bi = row / m, lr = row - bi * m;
+        int nblk = npad / m;
+        for (int t = lane; t < 3 * m; t += 64) {
+            int which = t / m, c = t - which * m;
+            int bj = bi + which - 1;
+            if (bj < 0 || bj >= nblk) continue;
+            const double *base = which == 0 ? g.A : (which == 1 ? g.B : g.C);
+            acc += base[((size_t)bi * m + lr) * m + c] * x[(size_t)bj * m + c];
+        }
+    } else {
+        int hb = g.hb;
+        for (int k = lane; k <= 2 * hb; k += 64) {
+            int c = row - hb + k;
+            if (c < 0 || c >= npad) continue;
+            acc += g.ab[(size_t)row * (size_t)(2 * hb) + (size_t)(c + hb)] * x[c];
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane == 0) s[row] = r[row] - acc;
+}
+
+__global__ void scale_kernel(int n, const double *D, const double *v, double *w)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) w[i] = D[i] * v[i];
+}
+
+// x += D w; out[0] = max|D w|, out[1] = max|x| (as bit patterns of non-negative doubles)
+__global__ void correct_kernel(int n, const double *D, const double *w, double *x, unsigned long long *out)
+{
+    double dmax = 0.0, xmax = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double dx = D[i] * w[i];
+        double xn = x[i] + dx;
+        x[i] = xn;
+        double a = fabs(dx);
+        dmax = (a == a) ? fmax(dmax, a) : 1e308;
+        xmax = fmax(xmax, fabs(xn));
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        dmax = fmax(dmax, __shfl_xor(dmax, off));
+        xmax = fmax(xmax, __shfl_xor(xmax, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(out + 0, (unsigned long long)__double_as_longlong(dmax));
+        atomicMax(out + 1, (unsigned long long)__double_as_longlong(xmax));
+    }
+}
+
+inline int grid_for(size_t n, int block, int cap = 4096)
+{
+    size_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > (size_t)cap) g = cap;
+    return (int)g;
+}
+
+}  // namespace
+
+int gf_mixed_alloc(gf_model *m)
+{
+    size_t n = (size_t)m->npad;
+    GF_HIP(hipMalloc(&m->d_scale, n * 8));
+    GF_HIP(hipMalloc(&m->d_res, n * 8));
+    GF_HIP(hipMalloc(&m->d_w, n * 8));
+    if (m->layout == GM_BTD) {
+        int st = gf_btd_alloc(m->btd32, m->btd.m, m->btd.nblk, 4);
+        if (st) return st;
+    } else {
+        GF_HIP(hipMalloc(&m->d_band32, m->band_elems * 4));
+    }
+    return GF_OK;
+}
+
+void gf_mixed_free(gf_model *m)
+{
+    void *p[] = {m->d_scale, m->d_res, m->d_w, m->d_band32};
+    for (void *q : p)
+        if (q) (void)hipFree(q);
+    if (m->btd32.A) gf_btd_free(m->btd32);
+}
+
+// Factor the equilibrated FP32 copy; returns launches or a negative status.
+int gf_mixed_factor(gf_model *m)
+{
+    gf_gmat g = gf_model_gmat(m);
+    int npad = m->npad;
+    hipLaunchKernelGGL(diag_scale_kernel, dim3((npad + 255) / 256), dim3(256), 0, m->stream, g, m->n_free, npad,
+                       m->d_scale);
+    if (m->layout == GM_BTD) {
+        size_t total = (size_t)m->btd.nblk * m->btd.bsz;
+        hipLaunchKernelGGL(convert_btd_kernel, dim3(grid_for(total, 256)), dim3(256), 0, m->stream,
+                           (const double *)m->btd.A, (const double *)m->btd.B, (const double *)m->btd.C,
+                           (float *)m->btd32.A, (float *)m->btd32.B, (float *)m->btd32.C, m->btd.m, m->btd.nblk,
+                           m->d_scale);
+        GF_HIP(hipGetLastError());
+        int l = gf_btd_factor(m->btd32, m->stream, m->d_flag);
+        return l < 0 ? l : l + 2;
+    }
+    hipLaunchKernelGGL(convert_band_kernel, dim3(grid_for(m->band_elems, 256)), dim3(256), 0, m->stream, m->d_band,
+                       m->d_band32, npad, m->half_band, m->d_scale);
+    GF_HIP(hipGetLastError());
+    int l = gf_band_factor(m->d_band32, 4, npad, m->half_band, m->stream, m->d_flag);
+    return l < 0 ? l : l + 2;
+}
+
+// Refined solve of K x = r (r = d_r, x = d_x).  Returns launches, or a negative
+// status; *fallback = 1 when the FP32 factor did not converge and the FP64
+// factorisation was used instead.
+int gf_mixed_solve(gf_model *m, int *fallback, int *sweeps)
+{
+    int npad = m->npad, launches = 0;
+    gf_gmat g = gf_model_gmat(m);
+    *fallback = 0;
+    *sweeps = 0;
+    GF_HIP(hipMemcpyAsync(m->d_res, m->d_r, sizeof(double) * npad, hipMemcpyDeviceToDevice, m->stream));
+    int st = gf_launch_fill(m->stream, m->d_x, (size_t)npad, 0.0);
+    if (st) return st;
+    double prev = 1e308;
+    bool ok = false;
+    for (int it = 0; it < std::max(1, m->refine_max); it++) {
+        hipLaunchKernelGGL(scale_kernel, dim3((npad + 255) / 256), dim3(256), 0, m->stream, npad, m->d_scale, m->d_res,
+                           m->d_w);
+        // BTD: solution in d_swork (d_w is reduced in place); band: in place in d_w
+        int l = m->layout == GM_BTD ? gf_btd_solve(m->btd32, m->stream, m->d_w, m->d_swork, npad, m->d_w)
+                                    : gf_band_solve(m->d_band32, 4, npad, m->half_band, m->stream, m->d_w, m->d_swork);
+        if (l < 0) return l;
+        const double *sol = m->layout == GM_BTD ? m->d_swork : m->d_w;
+        st = gf_launch_fill_bytes(m->stream, m->d_scal + 4, 2 * sizeof(double));
+        if (st) return st;
+        hipLaunchKernelGGL(correct_kernel, dim3(grid_for(npad, 256, 512)), dim3(256), 0, m->stream, npad, m->d_scale,
+                           sol, m->d_x, (unsigned long long *)(m->d_scal + 4));
+        hipLaunchKernelGGL(residual_kernel, dim3((npad * 64 + 255) / 256), dim3(256), 0, m->stream, g, npad, m->d_r,
+                           m->d_x, m->d_res);
+        GF_HIP(hipGetLastError());
+        launches += l + 3;
+        GF_HIP(hipMemcpyAsync(m->h_scal + 4, m->d_scal + 4, 2 * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+        GF_HIP(hipStreamSynchronize(m->stream));
+        double dn, xn;
+        memcpy(&dn, m->h_scal + 4, 8);
+        memcpy(&xn, m->h_scal + 5, 8);
+        (*sweeps)++;
+        if (!(dn == dn) || !(xn == xn)) break;
+        if (dn <= m->refine_tol * xn) { ok = true; break; }
+        if (it > 0 && dn > 0.5 * prev) {
+            // stagnation at the FP64 rounding floor counts as converged once the
+            // correction is already tiny; otherwise the FP32 factor is too poor
+            if (dn <= 1e-10 * xn) ok = true;
+            break;
+        }
+        prev = dn;
+    }
+    if (ok) return launches;
+    // fall back to the FP64 factorisation of the FP64 tangent
+    *fallback = 1;
+    if (m->layout == GM_BTD) {
+        int l = gf_btd_factor_solve(m->btd, m->stream, m->d_flag, m->d_r, m->d_x, npad, m->d_res);
+        return l < 0 ? l : launches + l;
+    }
+    int l = gf_band_factor(m->d_band, 8, npad, m->half_band, m->stream, m->d_flag);
+    if (l < 0) return l;
+    GF_HIP(hipMemcpyAsync(m->d_x, m->d_r, sizeof(double) * npad, hipMemcpyDeviceToDevice, m->stream));
+    int l2 = gf_band_solve(m->d_band, 8, npad, m->half_band, m->stream, m->d_x, m->d_swork);
+    if (l2 < 0) return l2;
+    return launches + l + l2;
+}

@@ -104,11 +104,12 @@ static void release(gf_model *m)
     void *bufs[] = {m->d_disp, m->d_copy, m->d_euler, m->d_dof, m->d_row_nc, m->d_bconn, m->d_bconst, m->d_bgp,
                     m->d_ln_node, m->d_ln_off, m->d_ln_ids, m->d_lrec, m->d_tab_t, m->d_tab_v, m->d_r, m->d_x,
                     m->d_scal, m->d_flag, m->d_copy0, m->d_bgp0, m->d_sconn, m->d_sconst, m->d_spconst, m->d_sgp,
-                    m->d_sgp0, m->d_spload, m->d_spflag, m->d_band};
+                    m->d_sgp0, m->d_spload, m->d_spflag, m->d_band, m->d_swork};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (m->h_scal) (void)hipHostFree(m->h_scal);
     gf_btd_free(m->btd);
+    gf_mixed_free(m);
     if (m->own_stream && m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
 }
@@ -229,9 +230,9 @@ static int create(const gf_model_desc *d, const gf_options *opt, gf_model *m)
         m->own_stream = true;
     }
     m->precision = opt ? opt->precision : GF_PREC_FP64;
-    if (m->precision != GF_PREC_FP64) {
-        gf_set_error("only GF_PREC_FP64 is built in this library version");
-        return GF_E_UNSUPPORTED;
+    if (m->precision != GF_PREC_FP64 && m->precision != GF_PREC_MIXED) {
+        gf_set_error("unknown precision");
+        return GF_E_ARG;
     }
     // node state
     std::vector<double> copy(6 * (size_t)nn, 0.0);
@@ -388,12 +389,14 @@ static int create(const gf_model_desc *d, const gf_options *opt, gf_model *m)
     size_t npad = (size_t)m->npad;
     GF_TRY(dalloc(&m->d_r, npad));
     GF_TRY(dalloc(&m->d_x, npad));
+    GF_TRY(dalloc(&m->d_swork, npad));
     if (m->layout == GM_BTD) {
         GF_TRY(gf_btd_alloc(m->btd, bm, nblk, 8));
     } else {
         m->band_elems = npad * (size_t)(2 * hb + 1);
         GF_TRY(dalloc(&m->d_band, m->band_elems));
     }
+    if (m->precision == GF_PREC_MIXED) GF_TRY(gf_mixed_alloc(m));
     GF_TRY(dalloc(&m->d_scal, 16));
     GF_TRY(dalloc(&m->d_flag, 4));
     GF_HIP(hipHostMalloc((void **)&m->h_scal, 16 * sizeof(double), hipHostMallocDefault));
@@ -482,6 +485,16 @@ extern "C" int gf_model_info(const gf_model *m, long long *o)
     return GF_OK;
 }
 
+extern "C" int gf_model_stats(const gf_model *m, long long *out)
+{
+    if (!m || !out) return GF_E_ARG;
+    out[0] = m->n_solves;
+    out[1] = m->n_sweeps;
+    out[2] = m->n_fallbacks;
+    out[3] = 0;
+    return GF_OK;
+}
+
 extern "C" int gf_begin_step(gf_model *m, double t_last, double dt)
 {
     if (!m) return GF_E_ARG;
@@ -548,19 +561,50 @@ extern "C" int gf_solve(gf_model *m, int *info_fail)
     *info_fail = 0;
     GF_TRY(gf_launch_fill_bytes(m->stream, m->d_flag, 4));
     size_t npad = (size_t)m->npad;
+    m->n_solves++;
+    if (m->precision == GF_PREC_MIXED) {
+        tick(m, GF_PH_FACTOR, true);
+        GF_TRYN(nl_f32, gf_mixed_factor(m));
+        tick(m, GF_PH_FACTOR, false);
+        int flag32 = 0;
+        GF_HIP(hipMemcpyAsync(&m->h_scal[8], m->d_flag, 4, hipMemcpyDeviceToHost, m->stream));
+        GF_HIP(hipStreamSynchronize(m->stream));
+        memcpy(&flag32, &m->h_scal[8], 4);
+        GF_TRY(tock(m, GF_PH_FACTOR, nl_f32));
+        if (!flag32) {
+            int fb = 0, sw = 0;
+            tick(m, GF_PH_SOLVE, true);
+            GF_TRYN(nl_ref, gf_mixed_solve(m, &fb, &sw));
+            tick(m, GF_PH_SOLVE, false);
+            GF_TRY(tock(m, GF_PH_SOLVE, nl_ref));
+            m->n_sweeps += sw;
+            m->n_fallbacks += fb;
+            m->factored = true;
+            int flag = 0;
+            GF_HIP(hipMemcpyAsync(&m->h_scal[8], m->d_flag, 4, hipMemcpyDeviceToHost, m->stream));
+            GF_HIP(hipStreamSynchronize(m->stream));
+            memcpy(&flag, &m->h_scal[8], 4);
+            if (flag) *info_fail = 1;
+            return GF_OK;
+        }
+        // an unusable FP32 pivot: FP64 factorisation below
+        m->n_fallbacks++;
+        GF_TRY(gf_launch_fill_bytes(m->stream, m->d_flag, 4));
+    }
+    // block-tridiagonal: the right-hand side is reduced alongside the factorisation
+    // (the factor phase covers the forward sweep too; the solve phase is empty)
     tick(m, GF_PH_FACTOR, true);
-    int nl_fac, nl_sol;
-    if (m->layout == GM_BTD) nl_fac = gf_btd_factor(m->btd, m->stream, m->d_flag);
+    int nl_fac, nl_sol = 0;
+    if (m->layout == GM_BTD)
+        nl_fac = gf_btd_factor_solve(m->btd, m->stream, m->d_flag, m->d_r, m->d_x, (int)npad, m->d_swork);
     else nl_fac = gf_band_factor(m->d_band, 8, m->npad, m->half_band, m->stream, m->d_flag);
     if (nl_fac < 0) return nl_fac;
     tick(m, GF_PH_FACTOR, false);
     m->factored = true;
     tick(m, GF_PH_SOLVE, true);
-    if (m->layout == GM_BTD) {
-        nl_sol = gf_btd_solve(m->btd, m->stream, m->d_r, m->d_x, (int)npad);
-    } else {
+    if (m->layout != GM_BTD) {
         GF_HIP(hipMemcpyAsync(m->d_x, m->d_r, sizeof(double) * npad, hipMemcpyDeviceToDevice, m->stream));
-        nl_sol = gf_band_solve(m->d_band, 8, m->npad, m->half_band, m->stream, m->d_x);
+        nl_sol = gf_band_solve(m->d_band, 8, m->npad, m->half_band, m->stream, m->d_x, m->d_swork);
     }
     if (nl_sol < 0) return nl_sol;
     tick(m, GF_PH_SOLVE, false);
@@ -708,12 +752,13 @@ extern "C" int gf_bandsystem(int n, int hb, const double *ab_rows, const double 
     int npad = gf_band_pad(n);
     size_t w = (size_t)(2 * hb + 1);
     size_t elems = (size_t)npad * w;
-    double *d_ab = nullptr, *d_x = nullptr;
+    double *d_ab = nullptr, *d_x = nullptr, *d_z = nullptr;
     int *d_flag = nullptr;
     hipStream_t s;
     GF_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     GF_HIP(hipMalloc(&d_ab, elems * 8));
     GF_HIP(hipMalloc(&d_x, (size_t)npad * 8));
+    GF_HIP(hipMalloc(&d_z, (size_t)npad * 8));
     GF_HIP(hipMalloc(&d_flag, 4));
     GF_TRY(gf_launch_fill(s, d_ab, elems, 0.0));
     GF_TRY(gf_launch_fill(s, d_x, (size_t)npad, 0.0));
@@ -731,7 +776,7 @@ extern "C" int gf_bandsystem(int n, int hb, const double *ab_rows, const double 
     if (npad > n) GF_HIP(hipMemcpyAsync(d_ab + (size_t)n * w, diag.data(), diag.size() * 8, hipMemcpyHostToDevice, s));
     GF_HIP(hipMemcpyAsync(d_x, b, (size_t)n * 8, hipMemcpyHostToDevice, s));
     int st = gf_band_factor(d_ab, 8, npad, hb, s, d_flag);
-    if (st >= 0) st = gf_band_solve(d_ab, 8, npad, hb, s, d_x);
+    if (st >= 0) st = gf_band_solve(d_ab, 8, npad, hb, s, d_x, d_z);
     int flag = 0;
     if (st >= 0) {
         GF_HIP(hipMemcpyAsync(x, d_x, (size_t)n * 8, hipMemcpyDeviceToHost, s));
@@ -739,7 +784,7 @@ extern "C" int gf_bandsystem(int n, int hb, const double *ab_rows, const double 
         GF_HIP(hipStreamSynchronize(s));
         st = GF_OK;
     }
-    (void)hipFree(d_ab); (void)hipFree(d_x); (void)hipFree(d_flag);
+    (void)hipFree(d_ab); (void)hipFree(d_x); (void)hipFree(d_z); (void)hipFree(d_flag);
     (void)hipStreamDestroy(s);
     if (flag) *info_fail = 1;
     return st;

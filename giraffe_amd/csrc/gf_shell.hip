@@ -68,6 +68,21 @@ __device__ __forceinline__ m3<double> E3m(const m3<double> &X)
     return r;
 }
 
+// Geometric block G(idx, 4) (row4 = false) or G(4, idx) (row4 = true), idx < 4:
+// G(R,4) = Gua1, Vm1^T, Gua2, Vm2^T and G(4,S) = Gua1^T, Vm1, Gua2^T, Vm2, read
+// from the Gua1 | Gua2 | Vm1 | Vm2 blocks at L_G with one indexed load per entry.
+__device__ __forceinline__ m3<double> g_block(const double *o, int idx, bool row4)
+{
+    const int off = L_G + ((idx & 1) ? 18 + 9 * (idx >> 1) : 9 * (idx >> 1));
+    const bool transpose = ((idx & 1) != 0) != row4;
+    m3<double> r;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) r.a[3 * i + j] = o[off + (transpose ? 3 * j + i : 3 * i + j)];
+    return r;
+}
+
 struct ShellKin {
     v3<double> u, u1, u2, a, a1, a2, z1, z2, eta1, eta2, k1, k2;
     m3<double> Q, Xi, Xi1, Xi2, QQ;
@@ -300,19 +315,13 @@ __global__ __launch_bounds__(SWG) void shell_assemble_kernel(
 #pragma unroll
             for (int j = 1; j < 4; j++) V = madd(V, mm3(ld3(o + L_D + 9 * (4 * R + j)), ld3(o + L_W + 9 * j)));
             Mb = mtm3(ld3(o + L_P + 9 * R), V);
-            // G(R,4): Gua1, Vm1^T, Gua2, Vm2^T
-            m3<double> G = R == 0 ? ld3(o + L_G) : R == 1 ? tr3(ld3(o + L_G + 18)) : R == 2 ? ld3(o + L_G + 9)
-                                                                                    : tr3(ld3(o + L_G + 27));
-            Mb = madd(Mb, G);
+            Mb = madd(Mb, g_block(o, R, false));
         } else if (S < 4) {
             m3<double> V = mtm3(ld3(o + L_W), ld3(o + L_D + 9 * S));
 #pragma unroll
             for (int i = 1; i < 4; i++) V = madd(V, mtm3(ld3(o + L_W + 9 * i), ld3(o + L_D + 9 * (4 * i + S))));
             Mb = mm3(V, ld3(o + L_P + 9 * S));
-            // G(4,S): Gua1^T, Vm1, Gua2^T, Vm2
-            m3<double> G = S == 0 ? tr3(ld3(o + L_G)) : S == 1 ? ld3(o + L_G + 18) : S == 2 ? tr3(ld3(o + L_G + 9))
-                                                                                    : ld3(o + L_G + 27);
-            Mb = madd(Mb, G);
+            Mb = madd(Mb, g_block(o, S, true));
         } else {
             m3<double> acc;
 #pragma unroll

@@ -46,6 +46,7 @@ def load():
     L.gf_model_create.argtypes = [C.POINTER(abi.GfModelDesc), C.POINTER(abi.GfOptions), C.POINTER(vp)]
     L.gf_model_destroy.argtypes = [vp]
     L.gf_model_info.argtypes = [vp, C.POINTER(C.c_longlong)]
+    L.gf_model_stats.argtypes = [vp, C.POINTER(C.c_longlong)]
     L.gf_set_solution_window.argtypes = [vp, C.c_double, C.c_double]
     L.gf_begin_step.argtypes = [vp, C.c_double, C.c_double]
     L.gf_assemble.argtypes = [vp, dp]
@@ -116,6 +117,11 @@ class DeviceModel:
         check(load().gf_model_info(self.h, o), "gf_model_info")
         keys = ["n_free", "half_band", "block", "n_blocks", "colors", "factor_bytes", "levels", "precision"]
         return dict(zip(keys, list(o)))
+
+    def stats(self):
+        o = (C.c_longlong * 4)()
+        check(load().gf_model_stats(self.h, o), "gf_model_stats")
+        return {"solves": o[0], "refine_sweeps": o[1], "fallbacks": o[2]}
 
     def set_window(self, start, end):
         check(load().gf_set_solution_window(self.h, start, end), "gf_set_solution_window")

@@ -143,6 +143,9 @@ int gf_model_destroy(gf_model *m);
 /* n_free, half-bandwidth, BTD block size (0 for the band layout), block count or
  * padded order, colours, bytes of the tangent storage, layout (0 BTD, 1 band), precision. */
 int gf_model_info(const gf_model *m, long long *info_out /* [8] */);
+/* Tangent solves so far, FP64 refinement sweeps (mixed precision), solves that
+ * fell back to the FP64 factorisation, reserved. */
+int gf_model_stats(const gf_model *m, long long *stats_out /* [4] */);
 
 /* ---- Newton step pieces --------------------------------------------------- */
 /* Time window of the current solution step; the gravity BoolTable ramp
