@@ -31,6 +31,10 @@ def workload(name):
     if name == "catenary":
         m, p, info = M.catenary(n_elem=10000)
         desc = "riser catenary: 10,000 Beam_1 (20,001 nodes), self-weight + N(0,500 N) lateral nodal loads, 10 load steps"
+    elif name == "plate":
+        m, p, info = M.shell_plate(nq=64)
+        desc = ("clamped Shell_1 plate: 64x64 quads = 8,192 triangles (16,641 nodes), t = 5 mm, follower pressure "
+                "ramped over 15 load steps, seeded N(0, 1e-4 m) imperfections")
     elif name == "rollup":
         m, p, info = M.cantilever_rollup()
         desc = "cantilever roll-up: 100 Beam_1, follower tip moment to 2*pi*EI/L, 20 load steps"
@@ -43,7 +47,11 @@ def log(msg):
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(m, p, kind):
+def cpu_baseline(m, p, kind, gpu_iterations, budget_s=30.0):
+    """The oracle (FP64 restatement, MKL PARDISO per iteration as sparsesystem
+    does) on this host's cores.  Bounded sample: whole load steps are run until
+    about budget_s has passed; solves/s is scaled from the time per Newton
+    iteration and the full solve's iteration count on the GPU."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     os.environ["GFO_VERBOSE"] = "2"  # the oracle prints a line per Newton iteration and per load step
     import oracle
@@ -52,15 +60,224 @@ def cpu_baseline(m, p, kind):
     if kind in ("auto", "pardiso") and oracle.pardiso_available():
         solver = oracle.SOLVER_PARDISO
         label = "oracle FP64 restatement, MKL PARDISO phase 13 per iteration (as sparsesystem)"
-    om = oracle.OracleModel(m, solver)
+    import dataclasses
+    n_steps = int(round((p.end_time - p.start_time) / p.time_step))
+
+    def run(k):
+        """The first k load steps of the same Static loop (load tables are in absolute time)."""
+        q = dataclasses.replace(p, end_time=p.start_time + k * p.time_step) if k < n_steps else p
+        om = oracle.OracleModel(m, solver)
+        t0 = time.perf_counter()
+        rep, _ = om.static_solve(q)
+        return time.perf_counter() - t0, int(rep.total_iterations), k >= n_steps
+
     log(f"cpu baseline: {label}")
+    dt, iters, full = run(1)
+    k = max(1, min(n_steps, int(budget_s / max(dt, 1e-3))))
+    if k > 1:
+        dt, iters, full = run(k)
+    per_iter = dt / max(1, iters)
+    value = 1.0 / (per_iter * gpu_iterations)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    what = "the full solve" if full else f"the first {k} of {n_steps} load steps"
+    return {"value": value, "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": f"{what}: {iters} Newton iterations in {dt:.1f} s ({per_iter * 1e3:.0f} ms/iteration), "
+                      f"scaled to the {gpu_iterations} iterations of one full solve; {label}",
+            "iterations_timed": iters}
+
+
+def measure(m, p, precision, steps, warmup, timing):
+    """Warmup + `steps` timed full solves of one resident model; the state is
+    reset on the device before each solve."""
+    import torch
+    from giraffe_amd import runtime
+
+    prec = abi.GF_PREC_FP64 if precision == "fp64" else abi.GF_PREC_MIXED
+    dm = runtime.DeviceModel(m, precision=prec)
+    info = dm.info()
+    log(f"[{precision}] model resident: {info}")
+    for w in range(warmup):
+        dm.reset()
+        ts = time.perf_counter()
+        dm.static_solve(p)
+        log(f"[{precision}] warmup solve {w} {1e3 * (time.perf_counter() - ts):.1f} ms")
+    if timing:
+        dm.timing(True)
+        dm.timing_read(reset=True)
+    st0 = dm.stats()
+    lat, reps = [], []
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    rep, iters = om.static_solve(p)
+    for _ in range(steps):
+        ts = time.perf_counter()
+        dm.reset()
+        rep, iters = dm.static_solve(p)
+        lat.append(time.perf_counter() - ts)
+        reps.append((rep.total_iterations, rep.total_solves, rep.max_rel_residual, rep.aborted, list(iters)))
+        log(f"[{precision}] timed solve {len(lat) - 1}: {1e3 * lat[-1]:.1f} ms, {rep.total_iterations} iterations")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ph = dm.timing_read(reset=True) if timing else {}
+    st1 = dm.stats()
+    lat_ms = np.array(lat) * 1e3
+    return {"info": info, "wall": wall, "lat_ms": lat_ms, "reps": reps, "phases": ph,
+            "refine_sweeps": st1["refine_sweeps"] - st0["refine_sweeps"],
+            "fallbacks": st1["fallbacks"] - st0["fallbacks"], "solves": st1["solves"] - st0["solves"]}
+
+
+def roofline(m, r, steps):
+    """Roofline of the dominant device phase: algorithmic work of one unit (an
+    assembly launch, one factorisation, one solve) over its HIP-event time."""
+    ph, info = r["phases"], r["info"]
+    n_free, b = info["n_free"], info["half_band"]
+    solves = r["reps"][-1][1]
+    dom = max(ph, key=lambda k: ph[k][0])
+    ms, launches = ph[dom]
+    if dom == "assemble":
+        if m.n_beams and not m.n_shells:
+            per_launch_elems = m.n_beams / max(1, info["colors"])
+            unit_bytes = FM.BEAM_BYTES_FP64 * per_launch_elems
+            avg_s = ms / launches * 1e-3
+            ach = unit_bytes / avg_s / 1e9
+            return {"kernel": "beam_assemble_kernel<0>", "bound": "hbm", "achieved": ach, "peak": FM.HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": ach / FM.HBM_PEAK_GBS, "traffic": None, "avg_launch_us": avg_s * 1e6,
+                    "algorithmic_bytes_per_launch": unit_bytes}
+        return {"kernel": "assembly", "bound": None, "note": "shell assembly roofline not modelled"}
+    per = ms / max(1, solves * steps) * 1e-3  # one factorisation (or solve) per tangent solve
+    if info["block"]:  # block-tridiagonal: factorisation with the forward sweep and back-substitution fused
+        work = FM.band_lu(n_free, b)
+        ach = work["bytes"] / per / 1e9
+        return {"kernel": "block-tridiagonal cyclic reduction (factor + solve, all levels)", "bound": "hbm",
+                "achieved": ach, "peak": FM.HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / FM.HBM_PEAK_GBS,
+                "traffic": None, "time_per_unit_us": per * 1e6, "launches_per_unit": launches / max(1, solves * steps),
+                "algorithmic_bytes_per_unit": work["bytes"], "unit_of_work": "one factorisation + solve"}
+    if dom == "factor":
+        work = FM.band_lu(n_free, b)
+        ach = work["flops"] / per / 1e12
+        peak = FM.FP64_PEAK_TFLOPS if info["precision"] == 0 else FM.FP32_PEAK_TFLOPS
+        return {"kernel": "band LU (panel + MFMA trailing update)", "bound": "mfma", "achieved": ach, "peak": peak,
+                "unit": "TFLOP/s", "frac": ach / peak, "traffic": None, "time_per_unit_us": per * 1e6,
+                "launches_per_unit": launches / max(1, solves * steps), "algorithmic_flops_per_unit": work["flops"],
+                "unit_of_work": "one factorisation"}
+    work = FM.band_trsv(n_free, b)
+    ach = work["bytes"] / per / 1e9
+    return {"kernel": "band triangular solves", "bound": "hbm", "achieved": ach, "peak": FM.HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": ach / FM.HBM_PEAK_GBS, "traffic": None, "time_per_unit_us": per * 1e6,
+            "algorithmic_bytes_per_unit": work["bytes"], "unit_of_work": "one solve"}
+
+
+def serve(args):
+    """Catenary request stream: riser requests (giraffe_amd.model.riser_request,
+    seed 1004) with Poisson arrivals at `load` x the closed-loop capacity, served
+    by gf_server (continuous admission by slot and device memory).  Latency =
+    submit to final commit."""
+    import torch
+    from giraffe_amd import runtime
+
+    rng = np.random.default_rng(1004)
+    t0 = time.perf_counter()
+    reqs = [M.riser_request(rng)[:2] for _ in range(args.requests)]
+    for m, _ in reqs:
+        m.freeze()
+    log(f"generated {len(reqs)} requests in {time.perf_counter() - t0:.1f} s "
+        f"({sum(m.n_beams for m, _ in reqs)} Beam_1 in total)")
+    prec = abi.GF_PREC_FP64 if args.precision == "fp64" else abi.GF_PREC_MIXED
+
+    def drain(srv, pending, lat, t_sub):
+        while pending:
+            srv.pump(1000)
+            for rid in list(pending):
+                try:
+                    rep, it, nodes, times = srv.take(rid)
+                except runtime.GiraffeError:
+                    continue
+                pending.discard(rid)
+                lat[rid] = (rep, times[1])
+
+    # warm-up and closed-loop capacity on the first requests
+    nwarm = min(len(reqs), 32)
+    srv = runtime.Server(precision=prec, max_inflight=args.max_inflight)
+    ids = {srv.submit(m, p) for m, p in reqs[:nwarm]}
+    drain(srv, ids, {}, None)
+    ncap = min(len(reqs), 200)
+    torch.cuda.synchronize()
+    tc = time.perf_counter()
+    ids = {srv.submit(m, p) for m, p in reqs[:ncap]}
+    drain(srv, ids, {}, None)
+    cap = ncap / (time.perf_counter() - tc)
+    log(f"closed-loop capacity {cap:.1f} solves/s over {ncap} requests")
+    # open loop: Poisson arrivals at load x capacity
+    rate = args.load * cap
+    gaps = rng.exponential(1.0 / rate, len(reqs))
+    arrivals = np.cumsum(gaps)
+    lat = {}
+    pending = set()
+    t_start = time.perf_counter()
+    for i, (m, p) in enumerate(reqs):
+        while time.perf_counter() - t_start < arrivals[i]:
+            srv.pump(200)
+            for rid in list(pending):
+                try:
+                    rep, it, nodes, times = srv.take(rid)
+                except runtime.GiraffeError:
+                    continue
+                pending.discard(rid)
+                lat[rid] = (rep, times[1])
+        pending.add(srv.submit(m, p))
+        if (i + 1) % 200 == 0:
+            log(f"submitted {i + 1} requests, {len(lat)} finished, {srv.counts()}")
+    drain(srv, pending, lat, None)
+    makespan = time.perf_counter() - t_start
+    srv.close()
+    lat_ms = np.array([v[1] for v in lat.values()])
+    aborted = sum(1 for v in lat.values() if v[0].aborted)
+    out = {
+        "metric": METRIC,
+        "value": len(lat) / makespan,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": len(reqs),
+        "warmup": nwarm,
+        "ms_per_step": makespan / len(lat) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f64 tangent/residual, f32 factor + f64 refinement",
+        "data": "synthetic (riser request stream, seed 1004, giraffe_amd/model.py)",
+        "config": {"workload": f"serving stream: {len(reqs)} Beam_1 riser requests (E log-U[50,5000]), Poisson "
+                               f"arrivals at {args.load:.2f} x closed-loop capacity, {args.max_inflight} resident",
+                   "precision": args.precision, "arrival_rate_per_s": rate, "closed_loop_capacity": cap},
+        "p50_ms": float(np.percentile(lat_ms, 50)),
+        "p99_ms": float(np.percentile(lat_ms, 99)),
+        "aborted": aborted,
+        "max_rel_residual": max(v[0].max_rel_residual for v in lat.values()),
+    }
+    if args.cpu_baseline != "none":
+        out["cpu_baseline"] = serve_cpu_baseline(reqs, args.cpu_budget)
+    print(json.dumps(out))
+    return 0
+
+
+def serve_cpu_baseline(reqs, budget_s):
+    """The oracle solving the same stream's requests one after another (closed loop)
+    until about budget_s has passed."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    os.environ["GFO_VERBOSE"] = "1"
+    import oracle
+    solver = oracle.SOLVER_PARDISO if oracle.pardiso_available() else oracle.SOLVER_BAND
+    t0 = time.perf_counter()
+    n = 0
+    for m, p in reqs:
+        oracle.OracleModel(m, solver).static_solve(p)
+        n += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
     dt = time.perf_counter() - t0
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": 1.0 / dt, "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": f"one full solve of the same workload ({dt:.1f} s, {rep.total_iterations} Newton iterations); {label}",
-            "iterations": int(rep.total_iterations)}
+    label = "MKL PARDISO" if solver == oracle.SOLVER_PARDISO else "band LU"
+    return {"value": n / dt, "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": f"the first {n} requests of the stream solved one after another in {dt:.1f} s "
+                      f"(oracle FP64 restatement, {label})"}
 
 
 def main():
@@ -68,9 +285,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="catenary")
+    ap.add_argument("--workload", default="catenary", choices=["catenary", "plate", "rollup", "serve"])
+    ap.add_argument("--requests", type=int, default=2000, help="serve: requests in the stream")
+    ap.add_argument("--max-inflight", type=int, default=16, help="serve: resident requests")
+    ap.add_argument("--load", type=float, default=0.8, help="serve: arrival rate / closed-loop capacity")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--compare-steps", type=int, default=2,
+                    help="solves of the other precision, reported beside the main figure (0: skip)")
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "pardiso", "band", "none"])
+    ap.add_argument("--cpu-budget", type=float, default=30.0, help="seconds of CPU baseline sample")
     ap.add_argument("--no-timing", action="store_true", help="do not record per-phase HIP events")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -79,88 +302,59 @@ def main():
         return 2
     log("importing torch")
     import torch
-    from giraffe_amd import runtime
 
     log("torch imported")
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
         return 2
+    if args.workload == "serve":
+        return serve(args)
     m, p, desc = workload(args.workload)
-    prec = abi.GF_PREC_FP64 if args.precision == "fp64" else abi.GF_PREC_MIXED
-    dm = runtime.DeviceModel(m, precision=prec)
-    info = dm.info()
-    log(f"model resident: {info}")
-    for w in range(args.warmup):
-        dm.reset()
-        ts = time.perf_counter()
-        dm.static_solve(p)
-        log(f"warmup solve {w} {1e3 * (time.perf_counter() - ts):.1f} ms")
-    if not args.no_timing:
-        dm.timing(True)
-        dm.timing_read(reset=True)
-    lat = []
-    reps = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ts = time.perf_counter()
-        dm.reset()
-        rep, iters = dm.static_solve(p)
-        lat.append(time.perf_counter() - ts)
-        reps.append((rep.total_iterations, rep.total_solves, rep.max_rel_residual, rep.aborted, list(iters)))
-        log(f"timed solve {len(lat) - 1}: {1e3 * lat[-1]:.1f} ms, {rep.total_iterations} iterations")
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ph = dm.timing_read(reset=True) if not args.no_timing else {}
-    lat_ms = np.array(lat) * 1e3
-    n_free, b = info["n_free"], info["half_band"]
+    r = measure(m, p, args.precision, args.steps, args.warmup, not args.no_timing)
+    info, reps = r["info"], r["reps"]
     its, solves = reps[-1][0], reps[-1][1]
     out = {
         "metric": METRIC,
-        "value": args.steps / wall,
+        "value": args.steps / r["wall"],
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": r["wall"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32 tangent/factor + f64 residual/refinement",
+        "dtype": "f64" if args.precision == "fp64" else "f64 tangent/residual, f32 factor + f64 refinement",
         "data": "synthetic (seeded generator, giraffe_amd/model.py)",
-        "config": {"workload": desc, "precision": args.precision, "free_dofs": n_free, "half_bandwidth": b,
-                   "block": info["block"], "newton_iterations": its, "tangent_solves": solves,
-                   "step_iterations": [int(v) for v in reps[-1][4]]},
-        "p50_ms": float(np.percentile(lat_ms, 50)),
-        "p99_ms": float(np.percentile(lat_ms, 99)),
-        "max_rel_residual": max(r[2] for r in reps),
-        "aborted": any(r[3] for r in reps),
+        "config": {"workload": desc, "precision": args.precision, "free_dofs": info["n_free"],
+                   "half_bandwidth": info["half_band"], "block": info["block"], "newton_iterations": its,
+                   "tangent_solves": solves, "step_iterations": [int(v) for v in reps[-1][4]]},
+        "p50_ms": float(np.percentile(r["lat_ms"], 50)),
+        "p99_ms": float(np.percentile(r["lat_ms"], 99)),
+        "max_rel_residual": max(x[2] for x in reps),
+        "aborted": any(x[3] for x in reps),
     }
-    if ph:
+    if args.precision == "mixed":
+        out["refinement"] = {"sweeps_per_solve": r["refine_sweeps"] / max(1, r["solves"]),
+                             "fp64_fallbacks": r["fallbacks"], "tangent_solves": r["solves"]}
+    if r["phases"]:
+        ph = r["phases"]
         out["phases_ms_per_solve"] = {k: v[0] / args.steps for k, v in ph.items()}
         out["phase_launches_per_solve"] = {k: v[1] / args.steps for k, v in ph.items()}
-        # dominant device phase and its roofline (algorithmic work / measured time)
-        dom = max(ph, key=lambda k: ph[k][0])
-        ms, launches = ph[dom]
-        if dom == "assemble":
-            per_launch_elems = m.n_beams / max(1, info["colors"])
-            unit_bytes = FM.BEAM_BYTES_FP64 * per_launch_elems
-            avg_s = ms / launches * 1e-3
-            ach = unit_bytes / avg_s / 1e9
-            out["roofline"] = {"kernel": "beam_assemble_kernel<0>", "bound": "hbm", "achieved": ach,
-                               "peak": FM.HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / FM.HBM_PEAK_GBS,
-                               "traffic": None, "avg_launch_us": avg_s * 1e6,
-                               "algorithmic_bytes_per_launch": unit_bytes}
-        else:
-            work = FM.band_lu(n_free, b) if dom == "factor" else FM.band_trsv(n_free, b)
-            per = ms / max(1, solves * args.steps) * 1e-3  # one factorisation / one solve per tangent solve
-            ach = work["bytes"] / per / 1e9
-            out["roofline"] = {"kernel": f"block-tridiagonal {dom} (all levels)", "bound": "hbm", "achieved": ach,
-                               "peak": FM.HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / FM.HBM_PEAK_GBS,
-                               "traffic": None, "avg_launch_us": ms / launches * 1e3,
-                               "algorithmic_bytes_per_unit": work["bytes"], "unit_of_work": f"one {dom}"}
+        out["roofline"] = roofline(m, r, args.steps)
+    if args.compare_steps > 0:
+        other = "mixed" if args.precision == "fp64" else "fp64"
+        c = measure(m, p, other, args.compare_steps, 1, False)
+        out["compare"] = {"precision": other, "solves_per_s": args.compare_steps / c["wall"],
+                          "p50_ms": float(np.percentile(c["lat_ms"], 50)),
+                          "p99_ms": float(np.percentile(c["lat_ms"], 99)),
+                          "newton_iterations": c["reps"][-1][0],
+                          "max_rel_residual": max(x[2] for x in c["reps"]),
+                          "refine_sweeps_per_solve": c["refine_sweeps"] / max(1, c["solves"]),
+                          "fp64_fallbacks": c["fallbacks"]}
+        out["compare"]["same_iterations"] = c["reps"][-1][4] == reps[-1][4]
     if args.cpu_baseline != "none":
-        out["cpu_baseline"] = cpu_baseline(m, p, args.cpu_baseline)
+        out["cpu_baseline"] = cpu_baseline(m, p, args.cpu_baseline, its, args.cpu_budget)
     print(json.dumps(out))
     return 0
 

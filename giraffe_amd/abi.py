@@ -109,6 +109,15 @@ class GfOptions(C.Structure):
     ]
 
 
+class GfServerOptions(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int),
+        ("device", C.c_int),
+        ("max_inflight", C.c_int),
+        ("mem_cap", C.c_ulonglong),
+    ]
+
+
 def ptr(a, ctype):
     """Pointer to a contiguous numpy array (the caller keeps `a` alive)."""
     if a is None:

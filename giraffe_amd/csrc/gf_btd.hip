@@ -527,7 +527,11 @@ int gf_btd_alloc(gf_btd &b, int m, int nblk, int elem_size)
     if (b.tail_level < 0) b.tail_level = L;
     size_t blocks = (size_t)nblk * b.bsz * elem_size;
     size_t sv = (save ? save : 1) * b.bsz * elem_size;
-    void **arrs[] = {&b.A, &b.B, &b.C, &b.Bi, &b.X, &b.Y};
+    // A, B, C contiguous: the assembly clears them with one fill
+    GF_HIP(hipMalloc(&b.A, 3 * blocks));
+    b.B = (char *)b.A + blocks;
+    b.C = (char *)b.A + 2 * blocks;
+    void **arrs[] = {&b.Bi, &b.X, &b.Y};
     for (void **a : arrs) GF_HIP(hipMalloc(a, blocks));
     GF_HIP(hipMalloc(&b.Lsv, sv));
     GF_HIP(hipMalloc(&b.Rsv, sv));
@@ -543,7 +547,7 @@ int gf_btd_alloc(gf_btd &b, int m, int nblk, int elem_size)
 
 void gf_btd_free(gf_btd &b)
 {
-    void *arrs[] = {b.A, b.B, b.C, b.Bi, b.X, b.Y, b.Lsv, b.Rsv, b.d_save};
+    void *arrs[] = {b.A, b.Bi, b.X, b.Y, b.Lsv, b.Rsv, b.d_save};  // B and C live inside A's allocation
     for (void *a : arrs)
         if (a) (void)hipFree(a);
     b.A = b.B = b.C = b.Bi = b.X = b.Y = b.Lsv = b.Rsv = nullptr;

@@ -1,6 +1,7 @@
 // Internal definitions of libgiraffe_amd (host side + kernel launch prototypes).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <chrono>
 #include <string>
 #include <vector>
 #include "../../include/giraffe_amd.h"
@@ -14,6 +15,12 @@ void gf_set_error(const std::string &msg);
             gf_set_error(std::string(#call) + ": " + hipGetErrorString(e_));           \
             return GF_E_DEVICE;                                                       \
         }                                                                             \
+    } while (0)
+
+#define GF_TRY(x)                   \
+    do {                            \
+        int s_ = (x);               \
+        if (s_ != GF_OK) return s_; \
     } while (0)
 
 // ---- per-element constant record of Beam_1 (structure of arrays, [BC_N][nb]) ----
@@ -171,6 +178,10 @@ struct gf_model {
     float *d_band32 = nullptr;
     double *d_scale = nullptr, *d_res = nullptr, *d_w = nullptr;
     long long n_solves = 0, n_sweeps = 0, n_fallbacks = 0;
+    // enqueue/result pairs (asynchronous drivers)
+    bool ev_pending[GF_NPHASE] = {};
+    long long pending_launches[GF_NPHASE] = {};
+    int pending_info_fail = 0;
     int npad = 0;                   // padded system size
     double *d_scal = nullptr;       // small reduction results
     double *h_scal = nullptr;       // pinned mirror
@@ -214,6 +225,37 @@ int gf_btd_factor(gf_btd &b, hipStream_t s, int *d_flag);
 int gf_btd_solve(gf_btd &b, hipStream_t s, const double *rhs, double *x, int n, double *work);
 // factorisation with the right-hand side reduced alongside: x = K^-1 rhs
 int gf_btd_factor_solve(gf_btd &b, hipStream_t s, int *d_flag, const double *rhs, double *x, int n, double *work);
+
+// Asynchronous halves of gf_assemble and gf_solve + gf_update: *_enqueue puts the
+// work on the model's stream, *_result reads the norms once the stream passed it.
+int gf_assemble_enqueue(gf_model *m);
+int gf_assemble_result(gf_model *m, double *res_out);
+int gf_solve_update_enqueue(gf_model *m);
+int gf_solve_update_result(gf_model *m, int *info_fail, double *inc_out);
+
+int gf_commit_enqueue(gf_model *m);
+
+// The Static loop as a resumable state machine (gf_static.cpp).
+struct gf_static_run {
+    enum Phase { PH_STEP, PH_ASSEMBLED, PH_SOLVED, PH_ITER_END, PH_DONE };
+    gf_model *m = nullptr;
+    gf_static_params p{};
+    gf_static_report r{};
+    Phase phase = PH_DONE;
+    hipEvent_t ev = nullptr;
+    bool has_f = false, has_m = false, aborted = false, diverged = false, res_conv = false, gl_conv = false;
+    double force_crit = 0, moment_crit = 0, time_step = 0, time = 0, last_converged = 0, last_rel = 0;
+    int n_conv_evaluated = 0, convergence_counter = 0, it = 0, info_fail = 0;
+    std::chrono::steady_clock::time_point t0, t_mark;
+    int start(gf_model *model, const gf_static_params &params, int *step_iters, int cap);
+    int advance();   // 1: device work pending (event ev), 0: finished, < 0: error
+    int wait();      // blocks on ev
+    ~gf_static_run();
+
+  private:
+    int enqueue_assemble();
+    int record();
+};
 
 int gf_mixed_alloc(gf_model *m);
 void gf_mixed_free(gf_model *m);

@@ -54,12 +54,6 @@ template <typename T> static int upload(gf_model *m, T *dst, const T *src, size_
     return GF_OK;
 }
 
-#define GF_TRY(x)                 \
-    do {                          \
-        int s_ = (x);             \
-        if (s_ != GF_OK) return s_; \
-    } while (0)
-
 // Launchers that return a launch count (>= 0) or a negative status.
 #define GF_TRYN(var, x)              \
     int var = (x);                   \
@@ -69,7 +63,9 @@ template <typename T> static int upload(gf_model *m, T *dst, const T *src, size_
 
 static void tick(gf_model *m, int ph, bool start)
 {
-    if (m->timing) (void)hipEventRecord(m->ev[2 * ph + (start ? 0 : 1)], m->stream);
+    if (!m->timing) return;
+    (void)hipEventRecord(m->ev[2 * ph + (start ? 0 : 1)], m->stream);
+    if (!start) m->ev_pending[ph] = true;
 }
 // Call after the stream work of the phase was enqueued; waits for its end event.
 static int tock(gf_model *m, int ph, int launches)
@@ -80,8 +76,27 @@ static int tock(gf_model *m, int ph, int launches)
     GF_HIP(hipEventElapsedTime(&ms, m->ev[2 * ph], m->ev[2 * ph + 1]));
     m->t_ms[ph] += ms;
     m->launches[ph] += launches;
+    m->ev_pending[ph] = false;
     return GF_OK;
 }
+// Accumulates the phases whose end events the stream has passed (enqueue/result pairs).
+static int flush_timing(gf_model *m)
+{
+    for (int ph = 0; ph < GF_NPHASE; ph++) {
+        if (m->timing && m->ev_pending[ph]) {
+            float ms = 0.f;
+            GF_HIP(hipEventSynchronize(m->ev[2 * ph + 1]));
+            GF_HIP(hipEventElapsedTime(&ms, m->ev[2 * ph], m->ev[2 * ph + 1]));
+            m->t_ms[ph] += ms;
+            m->launches[ph] += m->pending_launches[ph];
+        }
+        m->ev_pending[ph] = false;
+        m->pending_launches[ph] = 0;
+    }
+    return GF_OK;
+}
+
+static int solve_fp64_enqueue(gf_model *m);
 
 gf_gmat gf_model_gmat(const gf_model *m)
 {
@@ -143,7 +158,9 @@ static int colour_elements(const int *conn, int ne, int nper, int nn, std::vecto
     return GF_OK;
 }
 
-static int create(const gf_model_desc *d, const gf_options *opt, gf_model *m)
+// Host-side planning shared by create() and gf_model_footprint: validation,
+// DOFsActive + SetGlobalDOFs, half-bandwidth, storage layout, colourings.
+static int plan(const gf_model_desc *d, gf_model *m, int &bm, int &nblk)
 {
     if (d->n_nodes <= 0 || !d->node_xyz) { gf_set_error("model has no nodes"); return GF_E_ARG; }
     m->nn = d->n_nodes;
@@ -203,7 +220,7 @@ static int create(const gf_model_desc *d, const gf_options *opt, gf_model *m)
     }
     m->half_band = hb;
     // narrow bands: block-tridiagonal form with the smallest block size m <= 32
-    int bm = -1;
+    bm = -1;
     for (int cand = std::max(1, (hb + 2) / 2); cand <= 32 && bm < 0; cand++) {
         bool ok = true;
         for (int e = 0; e < nb + ns && ok; e++)
@@ -211,7 +228,7 @@ static int create(const gf_model_desc *d, const gf_options *opt, gf_model *m)
         if (ok) bm = cand;
     }
     m->layout = bm > 0 ? GM_BTD : GM_BAND;
-    int nblk = 0;
+    nblk = 0;
     if (m->layout == GM_BTD) {
         nblk = (nf + bm - 1) / bm;
         m->npad = nblk * bm;
@@ -220,6 +237,40 @@ static int create(const gf_model_desc *d, const gf_options *opt, gf_model *m)
     }
     GF_TRY(colour_elements(d->beam_conn, nb, 3, nn, m->beam_order, m->beam_color_off));
     GF_TRY(colour_elements(d->shell_conn, ns, 6, nn, m->shell_order, m->shell_color_off));
+    return GF_OK;
+}
+
+// Device bytes create() allocates for a planned model.
+static size_t footprint(const gf_model_desc *d, const gf_model *m, int precision, int bm, int nblk)
+{
+    const size_t nn = (size_t)m->nn, nb = (size_t)m->nb, ns = (size_t)m->ns, nf = (size_t)m->n_free;
+    const size_t npad = (size_t)m->npad;
+    size_t b = 0;
+    b += nn * (6 + 6 + 6 + 3) * 8 + nn * 6 * 4 + nf * 4;                       // node state, DOF map
+    b += nb * (3 * 4 + BC_COUNT * 8 + 2 * BG_COUNT * 8);                      // Beam_1 records
+    b += ns * (6 * 4 + SC_COUNT * 8 + SP_COUNT * 8 + 2 * SG_COUNT * 8 + 8);   // Shell_1 records
+    size_t items = d->n_loads ? (size_t)d->load_set_off[d->n_loads] : 0;
+    size_t rows = d->n_loads ? (size_t)d->load_tab_off[d->n_loads] : 0;
+    b += (size_t)d->n_loads * NL_REC * 8 + items * 12 + rows * 7 * 8;         // loads and tables
+    b += 3 * npad * 8 + 256;                                                   // r, x, work, scalars
+    size_t tangent;
+    if (m->layout == GM_BTD) {
+        size_t blocks = 6 * (size_t)nblk + 2 * (size_t)nblk;                  // A B C Bi X Y + saved couplings
+        tangent = blocks * (size_t)bm * bm;
+    } else {
+        tangent = npad * (size_t)(2 * m->half_band + 1);
+    }
+    b += tangent * 8;
+    if (precision == GF_PREC_MIXED) b += tangent * 4 + 3 * npad * 8;
+    return b;
+}
+
+static int create(const gf_model_desc *d, const gf_options *opt, gf_model *m)
+{
+    int bm = -1, nblk = 0;
+    GF_TRY(plan(d, m, bm, nblk));
+    const int nn = m->nn, nb = m->nb, ns = m->ns, nf = m->n_free;
+    const int hb = m->half_band;
     // device + stream
     m->device = opt ? opt->device : 0;
     GF_HIP(hipSetDevice(m->device));
@@ -442,6 +493,16 @@ extern "C" int gf_timing_read(gf_model *m, double *ms_out, long long *launches_o
     return GF_OK;
 }
 
+extern "C" int gf_model_footprint(const gf_model_desc *desc, const gf_options *opt, unsigned long long *bytes)
+{
+    if (!desc || !bytes) return GF_E_ARG;
+    gf_model tmp;
+    int bm = -1, nblk = 0;
+    GF_TRY(plan(desc, &tmp, bm, nblk));
+    *bytes = footprint(desc, &tmp, opt ? opt->precision : GF_PREC_FP64, bm, nblk);
+    return GF_OK;
+}
+
 extern "C" int gf_model_create(const gf_model_desc *desc, const gf_options *opt, gf_model **out)
 {
     if (!desc || !out) { gf_set_error("null argument"); return GF_E_ARG; }
@@ -514,17 +575,15 @@ extern "C" int gf_set_solution_window(gf_model *m, double start, double end)
     return GF_OK;
 }
 
-extern "C" int gf_assemble(gf_model *m, double *res_out)
+// Enqueues Clear + MountLocal/ElementLoads/Loads + MountGlobal and the residual
+// max-norm scan; the norms land in h_scal[0..2] when the stream reaches them.
+int gf_assemble_enqueue(gf_model *m)
 {
-    if (!m) return GF_E_ARG;
     GF_HIP(hipSetDevice(m->device));
     size_t npad = (size_t)m->npad;
     GF_TRY(gf_launch_fill(m->stream, m->d_r, npad, 0.0));
     if (m->layout == GM_BTD) {
-        size_t nbl = (size_t)m->btd.nblk * m->btd.bsz;
-        GF_TRY(gf_launch_fill(m->stream, (double *)m->btd.A, nbl, 0.0));
-        GF_TRY(gf_launch_fill(m->stream, (double *)m->btd.B, nbl, 0.0));
-        GF_TRY(gf_launch_fill(m->stream, (double *)m->btd.C, nbl, 0.0));
+        GF_TRY(gf_launch_fill(m->stream, (double *)m->btd.A, 3 * (size_t)m->btd.nblk * m->btd.bsz, 0.0));
     } else {
         GF_TRY(gf_launch_fill(m->stream, m->d_band, m->band_elems, 0.0));
     }
@@ -536,21 +595,101 @@ extern "C" int gf_assemble(gf_model *m, double *res_out)
     GF_TRYN(nl_beam, gf_beam_assemble(m, lf));
     GF_TRYN(nl_shell, gf_shell_assemble(m, lf));
     tick(m, GF_PH_ASSEMBLE, false);
-    GF_TRY(tock(m, GF_PH_ASSEMBLE, nl_beam + nl_shell));
+    m->pending_launches[GF_PH_ASSEMBLE] += nl_beam + nl_shell;
     GF_TRY(gf_shell_pressure(m, t));
     m->factored = false;
-    if (res_out) {
-        GF_TRY(gf_scan_residual(m));
-        GF_HIP(hipMemcpyAsync(m->h_scal, m->d_scal, 3 * sizeof(double), hipMemcpyDeviceToHost, m->stream));
-        GF_HIP(hipStreamSynchronize(m->stream));
-        unsigned long long *u = (unsigned long long *)m->h_scal;
-        double v0, v1;
-        memcpy(&v0, &u[0], 8);
-        memcpy(&v1, &u[1], 8);
-        res_out[0] = v0;
-        res_out[1] = v1;
-        res_out[2] = u[2] ? 1.0 : 0.0;
+    GF_TRY(gf_scan_residual(m));
+    GF_HIP(hipMemcpyAsync(m->h_scal, m->d_scal, 3 * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+    return GF_OK;
+}
+
+// Reads what gf_assemble_enqueue left in h_scal (the stream must have passed it).
+int gf_assemble_result(gf_model *m, double *res_out)
+{
+    GF_TRY(flush_timing(m));
+    unsigned long long *u = (unsigned long long *)m->h_scal;
+    double v0, v1;
+    memcpy(&v0, &u[0], 8);
+    memcpy(&v1, &u[1], 8);
+    res_out[0] = v0;
+    res_out[1] = v1;
+    res_out[2] = u[2] ? 1.0 : 0.0;
+    return GF_OK;
+}
+
+extern "C" int gf_assemble(gf_model *m, double *res_out)
+{
+    if (!m) return GF_E_ARG;
+    GF_TRY(gf_assemble_enqueue(m));
+    GF_HIP(hipStreamSynchronize(m->stream));
+    double res[3];
+    GF_TRY(gf_assemble_result(m, res));
+    if (res_out) memcpy(res_out, res, sizeof(res));
+    return GF_OK;
+}
+
+// FP64: enqueues the tangent solve, the failure-flag copy, UpdateDisps and the
+// increment norms (h_scal[0..4], flag at h_scal[8]).  Mixed precision: the
+// solve runs to completion here (its refinement is host-steered), the update
+// is enqueued after it.
+int gf_solve_update_enqueue(gf_model *m)
+{
+    GF_HIP(hipSetDevice(m->device));
+    if (m->precision == GF_PREC_MIXED) {
+        int info_fail = 0;
+        GF_TRY(gf_solve(m, &info_fail));
+        m->pending_info_fail = info_fail;
+    } else {
+        GF_TRY(gf_launch_fill_bytes(m->stream, m->d_flag, 4));
+        m->n_solves++;
+        GF_TRY(solve_fp64_enqueue(m));
+        GF_HIP(hipMemcpyAsync(&m->h_scal[8], m->d_flag, 4, hipMemcpyDeviceToHost, m->stream));
+        m->pending_info_fail = -1;
     }
+    GF_TRY(gf_update_disp(m));
+    GF_HIP(hipMemcpyAsync(m->h_scal, m->d_scal, 5 * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+    return GF_OK;
+}
+
+int gf_solve_update_result(gf_model *m, int *info_fail, double *inc_out)
+{
+    GF_TRY(flush_timing(m));
+    if (m->pending_info_fail >= 0) {
+        *info_fail = m->pending_info_fail;
+    } else {
+        int flag = 0;
+        memcpy(&flag, &m->h_scal[8], 4);
+        *info_fail = flag ? 1 : 0;
+    }
+    unsigned long long *u = (unsigned long long *)m->h_scal;
+    for (int i = 0; i < 4; i++) memcpy(&inc_out[i], &u[i], 8);
+    inc_out[4] = u[4] ? 1.0 : 0.0;
+    return GF_OK;
+}
+
+// FP64 factorisation + solve of K x = r on the model's stream (x in d_x).
+// Block-tridiagonal: the right-hand side is reduced alongside the factorisation
+// (the factor phase covers the forward sweep too; the solve phase is empty).
+static int solve_fp64_enqueue(gf_model *m)
+{
+    size_t npad = (size_t)m->npad;
+    tick(m, GF_PH_FACTOR, true);
+    int nl_fac, nl_sol = 0;
+    if (m->layout == GM_BTD)
+        nl_fac = gf_btd_factor_solve(m->btd, m->stream, m->d_flag, m->d_r, m->d_x, (int)npad, m->d_swork);
+    else nl_fac = gf_band_factor(m->d_band, 8, m->npad, m->half_band, m->stream, m->d_flag);
+    if (nl_fac < 0) return nl_fac;
+    tick(m, GF_PH_FACTOR, false);
+    m->factored = true;
+    tick(m, GF_PH_SOLVE, true);
+    if (m->layout != GM_BTD) {
+        GF_HIP(hipMemcpyAsync(m->d_x, m->d_r, sizeof(double) * npad, hipMemcpyDeviceToDevice, m->stream));
+        nl_sol = gf_band_solve(m->d_band, 8, m->npad, m->half_band, m->stream, m->d_x, m->d_swork);
+    }
+    if (nl_sol < 0) return nl_sol;
+    tick(m, GF_PH_SOLVE, false);
+    m->pending_launches[GF_PH_FACTOR] += nl_fac;
+    m->pending_launches[GF_PH_SOLVE] += nl_sol;
     return GF_OK;
 }
 
@@ -560,7 +699,6 @@ extern "C" int gf_solve(gf_model *m, int *info_fail)
     GF_HIP(hipSetDevice(m->device));
     *info_fail = 0;
     GF_TRY(gf_launch_fill_bytes(m->stream, m->d_flag, 4));
-    size_t npad = (size_t)m->npad;
     m->n_solves++;
     if (m->precision == GF_PREC_MIXED) {
         tick(m, GF_PH_FACTOR, true);
@@ -591,25 +729,8 @@ extern "C" int gf_solve(gf_model *m, int *info_fail)
         m->n_fallbacks++;
         GF_TRY(gf_launch_fill_bytes(m->stream, m->d_flag, 4));
     }
-    // block-tridiagonal: the right-hand side is reduced alongside the factorisation
-    // (the factor phase covers the forward sweep too; the solve phase is empty)
-    tick(m, GF_PH_FACTOR, true);
-    int nl_fac, nl_sol = 0;
-    if (m->layout == GM_BTD)
-        nl_fac = gf_btd_factor_solve(m->btd, m->stream, m->d_flag, m->d_r, m->d_x, (int)npad, m->d_swork);
-    else nl_fac = gf_band_factor(m->d_band, 8, m->npad, m->half_band, m->stream, m->d_flag);
-    if (nl_fac < 0) return nl_fac;
-    tick(m, GF_PH_FACTOR, false);
-    m->factored = true;
-    tick(m, GF_PH_SOLVE, true);
-    if (m->layout != GM_BTD) {
-        GF_HIP(hipMemcpyAsync(m->d_x, m->d_r, sizeof(double) * npad, hipMemcpyDeviceToDevice, m->stream));
-        nl_sol = gf_band_solve(m->d_band, 8, m->npad, m->half_band, m->stream, m->d_x, m->d_swork);
-    }
-    if (nl_sol < 0) return nl_sol;
-    tick(m, GF_PH_SOLVE, false);
-    GF_TRY(tock(m, GF_PH_FACTOR, nl_fac));
-    GF_TRY(tock(m, GF_PH_SOLVE, nl_sol));
+    GF_TRY(solve_fp64_enqueue(m));
+    GF_TRY(flush_timing(m));
     int flag = 0;
     GF_HIP(hipMemcpyAsync(&m->h_scal[8], m->d_flag, 4, hipMemcpyDeviceToHost, m->stream));
     GF_HIP(hipStreamSynchronize(m->stream));
@@ -633,13 +754,19 @@ extern "C" int gf_update(gf_model *m, double *inc_out)
     return GF_OK;
 }
 
-extern "C" int gf_commit(gf_model *m)
+int gf_commit_enqueue(gf_model *m)
 {
-    if (!m) return GF_E_ARG;
     GF_HIP(hipSetDevice(m->device));
     GF_TRY(gf_node_commit(m));
     GF_TRY(gf_beam_commit(m));
     GF_TRY(gf_shell_commit(m));
+    return GF_OK;
+}
+
+extern "C" int gf_commit(gf_model *m)
+{
+    if (!m) return GF_E_ARG;
+    GF_TRY(gf_commit_enqueue(m));
     GF_HIP(hipStreamSynchronize(m->stream));
     return GF_OK;
 }

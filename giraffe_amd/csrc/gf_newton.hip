@@ -111,59 +111,57 @@ __device__ __forceinline__ void amax(unsigned long long *p, double v)
     atomicMax(p, (unsigned long long)__double_as_longlong(v));
 }
 
-__global__ void scan_kernel(int n, const double *__restrict__ v, const int *__restrict__ row_nc,
-                            unsigned long long *out)
+// Workgroup max of NV non-negative values (256 lanes): wave shuffles, then LDS
+// across the 4 waves, then one atomic per value and workgroup.
+template <int NV> __device__ __forceinline__ void block_amax(double (&v)[NV], unsigned long long *out)
 {
-    double mt = 0.0, mr = 0.0;
-    int badv = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        double a = fabs(v[i]);
-        if (!(a == a) || a >= 1e300) badv = 1;
-        else if (row_nc[i] % 6 < 3) mt = fmax(mt, a);
-        else mr = fmax(mr, a);
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        mt = fmax(mt, __shfl_xor(mt, off));
-        mr = fmax(mr, __shfl_xor(mr, off));
-        badv |= __shfl_xor(badv, off);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        amax(out + 0, mt);
-        amax(out + 1, mr);
-        if (badv) atomicMax(out + 2, 1ull);
+    __shared__ double sred[4][NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++)
+        for (int off = 32; off > 0; off >>= 1) v[k] = fmax(v[k], __shfl_xor(v[k], off));
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; k++) sred[wave][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double mx = sred[0][threadIdx.x];
+        for (int w = 1; w < 4; w++) mx = fmax(mx, sred[w][threadIdx.x]);
+        amax(out + threadIdx.x, mx);
     }
 }
 
-// d += x on free DOFs; scans |x| and the updated |d| (CheckGLConvergence).
-__global__ void update_kernel(int n, const double *__restrict__ x, const int *__restrict__ row_nc,
-                              double *__restrict__ disp, unsigned long long *out)
+// max |r| over translational and rotational free DOFs, and a non-finite flag.
+__global__ __launch_bounds__(256) void scan_kernel(int n, const double *__restrict__ v,
+                                                   const int *__restrict__ row_nc, unsigned long long *out)
 {
-    double xt = 0.0, xr = 0.0, dt = 0.0, dr = 0.0;
-    int badv = 0;
+    double m[3] = {0.0, 0.0, 0.0};  // translations, rotations, non-finite
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double a = fabs(v[i]);
+        if (!(a == a) || a >= 1e300) m[2] = 1.0;
+        else if (row_nc[i] % 6 < 3) m[0] = fmax(m[0], a);
+        else m[1] = fmax(m[1], a);
+    }
+    block_amax<3>(m, out);
+}
+
+// d += x on free DOFs; scans |x| and the updated |d| (CheckGLConvergence).
+__global__ __launch_bounds__(256) void update_kernel(int n, const double *__restrict__ x,
+                                                     const int *__restrict__ row_nc, double *__restrict__ disp,
+                                                     unsigned long long *out)
+{
+    double m[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // |x| t/r, |d| t/r, non-finite
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         int nc = row_nc[i];
         double xi = x[i];
         double dn = disp[nc] + xi;
         disp[nc] = dn;
         double ax = fabs(xi), ad = fabs(dn);
-        if (!(ax == ax) || ax >= 1e300) badv = 1;
-        if (nc % 6 < 3) { xt = fmax(xt, ax); dt = fmax(dt, ad); }
-        else { xr = fmax(xr, ax); dr = fmax(dr, ad); }
+        if (!(ax == ax) || ax >= 1e300) m[4] = 1.0;
+        if (nc % 6 < 3) { m[0] = fmax(m[0], ax); m[2] = fmax(m[2], ad); }
+        else { m[1] = fmax(m[1], ax); m[3] = fmax(m[3], ad); }
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        xt = fmax(xt, __shfl_xor(xt, off));
-        xr = fmax(xr, __shfl_xor(xr, off));
-        dt = fmax(dt, __shfl_xor(dt, off));
-        dr = fmax(dr, __shfl_xor(dr, off));
-        badv |= __shfl_xor(badv, off);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        amax(out + 0, xt);
-        amax(out + 1, xr);
-        amax(out + 2, dt);
-        amax(out + 3, dr);
-        if (badv) atomicMax(out + 4, 1ull);
-    }
+    block_amax<5>(m, out);
 }
 
 // Node::SaveConfiguration: x += u, alpha <- compose(alpha_delta, alpha), Euler vector.
@@ -234,7 +232,7 @@ int gf_scan_residual(gf_model *m)
 {
     int st = gf_launch_fill_bytes(m->stream, m->d_scal, 8 * sizeof(double));
     if (st) return st;
-    hipLaunchKernelGGL(scan_kernel, dim3(grid_for(m->n_free, 256, 1024)), dim3(256), 0, m->stream, m->n_free, m->d_r,
+    hipLaunchKernelGGL(scan_kernel, dim3(grid_for(m->n_free, 256, 128)), dim3(256), 0, m->stream, m->n_free, m->d_r,
                        m->d_row_nc, (unsigned long long *)m->d_scal);
     GF_HIP(hipGetLastError());
     return GF_OK;
@@ -244,7 +242,7 @@ int gf_update_disp(gf_model *m)
 {
     int st = gf_launch_fill_bytes(m->stream, m->d_scal, 8 * sizeof(double));
     if (st) return st;
-    hipLaunchKernelGGL(update_kernel, dim3(grid_for(m->n_free, 256, 1024)), dim3(256), 0, m->stream, m->n_free,
+    hipLaunchKernelGGL(update_kernel, dim3(grid_for(m->n_free, 256, 128)), dim3(256), 0, m->stream, m->n_free,
                        m->d_x, m->d_row_nc, m->d_disp, (unsigned long long *)m->d_scal);
     GF_HIP(hipGetLastError());
     return GF_OK;

@@ -51,6 +51,21 @@ def cs_rows(E1, E3):
     return np.concatenate([E1, E2, E3])
 
 
+def cs_rows_many(E1, E3):
+    """cs_rows for many E3 directions (rows [n][3], normalised first) and one E1."""
+    E3 = np.asarray(E3, float)
+    E3 = E3 / np.linalg.norm(E3, axis=1)[:, None]
+    E1 = np.broadcast_to(np.asarray(E1, float), E3.shape)
+    E2 = np.cross(E3, E1)
+    n1 = np.linalg.norm(E1, axis=1)[:, None]
+    n2 = np.linalg.norm(E2, axis=1)[:, None]
+    n3 = np.linalg.norm(E3, axis=1)[:, None]
+    E1 = np.where(n1 != 1.0, E1 / n1, E1)
+    E2 = np.where(n2 != 1.0, E2 / n2, E2)
+    E3 = np.where(n3 != 1.0, E3 / n3, E3)
+    return np.concatenate([E1, E2, E3], 1)
+
+
 GLOBAL_CS = cs_rows([1, 0, 0], [0, 0, 1])
 
 
@@ -125,8 +140,31 @@ class Model:
 
     def add_load(self, kind, items, times, vals, cs9=GLOBAL_CS, flags=0):
         vals = np.asarray(vals, float).reshape(len(times), 6)
-        self.loads.append((kind, list(items), np.asarray(cs9, float), int(flags),
-                           np.asarray(times, float), vals))
+        items = np.asarray(list(items), np.int32)
+        self.loads.append({"kind": np.array([kind], np.int32), "set_off": np.array([0, len(items)], np.int32),
+                           "items": items, "cs": np.asarray(cs9, float).reshape(1, 9),
+                           "flags": np.array([int(flags)], np.int32),
+                           "tab_off": np.array([0, len(times)], np.int32),
+                           "tab_time": np.asarray(times, float), "tab_val": vals})
+        self._frozen = None
+
+    def add_nodal_loads(self, kind, nodes, times, vals, cs9=GLOBAL_CS):
+        """One load per node (load i on nodes[i]) sharing the instants `times`;
+        vals [len(nodes)][len(times)][6].  Same as add_load per node, in order."""
+        nodes = np.asarray(nodes, np.int32)
+        n, nt = len(nodes), len(times)
+        vals = np.asarray(vals, float).reshape(n, nt, 6)
+        self.loads.append({"kind": np.full(n, kind, np.int32), "set_off": np.arange(n + 1, dtype=np.int32),
+                           "items": nodes, "cs": np.tile(np.asarray(cs9, float).reshape(1, 9), (n, 1)),
+                           "flags": np.zeros(n, np.int32), "tab_off": (np.arange(n + 1) * nt).astype(np.int32),
+                           "tab_time": np.tile(np.asarray(times, float), n), "tab_val": vals.reshape(n * nt, 6)})
+        self._frozen = None
+
+    def add_beams(self, conn, cs, prop):
+        """Bulk add_beam: conn [n][3], cs [n][9], prop [n][GF_BEAM_NPROP]."""
+        self.beam_conn.extend(np.asarray(conn, np.int64).reshape(-1, 3).tolist())
+        self.beam_cs.extend(np.asarray(cs, float).reshape(-1, 9).tolist())
+        self.beam_prop.extend(np.asarray(prop, float).reshape(-1, abi.GF_BEAM_NPROP).tolist())
         self._frozen = None
 
     # -- sizes ------------------------------------------------------------
@@ -142,6 +180,10 @@ class Model:
     def n_shells(self):
         return len(self.shell_conn)
 
+    @property
+    def n_loads(self):
+        return int(sum(len(c["kind"]) for c in self.loads))
+
     def freeze(self):
         if self._frozen is not None:
             return self._frozen
@@ -153,25 +195,33 @@ class Model:
         f["beam_prop"] = np.ascontiguousarray(np.asarray(self.beam_prop, float).reshape(-1, abi.GF_BEAM_NPROP))
         f["shell_conn"] = np.ascontiguousarray(np.asarray(self.shell_conn, np.int32).reshape(-1, 6))
         f["shell_prop"] = np.ascontiguousarray(np.asarray(self.shell_prop, float).reshape(-1, abi.GF_SHELL_NPROP))
-        nl = len(self.loads)
-        f["load_kind"] = np.asarray([l[0] for l in self.loads], np.int32)
-        off = [0]
-        items = []
-        toff = [0]
-        tt, tv = [], []
-        for l in self.loads:
-            items.extend(l[1])
-            off.append(len(items))
-            tt.extend(l[4].tolist())
-            tv.append(l[5])
-            toff.append(len(tt))
-        f["load_set_off"] = np.asarray(off, np.int32)
-        f["load_set_items"] = np.asarray(items if items else [0], np.int32)
-        f["load_cs"] = np.ascontiguousarray(np.asarray([l[2] for l in self.loads], float).reshape(-1, 9)) if nl else np.zeros((1, 9))
-        f["load_flags"] = np.asarray([l[3] for l in self.loads] if nl else [0], np.int32)
-        f["load_tab_off"] = np.asarray(toff, np.int32)
-        f["tab_time"] = np.asarray(tt if tt else [0.0], float)
-        f["tab_val"] = np.ascontiguousarray(np.concatenate(tv, 0)) if tv else np.zeros((1, 6))
+        L = self.loads
+        nl = self.n_loads
+        if nl:
+            f["load_kind"] = np.concatenate([c["kind"] for c in L])
+            so, to, items_base, tab_base = [np.zeros(1, np.int32)], [np.zeros(1, np.int32)], 0, 0
+            for c in L:
+                so.append(c["set_off"][1:] + items_base)
+                to.append(c["tab_off"][1:] + tab_base)
+                items_base += int(c["set_off"][-1])
+                tab_base += int(c["tab_off"][-1])
+            f["load_set_off"] = np.ascontiguousarray(np.concatenate(so), np.int32)
+            items = np.concatenate([c["items"] for c in L])
+            f["load_set_items"] = np.ascontiguousarray(items if len(items) else np.zeros(1), np.int32)
+            f["load_cs"] = np.ascontiguousarray(np.concatenate([c["cs"] for c in L]), float)
+            f["load_flags"] = np.ascontiguousarray(np.concatenate([c["flags"] for c in L]), np.int32)
+            f["load_tab_off"] = np.ascontiguousarray(np.concatenate(to), np.int32)
+            f["tab_time"] = np.ascontiguousarray(np.concatenate([c["tab_time"] for c in L]), float)
+            f["tab_val"] = np.ascontiguousarray(np.concatenate([c["tab_val"] for c in L]), float)
+        else:
+            f["load_kind"] = np.zeros(1, np.int32)
+            f["load_set_off"] = np.zeros(1, np.int32)
+            f["load_set_items"] = np.zeros(1, np.int32)
+            f["load_cs"] = np.zeros((1, 9))
+            f["load_flags"] = np.zeros(1, np.int32)
+            f["load_tab_off"] = np.zeros(1, np.int32)
+            f["tab_time"] = np.zeros(1)
+            f["tab_val"] = np.zeros((1, 6))
         self._frozen = f
         return f
 
@@ -188,7 +238,7 @@ class Model:
         d.n_shells = self.n_shells
         d.shell_conn = abi.ptr(f["shell_conn"], C.c_int)
         d.shell_prop = abi.ptr(f["shell_prop"], C.c_double)
-        d.n_loads = len(self.loads)
+        d.n_loads = self.n_loads
         d.load_kind = abi.ptr(f["load_kind"], C.c_int)
         d.load_set_off = abi.ptr(f["load_set_off"], C.c_int)
         d.load_set_items = abi.ptr(f["load_set_items"], C.c_int)
@@ -289,20 +339,18 @@ def catenary(n_elem=10000, span_h=1500.0, span_v=1000.0, arc_ratio=1.15, De=0.3,
     m.add_nodes(xyz)
     A, I11, I22, I12, It = tube_section(De, Di)
     rho = w / (g * A)
-    for e in range(ne):
-        p1, p3 = xyz[2 * e], xyz[2 * e + 2]
-        e3 = p3 - p1
-        cs = cs_rows([0, 1, 0], e3 / np.linalg.norm(e3))
-        m.add_beam(2 * e, 2 * e + 1, 2 * e + 2, cs, E, nu, rho, A, I11, I22, I12, It)
+    e = np.arange(ne)
+    m.add_beams(np.stack([2 * e, 2 * e + 1, 2 * e + 2], 1), cs_rows_many([0.0, 1.0, 0.0], xyz[2::2] - xyz[:-2:2]),
+                np.tile([E, nu, rho, A, I11, I22, I12, It, 0.0], (ne, 1)))
     m.fix_dofs(0, [0, 1, 2, 3])
     m.fix_dofs(nn - 1, [0, 1, 2, 3])
     m.gravity = (0.0, 0.0, -g)
     m.gravity_on = 1
     rng = np.random.default_rng(seed)
     fy = rng.normal(0.0, sigma, nn)
-    for i in range(nn):
-        m.add_load(abi.GF_LOAD_NODAL, [i], [0.0, lateral_start, 1.0],
-                   [[0] * 6, [0] * 6, [0, fy[i], 0, 0, 0, 0]], GLOBAL_CS)
+    vals = np.zeros((nn, 3, 6))
+    vals[:, 2, 1] = fy
+    m.add_nodal_loads(abi.GF_LOAD_NODAL, np.arange(nn), [0.0, lateral_start, 1.0], vals, GLOBAL_CS)
     dt = 1.0 / n_steps
     Wtot = w * S
     p = StaticParams(end_time=1.0, time_step=dt, max_time_step=dt, min_time_step=dt / 1024.0,
@@ -377,3 +425,21 @@ def shell_plate(nq=64, L=1.0, t=0.005, E=70e9, nu=0.33, rho=2700.0, p_max=1.7e7,
     center = nid(nq, nq)
     info = {"center": center, "D": Dflex, "ng": ng, "n_free_expected": None}
     return m, p, info
+
+
+def riser_request(rng, n_elem=None, steps=10):
+    """One request of the catenary serving stream: Beam_1 riser with E ~ log-U[50, 5000]
+    elements, horizontal span U[200, 2000] m (vertical 2/3 of it), sag ratio U[0.2, 0.6]
+    (arc length ~ chord (1 + 8/3 sag^2)), OD U[0.1, 0.4] m, wall U[0.05, 0.1] OD, steel
+    rho = 7,850 kg/m^3 self-weight, N(0, 500 N) lateral nodal loads, `steps` load steps."""
+    if n_elem is None:
+        n_elem = int(round(math.exp(rng.uniform(math.log(50.0), math.log(5000.0)))))
+    span = rng.uniform(200.0, 2000.0)
+    sag = rng.uniform(0.2, 0.6)
+    od = rng.uniform(0.1, 0.4)
+    wall = rng.uniform(0.05, 0.1) * od
+    A = tube_section(od, od - 2.0 * wall)[0]
+    w = 7850.0 * 9.81 * A
+    seed = int(rng.integers(0, 2**31 - 1))
+    return catenary(n_elem=n_elem, span_h=span, span_v=2.0 * span / 3.0, arc_ratio=1.0 + 8.0 / 3.0 * sag * sag,
+                    De=od, Di=od - 2.0 * wall, w=w, seed=seed, n_steps=steps)

@@ -64,6 +64,15 @@ def load():
     L.gf_reset_state.argtypes = [vp]
     L.gf_timing_enable.argtypes = [vp, C.c_int]
     L.gf_timing_read.argtypes = [vp, dp, C.POINTER(C.c_longlong), C.c_int]
+    L.gf_model_footprint.argtypes = [C.POINTER(abi.GfModelDesc), C.POINTER(abi.GfOptions), C.POINTER(C.c_ulonglong)]
+    L.gf_default_server_options.argtypes = [C.POINTER(abi.GfServerOptions)]
+    L.gf_server_create.argtypes = [C.POINTER(abi.GfServerOptions), C.POINTER(vp)]
+    L.gf_server_destroy.argtypes = [vp]
+    L.gf_server_submit.argtypes = [vp, C.POINTER(abi.GfModelDesc), C.POINTER(abi.GfStaticParams),
+                                   C.POINTER(C.c_longlong)]
+    L.gf_server_pump.argtypes = [vp, C.c_int]
+    L.gf_server_counts.argtypes = [vp, C.POINTER(C.c_longlong)]
+    L.gf_server_take.argtypes = [vp, C.c_longlong, C.POINTER(abi.GfStaticReport), dp, dp]
     _lib = L
     return L
 
@@ -239,3 +248,68 @@ def bandsystem(A_rows, hb, b):
     check(load().gf_bandsystem(n, hb, _dp(A_rows), _dp(np.ascontiguousarray(b, float)), _dp(x), C.byref(fail)),
           "gf_bandsystem")
     return x, fail.value
+
+
+def footprint(model, precision=abi.GF_PREC_FP64):
+    """Device bytes a DeviceModel of `model` would hold (gf_model_footprint)."""
+    desc = model.desc()
+    o = options(precision)
+    b = C.c_ulonglong()
+    check(load().gf_model_footprint(C.byref(desc), C.byref(o), C.byref(b)), "gf_model_footprint")
+    return int(b.value)
+
+
+class Server:
+    """Request server (gf_server_*): independent static solves resident on one GPU,
+    admitted FIFO by slot and device memory, advanced one Newton step at a time."""
+
+    def __init__(self, precision=abi.GF_PREC_FP64, device=0, max_inflight=16, mem_cap=200 << 30):
+        L = load()
+        o = abi.GfServerOptions()
+        L.gf_default_server_options(C.byref(o))
+        o.precision, o.device, o.max_inflight, o.mem_cap = precision, device, max_inflight, int(mem_cap)
+        h = C.c_void_p()
+        check(L.gf_server_create(C.byref(o), C.byref(h)), "gf_server_create")
+        self.h = h
+        self.n_nodes = {}
+
+    def submit(self, model, params):
+        desc = model.desc()
+        pc = params.c()
+        rid = C.c_longlong()
+        check(load().gf_server_submit(self.h, C.byref(desc), C.byref(pc), C.byref(rid)), "gf_server_submit")
+        self.n_nodes[rid.value] = model.n_nodes
+        return rid.value
+
+    def pump(self, wait_us=0):
+        n = load().gf_server_pump(self.h, int(wait_us))
+        if n < 0:
+            check(n, "gf_server_pump")
+        return n
+
+    def counts(self):
+        o = (C.c_longlong * 4)()
+        check(load().gf_server_counts(self.h, o), "gf_server_counts")
+        return {"queued": o[0], "resident": o[1], "finished": o[2], "bytes": o[3]}
+
+    def take(self, rid, cap=4096):
+        """(report, step iterations, nodes [n][6], times_ms [submit->admit, submit->end, admit->end])."""
+        iters = np.zeros(cap, np.int32)
+        rep = abi.GfStaticReport()
+        rep.step_iters = iters.ctypes.data_as(C.POINTER(C.c_int))
+        rep.step_iters_cap = cap
+        nodes = np.zeros((self.n_nodes.pop(rid), 6))
+        times = np.zeros(3)
+        check(load().gf_server_take(self.h, rid, C.byref(rep), _dp(nodes), _dp(times)), "gf_server_take")
+        return rep, iters[:min(rep.n_step_iters, cap)].copy(), nodes, times
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().gf_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -146,6 +146,9 @@ int gf_model_info(const gf_model *m, long long *info_out /* [8] */);
 /* Tangent solves so far, FP64 refinement sweeps (mixed precision), solves that
  * fell back to the FP64 factorisation, reserved. */
 int gf_model_stats(const gf_model *m, long long *stats_out /* [4] */);
+/* Device bytes gf_model_create would allocate for this description (host-side
+ * planning only; no device call).  Used for admission by the request server. */
+int gf_model_footprint(const gf_model_desc *desc, const gf_options *opt, unsigned long long *bytes);
 
 /* ---- Newton step pieces --------------------------------------------------- */
 /* Time window of the current solution step; the gravity BoolTable ramp
@@ -223,6 +226,37 @@ int gf_bandsystem(int n, int hb, const double *ab_rows, const double *b, double 
  * 1..6, then alpha of nodes 4..6) and internal force 27. */
 int gf_shell_eval(int count, const double *x, const double *prop, const double *d,
                   const double *gp, double *K_out, double *f_out);
+
+/* ---- request server ----------------------------------------------------------
+ * No counterpart in the reference (it runs one Static::Solve per process); this is
+ * the serving loop around the same Static drop-in: many independent solves
+ * resident on one GPU, each with its own Newton state, tangent storage and
+ * stream, admitted FIFO while fewer than max_inflight are resident and their
+ * footprints fit under mem_cap, and advanced one Newton step at a time as their
+ * device work completes.  One host thread drives a server. */
+typedef struct gf_server gf_server;
+typedef struct gf_server_options {
+    int precision;                 /* GF_PREC_FP64 or GF_PREC_MIXED                  */
+    int device;                    /* HIP device ordinal                             */
+    int max_inflight;              /* resident requests at once (1..256)             */
+    unsigned long long mem_cap;    /* device bytes the resident requests may hold    */
+} gf_server_options;
+int gf_default_server_options(gf_server_options *opt);
+int gf_server_create(const gf_server_options *opt, gf_server **out);
+int gf_server_destroy(gf_server *s);
+/* Copies the description and parameters; *id_out identifies the request. */
+int gf_server_submit(gf_server *s, const gf_model_desc *desc, const gf_static_params *p, long long *id_out);
+/* Admits and advances requests for up to wait_us microseconds (returns earlier
+ * when nothing is queued or resident).  Returns the number of requests that
+ * finished during the call, or a negative status. */
+int gf_server_pump(gf_server *s, int wait_us);
+/* queued, resident, finished-not-taken, device bytes held by resident requests */
+int gf_server_counts(const gf_server *s, long long *out /* [4] */);
+/* Results of a finished request (then forgotten): report (step_iters filled up to
+ * its cap), node results [n_nodes][6] as gf_read_nodes (may be NULL), and
+ * times_ms[3] = submit->admit, submit->finish, admit->finish.  Returns the
+ * request's status (GF_OK or the error that ended it). */
+int gf_server_take(gf_server *s, long long id, gf_static_report *rep, double *nodes_out, double *times_ms);
 
 #ifdef __cplusplus
 }
