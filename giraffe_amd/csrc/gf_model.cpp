@@ -659,6 +659,10 @@ int gf_solve_update_result(gf_model *m, int *info_fail, double *inc_out)
     } else {
         int flag = 0;
         memcpy(&flag, &m->h_scal[8], 4);
+        if (flag == 2) {
+            gf_set_error("band solve: a block wait gave up");
+            return GF_E_DEVICE;
+        }
         *info_fail = flag ? 1 : 0;
     }
     unsigned long long *u = (unsigned long long *)m->h_scal;
@@ -722,6 +726,10 @@ extern "C" int gf_solve(gf_model *m, int *info_fail)
             GF_HIP(hipMemcpyAsync(&m->h_scal[8], m->d_flag, 4, hipMemcpyDeviceToHost, m->stream));
             GF_HIP(hipStreamSynchronize(m->stream));
             memcpy(&flag, &m->h_scal[8], 4);
+            if (flag == 2) {
+                gf_set_error("band solve: a block wait gave up");
+                return GF_E_DEVICE;
+            }
             if (flag) *info_fail = 1;
             return GF_OK;
         }
@@ -735,6 +743,10 @@ extern "C" int gf_solve(gf_model *m, int *info_fail)
     GF_HIP(hipMemcpyAsync(&m->h_scal[8], m->d_flag, 4, hipMemcpyDeviceToHost, m->stream));
     GF_HIP(hipStreamSynchronize(m->stream));
     memcpy(&flag, &m->h_scal[8], 4);
+    if (flag == 2) {
+        gf_set_error("band solve: a block wait gave up");
+        return GF_E_DEVICE;
+    }
     if (flag) *info_fail = 1;
     return GF_OK;
 }
@@ -879,17 +891,6 @@ extern "C" int gf_bandsystem(int n, int hb, const double *ab_rows, const double 
     int npad = gf_band_pad(n);
     size_t w = (size_t)(2 * hb + 1);
     size_t elems = (size_t)npad * w;
-    double *d_ab = nullptr, *d_x = nullptr, *d_z = nullptr;
-    int *d_flag = nullptr;
-    hipStream_t s;
-    GF_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    GF_HIP(hipMalloc(&d_ab, elems * 8));
-    GF_HIP(hipMalloc(&d_x, (size_t)npad * 8));
-    GF_HIP(hipMalloc(&d_z, (size_t)npad * 8));
-    GF_HIP(hipMalloc(&d_flag, 4));
-    GF_TRY(gf_launch_fill(s, d_ab, elems, 0.0));
-    GF_TRY(gf_launch_fill(s, d_x, (size_t)npad, 0.0));
-    GF_TRY(gf_launch_fill_bytes(s, d_flag, 4));
     // rows of the band: row i holds columns i-hb .. i+hb; columns outside [0, n) must be zero
     std::vector<double> rows((size_t)n * w);
     for (int i = 0; i < n; i++)
@@ -899,22 +900,45 @@ extern "C" int gf_bandsystem(int n, int hb, const double *ab_rows, const double 
         }
     std::vector<double> diag((size_t)(npad - n) * w, 0.0);
     for (int i = n; i < npad; i++) diag[(size_t)(i - n) * w + hb] = 1.0;
-    GF_HIP(hipMemcpyAsync(d_ab, rows.data(), rows.size() * 8, hipMemcpyHostToDevice, s));
-    if (npad > n) GF_HIP(hipMemcpyAsync(d_ab + (size_t)n * w, diag.data(), diag.size() * 8, hipMemcpyHostToDevice, s));
-    GF_HIP(hipMemcpyAsync(d_x, b, (size_t)n * 8, hipMemcpyHostToDevice, s));
-    int st = gf_band_factor(d_ab, 8, npad, hb, s, d_flag);
-    if (st >= 0) st = gf_band_solve(d_ab, 8, npad, hb, s, d_x, d_z);
+    double *d_ab = nullptr, *d_x = nullptr, *d_z = nullptr;
+    int *d_flag = nullptr;
+    hipStream_t s = nullptr;
     int flag = 0;
-    if (st >= 0) {
+    int st = GF_OK;
+    auto run = [&]() -> int {
+        GF_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        GF_HIP(hipMalloc(&d_ab, elems * 8));
+        GF_HIP(hipMalloc(&d_x, (size_t)npad * 8));
+        GF_HIP(hipMalloc(&d_z, (size_t)npad * 8));
+        GF_HIP(hipMalloc(&d_flag, 4));
+        GF_TRY(gf_launch_fill(s, d_ab, elems, 0.0));
+        GF_TRY(gf_launch_fill(s, d_x, (size_t)npad, 0.0));
+        GF_TRY(gf_launch_fill_bytes(s, d_flag, 4));
+        GF_HIP(hipMemcpyAsync(d_ab, rows.data(), rows.size() * 8, hipMemcpyHostToDevice, s));
+        if (npad > n)
+            GF_HIP(hipMemcpyAsync(d_ab + (size_t)n * w, diag.data(), diag.size() * 8, hipMemcpyHostToDevice, s));
+        GF_HIP(hipMemcpyAsync(d_x, b, (size_t)n * 8, hipMemcpyHostToDevice, s));
+        int l = gf_band_factor(d_ab, 8, npad, hb, s, d_flag);
+        if (l < 0) return l;
+        l = gf_band_solve(d_ab, 8, npad, hb, s, d_x, d_z);
+        if (l < 0) return l;
         GF_HIP(hipMemcpyAsync(x, d_x, (size_t)n * 8, hipMemcpyDeviceToHost, s));
         GF_HIP(hipMemcpyAsync(&flag, d_flag, 4, hipMemcpyDeviceToHost, s));
         GF_HIP(hipStreamSynchronize(s));
-        st = GF_OK;
+        return GF_OK;
+    };
+    st = run();
+    void *bufs[] = {d_ab, d_x, d_z, d_flag};
+    for (void *p : bufs)
+        if (p) (void)hipFree(p);
+    if (s) (void)hipStreamDestroy(s);
+    if (st) return st;
+    if (flag == 2) {
+        gf_set_error("band solve: a block wait gave up");
+        return GF_E_DEVICE;
     }
-    (void)hipFree(d_ab); (void)hipFree(d_x); (void)hipFree(d_z); (void)hipFree(d_flag);
-    (void)hipStreamDestroy(s);
     if (flag) *info_fail = 1;
-    return st;
+    return GF_OK;
 }
 
 extern "C" int gf_shell_eval(int count, const double *x, const double *prop, const double *d, const double *gp,

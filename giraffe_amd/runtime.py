@@ -73,6 +73,10 @@ def load():
     L.gf_server_pump.argtypes = [vp, C.c_int]
     L.gf_server_counts.argtypes = [vp, C.POINTER(C.c_longlong)]
     L.gf_server_take.argtypes = [vp, C.c_longlong, C.POINTER(abi.GfStaticReport), dp, dp]
+    L.gf_deck_read.argtypes = [C.c_char_p, C.POINTER(vp)]
+    L.gf_deck_desc.argtypes = [vp, C.POINTER(abi.GfModelDesc), C.POINTER(abi.GfStaticParams)]
+    L.gf_deck_destroy.argtypes = [vp]
+    L.gf_deck_write.argtypes = [C.c_char_p, C.POINTER(abi.GfModelDesc), C.POINTER(abi.GfStaticParams)]
     _lib = L
     return L
 
@@ -313,3 +317,58 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+class Deck:
+    """A GIRAFFE input deck read by gf_deck_read (the static hot-path subset).
+    Behaves like a Model for DeviceModel / Server (desc(), n_nodes); `params`
+    holds its Static and ConvergenceCriteria parameters."""
+
+    def __init__(self, path):
+        from .model import StaticParams
+
+        L = load()
+        h = C.c_void_p()
+        check(L.gf_deck_read(os.fsencode(path), C.byref(h)), "gf_deck_read")
+        self.h = h
+        d = abi.GfModelDesc()
+        gp = abi.GfStaticParams()
+        check(L.gf_deck_desc(self.h, C.byref(d), C.byref(gp)), "gf_deck_desc")
+        self._desc = d
+        self.params = StaticParams(**{f: getattr(gp, f) for f, _ in abi.GfStaticParams._fields_})
+
+    @property
+    def n_nodes(self):
+        return self._desc.n_nodes
+
+    @property
+    def n_beams(self):
+        return self._desc.n_beams
+
+    @property
+    def n_shells(self):
+        return self._desc.n_shells
+
+    def desc(self):
+        d = abi.GfModelDesc()
+        C.pointer(d)[0] = self._desc
+        d._keep = self  # the arrays belong to the deck
+        return d
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().gf_deck_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def write_deck(path, model, params):
+    """Writes `model` (Model or Deck) and `params` as a GIRAFFE deck (gf_deck_write)."""
+    desc = model.desc()
+    pc = params.c()
+    check(load().gf_deck_write(os.fsencode(path), C.byref(desc), C.byref(pc)), "gf_deck_write")

@@ -227,6 +227,22 @@ int gf_bandsystem(int n, int hb, const double *ab_rows, const double *b, double 
 int gf_shell_eval(int count, const double *x, const double *prop, const double *d,
                   const double *gp, double *K_out, double *f_out);
 
+/* ---- input decks (src/IO.cpp:193-770 dispatch, hot-path subset) -------------
+ * gf_deck_read parses a GIRAFFE .inp deck restricted to what this path runs:
+ * Nodes, NodeSets, ElementSets, Elements (Beam_1, Shell_1), Materials (Hooke),
+ * Sections (Tube), ShellSections (Homogeneous), CoordinateSystems, Loads
+ * (NodalLoad, NodalFollowerLoad, ShellLoad with NTimes tables), Constraints
+ * (NodalConstraint), Environment (GravityData), SolutionSteps (one Static) and
+ * ConvergenceCriteria; Monitor, PostFiles, SolverOptions, ExecutionData and
+ * ConfigurationSave are skipped; anything else is refused with a message.
+ * gf_deck_desc returns the resolved description (arrays owned by the deck) and
+ * the Static parameters.  gf_deck_write emits a description as such a deck. */
+typedef struct gf_deck gf_deck;
+int gf_deck_read(const char *path, gf_deck **out);
+int gf_deck_desc(const gf_deck *deck, gf_model_desc *desc_out, gf_static_params *params_out);
+int gf_deck_destroy(gf_deck *deck);
+int gf_deck_write(const char *path, const gf_model_desc *desc, const gf_static_params *params);
+
 /* ---- request server ----------------------------------------------------------
  * No counterpart in the reference (it runs one Static::Solve per process); this is
  * the serving loop around the same Static drop-in: many independent solves

@@ -1,0 +1,165 @@
+"""GIRAFFE input decks (gf_deck_read / gf_deck_write, hot-path subset of the
+reference grammar, src/IO.cpp:290-657): write -> read round trips reproduce the
+description and the parameters, a deck solves exactly like the model it was
+written from (oracle), a hand-written deck in the reference's own layout
+(comments, Sequence sets, ignored output blocks) parses, and physics outside
+the path is refused with a message."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from giraffe_amd import abi, model as M, runtime
+
+
+def _arr(ptr, n, dtype):
+    if n == 0:
+        return np.zeros(0, dtype)
+    ct = {np.float64: C.c_double, np.int32: C.c_int, np.uint8: C.c_ubyte}[dtype]
+    return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ct)), shape=(n,)).copy()
+
+
+def _desc_arrays(d):
+    nn, nb, ns, nl = d.n_nodes, d.n_beams, d.n_shells, d.n_loads
+    off = _arr(d.load_set_off, nl + 1, np.int32)
+    toff = _arr(d.load_tab_off, nl + 1, np.int32)
+    rows = int(toff[-1]) if nl else 0
+    return {
+        "xyz": _arr(d.node_xyz, 3 * nn, np.float64), "fix": _arr(d.node_fix, 6 * nn, np.uint8),
+        "beam_conn": _arr(d.beam_conn, 3 * nb, np.int32), "beam_cs": _arr(d.beam_cs, 9 * nb, np.float64),
+        "beam_prop": _arr(d.beam_prop, abi.GF_BEAM_NPROP * nb, np.float64),
+        "shell_conn": _arr(d.shell_conn, 6 * ns, np.int32),
+        "shell_prop": _arr(d.shell_prop, abi.GF_SHELL_NPROP * ns, np.float64),
+        "load_kind": _arr(d.load_kind, nl, np.int32), "load_set_off": off,
+        "load_set_items": _arr(d.load_set_items, int(off[-1]) if nl else 0, np.int32),
+        "load_cs": _arr(d.load_cs, 9 * nl, np.float64), "load_tab_off": toff,
+        "tab_time": _arr(d.tab_time, rows, np.float64), "tab_val": _arr(d.tab_val, 6 * rows, np.float64),
+        "gravity": np.array(list(d.gravity)), "gravity_on": np.array([d.gravity_on]),
+    }
+
+
+def _models():
+    return [M.catenary(n_elem=30)[:2], M.shell_plate(nq=4)[:2], M.cantilever_rollup(n_elem=10)[:2]]
+
+
+@pytest.mark.parametrize("which", [0, 1, 2])
+def test_round_trip(tmp_path, which):
+    m, p = _models()[which]
+    path = str(tmp_path / "deck.inp")
+    runtime.write_deck(path, m, p)
+    deck = runtime.Deck(path)
+    a, b = _desc_arrays(m.desc()), _desc_arrays(deck.desc())
+    for k in a:
+        if a[k].dtype.kind == "f":
+            # Tube properties are recomputed from De / Di: a few ulps
+            assert np.allclose(a[k], b[k], rtol=1e-14, atol=0), k
+        else:
+            assert np.array_equal(a[k], b[k]), k
+    for f in p.__dataclass_fields__:
+        assert getattr(deck.params, f) == pytest.approx(getattr(p, f), rel=1e-15), f
+
+
+@pytest.mark.parametrize("which", [1, 2])
+def test_deck_solves_like_its_model(tmp_path, oracle_lib, which):
+    m, p = _models()[which]
+    path = str(tmp_path / "deck.inp")
+    runtime.write_deck(path, m, p)
+    deck = runtime.Deck(path)
+    ra, ia = oracle_lib.OracleModel(m).static_solve(p)
+    od = oracle_lib.OracleModel(deck)
+    rb, ib = od.static_solve(deck.params)
+    assert list(ia) == list(ib)
+    om = oracle_lib.OracleModel(m)
+    om.static_solve(p)
+    na, nb = om.nodes(), od.nodes()
+    assert np.abs(na - nb).max() <= 1e-12 * np.abs(na).max()
+
+
+TUTORIAL_STYLE = """\
+/////////////////////////////////
+// cantilever, reference deck layout
+/////////////////////////////////
+Nodes\t5
+//Number\tX\tY\tZ
+Node\t1\t0\t0\t0.0
+Node\t2\t0\t0\t0.25
+Node\t3\t0\t0\t0.5
+Node\t4\t0\t0\t0.75
+Node\t5\t0\t0\t1.0
+NodeSets\t2
+NodeSet\t1\tNodes\t1\tList\t1
+NodeSet\t2\tNodes\t2\tSequence\tInitial\t4\tIncrement\t1
+Elements\t2
+Beam_1\t1\tMat\t1\tSec\t1\tCS\t1\tNodes\t1\t2\t3
+Beam_1\t2\tMat\t1\tSec\t1\tCS\t1\tNodes\t3\t4\t5\tPreTension\t10.0
+Materials\t1
+Hooke\t1\tE\t2.1e11\tNu\t0.3\tRho\t7850
+Sections\t1
+Tube\t1\tDe\t0.1\tDi\t0.08
+CoordinateSystems\t1
+CS\t1\tE1\t1\t0\t0\tE3\t0\t0\t1
+SolutionSteps\t1
+Static\t1
+EndTime\t1
+TimeStep\t0.1
+MaxTimeStep\t0.1
+MinTimeStep\t0.01
+MaxIt\t20
+MinIt\t3
+ConvIncrease\t4
+IncFactor\t1.0
+Sample\t2
+Loads\t1
+NodalLoad\t1\tNodeSet\t2\tCS\t1\tNTimes\t2
+//Time FX FY FZ MX MY MZ
+0\t0\t0\t0\t0\t0\t0
+1\t1000\t0\t0\t0\t0\t0
+Constraints\t1
+NodalConstraint\t1\tNodeSet\t1
+\tUX\t\tBoolTable\t1
+\tUY\t\tBoolTable\t1
+\tUZ\t\tBoolTable\t1
+\tROTX\tBoolTable\t1
+\tROTY\tBoolTable\t1
+\tROTZ\tBoolTable\t0 1
+Environment
+GravityData\tG\t0\t0\t-9.81\tBoolTable\t1
+SolverOptions
+Processors\t1\tLinSys\tDirect
+Monitor\tSample\t1
+MonitorNodes\t1\t5
+PostFiles
+MagFactor\t1
+WriteMesh\t1
+"""
+
+
+def test_reference_layout_deck(tmp_path):
+    path = tmp_path / "t.inp"
+    path.write_text(TUTORIAL_STYLE)
+    deck = runtime.Deck(str(path))
+    a = _desc_arrays(deck.desc())
+    assert deck.n_nodes == 5 and deck.n_beams == 2
+    assert list(a["fix"][:6]) == [1, 1, 1, 1, 1, 0]   # ROTZ's table is 0 for the first step
+    assert a["fix"][6:].sum() == 0
+    assert list(a["load_set_items"]) == [3, 4]         # Sequence Initial 4 Increment 1, 0-based
+    assert a["beam_prop"][abi.GF_BEAM_NPROP + 8] == 10.0
+    A = np.pi / 4 * (0.1 ** 2 - 0.08 ** 2)
+    assert a["beam_prop"][3] == pytest.approx(A, rel=1e-15)
+    assert list(a["gravity"]) == [0, 0, -9.81] and a["gravity_on"][0] == 1
+    assert deck.params.max_it == 20 and deck.params.min_it == 3 and deck.params.time_step == 0.1
+    assert deck.params.force_tol == 1e-4  # ConvergenceCriteria defaults
+
+
+@pytest.mark.parametrize("edit,msg", [
+    (("Tube\t1\tDe\t0.1\tDi\t0.08", "Rectangle\t1\tB\t0.1\tH\t0.1"), "Rectangle"),
+    (("SolverOptions", "Contacts\t0\nSolverOptions"), "Contacts"),
+    (("SolutionSteps\t1", "SolutionSteps\t2"), "one solution step"),
+    (("NodalLoad\t1\tNodeSet\t2\tCS\t1\tNTimes\t2", "NodalLoad\t1\tNodeSet\t2\tCS\t1\tMathCode"), "MathCode"),
+])
+def test_refusals(tmp_path, edit, msg):
+    path = tmp_path / "t.inp"
+    path.write_text(TUTORIAL_STYLE.replace(*edit))
+    with pytest.raises(runtime.GiraffeError, match=msg):
+        runtime.Deck(str(path))
