@@ -7,8 +7,9 @@ elements (20,001 nodes), self-weight plus seeded N(0, 500 N) lateral nodal
 loads over 10 load steps, Newton tolerance 1e-10.
 
 Prints one JSON line (see the repository's measurement contract): value =
-solves/s, p50/p99 latency, the roofline of the dominant device phase measured
-live with HIP events on the model's stream, and the host-CPU baseline (the
+solves/s, p50/p99 latency, the roofline of the dominant kernel measured live
+with HIP event pairs around each of its launches on the model's stream, and
+the host-CPU baseline (the
 oracle's FP64 restatement with MKL PARDISO, timed on this host's cores).
 """
 import argparse
@@ -104,6 +105,8 @@ def measure(m, p, precision, steps, warmup, timing):
     if timing:
         dm.timing(True)
         dm.timing_read(reset=True)
+        dm.kernel_probe(PROBES_BTD if info["block"] else PROBES_BAND)
+        dm.kernel_probe_read(reset=True)
     st0 = dm.stats()
     lat, reps = [], []
     torch.cuda.synchronize()
@@ -118,52 +121,90 @@ def measure(m, p, precision, steps, warmup, timing):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ph = dm.timing_read(reset=True) if timing else {}
+    kp = dm.kernel_probe_read(reset=True) if timing else {}
+    if timing:
+        dm.kernel_probe([])
     st1 = dm.stats()
     lat_ms = np.array(lat) * 1e3
-    return {"info": info, "wall": wall, "lat_ms": lat_ms, "reps": reps, "phases": ph,
+    return {"info": info, "wall": wall, "lat_ms": lat_ms, "reps": reps, "phases": ph, "kernels": kp,
             "refine_sweeps": st1["refine_sweeps"] - st0["refine_sweeps"],
             "fallbacks": st1["fallbacks"] - st0["fallbacks"], "solves": st1["solves"] - st0["solves"]}
 
 
-def roofline(m, r, steps):
-    """Roofline of the dominant device phase: algorithmic work of one unit (an
-    assembly launch, one factorisation, one solve) over its HIP-event time."""
-    ph, info = r["phases"], r["info"]
-    n_free, b = info["n_free"], info["half_band"]
-    solves = r["reps"][-1][1]
-    dom = max(ph, key=lambda k: ph[k][0])
-    ms, launches = ph[dom]
-    if dom == "assemble":
-        if m.n_beams and not m.n_shells:
-            per_launch_elems = m.n_beams / max(1, info["colors"])
-            unit_bytes = FM.BEAM_BYTES_FP64 * per_launch_elems
-            avg_s = ms / launches * 1e-3
-            ach = unit_bytes / avg_s / 1e9
-            return {"kernel": "beam_assemble_kernel<0>", "bound": "hbm", "achieved": ach, "peak": FM.HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": ach / FM.HBM_PEAK_GBS, "traffic": None, "avg_launch_us": avg_s * 1e6,
-                    "algorithmic_bytes_per_launch": unit_bytes}
-        return {"kernel": "assembly", "bound": None, "note": "shell assembly roofline not modelled"}
-    per = ms / max(1, solves * steps) * 1e-3  # one factorisation (or solve) per tangent solve
-    if info["block"]:  # block-tridiagonal: factorisation with the forward sweep and back-substitution fused
-        work = FM.band_lu(n_free, b)
-        ach = work["bytes"] / per / 1e9
-        return {"kernel": "block-tridiagonal cyclic reduction (factor + solve, all levels)", "bound": "hbm",
-                "achieved": ach, "peak": FM.HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / FM.HBM_PEAK_GBS,
-                "traffic": None, "time_per_unit_us": per * 1e6, "launches_per_unit": launches / max(1, solves * steps),
-                "algorithmic_bytes_per_unit": work["bytes"], "unit_of_work": "one factorisation + solve"}
-    if dom == "factor":
-        work = FM.band_lu(n_free, b)
-        ach = work["flops"] / per / 1e12
-        peak = FM.FP64_PEAK_TFLOPS if info["precision"] == 0 else FM.FP32_PEAK_TFLOPS
-        return {"kernel": "band LU (panel + MFMA trailing update)", "bound": "mfma", "achieved": ach, "peak": peak,
-                "unit": "TFLOP/s", "frac": ach / peak, "traffic": None, "time_per_unit_us": per * 1e6,
-                "launches_per_unit": launches / max(1, solves * steps), "algorithmic_flops_per_unit": work["flops"],
-                "unit_of_work": "one factorisation"}
-    work = FM.band_trsv(n_free, b)
-    ach = work["bytes"] / per / 1e9
-    return {"kernel": "band triangular solves", "bound": "hbm", "achieved": ach, "peak": FM.HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": ach / FM.HBM_PEAK_GBS, "traffic": None, "time_per_unit_us": per * 1e6,
-            "algorithmic_bytes_per_unit": work["bytes"], "unit_of_work": "one solve"}
+# Kernel classes bracketed by HIP events in the timed region (per layout).
+PROBES_BTD = ("btd_level", "btd_tail", "beam_assemble", "shell_assemble")
+PROBES_BAND = ("band_panel", "band_update", "band_sweep", "beam_assemble", "shell_assemble")
+
+KERNEL_NAMES = {"btd_level": "btd_level_kernel", "btd_tail": "btd_tail_kernel", "band_panel": "band_panel_kernel",
+                "band_update": "band_update_kernel", "band_sweep": "band_sweep_kernel",
+                "beam_assemble": "beam_assemble_kernel<0>", "shell_assemble": "shell_assemble_kernel<0>"}
+
+
+def kernel_work(cls, info):
+    """Algorithmic work of one unit of a kernel class: (bound, amount, unit, what)."""
+    n, b, mblk = info["n_free"], info["half_band"], max(1, info["block"])
+    s = 8 if info["precision"] == abi.GF_PREC_FP64 else 4
+    lu = FM.band_lu(n, b, s)
+    if cls in ("btd_level", "btd_tail"):
+        # the band factorisation's bytes apportioned per block row of m rows
+        return "hbm", lu["bytes"] / n * mblk, "B", "block row eliminated: 2 m (2b+1) s bytes of the band LU"
+    if cls == "band_update":
+        return "mfma", lu["flops"] / n, "flop", "band column eliminated: 2 b^2 flops of the band LU"
+    if cls == "band_panel":
+        return "hbm", 4.0 * b * s, "B", "band column eliminated: read + write one column of L21 and one row of U12"
+    if cls == "band_sweep":
+        return "hbm", FM.band_trsv(n, b, s)["bytes"] / n / 2, "B", "row swept: half of the triangular-solve bytes"
+    if cls == "beam_assemble":
+        return "hbm", FM.BEAM_BYTES_FP64, "B", "element (FP64 tangent)"
+    return "hbm", FM.SHELL_BYTES_FP64, "B", "element (FP64 tangent)"
+
+
+def kernel_roofline(cls, rec, info):
+    ms, launches, units = rec
+    bound, per_unit, u, what = kernel_work(cls, info)
+    avg_s = ms / launches * 1e-3
+    per_launch = per_unit * units / launches
+    if bound == "hbm":
+        ach, peak, unit = per_launch / avg_s / 1e9, FM.HBM_PEAK_GBS, "GB/s"
+    else:
+        ach = per_launch / avg_s / 1e12
+        peak = FM.FP64_PEAK_TFLOPS if info["precision"] == abi.GF_PREC_FP64 else FM.FP32_PEAK_TFLOPS
+        unit = "TFLOP/s"
+    return {"kernel": KERNEL_NAMES[cls], "bound": bound, "achieved": ach, "peak": peak, "unit": unit,
+            "frac": ach / peak, "traffic": None, "avg_launch_us": avg_s * 1e6, "launches": launches,
+            "units_per_launch": units / launches, "algorithmic_per_unit": per_unit,
+            "algorithmic_per_launch": per_launch, "algorithmic_unit": u, "unit_of_work": what,
+            "kernel_ms_total": ms}
+
+
+def traffic_from_profile(kernel, workload):
+    """HBM bytes per launch of `kernel` recorded under profiles/ from rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (line `traffic_bytes_per_launch: <kernel> <bytes>`)."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.strip().strip("`").split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch:" and parts[1] == kernel:
+                return float(parts[2])
+    return None
+
+
+def roofline(m, r, workload_name):
+    """Roofline of the dominant kernel: the algorithmic work of one launch
+    (per-unit figure x the units it processed) over its average duration, both
+    from the HIP event pairs around every launch of the class in the timed region."""
+    kp, info = r["kernels"], r["info"]
+    if not kp:
+        return None
+    dom = max(kp, key=lambda c: kp[c][0])
+    out = kernel_roofline(dom, kp[dom], info)
+    out["traffic"] = traffic_from_profile(out["kernel"], workload_name)
+    out["other_kernels"] = {KERNEL_NAMES[c]: {k: v for k, v in kernel_roofline(c, rec, info).items()
+                                              if k in ("achieved", "unit", "frac", "avg_launch_us", "launches")}
+                            for c, rec in kp.items() if c != dom}
+    return out
 
 
 def serve(args):
@@ -295,6 +336,9 @@ def main():
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "pardiso", "band", "none"])
     ap.add_argument("--cpu-budget", type=float, default=30.0, help="seconds of CPU baseline sample")
     ap.add_argument("--no-timing", action="store_true", help="do not record per-phase HIP events")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="before the workload, copy a buffer of this many MiB once on the device (a plain copy "
+                         "kernel of known size, the reference point for scaling HBM counter readings)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -307,6 +351,14 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
         return 2
+    if args.copy_mib > 0:
+        src = torch.ones(args.copy_mib << 17, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copied {args.copy_mib} MiB device to device")
+        del src, dst
     if args.workload == "serve":
         return serve(args)
     m, p, desc = workload(args.workload)
@@ -341,7 +393,7 @@ def main():
         ph = r["phases"]
         out["phases_ms_per_solve"] = {k: v[0] / args.steps for k, v in ph.items()}
         out["phase_launches_per_solve"] = {k: v[1] / args.steps for k, v in ph.items()}
-        out["roofline"] = roofline(m, r, args.steps)
+        out["roofline"] = roofline(m, r, args.workload)
     if args.compare_steps > 0:
         other = "mixed" if args.precision == "fp64" else "fp64"
         c = measure(m, p, other, args.compare_steps, 1, False)

@@ -8,28 +8,32 @@
 // every load masks them to zero and every store skips them.
 //
 // Right-looking, no row interchanges (PARDISO's mtype 11 default is static
-// pivoting), panel width NB = 64 over the padded order n (multiple of NB):
-//   diag   LU of the NB x NB diagonal block            (one workgroup)
-//   trsm   L21 = A21 U11^-1 and U12 = L11^-1 A12         (64-row / 64-column tiles)
-//   update A22 -= L21 U12 on the band window, MFMA 16x16x4 tiles of 64x64;
-//          the workgroup owning the next diagonal block factors it in place.
+// pivoting), panel width NB = 64 over the padded order n (multiple of NB).
+// Per panel p (columns k0 = 64 p ..):
+//   panel  L21 = A21 U11^-1 and U12 = L11^-1 A12 as MFMA products with the
+//          panel's stored inverses (no sequential substitution);
+//   update A22 -= L21 U12 on the band window, MFMA 16x16x4 tiles of 64 x 64;
+//          the workgroup owning the next diagonal block factors it in LDS
+//          (register-blocked, one barrier per pivot) and writes its triangular
+//          inverses, which the next panel and both triangular solves use.
 // Zero or non-finite pivots raise the failure flag (info_fail).
-// Triangular solves: one launch per panel (block of NB unknowns solved by one
-// workgroup, the coupled rows below / above updated by the rest).
+//
+// Triangular solves: one single-workgroup launch per sweep over 64-row blocks,
+// each block finished by its stored inverse (no other workgroup takes part).
 #include "gf_internal.h"
 
 namespace {
 
 constexpr int NB = 64;
-constexpr int KT = 32;  // k-chunk of the update tiles
+constexpr int KT = 32;  // k-chunk of the MFMA tiles
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-template <typename T> struct mfma;
 // Operands of both: A(i, k) in lane i + 16 k, B(k, j) in lane j + 16 k.  The
 // 16 x 16 result differs: item v of lane l holds row 4 v + l / 16 for f64 and
 // row 4 (l / 16) + v for f32, column l % 16 in both (checked on gfx950).
+template <typename T> struct mfma;
 template <> struct mfma<double> {
     typedef v4d acc_t;
     static __device__ __forceinline__ acc_t run(double a, double b, acc_t c)
@@ -50,136 +54,176 @@ template <> struct mfma<float> {
 __device__ __forceinline__ size_t bidx(int i, int c, int hb) { return (size_t)i * (size_t)(2 * hb) + (size_t)(c + hb); }
 __device__ __forceinline__ bool inband(int i, int c, int hb) { return i - c <= hb && c - i <= hb; }
 
+// Masked band load without a branch around it: the address is clamped into the
+// stored band of row min(i, n - 1) (always inside the n (2hb+1) allocation) and
+// the value is selected afterwards, so a thread's loads are all in flight
+// together instead of each waiting in its own branch.  i, c >= 0.
+template <typename T> __device__ __forceinline__ T band_ld(const T *ab, int i, int c, int hb, int n, bool ok)
+{
+    const int ii = min(i, n - 1);
+    const int cc = min(max(c, ii - hb), ii + hb);
+    const T v = ab[bidx(ii, cc, hb)];
+    return ok && i < n && inband(i, c, hb) ? v : T(0);
+}
+
 template <typename T> __device__ __forceinline__ bool bad_pivot(T v)
 {
     return !(v == v) || v == T(0) || fabs((double)v) >= 1e300;
 }
 
-// In-LDS LU (no pivoting) of a 64 x 64 tile with leading dimension ld.
-template <typename T> __device__ void lds_lu64(T *s, int ld, int *flag)
+// Per-panel inverses: inv + p * 2 * NB * NB holds Uinv (row-major), then Linv (unit lower).
+template <typename T> __device__ __forceinline__ T *uinv_of(T *inv, int p) { return inv + (size_t)p * 2 * NB * NB; }
+template <typename T> __device__ __forceinline__ T *linv_of(T *inv, int p) { return inv + (size_t)p * 2 * NB * NB + NB * NB; }
+
+// LU (no pivoting) of the 64 x 64 tile s (leading dimension ld) by 256 threads,
+// each holding a 4 x 4 sub-block in registers; one barrier per pivot; then the
+// triangular inverses on the same blocks.  On return s holds L\U; Uinv / Linv
+// are written row-major to uinv / linv.
+template <typename T>
+__device__ void tile_lu_inv(T *s, int ld, T *rowbuf /* [2][64] */, T *colbuf /* [2][64] */, T *uinv, T *linv,
+                            int *flag)
 {
     const int tid = threadIdx.x;
+    const int r0 = 4 * (tid >> 4), c0 = 4 * (tid & 15);
+    T a[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) a[i][j] = s[(r0 + i) * ld + c0 + j];
+    // publish row 0 and column 0
+    if (r0 == 0)
+#pragma unroll
+        for (int j = 0; j < 4; j++) rowbuf[c0 + j] = a[0][j];
+    if (c0 == 0)
+#pragma unroll
+        for (int i = 0; i < 4; i++) colbuf[r0 + i] = a[i][0];
+    __syncthreads();
     for (int k = 0; k < NB; k++) {
-        T piv = s[k * ld + k];
+        const T *rb = rowbuf + (k & 1) * NB, *cb = colbuf + (k & 1) * NB;
+        T piv = rb[k];
         if (bad_pivot(piv)) {
             if (tid == 0) atomicExch(flag, 1);
             piv = T(1);
         }
-        if (tid > k && tid < NB) s[tid * ld + k] /= piv;
-        __syncthreads();
-        int w = NB - 1 - k;
-        for (int t = tid; t < w * w; t += 256) {
-            int i = k + 1 + t / w, j = k + 1 + t % w;
-            s[i * ld + j] -= s[i * ld + k] * s[k * ld + j];
+        const T rp = T(1) / piv;
+        T l[4], u[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) l[i] = cb[r0 + i] * rp;
+#pragma unroll
+        for (int j = 0; j < 4; j++) u[j] = rb[c0 + j];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int gi = r0 + i, gj = c0 + j;
+                if (gi > k && gj > k) a[i][j] -= l[i] * u[j];
+                else if (gi > k && gj == k) a[i][j] = l[i];
+            }
+        // publish row k+1 (columns >= k+1) and column k+1 (rows > k+1) for the next pivot
+        const int k1 = k + 1;
+        if (k1 < NB) {
+            T *rbn = rowbuf + (k1 & 1) * NB, *cbn = colbuf + (k1 & 1) * NB;
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                if (r0 + i == k1)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) rbn[c0 + j] = a[i][j];
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (c0 + j == k1)
+#pragma unroll
+                    for (int i = 0; i < 4; i++) cbn[r0 + i] = a[i][j];
         }
         __syncthreads();
     }
-}
-
-template <typename T> __global__ __launch_bounds__(256) void band_diag_kernel(T *ab, int hb, int k0, int *flag)
-{
-    __shared__ T s[NB * NB];
-    for (int t = threadIdx.x; t < NB * NB; t += 256) {
-        int i = k0 + t / NB, c = k0 + t % NB;
-        s[t] = inband(i, c, hb) ? ab[bidx(i, c, hb)] : T(0);
-    }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) s[(r0 + i) * ld + c0 + j] = a[i][j];
     __syncthreads();
-    lds_lu64(s, NB, flag);
-    for (int t = threadIdx.x; t < NB * NB; t += 256) {
-        int i = k0 + t / NB, c = k0 + t % NB;
-        if (inband(i, c, hb)) ab[bidx(i, c, hb)] = s[t];
-    }
-}
-
-// blockIdx.x < nrt: a 64-row tile of L21; otherwise a 64-column tile of U12.
-template <typename T> __global__ __launch_bounds__(256) void band_trsm_kernel(T *ab, int hb, int k0, int n, int nrt)
-{
-    __shared__ T sd[NB * NB];  // diagonal block (L11 \ U11)
-    __shared__ T sx[NB * NB];  // tile
-    const int tid = threadIdx.x;
-    const int lim = min(n, k0 + NB + hb);
-    for (int t = tid; t < NB * NB; t += 256) {
-        int i = k0 + t / NB, c = k0 + t % NB;
-        sd[t] = inband(i, c, hb) ? ab[bidx(i, c, hb)] : T(0);
-    }
-    if ((int)blockIdx.x < nrt) {
-        const int r0 = k0 + NB + blockIdx.x * NB;
-        for (int t = tid; t < NB * NB; t += 256) {
-            int i = r0 + t / NB, c = k0 + t % NB;
-            sx[t] = (i < lim && inband(i, c, hb)) ? ab[bidx(i, c, hb)] : T(0);
+    // Inverses by column-oriented substitution on the same 4 x 4 register blocks,
+    // one barrier per step: Uinv pivots descend (x_k = b_k / U_kk, b_i -= U_ik x_k,
+    // i < k), Linv pivots ascend (x_k = b_k, b_i -= L_ik x_k, i > k).  The owners
+    // of the pivot row publish it; rowbuf / colbuf are reused as the two buffers.
+    T bu[4][4], bl[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) bu[i][j] = bl[i][j] = (r0 + i == c0 + j) ? T(1) : T(0);
+    for (int step = 0; step < NB; step++) {
+        const int ku = NB - 1 - step, kl = step;
+        T *xu = rowbuf + (step & 1) * NB, *xl = colbuf + (step & 1) * NB;
+        if (ku >= r0 && ku < r0 + 4) {
+            T d = s[ku * ld + ku];
+            if (bad_pivot(d)) d = T(1);
+            const T rd = T(1) / d;
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                if (r0 + i == ku)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        bu[i][j] *= rd;
+                        xu[c0 + j] = bu[i][j];
+                    }
+        }
+        if (kl >= r0 && kl < r0 + 4) {
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                if (r0 + i == kl)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) xl[c0 + j] = bl[i][j];
         }
         __syncthreads();
-        // X U11 = A21, column by column
-        for (int c = 0; c < NB; c++) {
-            if (tid < NB) sx[tid * NB + c] /= sd[c * NB + c];
-            __syncthreads();
-            int w = NB - 1 - c;
-            for (int t = tid; t < NB * w; t += 256) {
-                int i = t / w, j = c + 1 + t % w;
-                sx[i * NB + j] -= sx[i * NB + c] * sd[c * NB + j];
+        T uk[4], lk[4], xuv[4], xlv[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            uk[i] = s[(r0 + i) * ld + ku];
+            lk[i] = s[(r0 + i) * ld + kl];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            xuv[j] = xu[c0 + j];
+            xlv[j] = xl[c0 + j];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if (r0 + i < ku) bu[i][j] -= uk[i] * xuv[j];
+                if (r0 + i > kl) bl[i][j] -= lk[i] * xlv[j];
             }
-            __syncthreads();
-        }
-        for (int t = tid; t < NB * NB; t += 256) {
-            int i = r0 + t / NB, c = k0 + t % NB;
-            if (i < lim && inband(i, c, hb)) ab[bidx(i, c, hb)] = sx[t];
-        }
-    } else {
-        const int c0 = k0 + NB + (blockIdx.x - nrt) * NB;
-        for (int t = tid; t < NB * NB; t += 256) {
-            int i = k0 + t / NB, c = c0 + t % NB;
-            sx[t] = (c < lim && inband(i, c, hb)) ? ab[bidx(i, c, hb)] : T(0);
-        }
-        __syncthreads();
-        // L11 Y = A12 (unit lower), row by row
-        for (int r = 0; r < NB - 1; r++) {
-            int h = NB - 1 - r;
-            for (int t = tid; t < h * NB; t += 256) {
-                int i = r + 1 + t / NB, j = t % NB;
-                sx[i * NB + j] -= sd[i * NB + r] * sx[r * NB + j];
-            }
-            __syncthreads();
-        }
-        for (int t = tid; t < NB * NB; t += 256) {
-            int i = k0 + t / NB, c = c0 + t % NB;
-            if (c < lim && inband(i, c, hb)) ab[bidx(i, c, hb)] = sx[t];
-        }
     }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            uinv[(r0 + i) * NB + c0 + j] = bu[i][j];
+            linv[(r0 + i) * NB + c0 + j] = bl[i][j];
+        }
 }
 
-// A22 -= L21 U12 on 64 x 64 tiles of the window [k0+NB, lim)^2, band entries only.
-template <typename T> __global__ __launch_bounds__(256) void band_update_kernel(T *ab, int hb, int k0, int n, int nt,
-                                                                                 int *flag)
+// 64 x 64 x 64 product on 256 threads (4 waves, 2 x 2 of 32 x 32):
+// acc = P Q with P and Q staged chunk by chunk through sa [64][KT+1], sb [KT][64].
+template <typename T, typename LP, typename LQ>
+__device__ __forceinline__ void tile_mm(typename mfma<T>::acc_t (&acc)[2][2], T *sa, T *sb, LP loadP, LQ loadQ)
 {
-    typedef typename mfma<T>::acc_t acc_t;
-    // staging for the A ([row][k], padded) and B ([k][col]) chunks; after the
-    // k loop the same storage holds the tile of the fused diagonal LU
-    __shared__ T smem[NB * (KT + 1) + KT * NB];
-    static_assert(NB * (KT + 1) + KT * NB >= NB * (NB + 1), "fused tile must fit the staging buffers");
-    T *sa = smem, *sb = smem + NB * (KT + 1), *sc = smem;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int ti = blockIdx.x / nt, tj = blockIdx.x % nt;
-    const int base = k0 + NB;
-    const int lim = min(n, k0 + NB + hb);
-    const int R0 = base + ti * NB, C0 = base + tj * NB;
-    // skip tiles that do not meet the band
-    if (R0 - (C0 + NB - 1) > hb || C0 - (R0 + NB - 1) > hb) return;
     const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-    acc_t acc[2][2];
-#pragma unroll
-    for (int a = 0; a < 2; a++)
-#pragma unroll
-        for (int b = 0; b < 2; b++) acc[a][b] = acc_t{};
     for (int kc = 0; kc < NB; kc += KT) {
-        for (int t = tid; t < NB * KT; t += 256) {
-            int i = t / KT, kk = t % KT;
-            int gi = R0 + i, gk = k0 + kc + kk;
-            sa[i * (KT + 1) + kk] = (gi < lim && inband(gi, gk, hb)) ? ab[bidx(gi, gk, hb)] : T(0);
+        // all loads of the chunk first (in flight together), then the LDS stores
+        constexpr int PER = NB * KT / 256;
+        T pv[PER], qv[PER];
+#pragma unroll
+        for (int r = 0; r < PER; r++) {
+            const int t = tid + 256 * r;
+            pv[r] = loadP(t / KT, kc + t % KT);
+            qv[r] = loadQ(kc + t / NB, t % NB);
         }
-        for (int t = tid; t < KT * NB; t += 256) {
-            int kk = t / NB, j = t % NB;
-            int gk = k0 + kc + kk, gj = C0 + j;
-            sb[kk * NB + j] = (gj < lim && inband(gk, gj, hb)) ? ab[bidx(gk, gj, hb)] : T(0);
+#pragma unroll
+        for (int r = 0; r < PER; r++) {
+            const int t = tid + 256 * r;
+            sa[(t / KT) * (KT + 1) + t % KT] = pv[r];
+            sb[(t / NB) * NB + t % NB] = qv[r];
         }
         __syncthreads();
 #pragma unroll
@@ -196,6 +240,112 @@ template <typename T> __global__ __launch_bounds__(256) void band_update_kernel(
         }
         __syncthreads();
     }
+}
+
+template <typename T> struct TileSmem {
+    T ab[NB * (KT + 1) + KT * NB];  // staging of tile_mm; after it, the LU tile [64][65]
+    T rowbuf[2 * NB], colbuf[2 * NB];
+};
+
+// First diagonal block: LU in place and its inverses (panel 0).
+template <typename T>
+__global__ __launch_bounds__(256) void band_diag_kernel(T *ab, int hb, int n, T *inv, int *flag)
+{
+    __shared__ TileSmem<T> sm;
+    T *s = sm.ab;
+    T v[NB * NB / 256];
+#pragma unroll
+    for (int r = 0; r < NB * NB / 256; r++) {
+        const int t = threadIdx.x + 256 * r;
+        v[r] = band_ld(ab, t / NB, t % NB, hb, n, true);
+    }
+#pragma unroll
+    for (int r = 0; r < NB * NB / 256; r++) {
+        const int t = threadIdx.x + 256 * r;
+        s[(t / NB) * (NB + 1) + t % NB] = v[r];
+    }
+    __syncthreads();
+    tile_lu_inv<T>(s, NB + 1, sm.rowbuf, sm.colbuf, uinv_of(inv, 0), linv_of(inv, 0), flag);
+    __syncthreads();
+    for (int t = threadIdx.x; t < NB * NB; t += 256) {
+        int i = t / NB, c = t % NB;
+        if (inband(i, c, hb)) ab[bidx(i, c, hb)] = s[i * (NB + 1) + c];
+    }
+}
+
+// Panel p: blockIdx < nt: L21 tile = A21 Uinv; otherwise U12 tile = Linv A12.
+template <typename T> __global__ __launch_bounds__(256) void band_panel_kernel(T *ab, int hb, int p, int n, int nt,
+                                                                               const T *inv)
+{
+    typedef typename mfma<T>::acc_t acc_t;
+    __shared__ TileSmem<T> sm;
+    T *sa = sm.ab, *sb = sm.ab + NB * (KT + 1);
+    const int k0 = p * NB, lim = min(n, k0 + NB + hb);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    acc_t acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = acc_t{};
+    const T *U = uinv_of(const_cast<T *>(inv), p), *L = linv_of(const_cast<T *>(inv), p);
+    const bool lower = (int)blockIdx.x < nt;
+    const int o0 = k0 + NB + (lower ? blockIdx.x : blockIdx.x - nt) * NB;  // first row (lower) or column (upper)
+    if (lower) {
+        tile_mm<T>(acc, sa, sb,
+                   [&](int i, int k) { int gi = o0 + i; return band_ld(ab, gi, k0 + k, hb, n, gi < lim); },
+                   [&](int k, int j) { return U[k * NB + j]; });
+    } else {
+        tile_mm<T>(acc, sa, sb, [&](int i, int k) { return L[i * NB + k]; },
+                   [&](int k, int j) { int gj = o0 + j; return band_ld(ab, k0 + k, gj, hb, n, gj < lim); });
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                int i = wr + 16 * a + mfma<T>::row(lane, v), j = wc + 16 * b + (lane & 15);
+                int gi = lower ? o0 + i : k0 + i, gj = lower ? k0 + j : o0 + j;
+                bool ok = lower ? gi < lim : gj < lim;
+                if (ok && inband(gi, gj, hb)) ab[bidx(gi, gj, hb)] = acc[a][b][v];
+            }
+}
+
+// A22 -= L21 U12 on 64 x 64 tiles of the window [k0+NB, lim)^2; the tile that
+// holds the next diagonal block then factors it and writes panel p+1's inverses.
+template <typename T> __global__ __launch_bounds__(256) void band_update_kernel(T *ab, int hb, int p, int n, int nt,
+                                                                                T *inv, int *flag)
+{
+    typedef typename mfma<T>::acc_t acc_t;
+    __shared__ TileSmem<T> sm;
+    T *sa = sm.ab, *sb = sm.ab + NB * (KT + 1), *sc = sm.ab;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int k0 = p * NB, base = k0 + NB, lim = min(n, k0 + NB + hb);
+    const int ti = blockIdx.x / nt, tj = blockIdx.x % nt;
+    const int R0 = base + ti * NB, C0 = base + tj * NB;
+    if (R0 - (C0 + NB - 1) > hb || C0 - (R0 + NB - 1) > hb) return;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    acc_t acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = acc_t{};
+    // the old values of this thread's result entries, read before the product
+    // so their latency hides behind it
+    T old[2][2][4];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                int i = wr + 16 * a + mfma<T>::row(lane, v), j = wc + 16 * b + (lane & 15);
+                old[a][b][v] = band_ld(ab, R0 + i, C0 + j, hb, n, C0 + j < n);
+            }
+    tile_mm<T>(acc, sa, sb,
+               [&](int i, int k) { int gi = R0 + i; return band_ld(ab, gi, k0 + k, hb, n, gi < lim); },
+               [&](int k, int j) { int gj = C0 + j; return band_ld(ab, k0 + k, gj, hb, n, gj < lim); });
     const bool fuse = (ti == 0 && tj == 0 && base + NB <= n);
 #pragma unroll
     for (int a = 0; a < 2; a++)
@@ -206,17 +356,15 @@ template <typename T> __global__ __launch_bounds__(256) void band_update_kernel(
                 int i = wr + 16 * a + mfma<T>::row(lane, v), j = wc + 16 * b + (lane & 15);
                 int gi = R0 + i, gj = C0 + j;
                 // rows/columns past the window carry a zero update (masked operands)
-                if (gi < n && gj < n && inband(gi, gj, hb)) {
-                    T nv = ab[bidx(gi, gj, hb)] - acc[a][b][v];
-                    if (fuse) sc[i * (NB + 1) + j] = nv;
-                    else ab[bidx(gi, gj, hb)] = nv;
-                } else if (fuse) {
-                    sc[i * (NB + 1) + j] = T(0);
-                }
+                const bool in = gi < n && gj < n && inband(gi, gj, hb);
+                const T nv = old[a][b][v] - acc[a][b][v];
+                if (fuse) sc[i * (NB + 1) + j] = in ? nv : T(0);
+                else if (in) ab[bidx(gi, gj, hb)] = nv;
             }
     if (!fuse) return;
     __syncthreads();
-    lds_lu64(sc, NB + 1, flag);
+    tile_lu_inv<T>(sc, NB + 1, sm.rowbuf, sm.colbuf, uinv_of(inv, p + 1), linv_of(inv, p + 1), flag);
+    __syncthreads();
     for (int t = tid; t < NB * NB; t += 256) {
         int i = t / NB, j = t % NB;
         int gi = R0 + i, gj = C0 + j;
@@ -225,142 +373,115 @@ template <typename T> __global__ __launch_bounds__(256) void band_update_kernel(
 }
 
 // ---- triangular solves ------------------------------------------------------------
-// Forward, panel k0: every workgroup stages L11 in LDS and wave 0 solves
-// L11 y_p = b_p (unit lower) with register shuffles; workgroup 0 stores y_p in
-// z; then each wave takes rows below (one row per wave, coalesced along the
-// row) and subtracts L21(row, :) y_p from b.  b_p is only read and the rows
-// below are only written, so the workgroups of one launch never race.
-// Launches follow panel order; z holds the forward solution at the end.
-template <typename T, typename V>
-__global__ __launch_bounds__(256) void band_fwd_kernel(const T *ab, int hb, int k0, int n, V *b, V *z,
-                                                      int rows_per_wg)
+// One workgroup of 1024 lanes per sweep, blocks of 64 rows in order (forward:
+// L y = b with unit L; backward: U x = y).  Block r subtracts the couplings of
+// the blocks it reaches inside the band (wave w owns rows 4w..4w+3, lanes run
+// along the coupled block's 64 columns, so every load is a contiguous 512-byte
+// row segment) and then applies its stored 64 x 64 inverse.  Blocks depend on
+// the previous ones only through this workgroup's own writes, so barriers
+// are the only synchronisation.
+template <typename T, typename V, bool FWD>
+__global__ __launch_bounds__(1024) void band_sweep_kernel(const T *ab, int hb, int n, const T *inv, const V *x, V *y)
 {
-    __shared__ T sl[NB * (NB + 1)];
-    __shared__ V sy[NB];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int t = tid; t < NB * NB; t += 256) {
-        int i = t / NB, c = t % NB;
-        sl[i * (NB + 1) + c] = (c < i && inband(k0 + i, k0 + c, hb)) ? ab[bidx(k0 + i, k0 + c, hb)] : T(0);
-    }
-    __syncthreads();
-    if (wave == 0) {
-        V yi = b[k0 + lane];
-        for (int c = 0; c < NB - 1; c++) {
-            V yc = __shfl(yi, c);
-            if (lane > c) yi -= (V)sl[lane * (NB + 1) + c] * yc;
+    __shared__ V s_v[NB];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;  // 16 waves
+    const int nbk = n / NB;
+    const int reach = (hb + NB - 1) / NB;
+    for (int t = 0; t < nbk; t++) {
+        const int r = FWD ? t : nbk - 1 - t;
+        const int i0 = r * NB;
+        V acc[4] = {V(0), V(0), V(0), V(0)};
+        const int p_first = FWD ? max(0, r - reach) : r + 1;
+        const int p_last = FWD ? r - 1 : min(nbk - 1, r + reach);
+#pragma unroll 8
+        for (int p = p_first; p <= p_last; p++) {
+            const int c = p * NB + lane;
+            const V yc = y[c];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int i = i0 + 4 * wave + q;
+                acc[q] += (V)band_ld(ab, i, c, hb, n, true) * yc;
+            }
         }
-        sy[lane] = yi;
-        if (blockIdx.x == 0) z[k0 + lane] = yi;
-    }
-    __syncthreads();
-    const int lim = min(n, k0 + NB + hb);
-    const int rbeg = k0 + NB + blockIdx.x * rows_per_wg;
-    const int rend = min(lim, rbeg + rows_per_wg);
-    const V yl = sy[lane];
-    for (int r = rbeg + wave; r < rend; r += 4) {
-        V s = V(0);
-        int c = k0 + lane;
-        if (r - c <= hb) s = (V)ab[bidx(r, c, hb)] * yl;
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-        if (lane == 0) b[r] -= s;
-    }
-}
-
-// Backward, panel k0 (from the last): wave 0 solves U11 x_p = z_p and
-// workgroup 0 stores x_p in x; rows above that couple to the panel subtract
-// U(row, panel) x_p from z.  Same read/write split as the forward sweep.
-template <typename T, typename V>
-__global__ __launch_bounds__(256) void band_bwd_kernel(const T *ab, int hb, int k0, V *z, V *x, int rows_per_wg)
-{
-    __shared__ T su[NB * (NB + 1)];
-    __shared__ V sx[NB];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int t = tid; t < NB * NB; t += 256) {
-        int i = t / NB, c = t % NB;
-        su[i * (NB + 1) + c] = (c >= i && inband(k0 + i, k0 + c, hb)) ? ab[bidx(k0 + i, k0 + c, hb)] : T(0);
-    }
-    __syncthreads();
-    if (wave == 0) {
-        V yi = z[k0 + lane];
-        for (int c = NB - 1; c >= 0; c--) {
-            V xc = __shfl(yi, c) / (V)su[c * (NB + 1) + c];
-            if (lane == c) yi = xc;
-            else if (lane < c) yi -= (V)su[lane * (NB + 1) + c] * xc;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            V sum = acc[q];
+            for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+            if (lane == q) s_v[4 * wave + q] = x[i0 + 4 * wave + q] - sum;
         }
-        sx[lane] = yi;
-        if (blockIdx.x == 0) x[k0 + lane] = yi;
-    }
-    __syncthreads();
-    const int top = max(0, k0 - hb);
-    const int rbeg = top + blockIdx.x * rows_per_wg;
-    const int rend = min(k0, rbeg + rows_per_wg);
-    const V xl = sx[lane];
-    for (int r = rbeg + wave; r < rend; r += 4) {
-        V s = V(0);
-        int c = k0 + lane;
-        if (c - r <= hb) s = (V)ab[bidx(r, c, hb)] * xl;
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-        if (lane == 0) z[r] -= s;
+        __syncthreads();
+        const T *Ti = FWD ? linv_of(const_cast<T *>(inv), r) : uinv_of(const_cast<T *>(inv), r);
+        const V vl = s_v[lane];
+        V res = V(0);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int i = 4 * wave + q;
+            V sum = (V)Ti[i * NB + lane] * vl;
+            for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+            if (lane == q) res = sum;
+        }
+        if (lane < 4) y[i0 + 4 * wave + lane] = res;
+        __syncthreads();
     }
 }
 
 }  // namespace
 
 // ---- host -----------------------------------------------------------------------
-template <typename T> static int band_factor_t(T *ab, int n, int hb, hipStream_t s, int *flag)
+template <typename T> static int band_factor_t(T *ab, int n, int hb, hipStream_t s, int *flag, T *inv)
 {
     int launches = 0;
-    hipLaunchKernelGGL(band_diag_kernel<T>, dim3(1), dim3(256), 0, s, ab, hb, 0, flag);
+    hipLaunchKernelGGL(band_diag_kernel<T>, dim3(1), dim3(256), 0, s, ab, hb, n, inv, flag);
     launches++;
-    for (int k0 = 0; k0 < n; k0 += NB) {
+    for (int p = 0; p * NB < n; p++) {
+        int k0 = p * NB;
         int lim = std::min(n, k0 + NB + hb);
         int rem = lim - (k0 + NB);
         if (rem <= 0) break;
         int nt = (rem + NB - 1) / NB;
-        hipLaunchKernelGGL(band_trsm_kernel<T>, dim3(2 * nt), dim3(256), 0, s, ab, hb, k0, n, nt);
-        hipLaunchKernelGGL(band_update_kernel<T>, dim3(nt * nt), dim3(256), 0, s, ab, hb, k0, n, nt, flag);
+        int pr = gf_probe_begin(s, GF_KC_BAND_PANEL);
+        hipLaunchKernelGGL(band_panel_kernel<T>, dim3(2 * nt), dim3(256), 0, s, ab, hb, p, n, nt, (const T *)inv);
+        gf_probe_end(s, pr, NB);
+        pr = gf_probe_begin(s, GF_KC_BAND_UPDATE);
+        hipLaunchKernelGGL(band_update_kernel<T>, dim3(nt * nt), dim3(256), 0, s, ab, hb, p, n, nt, inv, flag);
+        gf_probe_end(s, pr, NB);
         launches += 2;
     }
     GF_HIP(hipGetLastError());
     return launches;
 }
 
-// x: right-hand side in, solution out; z: n entries of work space.
-template <typename T, typename V> static int band_solve_t(const T *ab, int n, int hb, hipStream_t s, V *x, V *z)
+template <typename T, typename V>
+static int band_solve_t(const T *ab, int n, int hb, hipStream_t s, V *x, V *z, const T *inv)
 {
-    const int RPW = 64;  // coupled rows per workgroup
-    int launches = 0;
-    for (int k0 = 0; k0 < n; k0 += NB) {
-        int lim = std::min(n, k0 + NB + hb);
-        int rows = std::max(0, lim - (k0 + NB));
-        int grid = std::max(1, (rows + RPW - 1) / RPW);
-        hipLaunchKernelGGL((band_fwd_kernel<T, V>), dim3(grid), dim3(256), 0, s, ab, hb, k0, n, x, z, RPW);
-        launches++;
-    }
-    for (int k0 = n - NB; k0 >= 0; k0 -= NB) {
-        int rows = k0 - std::max(0, k0 - hb);
-        int grid = std::max(1, (rows + RPW - 1) / RPW);
-        hipLaunchKernelGGL((band_bwd_kernel<T, V>), dim3(grid), dim3(256), 0, s, ab, hb, k0, z, x, RPW);
-        launches++;
-    }
+    int pr = gf_probe_begin(s, GF_KC_BAND_SWEEP);
+    hipLaunchKernelGGL((band_sweep_kernel<T, V, true>), dim3(1), dim3(1024), 0, s, ab, hb, n, inv, (const V *)x, z);
+    gf_probe_end(s, pr, n);
+    pr = gf_probe_begin(s, GF_KC_BAND_SWEEP);
+    hipLaunchKernelGGL((band_sweep_kernel<T, V, false>), dim3(1), dim3(1024), 0, s, ab, hb, n, inv, (const V *)z, x);
+    gf_probe_end(s, pr, n);
     GF_HIP(hipGetLastError());
-    return launches;
+    return 2;
 }
 
-int gf_band_factor(void *ab, int elem_size, int n, int hb, hipStream_t s, int *flag)
+size_t gf_band_inv_elems(int n) { return (size_t)(n / NB) * 2 * NB * NB; }
+
+int gf_band_factor(void *ab, int elem_size, int n, int hb, hipStream_t s, int *flag, void *inv)
 {
     if (n % NB != 0) {
         gf_set_error("band factor: order must be a multiple of 64");
         return GF_E_ARG;
     }
-    if (elem_size == 8) return band_factor_t<double>((double *)ab, n, hb, s, flag);
-    return band_factor_t<float>((float *)ab, n, hb, s, flag);
+    if (elem_size == 8) return band_factor_t<double>((double *)ab, n, hb, s, flag, (double *)inv);
+    return band_factor_t<float>((float *)ab, n, hb, s, flag, (float *)inv);
 }
 
-int gf_band_solve(const void *ab, int elem_size, int n, int hb, hipStream_t s, double *x, double *work)
+// x: right-hand side in, solution out; work: n entries.
+int gf_band_solve(const void *ab, int elem_size, int n, int hb, hipStream_t s, double *x, double *work, const void *inv)
 {
-    if (elem_size == 8) return band_solve_t<double, double>((const double *)ab, n, hb, s, x, work);
-    return band_solve_t<float, double>((const float *)ab, n, hb, s, x, work);
+    if (elem_size == 8)
+        return band_solve_t<double, double>((const double *)ab, n, hb, s, x, work, (const double *)inv);
+    return band_solve_t<float, double>((const float *)ab, n, hb, s, x, work, (const float *)inv);
 }
 
 int gf_band_pad(int n) { return (n + NB - 1) / NB * NB; }

@@ -347,9 +347,11 @@ int gf_beam_assemble(gf_model *m, double lf)
         if (e1 <= e0) continue;
         launches++;
         int grid = (e1 - e0 + EPB - 1) / EPB;
+        const int pr = gf_probe_begin(m->stream, GF_KC_BEAM_ASSEMBLE);
         hipLaunchKernelGGL(beam_assemble_kernel<0>, dim3(grid), dim3(WG), 0, m->stream, e0, e1, ld, m->d_bconn,
                            m->d_bconst, m->d_bgp, m->d_disp, m->d_dof, gf_model_gmat(m), m->d_r, lf, m->gravity[0],
                            m->gravity[1], m->gravity[2], (double *)nullptr, (double *)nullptr);
+        gf_probe_end(m->stream, pr, e1 - e0);
     }
     GF_HIP(hipGetLastError());
     return launches;

@@ -95,6 +95,8 @@ template <typename T, int MT> struct PairSmem {
 
 // Loads the augmented [B | I | A | C] of block row e into the wave's registers
 // (column c = lane + 64 s).  hasA / hasC: couplings that exist at this level.
+// Every lane loads from a clamped in-block address and selects afterwards, so
+// the MT loads of a lane are in flight together (no branch around a load).
 template <typename T, int MT>
 __device__ __forceinline__ void load_aug(T (&col)[cfg<MT>::S][MT], const T *A, const T *B, const T *C, int e, int m,
                                          bool hasA, bool hasC)
@@ -104,26 +106,29 @@ __device__ __forceinline__ void load_aug(T (&col)[cfg<MT>::S][MT], const T *A, c
 #pragma unroll
     for (int s = 0; s < cfg<MT>::S; s++) {
         const int c = lane + 64 * s, part = c / MT, cc = c % MT;
-        const T *src = part == 0 ? B : part == 2 ? A : C;
+        const T *src = (part == 2 ? A : part == 3 ? C : B) + o + (cc < m ? cc : m - 1);
         const bool live = part == 0 || (part == 2 && hasA) || (part == 3 && hasC);
+        T raw[MT];
+#pragma unroll
+        for (int i = 0; i < MT; i++) raw[i] = src[(size_t)(i < m ? i : m - 1) * m];
 #pragma unroll
         for (int i = 0; i < MT; i++) {
-            T v = T(0);
-            if (part == 1) v = i == cc ? T(1) : T(0);
-            else if (i < m && cc < m) { if (live) v = src[o + (size_t)i * m + cc]; }
-            else if (part == 0 && i == cc) v = T(1);
-            col[s][i] = v;
+            const T diag = i == cc ? T(1) : T(0);
+            col[s][i] = part == 1 ? diag : (i < m && cc < m) ? (live ? raw[i] : T(0)) : part == 0 ? diag : T(0);
         }
     }
 }
 
 // Gauss-Jordan on the wave-held augmented matrix: [B | I | A | C] -> [I | Binv | X | Y].
 // Pivot: the largest |entry| of column k in rows >= k unless |B_kk| >= half of it.
-template <typename T, int MT> __device__ void wave_gj(T (&col)[cfg<MT>::S][MT], int *flag)
+// Columns k >= m are the identity padding (zero elsewhere in rows and columns
+// >= m), so the sweep ends after m pivots.
+template <typename T, int MT> __device__ void wave_gj(T (&col)[cfg<MT>::S][MT], int *flag, int m)
 {
     constexpr int S = cfg<MT>::S;
 #pragma unroll
     for (int k = 0; k < MT; k++) {
+        if (k >= m) continue;  // wave-uniform; keeps the loop unrolled
         T mk[MT];
 #pragma unroll
         for (int i = 0; i < MT; i++) mk[i] = rdl(col[0][i], k);  // column k sits in lane k, slot 0
@@ -211,10 +216,9 @@ __device__ __forceinline__ void mma_acc(typename mma<T>::acc_t (&acc)[MT / 16][M
     }
 }
 
-// dst (global m x m block) = base - acc (base may be null: -acc).
+// dst (global m x m block) = acc.
 template <typename T, int MT>
-__device__ __forceinline__ void store_acc(const typename mma<T>::acc_t (&acc)[MT / 16][MT / 16], T *dst,
-                                          const T *base, int m)
+__device__ __forceinline__ void store_acc(const typename mma<T>::acc_t (&acc)[MT / 16][MT / 16], T *dst, int m)
 {
     const int lane = threadIdx.x & 63;
 #pragma unroll
@@ -224,10 +228,23 @@ __device__ __forceinline__ void store_acc(const typename mma<T>::acc_t (&acc)[MT
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 int i = 16 * ti + mma<T>::row(lane, v), j = 16 * tj + (lane & 15);
-                if (i < m && j < m) {
-                    size_t o = (size_t)i * m + j;
-                    dst[o] = (base ? base[o] : T(0)) - acc[ti][tj][v];
-                }
+                if (i < m && j < m) dst[(size_t)i * m + j] = acc[ti][tj][v];
+            }
+}
+
+// dst (global m x m block) = -acc.
+template <typename T, int MT>
+__device__ __forceinline__ void store_neg(const typename mma<T>::acc_t (&acc)[MT / 16][MT / 16], T *dst, int m)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int ti = 0; ti < MT / 16; ti++)
+#pragma unroll
+        for (int tj = 0; tj < MT / 16; tj++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                int i = 16 * ti + mma<T>::row(lane, v), j = 16 * tj + (lane & 15);
+                if (i < m && j < m) dst[(size_t)i * m + j] = -acc[ti][tj][v];
             }
 }
 
@@ -243,10 +260,15 @@ struct Fac {
 // with nothing to do pass active = false.
 // f: the right-hand side being reduced (null: factor only); zout: z of the
 // eliminated rows.
+// Every global read of the step (the eliminated neighbour's augmented row, the
+// kept row's couplings and diagonal block, both right-hand-side pieces) is
+// issued before the Gauss-Jordan sweep, so a step pays one memory round trip.
 template <typename T, int MT>
 __device__ void fac_step(const Fac &F, PairSmem<T, MT> &sm, bool active, int L, int nL, int q, T *Lsv, T *Rsv,
                          double *f, double *zout)
 {
+    typedef typename mma<T>::acc_t acc_t;
+    constexpr int NT = MT / 16, KV = MT * MT / 64;
     T *A = (T *)F.A, *B = (T *)F.B, *C = (T *)F.C;
     const int m = F.m;
     const size_t bs = (size_t)m * m;
@@ -257,76 +279,124 @@ __device__ void fac_step(const Fac &F, PairSmem<T, MT> &sm, bool active, int L, 
     const bool doE = active && (wv == 0 || hasR);
     const int e = wv == 0 ? l : r, pe = wv == 0 ? p - 1 : p + 1;
     const bool own = wv == 0 || p + 2 == nL;  // the right neighbour is written here only if it is the last row
+
+    // ---- reads
+    // (unconditional loads from clamped in-block addresses, selected afterwards;
+    // k, e and q are valid rows also for inactive pairs)
+    T kv[KV];  // kept-row coupling (wave 0: A_k, wave 1: C_k), element lane + 64 j
+    {
+        const T *src = (wv == 0 ? A : C) + (size_t)k * bs;
+        T raw[KV];
+#pragma unroll
+        for (int jv = 0; jv < KV; jv++) {
+            const int t = lane + 64 * jv, i = t / MT, jj = t % MT;
+            raw[jv] = src[(size_t)(i < m ? i : m - 1) * m + (jj < m ? jj : m - 1)];
+        }
+#pragma unroll
+        for (int jv = 0; jv < KV; jv++) {
+            const int t = lane + 64 * jv, i = t / MT, jj = t % MT;
+            kv[jv] = active && i < m && jj < m ? raw[jv] : T(0);
+        }
+    }
+    T bk[NT][NT][4];  // wave 0: B_k at its MFMA result positions
+#pragma unroll
+    for (int ti = 0; ti < NT; ti++)
+#pragma unroll
+        for (int tj = 0; tj < NT; tj++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int i = 16 * ti + mma<T>::row(lane, v), jj = 16 * tj + (lane & 15);
+                bk[ti][tj][v] = B[(size_t)k * bs + (size_t)(i < m ? i : m - 1) * m + (jj < m ? jj : m - 1)];
+            }
+#pragma unroll
+    for (int ti = 0; ti < NT; ti++)
+#pragma unroll
+        for (int tj = 0; tj < NT; tj++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int i = 16 * ti + mma<T>::row(lane, v), jj = 16 * tj + (lane & 15);
+                bk[ti][tj][v] = active && wv == 0 && i < m && jj < m ? bk[ti][tj][v] : T(0);
+            }
+    // right-hand side pieces: f_e (staged for z_e, read for j < m only) and f_k
+    // (used by wave 0's lanes < m); lanes >= m hold a duplicate entry
+    double fe = 0.0, fk = 0.0;
+    if (f) {
+        const int li = lane < m ? lane : m - 1;
+        fe = f[(size_t)e * m + li];
+        fk = f[(size_t)k * m + li];
+    }
+    T col[cfg<MT>::S][MT];  // (loaded by every wave: e is a valid row also when it is not eliminated here)
+    load_aug<T, MT>(col, A, B, C, e, m, pe >= 1, pe + 1 < nL);
+
+    // ---- kept-row couplings to LDS, saved for later solves
+    if (active) {
+        T *sv = (wv == 0 ? Lsv : Rsv) + (size_t)q * bs;
+        T *dst = sm.k[wv];
+#pragma unroll
+        for (int jv = 0; jv < KV; jv++) {
+            const int t = lane + 64 * jv, i = t / MT, jj = t % MT;
+            dst[i * cfg<MT>::LD + jj] = kv[jv];
+            if (i < m && jj < m) sv[(size_t)i * m + jj] = kv[jv];
+        }
+        if (f && lane < MT) sm.z[wv][lane] = T(fe);
+    }
+    // ---- eliminated neighbour
     if (doE) {
-        T col[cfg<MT>::S][MT];
-        load_aug<T, MT>(col, A, B, C, e, m, pe >= 1, pe + 1 < nL);
-        wave_gj<T, MT>(col, F.flag);
+        wave_gj<T, MT>(col, F.flag, m);
         store_elim<T, MT>(col, &sm.e[wv][0][0], (T *)F.Bi, (T *)F.X, (T *)F.Y, e, m, own);
     } else if (active) {
         // no right neighbour: zero X_r, Y_r so the products below need no special case
         for (int t = lane; t < 3 * cfg<MT>::BLK; t += 64) (&sm.e[1][0][0])[t] = T(0);
     }
-    if (active) {
-        // kept-row couplings (wave 0: A_k, wave 1: C_k), saved for later solves
-        const T *src = wv == 0 ? A : C;
-        T *sv = wv == 0 ? Lsv : Rsv;
-        T *dst = sm.k[wv];
-        for (int t = lane; t < MT * MT; t += 64) {
-            int i = t / MT, j = t % MT;
-            T v = T(0);
-            if (i < m && j < m) {
-                v = src[(size_t)k * bs + (size_t)i * m + j];
-                sv[(size_t)q * bs + (size_t)i * m + j] = v;
-            }
-            dst[i * cfg<MT>::LD + j] = v;
-        }
-    }
     __syncthreads();
-    if (active && f) {
-        // z of the eliminated neighbour: Binv_e f_e
-        if (doE && lane < MT) {
-            T s = T(0);
-            if (lane < m)
-                for (int j = 0; j < m; j++) s += sm.e[wv][0][lane * cfg<MT>::LD + j] * (T)f[(size_t)e * m + j];
-            sm.z[wv][lane] = s;
-            if (own && lane < m) zout[(size_t)e * m + lane] = (double)s;
-        } else if (!doE && lane < MT) {
-            sm.z[wv][lane] = T(0);
+    T zl = T(0);
+    if (active && f && lane < MT) {
+        // z of the eliminated neighbour: Binv_e f_e (f_e staged in sm.z)
+        if (doE && lane < m) {
+            for (int j = 0; j < m; j++) zl += sm.e[wv][0][lane * cfg<MT>::LD + j] * sm.z[wv][j];
+            if (own) zout[(size_t)e * m + lane] = (double)zl;
         }
     }
     if (active) {
-        typedef typename mma<T>::acc_t acc_t;
         if (wv == 0) {
-            acc_t acc[MT / 16][MT / 16];
+            acc_t acc[NT][NT];
 #pragma unroll
-            for (int a = 0; a < MT / 16; a++)
+            for (int a = 0; a < NT; a++)
 #pragma unroll
-                for (int b = 0; b < MT / 16; b++) acc[a][b] = acc_t{};
+                for (int b = 0; b < NT; b++) acc[a][b] = acc_t{};
             mma_acc<T, MT>(acc, sm.k[0], sm.e[0][2]);  // A_k Y_l
             mma_acc<T, MT>(acc, sm.k[1], sm.e[1][1]);  // C_k X_r
-            store_acc<T, MT>(acc, B + (size_t)k * bs, B + (size_t)k * bs, m);
+#pragma unroll
+            for (int ti = 0; ti < NT; ti++)
+#pragma unroll
+                for (int tj = 0; tj < NT; tj++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) acc[ti][tj][v] = bk[ti][tj][v] - acc[ti][tj][v];
+            store_acc<T, MT>(acc, B + (size_t)k * bs, m);
         } else {
-            acc_t acc[MT / 16][MT / 16];
+            acc_t acc[NT][NT];
 #pragma unroll
-            for (int a = 0; a < MT / 16; a++)
+            for (int a = 0; a < NT; a++)
 #pragma unroll
-                for (int b = 0; b < MT / 16; b++) acc[a][b] = acc_t{};
+                for (int b = 0; b < NT; b++) acc[a][b] = acc_t{};
             mma_acc<T, MT>(acc, sm.k[0], sm.e[0][1]);  // A_k X_l
-            store_acc<T, MT>(acc, A + (size_t)k * bs, (const T *)nullptr, m);
+            store_neg<T, MT>(acc, A + (size_t)k * bs, m);
 #pragma unroll
-            for (int a = 0; a < MT / 16; a++)
+            for (int a = 0; a < NT; a++)
 #pragma unroll
-                for (int b = 0; b < MT / 16; b++) acc[a][b] = acc_t{};
+                for (int b = 0; b < NT; b++) acc[a][b] = acc_t{};
             mma_acc<T, MT>(acc, sm.k[1], sm.e[1][2]);  // C_k Y_r
-            store_acc<T, MT>(acc, C + (size_t)k * bs, (const T *)nullptr, m);
+            store_neg<T, MT>(acc, C + (size_t)k * bs, m);
         }
     }
+    __syncthreads();  // sm.z is overwritten with z below (all readers of f_e are done)
+    if (active && f && lane < MT) sm.z[wv][lane] = zl;
     __syncthreads();
     if (active && f && wv == 0 && lane < m) {
         T s = T(0);
         for (int j = 0; j < m; j++)
             s += sm.k[0][lane * cfg<MT>::LD + j] * sm.z[0][j] + sm.k[1][lane * cfg<MT>::LD + j] * sm.z[1][j];
-        f[(size_t)k * m + lane] -= (double)s;
+        f[(size_t)k * m + lane] = fk - (double)s;
     }
     __syncthreads();
 }
@@ -341,7 +411,7 @@ __device__ void top_step(const Fac &F, PairSmem<T, MT> &sm, int levels, const do
     if (threadIdx.x < 64) {
         T col[cfg<MT>::S][MT];
         load_aug<T, MT>(col, (const T *)F.A, (const T *)F.B, (const T *)F.C, e, m, false, false);
-        wave_gj<T, MT>(col, F.flag);
+        wave_gj<T, MT>(col, F.flag, m);
         store_elim<T, MT>(col, &sm.e[0][0][0], (T *)F.Bi, (T *)F.X, (T *)F.Y, e, m, true);
         if (f && lane < m) {
             T s = T(0);
@@ -528,14 +598,14 @@ int gf_btd_alloc(gf_btd &b, int m, int nblk, int elem_size)
     size_t blocks = (size_t)nblk * b.bsz * elem_size;
     size_t sv = (save ? save : 1) * b.bsz * elem_size;
     // A, B, C contiguous: the assembly clears them with one fill
-    GF_HIP(hipMalloc(&b.A, 3 * blocks));
+    GF_TRY(gf_pool_alloc(&b.A, 3 * blocks));
     b.B = (char *)b.A + blocks;
     b.C = (char *)b.A + 2 * blocks;
     void **arrs[] = {&b.Bi, &b.X, &b.Y};
-    for (void **a : arrs) GF_HIP(hipMalloc(a, blocks));
-    GF_HIP(hipMalloc(&b.Lsv, sv));
-    GF_HIP(hipMalloc(&b.Rsv, sv));
-    GF_HIP(hipMalloc(&b.d_save, sizeof(long long) * (b.lvl_save.size() + 1)));
+    for (void **a : arrs) GF_TRY(gf_pool_alloc(a, blocks));
+    GF_TRY(gf_pool_alloc(&b.Lsv, sv));
+    GF_TRY(gf_pool_alloc(&b.Rsv, sv));
+    GF_TRY(gf_pool_alloc((void **)&b.d_save, sizeof(long long) * (b.lvl_save.size() + 1)));
     std::vector<long long> sv_h(b.lvl_save);
     sv_h.push_back((long long)save);
     GF_HIP(hipMemcpy(b.d_save, sv_h.data(), sizeof(long long) * sv_h.size(), hipMemcpyHostToDevice));
@@ -548,8 +618,7 @@ int gf_btd_alloc(gf_btd &b, int m, int nblk, int elem_size)
 void gf_btd_free(gf_btd &b)
 {
     void *arrs[] = {b.A, b.Bi, b.X, b.Y, b.Lsv, b.Rsv, b.d_save};  // B and C live inside A's allocation
-    for (void *a : arrs)
-        if (a) (void)hipFree(a);
+    for (void *a : arrs) gf_pool_free(a);
     b.A = b.B = b.C = b.Bi = b.X = b.Y = b.Lsv = b.Rsv = nullptr;
     b.d_save = nullptr;
 }
@@ -563,13 +632,17 @@ static int btd_factor_t(gf_btd &b, hipStream_t s, int *d_flag, double *f, double
     int launches = 0;
     for (int L = 0; L < b.tail_level; L++) {
         const int nL = b.lvl_n[L];
+        const int pr = gf_probe_begin(s, GF_KC_BTD_LEVEL);
         hipLaunchKernelGGL((btd_level_kernel<T, MT>), dim3(nL / 2), dim3(128), 0, s, F, L, nL, Lsv + b.lvl_save[L] * bs,
                            Rsv + b.lvl_save[L] * bs, f, x);
+        gf_probe_end(s, pr, nL / 2);
         launches++;
     }
     const int nL0 = b.tail_level < b.levels ? b.lvl_n[b.tail_level] : 1;
+    const int pr = gf_probe_begin(s, GF_KC_BTD_TAIL);
     hipLaunchKernelGGL((btd_tail_kernel<T, MT>), dim3(1), dim3(128 * cfg<MT>::PAIRS), 0, s, F, b.tail_level, b.levels,
                        nL0, Lsv, Rsv, (const long long *)b.d_save, f, x);
+    gf_probe_end(s, pr, nL0);
     launches++;
     if (f) {
         for (int L = b.tail_level - 1; L >= 0; L--) {

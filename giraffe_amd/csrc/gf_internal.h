@@ -23,6 +23,12 @@ void gf_set_error(const std::string &msg);
         if (s_ != GF_OK) return s_; \
     } while (0)
 
+// ---- device memory cache (gf_pool.cpp) ---------------------------------------
+int gf_pool_alloc(void **out, size_t bytes);
+void gf_pool_free(void *p);      // only after the last device use has completed
+int gf_pinned_alloc(void **out, size_t bytes);
+void gf_pinned_free(void *p);
+
 // ---- per-element constant record of Beam_1 (structure of arrays, [BC_N][nb]) ----
 enum {
     BC_T3 = 0,        // 9: transform3 rows E1,E2,E3
@@ -111,6 +117,41 @@ struct gf_btd {
     int elem_size = 8;
 };
 
+// ---- per-kernel timing --------------------------------------------------------
+// While a model's timing is on and its probe mask selects a kernel class, every
+// launch of that class is bracketed by a pair of HIP events on its stream; the
+// pairs are read back when the phase that issued them is accumulated.  `units`
+// counts the work of each launch in the class's unit (block rows eliminated,
+// band columns eliminated, elements assembled, rows swept).
+struct gf_kprobe {
+    unsigned mask = 0;
+    std::vector<hipEvent_t> ev;    // 2 per slot
+    std::vector<int> cls;
+    std::vector<double> units;
+    int used = 0;
+    double ms[GF_NKC] = {}, units_done[GF_NKC] = {};
+    long long launches[GF_NKC] = {};
+    int open_slot = -1;
+
+    bool wants(int c) const { return (mask >> c) & 1u; }
+    int begin(hipStream_t s, int c);           // before the launch; -1 when not probed
+    void end(hipStream_t s, int slot, double units);
+    int flush();                               // waits for the recorded pairs, accumulates
+    ~gf_kprobe();
+};
+// Probe of the model whose device work the calling thread is enqueuing (null when off).
+extern thread_local gf_kprobe *gf_probe;
+struct gf_probe_scope {
+    gf_kprobe *prev;
+    explicit gf_probe_scope(gf_kprobe *p) : prev(gf_probe) { gf_probe = p; }
+    ~gf_probe_scope() { gf_probe = prev; }
+};
+inline int gf_probe_begin(hipStream_t s, int c) { return gf_probe && gf_probe->wants(c) ? gf_probe->begin(s, c) : -1; }
+inline void gf_probe_end(hipStream_t s, int slot, double units)
+{
+    if (slot >= 0) gf_probe->end(s, slot, units);
+}
+
 struct gf_model {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -173,6 +214,8 @@ struct gf_model {
     gf_btd btd;
     double *d_band = nullptr;       // GM_BAND storage, FP64
     size_t band_elems = 0;
+    double *d_band_inv = nullptr;   // per-panel triangular inverses (FP64 factor)
+    float *d_band32_inv = nullptr;  // per-panel triangular inverses (FP32 factor)
     // mixed precision: equilibrated FP32 factor + FP64 refinement buffers
     gf_btd btd32;
     float *d_band32 = nullptr;
@@ -196,6 +239,7 @@ struct gf_model {
     hipEvent_t ev[2 * GF_NPHASE] = {};
     double t_ms[GF_NPHASE] = {0, 0, 0};
     long long launches[GF_NPHASE] = {0, 0, 0};
+    gf_kprobe kprobe;
 };
 
 // ---- kernel launchers (defined in the .hip files) ----------------------------
@@ -263,8 +307,10 @@ int gf_mixed_factor(gf_model *m);
 int gf_mixed_solve(gf_model *m, int *fallback, int *sweeps);
 
 int gf_band_pad(int n);
-int gf_band_factor(void *ab, int elem_size, int n, int hb, hipStream_t s, int *flag);
-int gf_band_solve(const void *ab, int elem_size, int n, int hb, hipStream_t s, double *x, double *work);
+size_t gf_band_inv_elems(int n);   // per-panel triangular inverses of a factor of order n
+int gf_band_factor(void *ab, int elem_size, int n, int hb, hipStream_t s, int *flag, void *inv);
+int gf_band_solve(const void *ab, int elem_size, int n, int hb, hipStream_t s, double *x, double *work,
+                  const void *inv);
 
 int gf_beam_eval_device(int count, const double *x, const double *cs, const double *prop,
                         const double *d, const double *gp, double *K_out, double *f_out);

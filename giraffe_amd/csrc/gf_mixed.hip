@@ -132,23 +132,23 @@ inline int grid_for(size_t n, int block, int cap = 4096)
 int gf_mixed_alloc(gf_model *m)
 {
     size_t n = (size_t)m->npad;
-    GF_HIP(hipMalloc(&m->d_scale, n * 8));
-    GF_HIP(hipMalloc(&m->d_res, n * 8));
-    GF_HIP(hipMalloc(&m->d_w, n * 8));
+    GF_TRY(gf_pool_alloc((void **)&m->d_scale, n * 8));
+    GF_TRY(gf_pool_alloc((void **)&m->d_res, n * 8));
+    GF_TRY(gf_pool_alloc((void **)&m->d_w, n * 8));
     if (m->layout == GM_BTD) {
         int st = gf_btd_alloc(m->btd32, m->btd.m, m->btd.nblk, 4);
         if (st) return st;
     } else {
-        GF_HIP(hipMalloc(&m->d_band32, m->band_elems * 4));
+        GF_TRY(gf_pool_alloc((void **)&m->d_band32, m->band_elems * 4));
+        GF_TRY(gf_pool_alloc((void **)&m->d_band32_inv, gf_band_inv_elems(m->npad) * 4));
     }
     return GF_OK;
 }
 
 void gf_mixed_free(gf_model *m)
 {
-    void *p[] = {m->d_scale, m->d_res, m->d_w, m->d_band32};
-    for (void *q : p)
-        if (q) (void)hipFree(q);
+    void *p[] = {m->d_scale, m->d_res, m->d_w, m->d_band32, m->d_band32_inv};
+    for (void *q : p) gf_pool_free(q);
     if (m->btd32.A) gf_btd_free(m->btd32);
 }
 
@@ -172,7 +172,7 @@ int gf_mixed_factor(gf_model *m)
     hipLaunchKernelGGL(convert_band_kernel, dim3(grid_for(m->band_elems, 256)), dim3(256), 0, m->stream, m->d_band,
                        m->d_band32, npad, m->half_band, m->d_scale);
     GF_HIP(hipGetLastError());
-    int l = gf_band_factor(m->d_band32, 4, npad, m->half_band, m->stream, m->d_flag);
+    int l = gf_band_factor(m->d_band32, 4, npad, m->half_band, m->stream, m->d_flag, m->d_band32_inv);
     return l < 0 ? l : l + 2;
 }
 
@@ -195,7 +195,8 @@ int gf_mixed_solve(gf_model *m, int *fallback, int *sweeps)
                            m->d_w);
         // BTD: solution in d_swork (d_w is reduced in place); band: in place in d_w
         int l = m->layout == GM_BTD ? gf_btd_solve(m->btd32, m->stream, m->d_w, m->d_swork, npad, m->d_w)
-                                    : gf_band_solve(m->d_band32, 4, npad, m->half_band, m->stream, m->d_w, m->d_swork);
+                                    : gf_band_solve(m->d_band32, 4, npad, m->half_band, m->stream, m->d_w, m->d_swork,
+                                                    m->d_band32_inv);
         if (l < 0) return l;
         const double *sol = m->layout == GM_BTD ? m->d_swork : m->d_w;
         st = gf_launch_fill_bytes(m->stream, m->d_scal + 4, 2 * sizeof(double));
@@ -229,10 +230,10 @@ int gf_mixed_solve(gf_model *m, int *fallback, int *sweeps)
         int l = gf_btd_factor_solve(m->btd, m->stream, m->d_flag, m->d_r, m->d_x, npad, m->d_res);
         return l < 0 ? l : launches + l;
     }
-    int l = gf_band_factor(m->d_band, 8, npad, m->half_band, m->stream, m->d_flag);
+    int l = gf_band_factor(m->d_band, 8, npad, m->half_band, m->stream, m->d_flag, m->d_band_inv);
     if (l < 0) return l;
     GF_HIP(hipMemcpyAsync(m->d_x, m->d_r, sizeof(double) * npad, hipMemcpyDeviceToDevice, m->stream));
-    int l2 = gf_band_solve(m->d_band, 8, npad, m->half_band, m->stream, m->d_x, m->d_swork);
+    int l2 = gf_band_solve(m->d_band, 8, npad, m->half_band, m->stream, m->d_x, m->d_swork, m->d_band_inv);
     if (l2 < 0) return l2;
     return launches + l + l2;
 }

@@ -40,12 +40,7 @@ extern "C" int gf_default_options(gf_options *o)
 
 template <typename T> static int dalloc(T **p, size_t n)
 {
-    hipError_t e = hipMalloc((void **)p, sizeof(T) * (n ? n : 1));
-    if (e != hipSuccess) {
-        gf_set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
-        return GF_E_NOMEM;
-    }
-    return GF_OK;
+    return gf_pool_alloc((void **)p, sizeof(T) * (n ? n : 1));
 }
 template <typename T> static int upload(gf_model *m, T *dst, const T *src, size_t n)
 {
@@ -77,7 +72,7 @@ static int tock(gf_model *m, int ph, int launches)
     m->t_ms[ph] += ms;
     m->launches[ph] += launches;
     m->ev_pending[ph] = false;
-    return GF_OK;
+    return m->kprobe.flush();
 }
 // Accumulates the phases whose end events the stream has passed (enqueue/result pairs).
 static int flush_timing(gf_model *m)
@@ -93,8 +88,60 @@ static int flush_timing(gf_model *m)
         m->ev_pending[ph] = false;
         m->pending_launches[ph] = 0;
     }
+    return m->kprobe.flush();
+}
+
+thread_local gf_kprobe *gf_probe = nullptr;
+
+int gf_kprobe::begin(hipStream_t s, int c)
+{
+    const int cap = 8192;
+    if (used == cap && flush() != GF_OK) return -1;
+    if (2 * used + 2 > (int)ev.size()) {
+        hipEvent_t a = nullptr, b = nullptr;
+        if (hipEventCreate(&a) != hipSuccess) return -1;
+        if (hipEventCreate(&b) != hipSuccess) {
+            (void)hipEventDestroy(a);
+            return -1;
+        }
+        ev.push_back(a);
+        ev.push_back(b);
+        cls.push_back(0);
+        units.push_back(0.0);
+    }
+    const int slot = used++;
+    cls[slot] = c;
+    (void)hipEventRecord(ev[2 * slot], s);
+    return slot;
+}
+
+void gf_kprobe::end(hipStream_t s, int slot, double u)
+{
+    units[slot] = u;
+    (void)hipEventRecord(ev[2 * slot + 1], s);
+}
+
+int gf_kprobe::flush()
+{
+    if (!used) return GF_OK;
+    GF_HIP(hipEventSynchronize(ev[2 * used - 1]));
+    for (int i = 0; i < used; i++) {
+        float t = 0.f;
+        GF_HIP(hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]));
+        ms[cls[i]] += t;
+        units_done[cls[i]] += units[i];
+        launches[cls[i]]++;
+    }
+    used = 0;
     return GF_OK;
 }
+
+gf_kprobe::~gf_kprobe()
+{
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+}
+
+static gf_kprobe *probe_of(gf_model *m) { return m->timing && m->kprobe.mask ? &m->kprobe : nullptr; }
 
 static int solve_fp64_enqueue(gf_model *m);
 
@@ -114,15 +161,16 @@ gf_gmat gf_model_gmat(const gf_model *m)
 static void release(gf_model *m)
 {
     if (!m) return;
+    // the cache hands freed blocks to the next model: nothing may still run on them
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
     for (int i = 0; i < 2 * GF_NPHASE; i++)
         if (m->ev[i]) (void)hipEventDestroy(m->ev[i]);
     void *bufs[] = {m->d_disp, m->d_copy, m->d_euler, m->d_dof, m->d_row_nc, m->d_bconn, m->d_bconst, m->d_bgp,
                     m->d_ln_node, m->d_ln_off, m->d_ln_ids, m->d_lrec, m->d_tab_t, m->d_tab_v, m->d_r, m->d_x,
                     m->d_scal, m->d_flag, m->d_copy0, m->d_bgp0, m->d_sconn, m->d_sconst, m->d_spconst, m->d_sgp,
-                    m->d_sgp0, m->d_spload, m->d_spflag, m->d_band, m->d_swork};
-    for (void *b : bufs)
-        if (b) (void)hipFree(b);
-    if (m->h_scal) (void)hipHostFree(m->h_scal);
+                    m->d_sgp0, m->d_spload, m->d_spflag, m->d_band, m->d_swork, m->d_band_inv};
+    for (void *b : bufs) gf_pool_free(b);
+    if (m->h_scal) gf_pinned_free(m->h_scal);
     gf_btd_free(m->btd);
     gf_mixed_free(m);
     if (m->own_stream && m->stream) (void)hipStreamDestroy(m->stream);
@@ -258,7 +306,7 @@ static size_t footprint(const gf_model_desc *d, const gf_model *m, int precision
         size_t blocks = 6 * (size_t)nblk + 2 * (size_t)nblk;                  // A B C Bi X Y + saved couplings
         tangent = blocks * (size_t)bm * bm;
     } else {
-        tangent = npad * (size_t)(2 * m->half_band + 1);
+        tangent = npad * (size_t)(2 * m->half_band + 1) + gf_band_inv_elems(m->npad);
     }
     b += tangent * 8;
     if (precision == GF_PREC_MIXED) b += tangent * 4 + 3 * npad * 8;
@@ -446,11 +494,12 @@ static int create(const gf_model_desc *d, const gf_options *opt, gf_model *m)
     } else {
         m->band_elems = npad * (size_t)(2 * hb + 1);
         GF_TRY(dalloc(&m->d_band, m->band_elems));
+        GF_TRY(dalloc(&m->d_band_inv, gf_band_inv_elems(m->npad)));
     }
     if (m->precision == GF_PREC_MIXED) GF_TRY(gf_mixed_alloc(m));
     GF_TRY(dalloc(&m->d_scal, 16));
     GF_TRY(dalloc(&m->d_flag, 4));
-    GF_HIP(hipHostMalloc((void **)&m->h_scal, 16 * sizeof(double), hipHostMallocDefault));
+    GF_TRY(gf_pinned_alloc((void **)&m->h_scal, 16 * sizeof(double)));
     for (int i = 0; i < 2 * GF_NPHASE; i++) GF_HIP(hipEventCreate(&m->ev[i]));
     GF_HIP(hipStreamSynchronize(m->stream));
     return GF_OK;
@@ -479,6 +528,28 @@ extern "C" int gf_timing_enable(gf_model *m, int on)
 {
     if (!m) return GF_E_ARG;
     m->timing = on != 0;
+    return GF_OK;
+}
+
+extern "C" int gf_kernel_probe(gf_model *m, unsigned mask)
+{
+    if (!m || mask >= (1u << GF_NKC)) return GF_E_ARG;
+    GF_TRY(m->kprobe.flush());
+    m->kprobe.mask = mask;
+    return GF_OK;
+}
+
+extern "C" int gf_kernel_probe_read(gf_model *m, double *ms_out, long long *launches_out, double *units_out, int reset)
+{
+    if (!m) return GF_E_ARG;
+    GF_TRY(m->kprobe.flush());
+    gf_kprobe &k = m->kprobe;
+    for (int c = 0; c < GF_NKC; c++) {
+        if (ms_out) ms_out[c] = k.ms[c];
+        if (launches_out) launches_out[c] = k.launches[c];
+        if (units_out) units_out[c] = k.units_done[c];
+        if (reset) { k.ms[c] = 0.0; k.launches[c] = 0; k.units_done[c] = 0.0; }
+    }
     return GF_OK;
 }
 
@@ -579,6 +650,7 @@ extern "C" int gf_set_solution_window(gf_model *m, double start, double end)
 // max-norm scan; the norms land in h_scal[0..2] when the stream reaches them.
 int gf_assemble_enqueue(gf_model *m)
 {
+    gf_probe_scope ps(probe_of(m));
     GF_HIP(hipSetDevice(m->device));
     size_t npad = (size_t)m->npad;
     GF_TRY(gf_launch_fill(m->stream, m->d_r, npad, 0.0));
@@ -634,6 +706,7 @@ extern "C" int gf_assemble(gf_model *m, double *res_out)
 // is enqueued after it.
 int gf_solve_update_enqueue(gf_model *m)
 {
+    gf_probe_scope ps(probe_of(m));
     GF_HIP(hipSetDevice(m->device));
     if (m->precision == GF_PREC_MIXED) {
         int info_fail = 0;
@@ -681,14 +754,14 @@ static int solve_fp64_enqueue(gf_model *m)
     int nl_fac, nl_sol = 0;
     if (m->layout == GM_BTD)
         nl_fac = gf_btd_factor_solve(m->btd, m->stream, m->d_flag, m->d_r, m->d_x, (int)npad, m->d_swork);
-    else nl_fac = gf_band_factor(m->d_band, 8, m->npad, m->half_band, m->stream, m->d_flag);
+    else nl_fac = gf_band_factor(m->d_band, 8, m->npad, m->half_band, m->stream, m->d_flag, m->d_band_inv);
     if (nl_fac < 0) return nl_fac;
     tick(m, GF_PH_FACTOR, false);
     m->factored = true;
     tick(m, GF_PH_SOLVE, true);
     if (m->layout != GM_BTD) {
         GF_HIP(hipMemcpyAsync(m->d_x, m->d_r, sizeof(double) * npad, hipMemcpyDeviceToDevice, m->stream));
-        nl_sol = gf_band_solve(m->d_band, 8, m->npad, m->half_band, m->stream, m->d_x, m->d_swork);
+        nl_sol = gf_band_solve(m->d_band, 8, m->npad, m->half_band, m->stream, m->d_x, m->d_swork, m->d_band_inv);
     }
     if (nl_sol < 0) return nl_sol;
     tick(m, GF_PH_SOLVE, false);
@@ -700,6 +773,7 @@ static int solve_fp64_enqueue(gf_model *m)
 extern "C" int gf_solve(gf_model *m, int *info_fail)
 {
     if (!m || !info_fail) return GF_E_ARG;
+    gf_probe_scope ps(probe_of(m));
     GF_HIP(hipSetDevice(m->device));
     *info_fail = 0;
     GF_TRY(gf_launch_fill_bytes(m->stream, m->d_flag, 4));
@@ -900,7 +974,7 @@ extern "C" int gf_bandsystem(int n, int hb, const double *ab_rows, const double 
         }
     std::vector<double> diag((size_t)(npad - n) * w, 0.0);
     for (int i = n; i < npad; i++) diag[(size_t)(i - n) * w + hb] = 1.0;
-    double *d_ab = nullptr, *d_x = nullptr, *d_z = nullptr;
+    double *d_ab = nullptr, *d_x = nullptr, *d_z = nullptr, *d_inv = nullptr;
     int *d_flag = nullptr;
     hipStream_t s = nullptr;
     int flag = 0;
@@ -910,6 +984,7 @@ extern "C" int gf_bandsystem(int n, int hb, const double *ab_rows, const double 
         GF_HIP(hipMalloc(&d_ab, elems * 8));
         GF_HIP(hipMalloc(&d_x, (size_t)npad * 8));
         GF_HIP(hipMalloc(&d_z, (size_t)npad * 8));
+        GF_HIP(hipMalloc(&d_inv, gf_band_inv_elems(npad) * 8));
         GF_HIP(hipMalloc(&d_flag, 4));
         GF_TRY(gf_launch_fill(s, d_ab, elems, 0.0));
         GF_TRY(gf_launch_fill(s, d_x, (size_t)npad, 0.0));
@@ -918,9 +993,9 @@ extern "C" int gf_bandsystem(int n, int hb, const double *ab_rows, const double 
         if (npad > n)
             GF_HIP(hipMemcpyAsync(d_ab + (size_t)n * w, diag.data(), diag.size() * 8, hipMemcpyHostToDevice, s));
         GF_HIP(hipMemcpyAsync(d_x, b, (size_t)n * 8, hipMemcpyHostToDevice, s));
-        int l = gf_band_factor(d_ab, 8, npad, hb, s, d_flag);
+        int l = gf_band_factor(d_ab, 8, npad, hb, s, d_flag, d_inv);
         if (l < 0) return l;
-        l = gf_band_solve(d_ab, 8, npad, hb, s, d_x, d_z);
+        l = gf_band_solve(d_ab, 8, npad, hb, s, d_x, d_z, d_inv);
         if (l < 0) return l;
         GF_HIP(hipMemcpyAsync(x, d_x, (size_t)n * 8, hipMemcpyDeviceToHost, s));
         GF_HIP(hipMemcpyAsync(&flag, d_flag, 4, hipMemcpyDeviceToHost, s));
@@ -928,7 +1003,7 @@ extern "C" int gf_bandsystem(int n, int hb, const double *ab_rows, const double 
         return GF_OK;
     };
     st = run();
-    void *bufs[] = {d_ab, d_x, d_z, d_flag};
+    void *bufs[] = {d_ab, d_x, d_z, d_inv, d_flag};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (s) (void)hipStreamDestroy(s);

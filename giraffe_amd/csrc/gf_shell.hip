@@ -644,9 +644,11 @@ int gf_shell_assemble(gf_model *m, double lf)
         if (e1 <= e0) continue;
         launches++;
         int grid = (e1 - e0 + SEPB - 1) / SEPB;
+        const int pr = gf_probe_begin(m->stream, GF_KC_SHELL_ASSEMBLE);
         hipLaunchKernelGGL(shell_assemble_kernel<0>, dim3(grid), dim3(SWG), 0, m->stream, e0, e1, ld, m->d_sconn,
                            m->d_sconst, m->d_sgp, m->d_disp, m->d_dof, gf_model_gmat(m), m->d_r, lf, m->gravity[0],
                            m->gravity[1], m->gravity[2], (double *)nullptr, (double *)nullptr);
+        gf_probe_end(m->stream, pr, e1 - e0);
     }
     GF_HIP(hipGetLastError());
     return launches;

@@ -16,6 +16,15 @@ Beam_1 assembly, per element and Newton iteration (FP64 tangent):
     operator and the nine 6x6 node blocks; the dense products as the reference
     writes them are 7.4e4).
 
+Shell_1 assembly, per element and Newton iteration (FP64 tangent):
+    node displacements  6 x 6 x 8         =   288 B
+    Gauss-point state   3 x 24 x 8        =   576 B
+    element constants                     ~   300 B
+    tangent scatter     729 x 8 x 2       = 11664 B
+    residual scatter    27 x 8            =   216 B
+                                            -------
+                                            13044 B
+
 Band factorisation (half-bandwidth b, n unknowns, element size s):
     flops 2 n b^2, bytes 2 n (2b+1) s  (read and write the band once)
 Triangular solves per right-hand side:
@@ -25,6 +34,7 @@ Triangular solves per right-hand side:
 BEAM_BYTES_FP64 = 6108
 BEAM_BYTES_FP32_TANGENT = 144 + 240 + 168 + 84 + 324 * 4 * 2 + 288
 BEAM_FLOPS = 1.5e4
+SHELL_BYTES_FP64 = 288 + 576 + 300 + 729 * 8 * 2 + 27 * 8
 
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E, GB/s
 FP64_PEAK_TFLOPS = 78.6      # dense FP64 (vector and MFMA)

@@ -64,6 +64,8 @@ def load():
     L.gf_reset_state.argtypes = [vp]
     L.gf_timing_enable.argtypes = [vp, C.c_int]
     L.gf_timing_read.argtypes = [vp, dp, C.POINTER(C.c_longlong), C.c_int]
+    L.gf_kernel_probe.argtypes = [vp, C.c_uint]
+    L.gf_kernel_probe_read.argtypes = [vp, dp, C.POINTER(C.c_longlong), dp, C.c_int]
     L.gf_model_footprint.argtypes = [C.POINTER(abi.GfModelDesc), C.POINTER(abi.GfOptions), C.POINTER(C.c_ulonglong)]
     L.gf_default_server_options.argtypes = [C.POINTER(abi.GfServerOptions)]
     L.gf_server_create.argtypes = [C.POINTER(abi.GfServerOptions), C.POINTER(vp)]
@@ -203,6 +205,25 @@ class DeviceModel:
         names = ["assemble", "factor", "solve"]
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(names)}
 
+    KERNEL_CLASSES = ("btd_level", "btd_tail", "band_panel", "band_update", "band_sweep", "beam_assemble",
+                      "shell_assemble")
+
+    def kernel_probe(self, classes):
+        """Bracket every launch of the named kernel classes with HIP events
+        while timing is on (KERNEL_CLASSES; empty to stop)."""
+        mask = 0
+        for c in classes:
+            mask |= 1 << self.KERNEL_CLASSES.index(c)
+        check(load().gf_kernel_probe(self.h, mask), "gf_kernel_probe")
+
+    def kernel_probe_read(self, reset=True):
+        """{class: (ms, launches, units)} for the classes that ran."""
+        k = len(self.KERNEL_CLASSES)
+        ms, units = np.zeros(k), np.zeros(k)
+        n = (C.c_longlong * k)()
+        check(load().gf_kernel_probe_read(self.h, _dp(ms), n, _dp(units), 1 if reset else 0), "gf_kernel_probe_read")
+        return {c: (float(ms[i]), int(n[i]), float(units[i])) for i, c in enumerate(self.KERNEL_CLASSES) if n[i]}
+
     def static_solve(self, params, cap=4096):
         iters = np.zeros(cap, np.int32)
         rep = abi.GfStaticReport()
@@ -302,9 +323,12 @@ class Server:
         rep = abi.GfStaticReport()
         rep.step_iters = iters.ctypes.data_as(C.POINTER(C.c_int))
         rep.step_iters_cap = cap
-        nodes = np.zeros((self.n_nodes.pop(rid), 6))
+        if rid not in self.n_nodes:
+            raise GiraffeError(f"unknown request {rid}")
+        nodes = np.zeros((self.n_nodes[rid], 6))
         times = np.zeros(3)
         check(load().gf_server_take(self.h, rid, C.byref(rep), _dp(nodes), _dp(times)), "gf_server_take")
+        del self.n_nodes[rid]
         return rep, iters[:min(rep.n_step_iters, cap)].copy(), nodes, times
 
     def close(self):

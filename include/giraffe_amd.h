@@ -187,6 +187,22 @@ int gf_timing_enable(gf_model *m, int on);
 /* ms_out[GF_NPHASE] accumulated milliseconds, launches_out[GF_NPHASE] kernel launches. */
 int gf_timing_read(gf_model *m, double *ms_out, long long *launches_out, int reset);
 
+/* Per-kernel timing (no counterpart in the reference; a measurement aid).  While
+ * timing is enabled, every launch of a kernel class whose bit is set in `mask`
+ * is bracketed by HIP events on the model's stream.  Read per class:
+ * accumulated milliseconds, launches, and work units (GF_KC_* comments). */
+#define GF_KC_BTD_LEVEL      0  /* btd_level_kernel; unit: block row eliminated     */
+#define GF_KC_BTD_TAIL       1  /* btd_tail_kernel; unit: block row eliminated      */
+#define GF_KC_BAND_PANEL     2  /* band_panel_kernel; unit: band column eliminated  */
+#define GF_KC_BAND_UPDATE    3  /* band_update_kernel; unit: band column eliminated */
+#define GF_KC_BAND_SWEEP     4  /* band_sweep_kernel; unit: row swept               */
+#define GF_KC_BEAM_ASSEMBLE  5  /* beam_assemble_kernel; unit: element              */
+#define GF_KC_SHELL_ASSEMBLE 6  /* shell_assemble_kernel; unit: element             */
+#define GF_NKC               7
+int gf_kernel_probe(gf_model *m, unsigned mask);
+/* ms_out, launches_out, units_out: GF_NKC entries each (any may be NULL). */
+int gf_kernel_probe_read(gf_model *m, double *ms_out, long long *launches_out, double *units_out, int reset);
+
 /* ---- results --------------------------------------------------------------- */
 /* Per node: X Y Z of the last converged configuration and the Euler rotation
  * vector (Node::WriteResults src/Node.cpp:208-219). */
