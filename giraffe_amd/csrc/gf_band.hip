@@ -75,131 +75,234 @@ template <typename T> __device__ __forceinline__ bool bad_pivot(T v)
 template <typename T> __device__ __forceinline__ T *uinv_of(T *inv, int p) { return inv + (size_t)p * 2 * NB * NB; }
 template <typename T> __device__ __forceinline__ T *linv_of(T *inv, int p) { return inv + (size_t)p * 2 * NB * NB + NB * NB; }
 
-// LU (no pivoting) of the 64 x 64 tile s (leading dimension ld) by 256 threads,
-// each holding a 4 x 4 sub-block in registers; one barrier per pivot; then the
-// triangular inverses on the same blocks.  On return s holds L\U; Uinv / Linv
-// are written row-major to uinv / linv.
-template <typename T>
-__device__ void tile_lu_inv(T *s, int ld, T *rowbuf /* [2][64] */, T *colbuf /* [2][64] */, T *uinv, T *linv,
-                            int *flag)
+// ---- factorisation of one 64 x 64 diagonal block --------------------------------
+// Blocked over 16 x 16 blocks D(a, b) of the LDS tile s (leading dimension ld),
+// by a workgroup of 4 wavefronts:
+//   for kb = 0..3:  wave 0 factors D(kb, kb) in registers (wave_lu16: L\U in
+//                   place, its triangular inverses into li[kb] / ui[kb]);
+//                   L(i, kb) = D(i, kb) Uinv(kb), U(kb, j) = Linv(kb) D(kb, j);
+//                   D(i, j) -= L(i, kb) U(kb, j)            (MFMA, all waves)
+//   then wave j assembles block column j of the 64 x 64 Linv and Uinv by block
+//   substitution (MFMA) and writes both to global memory.
+// Barriers: 3 per kb; the sequential pivot steps run inside one wavefront.
+constexpr int LB = 16, LLD = LB + 1;  // 16 x 16 blocks in LDS with leading dimension 17
+
+template <typename T> __device__ __forceinline__ T rdl(T v, int l);
+template <> __device__ __forceinline__ double rdl<double>(double v, int l)
 {
-    const int tid = threadIdx.x;
-    const int r0 = 4 * (tid >> 4), c0 = 4 * (tid & 15);
-    T a[4][4];
+    int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+template <> __device__ __forceinline__ float rdl<float>(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// One wavefront: LU (no pivoting) of the 16 x 16 block at s (leading dimension
+// ld) in place, with Linv (unit lower) and Uinv written row-major to li / ui
+// (leading dimension LLD).  Lane c of the first 16 holds column c of the block,
+// lanes 16..31 the identity that the forward elimination turns into Linv,
+// lanes 32..47 the identity that the backward substitution turns into Uinv.
+template <typename T> __device__ __forceinline__ void wave_lu16(T *s, int ld, T *li, T *ui, int *flag)
+{
+    const int lane = threadIdx.x & 63, part = lane >> 4, c = lane & 15;
+    T col[LB];
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < LB; i++) {
+        const T d = s[i * ld + c];
+        col[i] = part == 0 ? d : (part == 1 && i == c) ? T(1) : T(0);
+    }
+    T rp[LB];
 #pragma unroll
-        for (int j = 0; j < 4; j++) a[i][j] = s[(r0 + i) * ld + c0 + j];
-    // publish row 0 and column 0
-    if (r0 == 0)
-#pragma unroll
-        for (int j = 0; j < 4; j++) rowbuf[c0 + j] = a[0][j];
-    if (c0 == 0)
-#pragma unroll
-        for (int i = 0; i < 4; i++) colbuf[r0 + i] = a[i][0];
-    __syncthreads();
-    for (int k = 0; k < NB; k++) {
-        const T *rb = rowbuf + (k & 1) * NB, *cb = colbuf + (k & 1) * NB;
-        T piv = rb[k];
-        if (bad_pivot(piv)) {
-            if (tid == 0) atomicExch(flag, 1);
-            piv = T(1);
+    for (int k = 0; k < LB; k++) {
+        T p = rdl<T>(col[k], k);
+        if (bad_pivot(p)) {
+            if (lane == 0) atomicExch(flag, 1);
+            p = T(1);
         }
-        const T rp = T(1) / piv;
-        T l[4], u[4];
+        rp[k] = T(1) / p;
+        T l[LB];
 #pragma unroll
-        for (int i = 0; i < 4; i++) l[i] = cb[r0 + i] * rp;
+        for (int i = k + 1; i < LB; i++) l[i] = rdl<T>(col[i], k) * rp[k];
+        const T ck = col[k];
+        // the row operations act on the columns right of k (left of it the block
+        // holds the multipliers) and on every column of the identity part
+        if ((part == 0 && c > k) || part == 1) {
 #pragma unroll
-        for (int j = 0; j < 4; j++) u[j] = rb[c0 + j];
+            for (int i = k + 1; i < LB; i++) col[i] -= l[i] * ck;
+        }
+        if (lane == k) {
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+            for (int i = k + 1; i < LB; i++) col[i] = l[i];
+        }
+    }
+    if (part == 2) {
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int gi = r0 + i, gj = c0 + j;
-                if (gi > k && gj > k) a[i][j] -= l[i] * u[j];
-                else if (gi > k && gj == k) a[i][j] = l[i];
+        for (int i = 0; i < LB; i++) col[i] = i == c ? T(1) : T(0);
+    }
+#pragma unroll
+    for (int k = LB - 1; k >= 0; k--) {
+        T u[LB];
+#pragma unroll
+        for (int i = 0; i < k; i++) u[i] = rdl<T>(col[i], k);
+        if (part == 2) {
+            const T xk = col[k] * rp[k];
+            col[k] = xk;
+#pragma unroll
+            for (int i = 0; i < k; i++) col[i] -= u[i] * xk;
+        }
+    }
+    if (part < 3) {
+        T *dst = part == 0 ? s : part == 1 ? li : ui;
+        const int ldd = part == 0 ? ld : LLD;
+#pragma unroll
+        for (int i = 0; i < LB; i++) dst[i * ldd + c] = col[i];
+    }
+}
+
+// acc += P Q for 16 x 16 blocks in LDS (one wavefront, 4 MFMA 16x16x4).
+template <typename T>
+__device__ __forceinline__ void mm16(typename mfma<T>::acc_t &acc, const T *P, int ldp, const T *Q, int ldq)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k0 = 0; k0 < LB; k0 += 4) {
+        const T a = P[(lane & 15) * ldp + k0 + (lane >> 4)];
+        const T b = Q[(k0 + (lane >> 4)) * ldq + (lane & 15)];
+        acc = mfma<T>::run(a, b, acc);
+    }
+}
+
+// Stores sign * acc (result layout) as a row-major 16 x 16 block.
+template <typename T>
+__device__ __forceinline__ void st16(const typename mfma<T>::acc_t &acc, T *dst, int ld, T sign)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int v = 0; v < 4; v++) dst[mfma<T>::row(lane, v) * ld + (lane & 15)] = sign * acc[v];
+}
+
+template <typename T> __device__ __forceinline__ void wave_sync_lds()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+template <typename T> struct LuSmem {
+    T li[4][LB * LLD], ui[4][LB * LLD];  // inverses of the 16 x 16 diagonal blocks
+    T ws[4][LB * LLD];                   // per-wave staging block
+};
+
+// s: the 64 x 64 tile (ld) in LDS; uinv / linv: 64 x 64 row-major outputs.
+// All 256 threads call it.
+template <typename T>
+__device__ __forceinline__ void tile_lu_inv(T *s, int ld, LuSmem<T> &w, T *uinv, T *linv, int *flag)
+{
+    typedef typename mfma<T>::acc_t acc_t;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    auto D = [&](int a, int b) { return s + (LB * a) * ld + LB * b; };
+    for (int kb = 0; kb < 4; kb++) {
+        if (wave == 0) wave_lu16<T>(D(kb, kb), ld, w.li[kb], w.ui[kb], flag);
+        __syncthreads();
+        // L(i, kb) = D(i, kb) Uinv(kb) for i > kb; U(kb, j) = Linv(kb) D(kb, j) for j > kb
+        const int nr = 3 - kb;
+        for (int t = wave; t < 2 * nr; t += 4) {
+            acc_t acc = acc_t{};
+            T *dst;
+            if (t < nr) {
+                dst = D(kb + 1 + t, kb);
+                mm16<T>(acc, dst, ld, w.ui[kb], LLD);
+            } else {
+                dst = D(kb, kb + 1 + (t - nr));
+                mm16<T>(acc, w.li[kb], LLD, dst, ld);
             }
-        // publish row k+1 (columns >= k+1) and column k+1 (rows > k+1) for the next pivot
-        const int k1 = k + 1;
-        if (k1 < NB) {
-            T *rbn = rowbuf + (k1 & 1) * NB, *cbn = colbuf + (k1 & 1) * NB;
+            wave_sync_lds<T>();
+            st16<T>(acc, dst, ld, T(1));
+        }
+        __syncthreads();
+        // D(i, j) -= L(i, kb) U(kb, j)
+        for (int t = wave; t < nr * nr; t += 4) {
+            const int i = kb + 1 + t / nr, j = kb + 1 + t % nr;
+            acc_t acc = acc_t{};
+            mm16<T>(acc, D(i, kb), ld, D(kb, j), ld);
+            T *dst = D(i, j);
 #pragma unroll
-            for (int i = 0; i < 4; i++)
-                if (r0 + i == k1)
-#pragma unroll
-                    for (int j = 0; j < 4; j++) rbn[c0 + j] = a[i][j];
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-                if (c0 + j == k1)
-#pragma unroll
-                    for (int i = 0; i < 4; i++) cbn[r0 + i] = a[i][j];
+            for (int v = 0; v < 4; v++) {
+                T &e = dst[mfma<T>::row(lane, v) * ld + (lane & 15)];
+                e -= acc[v];
+            }
         }
         __syncthreads();
     }
+    // Block column j = wave of Linv (forward) and Uinv (backward) substitution.
+    const int j = wave;
+    T *ws = w.ws[j];
+    auto put = [&](T *out, int bi, const acc_t &acc, T sign) {
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+        for (int v = 0; v < 4; v++)
+            out[(LB * bi + mfma<T>::row(lane, v)) * NB + LB * j + (lane & 15)] = sign * acc[v];
+    };
+    auto put_lds = [&](T *out, int bi, const T *src) {  // copy a 16 x 16 LDS block (LLD) to the output
+        for (int t = lane; t < LB * LB; t += 64)
+            out[(LB * bi + t / LB) * NB + LB * j + t % LB] = src[(t / LB) * LLD + t % LB];
+    };
+    auto zero = [&](T *out, int bi) {
+        for (int t = lane; t < LB * LB; t += 64) out[(LB * bi + t / LB) * NB + LB * j + t % LB] = T(0);
+    };
+    // Linv(:, j): X(j) = Linv(j); X(i) = -Linv(i) sum_{k=j}^{i-1} L(i, k) X(k), i > j
+    // (loops unrolled over the block indices; j is wave-uniform)
+    {
+        for (int i = 0; i < j; i++) zero(linv, i);
+        put_lds(linv, j, w.li[j]);
+        acc_t acc[4];
 #pragma unroll
-        for (int j = 0; j < 4; j++) s[(r0 + i) * ld + c0 + j] = a[i][j];
-    __syncthreads();
-    // Inverses by column-oriented substitution on the same 4 x 4 register blocks,
-    // one barrier per step: Uinv pivots descend (x_k = b_k / U_kk, b_i -= U_ik x_k,
-    // i < k), Linv pivots ascend (x_k = b_k, b_i -= L_ik x_k, i > k).  The owners
-    // of the pivot row publish it; rowbuf / colbuf are reused as the two buffers.
-    T bu[4][4], bl[4][4];
+        for (int i = 0; i < 4; i++) acc[i] = acc_t{};
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+        for (int k = 0; k < 3; k++) {
+            if (k < j) continue;
+            const T *xk = k == j ? w.li[j] : ws;  // X(k) as a product operand
 #pragma unroll
-        for (int j = 0; j < 4; j++) bu[i][j] = bl[i][j] = (r0 + i == c0 + j) ? T(1) : T(0);
-    for (int step = 0; step < NB; step++) {
-        const int ku = NB - 1 - step, kl = step;
-        T *xu = rowbuf + (step & 1) * NB, *xl = colbuf + (step & 1) * NB;
-        if (ku >= r0 && ku < r0 + 4) {
-            T d = s[ku * ld + ku];
-            if (bad_pivot(d)) d = T(1);
-            const T rd = T(1) / d;
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-                if (r0 + i == ku)
-#pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        bu[i][j] *= rd;
-                        xu[c0 + j] = bu[i][j];
-                    }
+            for (int i = k + 1; i < 4; i++) mm16<T>(acc[i], D(i, k), ld, xk, LLD);
+            // X(k+1) = -Linv(k+1) acc[k+1]
+            wave_sync_lds<T>();
+            st16<T>(acc[k + 1], ws, LLD, T(1));
+            wave_sync_lds<T>();
+            acc_t x = acc_t{};
+            mm16<T>(x, w.li[k + 1], LLD, ws, LLD);
+            put(linv, k + 1, x, T(-1));
+            wave_sync_lds<T>();
+            st16<T>(x, ws, LLD, T(-1));
+            wave_sync_lds<T>();
         }
-        if (kl >= r0 && kl < r0 + 4) {
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-                if (r0 + i == kl)
-#pragma unroll
-                    for (int j = 0; j < 4; j++) xl[c0 + j] = bl[i][j];
-        }
-        __syncthreads();
-        T uk[4], lk[4], xuv[4], xlv[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            uk[i] = s[(r0 + i) * ld + ku];
-            lk[i] = s[(r0 + i) * ld + kl];
-        }
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            xuv[j] = xu[c0 + j];
-            xlv[j] = xl[c0 + j];
-        }
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                if (r0 + i < ku) bu[i][j] -= uk[i] * xuv[j];
-                if (r0 + i > kl) bl[i][j] -= lk[i] * xlv[j];
-            }
     }
+    wave_sync_lds<T>();
+    // Uinv(:, j): X(j) = Uinv(j); X(i) = -Uinv(i) sum_{k=i+1}^{j} U(i, k) X(k), i < j
+    {
+        for (int i = j + 1; i < 4; i++) zero(uinv, i);
+        put_lds(uinv, j, w.ui[j]);
+        acc_t acc[4];
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+        for (int i = 0; i < 4; i++) acc[i] = acc_t{};
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            uinv[(r0 + i) * NB + c0 + j] = bu[i][j];
-            linv[(r0 + i) * NB + c0 + j] = bl[i][j];
+        for (int k = 3; k > 0; k--) {
+            if (k > j) continue;
+            const T *xk = k == j ? w.ui[j] : ws;
+#pragma unroll
+            for (int i = 0; i < k; i++) mm16<T>(acc[i], D(i, k), ld, xk, LLD);
+            wave_sync_lds<T>();
+            st16<T>(acc[k - 1], ws, LLD, T(1));
+            wave_sync_lds<T>();
+            acc_t x = acc_t{};
+            mm16<T>(x, w.ui[k - 1], LLD, ws, LLD);
+            put(uinv, k - 1, x, T(-1));
+            wave_sync_lds<T>();
+            st16<T>(x, ws, LLD, T(-1));
+            wave_sync_lds<T>();
         }
+    }
+    __syncthreads();
 }
 
 // 64 x 64 x 64 product on 256 threads (4 waves, 2 x 2 of 32 x 32):
@@ -244,14 +347,17 @@ __device__ __forceinline__ void tile_mm(typename mfma<T>::acc_t (&acc)[2][2], T 
 
 template <typename T> struct TileSmem {
     T ab[NB * (KT + 1) + KT * NB];  // staging of tile_mm; after it, the LU tile [64][65]
-    T rowbuf[2 * NB], colbuf[2 * NB];
+};
+template <typename T> struct DiagSmem {
+    T ab[NB * (KT + 1) + KT * NB];
+    LuSmem<T> lu;
 };
 
 // First diagonal block: LU in place and its inverses (panel 0).
 template <typename T>
 __global__ __launch_bounds__(256) void band_diag_kernel(T *ab, int hb, int n, T *inv, int *flag)
 {
-    __shared__ TileSmem<T> sm;
+    __shared__ DiagSmem<T> sm;
     T *s = sm.ab;
     T v[NB * NB / 256];
 #pragma unroll
@@ -265,7 +371,7 @@ __global__ __launch_bounds__(256) void band_diag_kernel(T *ab, int hb, int n, T 
         s[(t / NB) * (NB + 1) + t % NB] = v[r];
     }
     __syncthreads();
-    tile_lu_inv<T>(s, NB + 1, sm.rowbuf, sm.colbuf, uinv_of(inv, 0), linv_of(inv, 0), flag);
+    tile_lu_inv<T>(s, NB + 1, sm.lu, uinv_of(inv, 0), linv_of(inv, 0), flag);
     __syncthreads();
     for (int t = threadIdx.x; t < NB * NB; t += 256) {
         int i = t / NB, c = t % NB;
@@ -318,7 +424,7 @@ template <typename T> __global__ __launch_bounds__(256) void band_update_kernel(
                                                                                 T *inv, int *flag)
 {
     typedef typename mfma<T>::acc_t acc_t;
-    __shared__ TileSmem<T> sm;
+    __shared__ DiagSmem<T> sm;
     T *sa = sm.ab, *sb = sm.ab + NB * (KT + 1), *sc = sm.ab;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int k0 = p * NB, base = k0 + NB, lim = min(n, k0 + NB + hb);
@@ -363,7 +469,7 @@ template <typename T> __global__ __launch_bounds__(256) void band_update_kernel(
             }
     if (!fuse) return;
     __syncthreads();
-    tile_lu_inv<T>(sc, NB + 1, sm.rowbuf, sm.colbuf, uinv_of(inv, p + 1), linv_of(inv, p + 1), flag);
+    tile_lu_inv<T>(sc, NB + 1, sm.lu, uinv_of(inv, p + 1), linv_of(inv, p + 1), flag);
     __syncthreads();
     for (int t = tid; t < NB * NB; t += 256) {
         int i = t / NB, j = t % NB;
@@ -393,15 +499,31 @@ __global__ __launch_bounds__(1024) void band_sweep_kernel(const T *ab, int hb, i
         V acc[4] = {V(0), V(0), V(0), V(0)};
         const int p_first = FWD ? max(0, r - reach) : r + 1;
         const int p_last = FWD ? r - 1 : min(nbk - 1, r + reach);
-#pragma unroll 8
-        for (int p = p_first; p <= p_last; p++) {
+        // Row pointers: entry (i, c) is rowp[q][c].  For 0 <= c < n the address
+        // lies inside the band storage even where (i, c) is outside the band, so
+        // only the two coupled blocks at the band's edge need the mask.
+        const T *rowp[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) rowp[q] = ab + (size_t)(i0 + 4 * wave + q) * (size_t)(2 * hb) + hb;
+        const int m_lo = FWD ? p_first : max(p_first, p_last - 1);      // masked blocks [m_lo, m_hi]
+        const int m_hi = FWD ? min(p_last, p_first + 1) : p_last;
+        const int u_lo = FWD ? m_hi + 1 : p_first;                      // unmasked blocks [u_lo, u_hi]
+        const int u_hi = FWD ? p_last : m_lo - 1;
+        for (int p = m_lo; p <= m_hi; p++) {
             const int c = p * NB + lane;
             const V yc = y[c];
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const int i = i0 + 4 * wave + q;
-                acc[q] += (V)band_ld(ab, i, c, hb, n, true) * yc;
+                const T v = rowp[q][c];
+                acc[q] += inband(i0 + 4 * wave + q, c, hb) ? (V)v * yc : V(0);
             }
+        }
+#pragma unroll 8
+        for (int p = u_lo; p <= u_hi; p++) {
+            const int c = p * NB + lane;
+            const V yc = y[c];
+#pragma unroll
+            for (int q = 0; q < 4; q++) acc[q] += (V)rowp[q][c] * yc;
         }
 #pragma unroll
         for (int q = 0; q < 4; q++) {
