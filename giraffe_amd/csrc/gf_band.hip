@@ -479,72 +479,81 @@ template <typename T> __global__ __launch_bounds__(256) void band_update_kernel(
 }
 
 // ---- triangular solves ------------------------------------------------------------
-// One workgroup of 1024 lanes per sweep, blocks of 64 rows in order (forward:
-// L y = b with unit L; backward: U x = y).  Block r subtracts the couplings of
-// the blocks it reaches inside the band (wave w owns rows 4w..4w+3, lanes run
-// along the coupled block's 64 columns, so every load is a contiguous 512-byte
-// row segment) and then applies its stored 64 x 64 inverse.  Blocks depend on
-// the previous ones only through this workgroup's own writes, so barriers
-// are the only synchronisation.
-template <typename T, typename V, bool FWD>
-__global__ __launch_bounds__(1024) void band_sweep_kernel(const T *ab, int hb, int n, const T *inv, const V *x, V *y)
+// Right-looking block sweeps, one launch per batch of K consecutive 64-row
+// blocks (forward: L y = b with unit L, blocks ascending; backward: U x = y,
+// blocks descending).  The right-hand side f is updated in place: when a batch
+// starts, every contribution of the blocks before it has been subtracted.
+// Every workgroup of the launch solves the batch itself from f and the stored
+// inverses (K inverse and K(K-1)/2 coupling products, the same data for all
+// workgroups, so it comes from L2); workgroup 0 writes the batch's solution and
+// workgroup g >= 1 subtracts the batch's couplings from block g past it.  No
+// workgroup reads what another workgroup of the same launch writes, so the
+// stream order of the launches is the only synchronisation, and the coupling
+// strips are streamed by as many CUs as the band reaches blocks.
+template <typename T, typename V>
+__device__ __forceinline__ V mv_part(const T *row, int c0, const V *v)  // 16 products of one row quarter
 {
-    __shared__ V s_v[NB];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;  // 16 waves
-    const int nbk = n / NB;
-    const int reach = (hb + NB - 1) / NB;
-    for (int t = 0; t < nbk; t++) {
-        const int r = FWD ? t : nbk - 1 - t;
-        const int i0 = r * NB;
-        V acc[4] = {V(0), V(0), V(0), V(0)};
-        const int p_first = FWD ? max(0, r - reach) : r + 1;
-        const int p_last = FWD ? r - 1 : min(nbk - 1, r + reach);
-        // Row pointers: entry (i, c) is rowp[q][c].  For 0 <= c < n the address
-        // lies inside the band storage even where (i, c) is outside the band, so
-        // only the two coupled blocks at the band's edge need the mask.
-        const T *rowp[4];
+    T a[16];
 #pragma unroll
-        for (int q = 0; q < 4; q++) rowp[q] = ab + (size_t)(i0 + 4 * wave + q) * (size_t)(2 * hb) + hb;
-        const int m_lo = FWD ? p_first : max(p_first, p_last - 1);      // masked blocks [m_lo, m_hi]
-        const int m_hi = FWD ? min(p_last, p_first + 1) : p_last;
-        const int u_lo = FWD ? m_hi + 1 : p_first;                      // unmasked blocks [u_lo, u_hi]
-        const int u_hi = FWD ? p_last : m_lo - 1;
-        for (int p = m_lo; p <= m_hi; p++) {
-            const int c = p * NB + lane;
-            const V yc = y[c];
+    for (int j = 0; j < 16; j++) a[j] = row[c0 + j];
+    V s = V(0);
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const T v = rowp[q][c];
-                acc[q] += inband(i0 + 4 * wave + q, c, hb) ? (V)v * yc : V(0);
-            }
-        }
-#pragma unroll 8
-        for (int p = u_lo; p <= u_hi; p++) {
-            const int c = p * NB + lane;
-            const V yc = y[c];
+    for (int j = 0; j < 16; j++) s += (V)a[j] * v[c0 + j];
+    return s;
+}
+
+template <typename V> __device__ __forceinline__ V quad_sum(V s)
+{
+    s += __shfl_xor(s, 1);
+    s += __shfl_xor(s, 2);
+    return s;
+}
+
+// Row i = tid / 4 of the 64 x 64 block coupling band rows R0.. to band columns
+// C0.., times v (LDS): the sum over the quad (masked to the band).
+template <typename T, typename V>
+__device__ __forceinline__ V coupling_row(const T *ab, int hb, int R0, int C0, const V *v)
+{
+    const int tid = threadIdx.x, i = tid >> 2, c0 = 16 * (tid & 3);
+    const T *rowp = ab + (size_t)(R0 + i) * (size_t)(2 * hb) + hb + C0;  // entry (R0+i, C0+c) at rowp[c]
+    T a[16];
 #pragma unroll
-            for (int q = 0; q < 4; q++) acc[q] += (V)rowp[q][c] * yc;
-        }
+    for (int j = 0; j < 16; j++) a[j] = rowp[c0 + j];
+    V s = V(0);
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            V sum = acc[q];
-            for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
-            if (lane == q) s_v[4 * wave + q] = x[i0 + 4 * wave + q] - sum;
-        }
+    for (int j = 0; j < 16; j++) s += inband(R0 + i, C0 + c0 + j, hb) ? (V)a[j] * v[c0 + j] : V(0);
+    return quad_sum(s);
+}
+
+template <typename T, typename V, bool FWD, int K>
+__global__ __launch_bounds__(256) void band_sweep_kernel(const T *ab, int hb, int n, const T *inv, int r0, int kc,
+                                                         V *f, V *out)
+{
+    __shared__ V fb[K][NB], ys[K][NB];
+    const int tid = threadIdx.x, i = tid >> 2, c0 = 16 * (tid & 3);
+    auto blk = [&](int k) { return FWD ? r0 + k : r0 - k; };
+    for (int t = tid; t < kc * NB; t += 256) fb[t / NB][t % NB] = f[(size_t)blk(t / NB) * NB + t % NB];
+    __syncthreads();
+    for (int k = 0; k < kc; k++) {
+        const int b = blk(k);
+        const T *Ti = FWD ? linv_of(const_cast<T *>(inv), b) : uinv_of(const_cast<T *>(inv), b);
+        const V yv = quad_sum(mv_part<T, V>(Ti + i * NB, c0, fb[k]));
+        if ((tid & 3) == 0) ys[k][i] = yv;
         __syncthreads();
-        const T *Ti = FWD ? linv_of(const_cast<T *>(inv), r) : uinv_of(const_cast<T *>(inv), r);
-        const V vl = s_v[lane];
-        V res = V(0);
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int i = 4 * wave + q;
-            V sum = (V)Ti[i * NB + lane] * vl;
-            for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
-            if (lane == q) res = sum;
+        for (int k2 = k + 1; k2 < kc; k2++) {
+            const V cv = coupling_row<T, V>(ab, hb, blk(k2) * NB, b * NB, ys[k]);
+            if ((tid & 3) == 0) fb[k2][i] -= cv;
         }
-        if (lane < 4) y[i0 + 4 * wave + lane] = res;
         __syncthreads();
     }
+    if (blockIdx.x == 0) {
+        for (int t = tid; t < kc * NB; t += 256) out[(size_t)blk(t / NB) * NB + t % NB] = ys[t / NB][t % NB];
+        return;
+    }
+    const int q = FWD ? r0 + kc - 1 + (int)blockIdx.x : r0 - kc + 1 - (int)blockIdx.x;  // target block
+    V acc = V(0);
+    for (int k = 0; k < kc; k++) acc += coupling_row<T, V>(ab, hb, q * NB, blk(k) * NB, ys[k]);
+    if ((tid & 3) == 0) f[(size_t)q * NB + i] -= acc;
 }
 
 }  // namespace
@@ -573,17 +582,35 @@ template <typename T> static int band_factor_t(T *ab, int n, int hb, hipStream_t
     return launches;
 }
 
+constexpr int SWEEP_K = 2;  // blocks per sweep launch
+
 template <typename T, typename V>
 static int band_solve_t(const T *ab, int n, int hb, hipStream_t s, V *x, V *z, const T *inv)
 {
-    int pr = gf_probe_begin(s, GF_KC_BAND_SWEEP);
-    hipLaunchKernelGGL((band_sweep_kernel<T, V, true>), dim3(1), dim3(1024), 0, s, ab, hb, n, inv, (const V *)x, z);
-    gf_probe_end(s, pr, n);
-    pr = gf_probe_begin(s, GF_KC_BAND_SWEEP);
-    hipLaunchKernelGGL((band_sweep_kernel<T, V, false>), dim3(1), dim3(1024), 0, s, ab, hb, n, inv, (const V *)z, x);
-    gf_probe_end(s, pr, n);
+    const int nbk = n / NB, reach = (hb + NB - 1) / NB;
+    int launches = 0;
+    // forward: L z = x, x updated in place as the right-hand side
+    for (int r0 = 0; r0 < nbk; r0 += SWEEP_K) {
+        const int kc = std::min(SWEEP_K, nbk - r0), last = r0 + kc - 1;
+        const int g = 1 + std::min(reach, nbk - 1 - last);
+        const int pr = gf_probe_begin(s, GF_KC_BAND_SWEEP);
+        hipLaunchKernelGGL((band_sweep_kernel<T, V, true, SWEEP_K>), dim3(g), dim3(256), 0, s, ab, hb, n, inv, r0, kc,
+                           x, z);
+        gf_probe_end(s, pr, kc * NB);
+        launches++;
+    }
+    // backward: U x = z, z updated in place
+    for (int r0 = nbk - 1; r0 >= 0; r0 -= SWEEP_K) {
+        const int kc = std::min(SWEEP_K, r0 + 1), last = r0 - kc + 1;
+        const int g = 1 + std::min(reach, last);
+        const int pr = gf_probe_begin(s, GF_KC_BAND_SWEEP);
+        hipLaunchKernelGGL((band_sweep_kernel<T, V, false, SWEEP_K>), dim3(g), dim3(256), 0, s, ab, hb, n, inv, r0,
+                           kc, z, x);
+        gf_probe_end(s, pr, kc * NB);
+        launches++;
+    }
     GF_HIP(hipGetLastError());
-    return 2;
+    return launches;
 }
 
 size_t gf_band_inv_elems(int n) { return (size_t)(n / NB) * 2 * NB * NB; }

@@ -5,7 +5,9 @@
 // MountFieldLoading (gravity) :838-878, MountGlobal :1429-1492,
 // SaveLagrange :1494-1506, PreCalc :501-692.
 //
-// Work split per workgroup of 256 lanes and EPB = 28 elements:
+// Work split per workgroup of one wavefront (64 lanes) and EPB = 7 elements
+// (a colour of 5,000 elements is 715 workgroups, so every CU holds several and
+// their latencies overlap):
 //   phase 1 -- one lane per (element, Gauss point): Rodrigues kinematics,
 //              back-rotated strains, stresses and the 9x9 operator
 //              M = B^T D B + G (block-sparse: 7 non-zero 3x3 blocks) plus
@@ -23,8 +25,8 @@
 
 namespace {
 
-constexpr int EPB = 28;      // elements per workgroup
-constexpr int WG = 256;
+constexpr int EPB = 7;       // elements per workgroup
+constexpr int WG = 64;
 constexpr int LDS_GP = 72;   // 7 blocks * 9 + 9 (B^T sigma)
 
 struct BeamGP {

@@ -133,13 +133,16 @@ template <typename T, int MT> __device__ void wave_gj(T (&col)[cfg<MT>::S][MT], 
 #pragma unroll
         for (int i = 0; i < MT; i++) mk[i] = rdl(col[0][i], k);  // column k sits in lane k, slot 0
         T amax = T(0);
-        int p = k;
 #pragma unroll
-        for (int i = k; i < MT; i++) {
-            T a = fabs(mk[i]);
-            if (a > amax) { amax = a; p = i; }
-        }
-        if (fabs(mk[k]) < T(0.5) * amax) {
+        for (int i = k; i < MT; i++) amax = fmax(amax, fabs(mk[i]));
+        if (fabs(mk[k]) < T(0.5) * amax) {  // wave-uniform; rarely taken
+            int p = k;
+            T best = T(0);
+#pragma unroll
+            for (int i = k; i < MT; i++) {
+                const T a = fabs(mk[i]);
+                if (a > best) { best = a; p = i; }
+            }
 #pragma unroll
             for (int s = 0; s < S; s++) {
                 T ck = col[s][k], cp = ck;

@@ -86,6 +86,36 @@ __device__ __forceinline__ double gf_table_value(const double *tt, const double 
     return Vv[6 * (idx - 1) + col] + fac * (Vv[6 * idx + col] - Vv[6 * (idx - 1) + col]);
 }
 
+// All six columns of the same table at time t: one segment search, the same
+// per-column arithmetic as gf_table_value.
+__device__ __forceinline__ void gf_table_values6(const double *tt, const double *tv, int r0, int nt, double t,
+                                                 double (&out)[6])
+{
+    if (nt == 0) {
+#pragma unroll
+        for (int c = 0; c < 6; c++) out[c] = 0.0;
+        return;
+    }
+    const double *T = tt + r0;
+    const double *Vv = tv + 6 * (size_t)r0;
+    int lo, hi;
+    double fac;
+    if (t >= T[nt - 1]) { lo = hi = nt - 1; fac = -1.0; }
+    else if (t <= T[0]) { lo = hi = 0; fac = -1.0; }
+    else {
+        int idx = 0;
+        while (t >= T[idx]) idx++;
+        lo = idx - 1;
+        hi = idx;
+        fac = (t - T[idx - 1]) / (T[idx] - T[idx - 1]);
+    }
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        const double a = Vv[6 * lo + c], b = Vv[6 * hi + c];
+        out[c] = fac < 0.0 ? a : a + fac * (b - a);
+    }
+}
+
 // ---- Shell_1 constant record ([SC_COUNT][ns]) and Gauss-point state ([SG_COUNT][ns]) ----
 enum {
     SC_T3 = 0,        // 9: rows e1r, e2r, e3r

@@ -26,10 +26,9 @@ __global__ void fill_u32_kernel(unsigned *p, size_t n)
     for (; i < n; i += stride) p[i] = 0u;
 }
 
-#define table_value gf_table_value
 
 // One thread per loaded node: all loads on that node, in load order.
-__global__ void nodal_loads_kernel(int n_ln, const int *__restrict__ ln_node, const int *__restrict__ ln_off,
+__global__ __launch_bounds__(128) void nodal_loads_kernel(int n_ln, const int *__restrict__ ln_node, const int *__restrict__ ln_off,
                                    const int *__restrict__ ln_ids, const double *__restrict__ rec,
                                    const double *__restrict__ tab_t, const double *__restrict__ tab_v,
                                    const double *__restrict__ disp, const double *__restrict__ copy,
@@ -49,12 +48,10 @@ __global__ void nodal_loads_kernel(int n_ln, const int *__restrict__ ln_node, co
 #pragma unroll
         for (int i = 0; i < 9; i++) CS.a[i] = L[NL_CS + i];
         int r0 = (int)L[NL_TAB0], nt = (int)L[NL_TABN];
-        v3<double> f = mk3(L[NL_MULT + 0] * table_value(tab_t, tab_v, r0, nt, t, 0),
-                           L[NL_MULT + 1] * table_value(tab_t, tab_v, r0, nt, t, 1),
-                           L[NL_MULT + 2] * table_value(tab_t, tab_v, r0, nt, t, 2));
-        v3<double> mo = mk3(L[NL_MULT + 3] * table_value(tab_t, tab_v, r0, nt, t, 3),
-                            L[NL_MULT + 4] * table_value(tab_t, tab_v, r0, nt, t, 4),
-                            L[NL_MULT + 5] * table_value(tab_t, tab_v, r0, nt, t, 5));
+        double tv6[6];
+        gf_table_values6(tab_t, tab_v, r0, nt, t, tv6);
+        v3<double> f = mk3(L[NL_MULT + 0] * tv6[0], L[NL_MULT + 1] * tv6[1], L[NL_MULT + 2] * tv6[2]);
+        v3<double> mo = mk3(L[NL_MULT + 3] * tv6[3], L[NL_MULT + 4] * tv6[4], L[NL_MULT + 5] * tv6[5]);
         if (kind == GF_LOAD_NODAL) {
             // dead force + pseudo-moment Xi^T m with stiffness -V(alpha, Xi^T m)
             f = mtv(CS, f);
