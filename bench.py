@@ -287,7 +287,8 @@ def serve(args):
         "data": "synthetic (riser request stream, seed 1004, giraffe_amd/model.py)",
         "config": {"workload": f"serving stream: {len(reqs)} Beam_1 riser requests (E log-U[50,5000]), Poisson "
                                f"arrivals at {args.load:.2f} x closed-loop capacity, {args.max_inflight} resident",
-                   "precision": args.precision, "arrival_rate_per_s": rate, "closed_loop_capacity": cap},
+                   "precision": args.precision, "arrival_rate_per_s": rate, "closed_loop_capacity": cap,
+                   "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))},
         "p50_ms": float(np.percentile(lat_ms, 50)),
         "p99_ms": float(np.percentile(lat_ms, 99)),
         "aborted": aborted,
@@ -330,6 +331,9 @@ def main():
     ap.add_argument("--requests", type=int, default=2000, help="serve: requests in the stream")
     ap.add_argument("--max-inflight", type=int, default=16, help="serve: resident requests")
     ap.add_argument("--load", type=float, default=0.8, help="serve: arrival rate / closed-loop capacity")
+    ap.add_argument("--hw-queues", type=int, default=16,
+                    help="serve: HIP hardware queues for the request streams (GPU_MAX_HW_QUEUES, at most 32; "
+                         "0 keeps the environment's setting)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--compare-steps", type=int, default=2,
                     help="solves of the other precision, reported beside the main figure (0: skip)")
@@ -344,6 +348,10 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(requests shard across GPUs with no collectives; see DESIGN.md).", file=sys.stderr)
         return 2
+    if args.workload == "serve" and args.hw_queues > 0:
+        # one stream per resident request; HIP spreads streams over this many
+        # hardware queues (its default is 4), set before the runtime starts
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, args.hw_queues))
     log("importing torch")
     import torch
 

@@ -68,7 +68,8 @@ template <typename T> __device__ __forceinline__ T band_ld(const T *ab, int i, i
 
 template <typename T> __device__ __forceinline__ bool bad_pivot(T v)
 {
-    return !(v == v) || v == T(0) || fabs((double)v) >= 1e300;
+    const double a = fabs((double)v);
+    return !(v == v) || a >= 1e300 || a < (sizeof(T) == 4 ? 1e-37 : 1e-300);
 }
 
 // Per-panel inverses: inv + p * 2 * NB * NB holds Uinv (row-major), then Linv (unit lower).
@@ -121,7 +122,7 @@ template <typename T> __device__ __forceinline__ void wave_lu16(T *s, int ld, T 
             if (lane == 0) atomicExch(flag, 1);
             p = T(1);
         }
-        rp[k] = T(1) / p;
+        rp[k] = pivot_rcp(p);
         T l[LB];
 #pragma unroll
         for (int i = k + 1; i < LB; i++) l[i] = rdl<T>(col[i], k) * rp[k];

@@ -207,6 +207,8 @@ __global__ __launch_bounds__(WG) void beam_assemble_kernel(
                 int rr[6], cc[6];
 #pragma unroll
                 for (int k = 0; k < 6; k++) { rr[k] = dof[6 * na + k]; cc[k] = dof[6 * nbn + k]; }
+                int R36[36], C36[36];
+                double V36[36];
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
                     int ro = (q >> 1) * 3, co = (q & 1) * 3;
@@ -214,9 +216,13 @@ __global__ __launch_bounds__(WG) void beam_assemble_kernel(
                     for (int i = 0; i < 3; i++)
 #pragma unroll
                         for (int j = 0; j < 3; j++) {
-                            gm_add(gm, rr[ro + i], cc[co + j], blk[q].a[3 * i + j]);
+                            const int k = 9 * q + 3 * i + j;
+                            R36[k] = rr[ro + i];
+                            C36[k] = cc[co + j];
+                            V36[k] = blk[q].a[3 * i + j];
                         }
                 }
+                gm_add_n<36>(gm, R36, C36, V36);
             } else {
                 double *Ke = Kout + (size_t)e * 324;
 #pragma unroll
@@ -257,11 +263,14 @@ __global__ __launch_bounds__(WG) void beam_assemble_kernel(
                 double mult = lf * w * cst[BC_RHOA * ld + e] * Nsum;
                 P[0] -= mult * gx; P[1] -= mult * gy; P[2] -= mult * gz;
                 int na = conn[a * ld + e];
+                int R6[6];
+                double V6[6];
 #pragma unroll
                 for (int k = 0; k < 6; k++) {
-                    int R = dof[6 * na + k];
-                    if (R >= 0) r[R] -= P[k];
+                    R6[k] = dof[6 * na + k];
+                    V6[k] = -P[k];
                 }
+                vec_add_n<6>(r, R6, V6);
             } else {
 #pragma unroll
                 for (int k = 0; k < 6; k++) fout[(size_t)e * 18 + 6 * a + k] = P[k];

@@ -76,7 +76,7 @@ __device__ __forceinline__ int lvl_row(int p, int L) { return ((p + 1) << L) - 1
 template <typename T> __device__ __forceinline__ bool bad(T v)
 {
     double a = fabs((double)v);
-    return !(a > 0.0) || a >= 1e300 || !(v == v) || (sizeof(T) == 4 && a >= 3.0e38);
+    return !(a >= (sizeof(T) == 4 ? 1e-37 : 1e-300)) || a >= 1e300 || !(v == v) || (sizeof(T) == 4 && a >= 3.0e38);
 }
 
 template <int MT> struct cfg {
@@ -164,7 +164,7 @@ template <typename T, int MT> __device__ void wave_gj(T (&col)[cfg<MT>::S][MT], 
             if ((threadIdx.x & 63) == 0) atomicExch(flag, 1);
             piv = T(1);
         }
-        const T inv = T(1) / piv;
+        const T inv = pivot_rcp(piv);
 #pragma unroll
         for (int s = 0; s < S; s++) {
             const T rk = col[s][k] * inv;
@@ -180,7 +180,7 @@ template <typename T, int MT> __device__ void wave_gj(T (&col)[cfg<MT>::S][MT], 
 // and, when `to_global`, into Bi/X/Y of block row e.
 template <typename T, int MT>
 __device__ __forceinline__ void store_elim(const T (&col)[cfg<MT>::S][MT], T *sE, T *Bi, T *X, T *Y, int e, int m,
-                                           bool to_global)
+                                           bool to_global, bool keep_bi)
 {
     const int lane = threadIdx.x & 63;
     const size_t bs = (size_t)m * m, o = (size_t)e * bs;
@@ -193,7 +193,7 @@ __device__ __forceinline__ void store_elim(const T (&col)[cfg<MT>::S][MT], T *sE
 #pragma unroll
         for (int i = 0; i < MT; i++) {
             dst[i * cfg<MT>::LD + cc] = col[s][i];
-            if (to_global && i < m && cc < m) g[o + (size_t)i * m + cc] = col[s][i];
+            if (to_global && (part != 1 || keep_bi) && i < m && cc < m) g[o + (size_t)i * m + cc] = col[s][i];
         }
     }
 }
@@ -282,6 +282,9 @@ __device__ void fac_step(const Fac &F, PairSmem<T, MT> &sm, bool active, int L, 
     const bool doE = active && (wv == 0 || hasR);
     const int e = wv == 0 ? l : r, pe = wv == 0 ? p - 1 : p + 1;
     const bool own = wv == 0 || p + 2 == nL;  // the right neighbour is written here only if it is the last row
+    // B^-1 and the kept-row couplings are stored only for later solve-only sweeps;
+    // a factorisation that reduces its right-hand side (f) needs X and Y alone.
+    const bool keep = f == nullptr;
 
     // ---- reads
     // (unconditional loads from clamped in-block addresses, selected afterwards;
@@ -339,14 +342,14 @@ __device__ void fac_step(const Fac &F, PairSmem<T, MT> &sm, bool active, int L, 
         for (int jv = 0; jv < KV; jv++) {
             const int t = lane + 64 * jv, i = t / MT, jj = t % MT;
             dst[i * cfg<MT>::LD + jj] = kv[jv];
-            if (i < m && jj < m) sv[(size_t)i * m + jj] = kv[jv];
+            if (keep && i < m && jj < m) sv[(size_t)i * m + jj] = kv[jv];
         }
         if (f && lane < MT) sm.z[wv][lane] = T(fe);
     }
     // ---- eliminated neighbour
     if (doE) {
         wave_gj<T, MT>(col, F.flag, m);
-        store_elim<T, MT>(col, &sm.e[wv][0][0], (T *)F.Bi, (T *)F.X, (T *)F.Y, e, m, own);
+        store_elim<T, MT>(col, &sm.e[wv][0][0], (T *)F.Bi, (T *)F.X, (T *)F.Y, e, m, own, keep);
     } else if (active) {
         // no right neighbour: zero X_r, Y_r so the products below need no special case
         for (int t = lane; t < 3 * cfg<MT>::BLK; t += 64) (&sm.e[1][0][0])[t] = T(0);
@@ -415,7 +418,7 @@ __device__ void top_step(const Fac &F, PairSmem<T, MT> &sm, int levels, const do
         T col[cfg<MT>::S][MT];
         load_aug<T, MT>(col, (const T *)F.A, (const T *)F.B, (const T *)F.C, e, m, false, false);
         wave_gj<T, MT>(col, F.flag, m);
-        store_elim<T, MT>(col, &sm.e[0][0][0], (T *)F.Bi, (T *)F.X, (T *)F.Y, e, m, true);
+        store_elim<T, MT>(col, &sm.e[0][0][0], (T *)F.Bi, (T *)F.X, (T *)F.Y, e, m, true, f == nullptr);
         if (f && lane < m) {
             T s = T(0);
             for (int j = 0; j < m; j++) s += sm.e[0][0][lane * cfg<MT>::LD + j] * (T)f[(size_t)e * m + j];

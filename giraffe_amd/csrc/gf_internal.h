@@ -72,6 +72,70 @@ __device__ __forceinline__ void gm_add(const gf_gmat &g, int R, int Cc, double v
     }
 }
 
+// Reciprocal of a pivot that is finite, non-zero and not subnormal (the solvers
+// check this first): the hardware estimate refined by Newton-Raphson steps
+// (each doubles the correct bits), within an ulp of 1/p at a fraction of the
+// latency of the IEEE division sequence, which sits on the pivot chain.
+__device__ __forceinline__ double pivot_rcp(double p)
+{
+    double r = __builtin_amdgcn_rcp(p);
+    double e = fma(-p, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-p, r, 1.0);
+    return fma(r, e, r);
+}
+__device__ __forceinline__ float pivot_rcp(float p)
+{
+    float r = __builtin_amdgcn_rcpf(p);
+    const float e = fmaf(-p, r, 1.0f);
+    return fmaf(r, e, r);
+}
+
+// Address of entry (R, Cc); for a constrained DOF (R or Cc negative) a valid
+// stand-in address whose value is never stored back.  Lets a batch of
+// read-modify-writes issue all its loads before any store (a load inside the
+// branch of gm_add waits for its own round trip).
+__device__ __forceinline__ double *gm_ptr(const gf_gmat &g, int R, int Cc)
+{
+    const bool ok = R >= 0 && Cc >= 0;
+    R = ok ? R : 0;
+    Cc = ok ? Cc : 0;
+    if (g.layout == GM_BTD) {
+        const int bi = R / g.m, bj = Cc / g.m;
+        const size_t off = ((size_t)bi * g.m + (R - bi * g.m)) * g.m + (Cc - bj * g.m);
+        double *base = bj == bi ? g.B : (bj < bi ? g.A : g.C);
+        return base + off;
+    }
+    return g.ab + (size_t)R * (size_t)(2 * g.hb) + (size_t)(Cc + g.hb);
+}
+
+// N entries added at once: all loads, then the stores of the unconstrained ones.
+template <int N>
+__device__ __forceinline__ void gm_add_n(const gf_gmat &g, const int (&R)[N], const int (&Cc)[N], const double (&v)[N])
+{
+    double *p[N];
+    double old[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) p[k] = gm_ptr(g, R[k], Cc[k]);
+#pragma unroll
+    for (int k = 0; k < N; k++) old[k] = *p[k];
+#pragma unroll
+    for (int k = 0; k < N; k++)
+        if (R[k] >= 0 && Cc[k] >= 0) *p[k] = old[k] + v[k];
+}
+
+// r[R[k]] += v[k] for the unconstrained rows, loads first.
+template <int N>
+__device__ __forceinline__ void vec_add_n(double *r, const int (&R)[N], const double (&v)[N])
+{
+    double old[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) old[k] = r[R[k] >= 0 ? R[k] : 0];
+#pragma unroll
+    for (int k = 0; k < N; k++)
+        if (R[k] >= 0) r[R[k]] = old[k] + v[k];
+}
+
 // Table::GetValueAt (src/Table.cpp:73-101): piecewise-linear, clamped at both ends.
 __device__ __forceinline__ double gf_table_value(const double *tt, const double *tv, int r0, int nt, double t, int col)
 {
@@ -182,6 +246,10 @@ inline void gf_probe_end(hipStream_t s, int slot, double units)
     if (slot >= 0) gf_probe->end(s, slot, units);
 }
 
+// scalar block of a model (doubles): increment norms, failure flag, residual
+// norms, refinement norms
+enum { GF_SCAL_INC = 0, GF_SCAL_FLAG = 8, GF_SCAL_RES = 16, GF_SCAL_REF = 24, GF_SCAL_N = 32 };
+
 struct gf_model {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -274,6 +342,8 @@ struct gf_model {
 
 // ---- kernel launchers (defined in the .hip files) ----------------------------
 int gf_launch_fill(hipStream_t s, double *p, size_t n, double v);
+// zeroes three double ranges in one launch
+int gf_launch_clear3(hipStream_t s, double *a, size_t na, double *b, size_t nb, double *c, size_t nc);
 int gf_launch_fill_bytes(hipStream_t s, void *p, size_t bytes);
 // Returns the number of kernel launches issued (>= 0) or a negative status.
 int gf_beam_assemble(gf_model *m, double lf);
@@ -288,7 +358,7 @@ void gf_shell_host_const(const double *x18, const double *prop, double *rec /*[S
 gf_gmat gf_model_gmat(const gf_model *m);
 int gf_nodal_loads(gf_model *m, double t);
 int gf_scan_residual(gf_model *m);
-int gf_update_disp(gf_model *m);
+int gf_update_disp(gf_model *m, bool zero_norms);
 int gf_node_commit(gf_model *m);
 
 int gf_btd_alloc(gf_btd &b, int m, int nblk, int elem_size);
@@ -302,9 +372,9 @@ int gf_btd_factor_solve(gf_btd &b, hipStream_t s, int *d_flag, const double *rhs
 
 // Asynchronous halves of gf_assemble and gf_solve + gf_update: *_enqueue puts the
 // work on the model's stream, *_result reads the norms once the stream passed it.
-int gf_assemble_enqueue(gf_model *m);
+int gf_assemble_enqueue(gf_model *m, bool copy_norms);
 int gf_assemble_result(gf_model *m, double *res_out);
-int gf_solve_update_enqueue(gf_model *m);
+int gf_solve_update_enqueue(gf_model *m, bool then_assemble);
 int gf_solve_update_result(gf_model *m, int *info_fail, double *inc_out);
 
 int gf_commit_enqueue(gf_model *m);

@@ -199,19 +199,19 @@ int gf_mixed_solve(gf_model *m, int *fallback, int *sweeps)
                                                     m->d_band32_inv);
         if (l < 0) return l;
         const double *sol = m->layout == GM_BTD ? m->d_swork : m->d_w;
-        st = gf_launch_fill_bytes(m->stream, m->d_scal + 4, 2 * sizeof(double));
+        st = gf_launch_fill_bytes(m->stream, m->d_scal + GF_SCAL_REF, 2 * sizeof(double));
         if (st) return st;
         hipLaunchKernelGGL(correct_kernel, dim3(grid_for(npad, 256, 512)), dim3(256), 0, m->stream, npad, m->d_scale,
-                           sol, m->d_x, (unsigned long long *)(m->d_scal + 4));
+                           sol, m->d_x, (unsigned long long *)(m->d_scal + GF_SCAL_REF));
         hipLaunchKernelGGL(residual_kernel, dim3((npad * 64 + 255) / 256), dim3(256), 0, m->stream, g, npad, m->d_r,
                            m->d_x, m->d_res);
         GF_HIP(hipGetLastError());
         launches += l + 3;
-        GF_HIP(hipMemcpyAsync(m->h_scal + 4, m->d_scal + 4, 2 * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+        GF_HIP(hipMemcpyAsync(m->h_scal + GF_SCAL_REF, m->d_scal + GF_SCAL_REF, 2 * sizeof(double), hipMemcpyDeviceToHost, m->stream));
         GF_HIP(hipStreamSynchronize(m->stream));
         double dn, xn;
-        memcpy(&dn, m->h_scal + 4, 8);
-        memcpy(&xn, m->h_scal + 5, 8);
+        memcpy(&dn, m->h_scal + GF_SCAL_REF, 8);
+        memcpy(&xn, m->h_scal + GF_SCAL_REF + 1, 8);
         (*sweeps)++;
         if (!(dn == dn) || !(xn == xn)) break;
         if (dn <= m->refine_tol * xn) { ok = true; break; }

@@ -167,7 +167,7 @@ static void release(gf_model *m)
         if (m->ev[i]) (void)hipEventDestroy(m->ev[i]);
     void *bufs[] = {m->d_disp, m->d_copy, m->d_euler, m->d_dof, m->d_row_nc, m->d_bconn, m->d_bconst, m->d_bgp,
                     m->d_ln_node, m->d_ln_off, m->d_ln_ids, m->d_lrec, m->d_tab_t, m->d_tab_v, m->d_r, m->d_x,
-                    m->d_scal, m->d_flag, m->d_copy0, m->d_bgp0, m->d_sconn, m->d_sconst, m->d_spconst, m->d_sgp,
+                    m->d_scal, m->d_copy0, m->d_bgp0, m->d_sconn, m->d_sconst, m->d_spconst, m->d_sgp,
                     m->d_sgp0, m->d_spload, m->d_spflag, m->d_band, m->d_swork, m->d_band_inv};
     for (void *b : bufs) gf_pool_free(b);
     if (m->h_scal) gf_pinned_free(m->h_scal);
@@ -497,9 +497,12 @@ static int create(const gf_model_desc *d, const gf_options *opt, gf_model *m)
         GF_TRY(dalloc(&m->d_band_inv, gf_band_inv_elems(m->npad)));
     }
     if (m->precision == GF_PREC_MIXED) GF_TRY(gf_mixed_alloc(m));
-    GF_TRY(dalloc(&m->d_scal, 16));
-    GF_TRY(dalloc(&m->d_flag, 4));
-    GF_TRY(gf_pinned_alloc((void **)&m->h_scal, 16 * sizeof(double)));
+    // scalar block (doubles): [0..4] increment norms, [8] solver failure flag
+    // (int), [16..18] residual norms, [24..25] refinement norms; one copy of
+    // [0..18] returns a whole Newton iteration's results
+    GF_TRY(dalloc(&m->d_scal, GF_SCAL_N));
+    m->d_flag = (int *)(m->d_scal + GF_SCAL_FLAG);
+    GF_TRY(gf_pinned_alloc((void **)&m->h_scal, GF_SCAL_N * sizeof(double)));
     for (int i = 0; i < 2 * GF_NPHASE; i++) GF_HIP(hipEventCreate(&m->ev[i]));
     GF_HIP(hipStreamSynchronize(m->stream));
     return GF_OK;
@@ -647,18 +650,16 @@ extern "C" int gf_set_solution_window(gf_model *m, double start, double end)
 }
 
 // Enqueues Clear + MountLocal/ElementLoads/Loads + MountGlobal and the residual
-// max-norm scan; the norms land in h_scal[0..2] when the stream reaches them.
-int gf_assemble_enqueue(gf_model *m)
+// max-norm scan (norms at d_scal[GF_SCAL_RES..+2]); copy_norms: also copy them
+// to h_scal (a caller that copies the scalar block itself passes false).
+int gf_assemble_enqueue(gf_model *m, bool copy_norms)
 {
     gf_probe_scope ps(probe_of(m));
     GF_HIP(hipSetDevice(m->device));
-    size_t npad = (size_t)m->npad;
-    GF_TRY(gf_launch_fill(m->stream, m->d_r, npad, 0.0));
-    if (m->layout == GM_BTD) {
-        GF_TRY(gf_launch_fill(m->stream, (double *)m->btd.A, 3 * (size_t)m->btd.nblk * m->btd.bsz, 0.0));
-    } else {
-        GF_TRY(gf_launch_fill(m->stream, m->d_band, m->band_elems, 0.0));
-    }
+    const size_t npad = (size_t)m->npad;
+    double *tan = m->layout == GM_BTD ? (double *)m->btd.A : m->d_band;
+    const size_t ntan = m->layout == GM_BTD ? 3 * (size_t)m->btd.nblk * m->btd.bsz : m->band_elems;
+    GF_TRY(gf_launch_clear3(m->stream, m->d_r, npad, tan, ntan, m->d_scal + GF_SCAL_RES, 8));
     GF_TRY(gf_pad_diag(m));
     double t = m->t_last + m->dt;
     GF_TRY(gf_nodal_loads(m, t));
@@ -671,7 +672,9 @@ int gf_assemble_enqueue(gf_model *m)
     GF_TRY(gf_shell_pressure(m, t));
     m->factored = false;
     GF_TRY(gf_scan_residual(m));
-    GF_HIP(hipMemcpyAsync(m->h_scal, m->d_scal, 3 * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+    if (copy_norms)
+        GF_HIP(hipMemcpyAsync(m->h_scal + GF_SCAL_RES, m->d_scal + GF_SCAL_RES, 3 * sizeof(double),
+                              hipMemcpyDeviceToHost, m->stream));
     return GF_OK;
 }
 
@@ -679,7 +682,7 @@ int gf_assemble_enqueue(gf_model *m)
 int gf_assemble_result(gf_model *m, double *res_out)
 {
     GF_TRY(flush_timing(m));
-    unsigned long long *u = (unsigned long long *)m->h_scal;
+    unsigned long long *u = (unsigned long long *)(m->h_scal + GF_SCAL_RES);
     double v0, v1;
     memcpy(&v0, &u[0], 8);
     memcpy(&v1, &u[1], 8);
@@ -692,7 +695,7 @@ int gf_assemble_result(gf_model *m, double *res_out)
 extern "C" int gf_assemble(gf_model *m, double *res_out)
 {
     if (!m) return GF_E_ARG;
-    GF_TRY(gf_assemble_enqueue(m));
+    GF_TRY(gf_assemble_enqueue(m, true));
     GF_HIP(hipStreamSynchronize(m->stream));
     double res[3];
     GF_TRY(gf_assemble_result(m, res));
@@ -700,11 +703,15 @@ extern "C" int gf_assemble(gf_model *m, double *res_out)
     return GF_OK;
 }
 
-// FP64: enqueues the tangent solve, the failure-flag copy, UpdateDisps and the
-// increment norms (h_scal[0..4], flag at h_scal[8]).  Mixed precision: the
-// solve runs to completion here (its refinement is host-steered), the update
-// is enqueued after it.
-int gf_solve_update_enqueue(gf_model *m)
+// FP64: enqueues the tangent solve, UpdateDisps and the increment norms
+// (d_scal[0..4], failure flag at d_scal[GF_SCAL_FLAG]).  Mixed precision: the
+// solve runs to completion here (its refinement is host-steered), the update is
+// enqueued after it.  then_assemble: the next iteration's assembly and residual
+// norms follow in the same batch (the Newton loop assembles after every solve
+// that does not end the load step; assembly changes no state, so a batch whose
+// step ends on the increments simply does not use it).  One copy brings the
+// scalar block back.
+int gf_solve_update_enqueue(gf_model *m, bool then_assemble)
 {
     gf_probe_scope ps(probe_of(m));
     GF_HIP(hipSetDevice(m->device));
@@ -712,15 +719,17 @@ int gf_solve_update_enqueue(gf_model *m)
         int info_fail = 0;
         GF_TRY(gf_solve(m, &info_fail));
         m->pending_info_fail = info_fail;
+        GF_TRY(gf_launch_fill_bytes(m->stream, m->d_scal, GF_SCAL_FLAG * sizeof(double)));
     } else {
-        GF_TRY(gf_launch_fill_bytes(m->stream, m->d_flag, 4));
+        GF_TRY(gf_launch_fill_bytes(m->stream, m->d_scal, (GF_SCAL_FLAG + 1) * sizeof(double)));
         m->n_solves++;
         GF_TRY(solve_fp64_enqueue(m));
-        GF_HIP(hipMemcpyAsync(&m->h_scal[8], m->d_flag, 4, hipMemcpyDeviceToHost, m->stream));
         m->pending_info_fail = -1;
     }
-    GF_TRY(gf_update_disp(m));
-    GF_HIP(hipMemcpyAsync(m->h_scal, m->d_scal, 5 * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+    GF_TRY(gf_update_disp(m, false));
+    if (then_assemble) GF_TRY(gf_assemble_enqueue(m, false));
+    const int nscal = then_assemble ? GF_SCAL_RES + 3 : GF_SCAL_FLAG + 1;
+    GF_HIP(hipMemcpyAsync(m->h_scal, m->d_scal, nscal * sizeof(double), hipMemcpyDeviceToHost, m->stream));
     return GF_OK;
 }
 
@@ -731,11 +740,7 @@ int gf_solve_update_result(gf_model *m, int *info_fail, double *inc_out)
         *info_fail = m->pending_info_fail;
     } else {
         int flag = 0;
-        memcpy(&flag, &m->h_scal[8], 4);
-        if (flag == 2) {
-            gf_set_error("band solve: a block wait gave up");
-            return GF_E_DEVICE;
-        }
+        memcpy(&flag, &m->h_scal[GF_SCAL_FLAG], 4);
         *info_fail = flag ? 1 : 0;
     }
     unsigned long long *u = (unsigned long long *)m->h_scal;
@@ -783,9 +788,9 @@ extern "C" int gf_solve(gf_model *m, int *info_fail)
         GF_TRYN(nl_f32, gf_mixed_factor(m));
         tick(m, GF_PH_FACTOR, false);
         int flag32 = 0;
-        GF_HIP(hipMemcpyAsync(&m->h_scal[8], m->d_flag, 4, hipMemcpyDeviceToHost, m->stream));
+        GF_HIP(hipMemcpyAsync(&m->h_scal[GF_SCAL_FLAG], m->d_flag, 4, hipMemcpyDeviceToHost, m->stream));
         GF_HIP(hipStreamSynchronize(m->stream));
-        memcpy(&flag32, &m->h_scal[8], 4);
+        memcpy(&flag32, &m->h_scal[GF_SCAL_FLAG], 4);
         GF_TRY(tock(m, GF_PH_FACTOR, nl_f32));
         if (!flag32) {
             int fb = 0, sw = 0;
@@ -797,13 +802,9 @@ extern "C" int gf_solve(gf_model *m, int *info_fail)
             m->n_fallbacks += fb;
             m->factored = true;
             int flag = 0;
-            GF_HIP(hipMemcpyAsync(&m->h_scal[8], m->d_flag, 4, hipMemcpyDeviceToHost, m->stream));
+            GF_HIP(hipMemcpyAsync(&m->h_scal[GF_SCAL_FLAG], m->d_flag, 4, hipMemcpyDeviceToHost, m->stream));
             GF_HIP(hipStreamSynchronize(m->stream));
-            memcpy(&flag, &m->h_scal[8], 4);
-            if (flag == 2) {
-                gf_set_error("band solve: a block wait gave up");
-                return GF_E_DEVICE;
-            }
+            memcpy(&flag, &m->h_scal[GF_SCAL_FLAG], 4);
             if (flag) *info_fail = 1;
             return GF_OK;
         }
@@ -814,13 +815,9 @@ extern "C" int gf_solve(gf_model *m, int *info_fail)
     GF_TRY(solve_fp64_enqueue(m));
     GF_TRY(flush_timing(m));
     int flag = 0;
-    GF_HIP(hipMemcpyAsync(&m->h_scal[8], m->d_flag, 4, hipMemcpyDeviceToHost, m->stream));
+    GF_HIP(hipMemcpyAsync(&m->h_scal[GF_SCAL_FLAG], m->d_flag, 4, hipMemcpyDeviceToHost, m->stream));
     GF_HIP(hipStreamSynchronize(m->stream));
-    memcpy(&flag, &m->h_scal[8], 4);
-    if (flag == 2) {
-        gf_set_error("band solve: a block wait gave up");
-        return GF_E_DEVICE;
-    }
+    memcpy(&flag, &m->h_scal[GF_SCAL_FLAG], 4);
     if (flag) *info_fail = 1;
     return GF_OK;
 }
@@ -829,7 +826,7 @@ extern "C" int gf_update(gf_model *m, double *inc_out)
 {
     if (!m) return GF_E_ARG;
     GF_HIP(hipSetDevice(m->device));
-    GF_TRY(gf_update_disp(m));
+    GF_TRY(gf_update_disp(m, true));
     if (inc_out) {
         GF_HIP(hipMemcpyAsync(m->h_scal, m->d_scal, 5 * sizeof(double), hipMemcpyDeviceToHost, m->stream));
         GF_HIP(hipStreamSynchronize(m->stream));

@@ -19,6 +19,15 @@ __global__ void fill_kernel(double *p, size_t n, double v)
     for (; i < n; i += stride) p[i] = v;
 }
 
+__global__ __launch_bounds__(256) void clear3_kernel(double *a, size_t na, double *b, size_t nb, double *c, size_t nc)
+{
+    const size_t n = na + nb + nc;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        double *p = i < na ? a + i : i < na + nb ? b + (i - na) : c + (i - na - nb);
+        *p = 0.0;
+    }
+}
+
 __global__ void fill_u32_kernel(unsigned *p, size_t n)
 {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -61,14 +70,19 @@ __global__ __launch_bounds__(128) void nodal_loads_kernel(int n_ln, const int *_
             rodrigues(alpha, Q, Xi, g);
             mo = mtv(Xi, mo);
             m3<double> S = opV(alpha, mo);
-            double q6[6] = {f.x, f.y, f.z, mo.x, mo.y, mo.z};
-#pragma unroll
-            for (int i = 0; i < 6; i++)
-                if (R[i] >= 0) r[R[i]] += q6[i];
+            const double q6[6] = {f.x, f.y, f.z, mo.x, mo.y, mo.z};
+            vec_add_n<6>(r, R, q6);
+            int R9[9], C9[9];
+            double V9[9];
 #pragma unroll
             for (int i = 0; i < 3; i++)
 #pragma unroll
-                for (int j = 0; j < 3; j++) gm_add(gm, R[3 + i], R[3 + j], -S.a[3 * i + j]);
+                for (int j = 0; j < 3; j++) {
+                    R9[3 * i + j] = R[3 + i];
+                    C9[3 * i + j] = R[3 + j];
+                    V9[3 * i + j] = -S.a[3 * i + j];
+                }
+            gm_add_n<9>(gm, R9, C9, V9);
         } else if (kind == GF_LOAD_NODAL_FOLLOWER) {
             v3<double> ai = mk3(copy[6 * node + 3], copy[6 * node + 4], copy[6 * node + 5]);
             m3<double> Qi, Xii, Q, Xi;
@@ -85,17 +99,22 @@ __global__ __launch_bounds__(128) void nodal_loads_kernel(int n_ln, const int *_
             for (int i = 0; i < 9; i++) dy.a[i] = 0.0;
             add_outer(dy, 1.0, Qim, alpha);
             m3<double> K22 = mscl(-0.5 * g, madd(skew3(Qim), mm3(Xi, dy)));
-            double q6[6] = {fip.x, fip.y, fip.z, mip.x, mip.y, mip.z};
-#pragma unroll
-            for (int i = 0; i < 6; i++)
-                if (R[i] >= 0) r[R[i]] += q6[i];
+            const double q6[6] = {fip.x, fip.y, fip.z, mip.x, mip.y, mip.z};
+            vec_add_n<6>(r, R, q6);
+            int R18[18], C18[18];
+            double V18[18];
 #pragma unroll
             for (int i = 0; i < 3; i++)
 #pragma unroll
                 for (int j = 0; j < 3; j++) {
-                    gm_add(gm, R[i], R[3 + j], -K12.a[3 * i + j]);
-                    gm_add(gm, R[3 + i], R[3 + j], -K22.a[3 * i + j]);
+                    R18[3 * i + j] = R[i];
+                    C18[3 * i + j] = R[3 + j];
+                    V18[3 * i + j] = -K12.a[3 * i + j];
+                    R18[9 + 3 * i + j] = R[3 + i];
+                    C18[9 + 3 * i + j] = R[3 + j];
+                    V18[9 + 3 * i + j] = -K22.a[3 * i + j];
                 }
+            gm_add_n<18>(gm, R18, C18, V18);
         }
     }
 }
@@ -206,6 +225,15 @@ int gf_launch_fill(hipStream_t s, double *p, size_t n, double v)
     return GF_OK;
 }
 
+int gf_launch_clear3(hipStream_t s, double *a, size_t na, double *b, size_t nb, double *c, size_t nc)
+{
+    const size_t n = na + nb + nc;
+    if (n == 0) return GF_OK;
+    hipLaunchKernelGGL(clear3_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, a, na, b, nb, c, nc);
+    GF_HIP(hipGetLastError());
+    return GF_OK;
+}
+
 int gf_launch_fill_bytes(hipStream_t s, void *p, size_t bytes)
 {
     size_t n = bytes / 4;
@@ -225,20 +253,22 @@ int gf_nodal_loads(gf_model *m, double t)
     return GF_OK;
 }
 
+// The norms land in d_scal[GF_SCAL_RES..+2], which the assembly's clear zeroed.
 int gf_scan_residual(gf_model *m)
 {
-    int st = gf_launch_fill_bytes(m->stream, m->d_scal, 8 * sizeof(double));
-    if (st) return st;
     hipLaunchKernelGGL(scan_kernel, dim3(grid_for(m->n_free, 256, 128)), dim3(256), 0, m->stream, m->n_free, m->d_r,
-                       m->d_row_nc, (unsigned long long *)m->d_scal);
+                       m->d_row_nc, (unsigned long long *)(m->d_scal + GF_SCAL_RES));
     GF_HIP(hipGetLastError());
     return GF_OK;
 }
 
-int gf_update_disp(gf_model *m)
+// zero_norms: clear d_scal[0..7] first (callers that cleared it pass false).
+int gf_update_disp(gf_model *m, bool zero_norms)
 {
-    int st = gf_launch_fill_bytes(m->stream, m->d_scal, 8 * sizeof(double));
-    if (st) return st;
+    if (zero_norms) {
+        int st = gf_launch_fill_bytes(m->stream, m->d_scal, GF_SCAL_FLAG * sizeof(double));
+        if (st) return st;
+    }
     hipLaunchKernelGGL(update_kernel, dim3(grid_for(m->n_free, 256, 128)), dim3(256), 0, m->stream, m->n_free,
                        m->d_x, m->d_row_nc, m->d_disp, (unsigned long long *)m->d_scal);
     GF_HIP(hipGetLastError());

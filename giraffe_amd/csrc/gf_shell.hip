@@ -364,10 +364,17 @@ __global__ __launch_bounds__(SWG) void shell_assemble_kernel(
             int na, ca, nb_, cb;
             dof_block(conn, ld, e, I, na, ca);
             dof_block(conn, ld, e, J, nb_, cb);
+            int R9[9], C9[9];
+            double V9[9];
 #pragma unroll
             for (int i = 0; i < 3; i++)
 #pragma unroll
-                for (int j = 0; j < 3; j++) gm_add(gm, dof[6 * na + ca + i], dof[6 * nb_ + cb + j], K.a[3 * i + j]);
+                for (int j = 0; j < 3; j++) {
+                    R9[3 * i + j] = dof[6 * na + ca + i];
+                    C9[3 * i + j] = dof[6 * nb_ + cb + j];
+                    V9[3 * i + j] = K.a[3 * i + j];
+                }
+            gm_add_n<9>(gm, R9, C9, V9);
         } else {
             double *Ke = Kout + (size_t)e * 729;
 #pragma unroll
@@ -411,12 +418,11 @@ __global__ __launch_bounds__(SWG) void shell_assemble_kernel(
             }
             int node, c0;
             dof_block(conn, ld, e, I, node, c0);
-            double P[3] = {f.x, f.y, f.z};
+            int R3[3];
+            double V3[3] = {-f.x, -f.y, -f.z};
 #pragma unroll
-            for (int i = 0; i < 3; i++) {
-                int R = dof[6 * node + c0 + i];
-                if (R >= 0) r[R] -= P[i];
-            }
+            for (int i = 0; i < 3; i++) R3[i] = dof[6 * node + c0 + i];
+            vec_add_n<3>(r, R3, V3);
         } else {
             fout[(size_t)e * 27 + 3 * I + 0] = f.x;
             fout[(size_t)e * 27 + 3 * I + 1] = f.y;
@@ -494,13 +500,22 @@ __global__ __launch_bounds__(256) void shell_pressure_kernel(
     int rr[3] = {dof[6 * na], dof[6 * na + 1], dof[6 * na + 2]};
     double Pv[3] = {Pa.x, Pa.y, Pa.z};
     // the element's unbalanced force grows by Pa; the residual is its negative sum
+    const double Pn[3] = {-Pv[0], -Pv[1], -Pv[2]};
+    vec_add_n<3>(r, rr, Pn);
 #pragma unroll
-    for (int i = 0; i < 3; i++)
-        if (rr[i] >= 0) r[rr[i]] -= Pv[i];
     for (int b = 0; b < 6; b++) {
         int nb_ = conn[b * ld + e];
+        int R9[9], C9[9];
+        double V9[9];
+#pragma unroll
         for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++) gm_add(gm, rr[i], dof[6 * nb_ + j], Kr[i][3 * b + j]);
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                R9[3 * i + j] = rr[i];
+                C9[3 * i + j] = dof[6 * nb_ + j];
+                V9[3 * i + j] = Kr[i][3 * b + j];
+            }
+        gm_add_n<9>(gm, R9, C9, V9);
     }
 }
 

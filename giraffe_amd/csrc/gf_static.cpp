@@ -15,7 +15,9 @@
 //
 // The loop is a resumable state machine (gf_static_run): every device step is
 // enqueued on the model's stream and followed by an event, and the host logic
-// resumes once the event has completed.  gf_static_solve drives one run to the
+// resumes once the event has completed.  A solve batch also assembles the next
+// iteration (the loop always assembles after a solve unless the step ends, and
+// assembly changes no state), so a Newton iteration costs one host round trip.  gf_static_solve drives one run to the
 // end, waiting on each event; the request server drives many runs at once,
 // polling their events, so the device interleaves the requests' work.
 #include "gf_internal.h"
@@ -123,7 +125,8 @@ int gf_static_run::advance()
             info_fail = 0;
             if (!(res_conv && gl_conv)) {
                 t_mark = std::chrono::steady_clock::now();
-                if ((st = gf_solve_update_enqueue(m))) return st;
+                // the next iteration's assembly rides in the same batch
+                if ((st = gf_solve_update_enqueue(m, true))) return st;
                 if ((st = record())) return st;
                 r.total_solves++;
                 res_conv = false;
@@ -157,8 +160,10 @@ int gf_static_run::advance()
             if (time_step < p.min_time_step && time != p.end_time) aborted = true;
             it++;
             if ((!res_conv || !gl_conv) && !diverged && !aborted) {
-                if ((st = enqueue_assemble())) return st;
-                return 1;
+                // every iteration that continues follows a solve (res_conv is reset
+                // there), whose batch already assembled and scanned the residual
+                phase = PH_ASSEMBLED;
+                break;
             }
             // end of the load step
             if (!diverged) {
@@ -195,7 +200,7 @@ int gf_static_run::advance()
 int gf_static_run::enqueue_assemble()
 {
     t_mark = std::chrono::steady_clock::now();
-    int st = gf_assemble_enqueue(m);
+    int st = gf_assemble_enqueue(m, true);
     if (st) return st;
     if ((st = record())) return st;
     phase = PH_ASSEMBLED;

@@ -58,6 +58,7 @@ def load():
     for f in ("gf_read_nodes", "gf_read_disp", "gf_read_residual", "gf_read_increment", "gf_write_disp"):
         getattr(L, f).argtypes = [vp, dp]
     L.gf_read_tangent_dense.argtypes = [vp, dp, C.c_longlong]
+    L.gf_write_results.argtypes = [vp, C.c_char_p]
     L.gf_beam_eval.argtypes = [C.c_int, dp, dp, dp, dp, dp, dp, dp]
     L.gf_shell_eval.argtypes = [C.c_int, dp, dp, dp, dp, dp, dp]
     L.gf_bandsystem.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.POINTER(C.c_int)]
@@ -172,6 +173,10 @@ class DeviceModel:
 
     def nodes(self):
         return self._read("gf_read_nodes", (self.nn, 6))
+
+    def write_results(self, path):
+        """Configuration results file (node section) of the last converged configuration."""
+        check(load().gf_write_results(self.h, os.fsencode(path)), "gf_write_results")
 
     def disp(self):
         return self._read("gf_read_disp", (self.nn, 6))

@@ -207,6 +207,11 @@ int gf_kernel_probe_read(gf_model *m, double *ms_out, long long *launches_out, d
 /* Per node: X Y Z of the last converged configuration and the Euler rotation
  * vector (Node::WriteResults src/Node.cpp:208-219). */
 int gf_read_nodes(gf_model *m, double *out /* [n_nodes][6] */);
+/* Configuration results text file of the last converged configuration, as
+ * PostFiles::WriteConfigurationResults writes it (src/PostFiles.cpp:1429-1464):
+ * the node section with one Node::WriteResults line per node
+ * (src/Node.cpp:208-219); the element section header only (no element lines). */
+int gf_write_results(gf_model *m, const char *path);
 /* Per node and DOF: incremental displacement d, or residual r (0 where not free). */
 int gf_read_disp(gf_model *m, double *out /* [n_nodes][6] */);
 int gf_read_residual(gf_model *m, double *out /* [n_nodes][6] */);

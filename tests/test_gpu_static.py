@@ -45,3 +45,32 @@ def test_catenary_full_size(oracle_lib):
     """Config 2 at its full size: 10,000 Beam_1, 120k DOFs."""
     m, p, _ = M.catenary(n_elem=10000)
     _compare(oracle_lib, m, p)
+
+
+def test_results_file_layout(oracle_lib, tmp_path):
+    """gf_write_results: the node section of PostFiles::WriteConfigurationResults
+    (Node::WriteResults lines, %.6e) for the converged configuration."""
+    m, p, info = M.cantilever_rollup()
+    dm = runtime.DeviceModel(m)
+    dm.static_solve(p)
+    path = tmp_path / "solution_1_configuration_1.txt"
+    dm.write_results(path)
+    lines = path.read_text().split("\n")
+    assert lines[0] == "RESULTS - NODES" and lines[1] == ""
+    nodes = dm.nodes()
+    for i in range(m.n_nodes):
+        f = lines[2 + i].split("\t")
+        assert f[0] == "Node" and int(f[1]) == i + 1
+        assert f[2::2] == ["X", "Y", "Z", "RX", "RY", "RZ"]
+        vals = np.array([float(v) for v in f[3::2]])
+        assert all(len(v.split("e")[0].split(".")[1]) == 6 for v in f[3::2])  # %.6e
+        np.testing.assert_allclose(vals, nodes[i], rtol=5e-7, atol=1e-300)
+    tail = lines[2 + m.n_nodes:]
+    assert tail[:4] == ["", "", "RESULTS - ELEMENTS", ""]
+    # against the oracle's converged configuration at the file's precision
+    om = oracle_lib.OracleModel(m)
+    om.static_solve(p)
+    no = om.nodes()
+    got = np.array([[float(v) for v in lines[2 + i].split("\t")[3::2]] for i in range(m.n_nodes)])
+    scale = np.abs(no).max()
+    assert np.abs(got - no).max() <= 1e-6 * scale
