@@ -215,13 +215,20 @@ def serve(args):
     import torch
     from giraffe_amd import runtime
 
-    rng = np.random.default_rng(1004)
+    mixed = args.workload == "serve-mixed"
+    rng = np.random.default_rng(1005 if mixed else 1004)
     t0 = time.perf_counter()
-    reqs = [M.riser_request(rng)[:2] for _ in range(args.requests)]
+    if mixed:  # 70 % riser chains, 30 % clamped plates
+        reqs = [M.riser_request(rng)[:2] if rng.uniform() < 0.7 else M.plate_request(rng)[:2]
+                for _ in range(args.requests)]
+    else:
+        reqs = [M.riser_request(rng)[:2] for _ in range(args.requests)]
     for m, _ in reqs:
         m.freeze()
+    n_plates = sum(1 for m, _ in reqs if m.n_shells)
     log(f"generated {len(reqs)} requests in {time.perf_counter() - t0:.1f} s "
-        f"({sum(m.n_beams for m, _ in reqs)} Beam_1 in total)")
+        f"({sum(m.n_beams for m, _ in reqs)} Beam_1, {n_plates} plates with "
+        f"{sum(m.n_shells for m, _ in reqs)} Shell_1 in total)")
     prec = abi.GF_PREC_FP64 if args.precision == "fp64" else abi.GF_PREC_MIXED
 
     def drain(srv, pending, lat, t_sub):
@@ -284,9 +291,13 @@ def serve(args):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f64 tangent/residual, f32 factor + f64 refinement",
-        "data": "synthetic (riser request stream, seed 1004, giraffe_amd/model.py)",
-        "config": {"workload": f"serving stream: {len(reqs)} Beam_1 riser requests (E log-U[50,5000]), Poisson "
-                               f"arrivals at {args.load:.2f} x closed-loop capacity, {args.max_inflight} resident",
+        "data": "synthetic (seeded request generators, giraffe_amd/model.py)",
+        "config": {"workload": (f"mixed serving stream: {len(reqs)} requests, {len(reqs) - n_plates} Beam_1 risers "
+                                f"(E log-U[50,5000]) and {n_plates} clamped Shell_1 plates ({{16,24,32,48,64}}^2 "
+                                f"quads, t U[2,10] mm, deflection U[5,25] t), seed 1005" if mixed else
+                                f"serving stream: {len(reqs)} Beam_1 riser requests (E log-U[50,5000]), seed 1004")
+                               + f"; Poisson arrivals at {args.load:.2f} x closed-loop capacity, "
+                                 f"{args.max_inflight} resident",
                    "precision": args.precision, "arrival_rate_per_s": rate, "closed_loop_capacity": cap,
                    "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))},
         "p50_ms": float(np.percentile(lat_ms, 50)),
@@ -327,7 +338,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="catenary", choices=["catenary", "plate", "rollup", "serve"])
+    ap.add_argument("--workload", default="catenary", choices=["catenary", "plate", "rollup", "serve", "serve-mixed"])
     ap.add_argument("--requests", type=int, default=2000, help="serve: requests in the stream")
     ap.add_argument("--max-inflight", type=int, default=16, help="serve: resident requests")
     ap.add_argument("--load", type=float, default=0.8, help="serve: arrival rate / closed-loop capacity")
@@ -348,7 +359,7 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(requests shard across GPUs with no collectives; see DESIGN.md).", file=sys.stderr)
         return 2
-    if args.workload == "serve" and args.hw_queues > 0:
+    if args.workload.startswith("serve") and args.hw_queues > 0:
         # one stream per resident request; HIP spreads streams over this many
         # hardware queues (its default is 4), set before the runtime starts
         os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, args.hw_queues))
@@ -367,7 +378,7 @@ def main():
         torch.cuda.synchronize()
         log(f"copied {args.copy_mib} MiB device to device")
         del src, dst
-    if args.workload == "serve":
+    if args.workload.startswith("serve"):
         return serve(args)
     m, p, desc = workload(args.workload)
     r = measure(m, p, args.precision, args.steps, args.warmup, not args.no_timing)

@@ -462,12 +462,15 @@ __global__ __launch_bounds__(128) void btd_level_kernel(Fac F, int L, int nL, T 
     fac_step<T, MT>(F, sm, true, L, nL, blockIdx.x, Lsv, Rsv, f, zout);
 }
 
-// Levels L0 .. levels-1, the top and (with f) the back-substitution of those
-// levels, in one workgroup of PAIRS wave pairs.
+// Levels L0 .. levels-1, the top and (with f) the back-substitution of levels
+// levels-1 .. Lb (Lb <= L0: the upper levels below the tail have so few
+// eliminated rows that their back-substitution costs less here than a launch),
+// in one workgroup of PAIRS wave pairs.  nblk: block rows of level 0 (level L
+// has nblk >> L active rows).
 template <typename T, int MT>
 __global__ __launch_bounds__(128 * cfg<MT>::PAIRS) void btd_tail_kernel(Fac F, int L0, int levels, int nL0,
                                                                         T *Lsv, T *Rsv, const long long *save,
-                                                                        double *f, double *x)
+                                                                        double *f, double *x, int Lb, int nblk)
 {
     constexpr int PAIRS = cfg<MT>::PAIRS;
     __shared__ PairSmem<T, MT> sm[PAIRS];
@@ -484,11 +487,9 @@ __global__ __launch_bounds__(128 * cfg<MT>::PAIRS) void btd_tail_kernel(Fac F, i
     }
     top_step<T, MT>(F, sm[0], levels, f, x);
     if (!f) return;
-    // back-substitution of the tail levels (nL of level L recomputed downwards)
-    for (int L = levels - 1; L >= L0; L--) {
-        int n = nL0;
-        for (int j = L0; j < L; j++) n /= 2;
-        back_rows<T>((const T *)F.X, (const T *)F.Y, F.m, L, n, x, threadIdx.x, blockDim.x);
+    // back-substitution of levels levels-1 .. Lb
+    for (int L = levels - 1; L >= Lb; L--) {
+        back_rows<T>((const T *)F.X, (const T *)F.Y, F.m, L, nblk >> L, x, threadIdx.x, blockDim.x);
         __syncthreads();
     }
 }
@@ -601,6 +602,9 @@ int gf_btd_alloc(gf_btd &b, int m, int nblk, int elem_size)
     }
     b.levels = L;
     if (b.tail_level < 0) b.tail_level = L;
+    // the back-substitution of the levels above the tail runs as one wide launch
+    // per level: inside the single tail workgroup it took longer than the launches
+    b.back_level = b.tail_level;
     size_t blocks = (size_t)nblk * b.bsz * elem_size;
     size_t sv = (save ? save : 1) * b.bsz * elem_size;
     // A, B, C contiguous: the assembly clears them with one fill
@@ -647,11 +651,11 @@ static int btd_factor_t(gf_btd &b, hipStream_t s, int *d_flag, double *f, double
     const int nL0 = b.tail_level < b.levels ? b.lvl_n[b.tail_level] : 1;
     const int pr = gf_probe_begin(s, GF_KC_BTD_TAIL);
     hipLaunchKernelGGL((btd_tail_kernel<T, MT>), dim3(1), dim3(128 * cfg<MT>::PAIRS), 0, s, F, b.tail_level, b.levels,
-                       nL0, Lsv, Rsv, (const long long *)b.d_save, f, x);
+                       nL0, Lsv, Rsv, (const long long *)b.d_save, f, x, b.back_level, b.nblk);
     gf_probe_end(s, pr, nL0);
     launches++;
     if (f) {
-        for (int L = b.tail_level - 1; L >= 0; L--) {
+        for (int L = b.back_level - 1; L >= 0; L--) {
             const int nL = b.lvl_n[L];
             const int work = (nL + 1) / 2 * b.m;
             hipLaunchKernelGGL((btd_back_kernel<T>), dim3((work + 255) / 256), dim3(256), 0, s, (const T *)b.X,

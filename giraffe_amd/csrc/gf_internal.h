@@ -201,6 +201,7 @@ enum { NL_KIND = 0, NL_CS = 1, NL_MULT = 10, NL_TAB0 = 16, NL_TABN = 17, NL_REC 
 // Block-tridiagonal factor workspace (cyclic reduction), precision T.
 struct gf_btd {
     int m = 0, mt = 16, nblk = 0, levels = 0, tail_level = 0;
+    int back_level = 0;             // lowest level whose back-substitution the tail kernel does
     size_t bsz = 0;                 // m*m
     void *A = nullptr, *B = nullptr, *C = nullptr;     // [nblk][m][m] assembled blocks (kept rows reduced in place)
     void *Bi = nullptr, *X = nullptr, *Y = nullptr;    // eliminated rows: B^-1, B^-1 A, B^-1 C

@@ -427,6 +427,23 @@ def shell_plate(nq=64, L=1.0, t=0.005, E=70e9, nu=0.33, rho=2700.0, p_max=1.7e7,
     return m, p, info
 
 
+def plate_request(rng, steps=15):
+    """One plate request of the mixed serving stream: clamped Shell_1 plate with a
+    mesh of nq x nq quads, nq uniform over {16, 24, 32, 48, 64}, thickness
+    U[2, 10] mm, follower pressure ramped so that the centre deflection reaches
+    about U[5, 25] thicknesses (the pressure of the 64x64, 5 mm plate that gives
+    20 thicknesses, scaled with the membrane law p ~ E t^4 (w/t)^3 / L^4), seeded
+    imperfections.  Returns (model, params, info)."""
+    nq = int(rng.choice([16, 24, 32, 48, 64]))
+    t = rng.uniform(0.002, 0.010)
+    k = rng.uniform(5.0, 25.0)
+    p_max = 1.7e7 * (k / 20.0) ** 3 * (t / 0.005) ** 4
+    seed = int(rng.integers(1 << 30))
+    m, p, info = shell_plate(nq=nq, t=t, p_max=p_max, n_steps=steps, seed=seed)
+    info = dict(info, nq=nq, t=t, target_ratio=k)
+    return m, p, info
+
+
 def riser_request(rng, n_elem=None, steps=10):
     """One request of the catenary serving stream: Beam_1 riser with E ~ log-U[50, 5000]
     elements, horizontal span U[200, 2000] m (vertical 2/3 of it), sag ratio U[0.2, 0.6]
