@@ -101,12 +101,17 @@ template <> __device__ __forceinline__ float rdl<float>(float v, int l)
 }
 
 // One wavefront: LU (no pivoting) of the 16 x 16 block at s (leading dimension
-// ld) in place, with Linv (unit lower) and Uinv written row-major to li / ui
-// (leading dimension LLD).  Lane c of the first 16 holds column c of the block,
-// lanes 16..31 the identity that the forward elimination turns into Linv,
-// lanes 32..47 the identity that the backward substitution turns into Uinv.
+// ld), with Linv (unit lower) and Uinv written row-major to li / ui (leading
+// dimension LLD).  Lane c of the first 16 holds column c of the block, lanes
+// 16..31 the identity that the forward elimination turns into Linv, lanes
+// 32..47 the identity that the backward substitution turns into Uinv.  On
+// return s holds U on and above the diagonal; below it the values are not the
+// multipliers (nothing reads them).
 template <typename T> __device__ __forceinline__ void wave_lu16(T *s, int ld, T *li, T *ui, int *flag)
 {
+    // Branch-free: the masks are applied arithmetically and a bad pivot is only
+    // recorded, so the unrolled sweep is one basic block and the scheduler can
+    // start the next pivot's reciprocal while the current updates issue.
     const int lane = threadIdx.x & 63, part = lane >> 4, c = lane & 15;
     T col[LB];
 #pragma unroll
@@ -115,45 +120,41 @@ template <typename T> __device__ __forceinline__ void wave_lu16(T *s, int ld, T 
         col[i] = part == 0 ? d : (part == 1 && i == c) ? T(1) : T(0);
     }
     T rp[LB];
+    bool any_bad = false;
 #pragma unroll
     for (int k = 0; k < LB; k++) {
         T p = rdl<T>(col[k], k);
-        if (bad_pivot(p)) {
-            if (lane == 0) atomicExch(flag, 1);
-            p = T(1);
-        }
+        const bool bad = bad_pivot(p);
+        any_bad |= bad;
+        p = bad ? T(1) : p;
         rp[k] = pivot_rcp(p);
-        T l[LB];
+        T a[LB];
 #pragma unroll
-        for (int i = k + 1; i < LB; i++) l[i] = rdl<T>(col[i], k) * rp[k];
-        const T ck = col[k];
-        // the row operations act on the columns right of k (left of it the block
-        // holds the multipliers) and on every column of the identity part
-        if ((part == 0 && c > k) || part == 1) {
+        for (int i = k + 1; i < LB; i++) a[i] = rdl<T>(col[i], k);
+        // row i -= (a_ik / a_kk) row k, applied to the columns right of k and to
+        // every column of the identity part (elsewhere the scale is zero); the
+        // multipliers themselves are not kept: only the inverses of the
+        // diagonal blocks are used downstream.
+        const bool act = (part == 0 && c > k) || part == 1;
+        const T ck = act ? col[k] * rp[k] : T(0);
 #pragma unroll
-            for (int i = k + 1; i < LB; i++) col[i] -= l[i] * ck;
-        }
-        if (lane == k) {
-#pragma unroll
-            for (int i = k + 1; i < LB; i++) col[i] = l[i];
-        }
+        for (int i = k + 1; i < LB; i++) col[i] -= a[i] * ck;
     }
-    if (part == 2) {
+    const bool inv_u = part == 2;
 #pragma unroll
-        for (int i = 0; i < LB; i++) col[i] = i == c ? T(1) : T(0);
-    }
+    for (int i = 0; i < LB; i++) col[i] = inv_u ? (i == c ? T(1) : T(0)) : col[i];
 #pragma unroll
     for (int k = LB - 1; k >= 0; k--) {
         T u[LB];
 #pragma unroll
         for (int i = 0; i < k; i++) u[i] = rdl<T>(col[i], k);
-        if (part == 2) {
-            const T xk = col[k] * rp[k];
-            col[k] = xk;
+        const T xk = col[k] * rp[k];
+        col[k] = inv_u ? xk : col[k];
+        const T xm = inv_u ? xk : T(0);
 #pragma unroll
-            for (int i = 0; i < k; i++) col[i] -= u[i] * xk;
-        }
+        for (int i = 0; i < k; i++) col[i] -= u[i] * xm;
     }
+    if (any_bad && lane == 0) atomicExch(flag, 1);
     if (part < 3) {
         T *dst = part == 0 ? s : part == 1 ? li : ui;
         const int ldd = part == 0 ? ld : LLD;

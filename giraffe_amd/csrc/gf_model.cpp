@@ -1005,10 +1005,51 @@ extern "C" int gf_bandsystem(int n, int hb, const double *ab_rows, const double 
         if (p) (void)hipFree(p);
     if (s) (void)hipStreamDestroy(s);
     if (st) return st;
-    if (flag == 2) {
-        gf_set_error("band solve: a block wait gave up");
-        return GF_E_DEVICE;
-    }
+    if (flag) *info_fail = 1;
+    return GF_OK;
+}
+
+extern "C" int gf_btdsystem(int m, int nblk, const double *A, const double *B, const double *C, const double *b,
+                            double *x, int *info_fail)
+{
+    if (m < 1 || m > 32 || nblk < 1 || !A || !B || !C || !b || !x || !info_fail) return GF_E_ARG;
+    *info_fail = 0;
+    const size_t bs = (size_t)m * m, blocks = (size_t)nblk * bs, n = (size_t)nblk * m;
+    gf_btd t;
+    double *d_r = nullptr, *d_x = nullptr, *d_w = nullptr;
+    int *d_flag = nullptr;
+    hipStream_t s = nullptr;
+    int flag = 0;
+    auto run = [&]() -> int {
+        GF_TRY(gf_btd_alloc(t, m, nblk, 8));
+        GF_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        GF_HIP(hipMalloc(&d_r, n * 8));
+        GF_HIP(hipMalloc(&d_x, n * 8));
+        GF_HIP(hipMalloc(&d_w, n * 8));
+        GF_HIP(hipMalloc(&d_flag, 4));
+        GF_TRY(gf_launch_fill_bytes(s, d_flag, 4));
+        GF_HIP(hipMemcpyAsync(t.A, A, blocks * 8, hipMemcpyHostToDevice, s));
+        GF_HIP(hipMemcpyAsync(t.B, B, blocks * 8, hipMemcpyHostToDevice, s));
+        GF_HIP(hipMemcpyAsync(t.C, C, blocks * 8, hipMemcpyHostToDevice, s));
+        // the couplings past either end do not exist
+        GF_HIP(hipMemsetAsync(t.A, 0, bs * 8, s));
+        GF_HIP(hipMemsetAsync((double *)t.C + (blocks - bs), 0, bs * 8, s));
+        GF_HIP(hipMemcpyAsync(d_r, b, n * 8, hipMemcpyHostToDevice, s));
+        int l = gf_btd_factor_solve(t, s, d_flag, d_r, d_x, (int)n, d_w);
+        if (l < 0) return l;
+        GF_HIP(hipMemcpyAsync(x, d_x, n * 8, hipMemcpyDeviceToHost, s));
+        GF_HIP(hipMemcpyAsync(&flag, d_flag, 4, hipMemcpyDeviceToHost, s));
+        GF_HIP(hipStreamSynchronize(s));
+        return GF_OK;
+    };
+    const int st = run();
+    if (s) (void)hipStreamSynchronize(s);
+    void *bufs[] = {d_r, d_x, d_w, d_flag};
+    for (void *p : bufs)
+        if (p) (void)hipFree(p);
+    gf_btd_free(t);
+    if (s) (void)hipStreamDestroy(s);
+    if (st) return st;
     if (flag) *info_fail = 1;
     return GF_OK;
 }

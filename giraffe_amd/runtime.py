@@ -62,6 +62,7 @@ def load():
     L.gf_beam_eval.argtypes = [C.c_int, dp, dp, dp, dp, dp, dp, dp]
     L.gf_shell_eval.argtypes = [C.c_int, dp, dp, dp, dp, dp, dp]
     L.gf_bandsystem.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.POINTER(C.c_int)]
+    L.gf_btdsystem.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp, dp, C.POINTER(C.c_int)]
     L.gf_reset_state.argtypes = [vp]
     L.gf_timing_enable.argtypes = [vp, C.c_int]
     L.gf_timing_read.argtypes = [vp, dp, C.POINTER(C.c_longlong), C.c_int]
@@ -277,6 +278,18 @@ def bandsystem(A_rows, hb, b):
     fail = C.c_int(0)
     check(load().gf_bandsystem(n, hb, _dp(A_rows), _dp(np.ascontiguousarray(b, float)), _dp(x), C.byref(fail)),
           "gf_bandsystem")
+    return x, fail.value
+
+
+def btdsystem(A_blocks, B_blocks, C_blocks, b):
+    """Solve a block-tridiagonal system with the device cyclic reduction;
+    blocks [nblk, m, m] (A[i] couples block row i to i-1, C[i] to i+1)."""
+    A_blocks, B_blocks, C_blocks = (np.ascontiguousarray(v, float) for v in (A_blocks, B_blocks, C_blocks))
+    nblk, m = B_blocks.shape[0], B_blocks.shape[1]
+    x = np.zeros(nblk * m)
+    fail = C.c_int(0)
+    check(load().gf_btdsystem(m, nblk, _dp(A_blocks), _dp(B_blocks), _dp(C_blocks),
+                              _dp(np.ascontiguousarray(b, float)), _dp(x), C.byref(fail)), "gf_btdsystem")
     return x, fail.value
 
 

@@ -237,6 +237,13 @@ int gf_beam_eval(int count, const double *x, const double *cs, const double *pro
  * ab_rows[i*(2hb+1) + (c - i + hb)] = A(i, c) for |i - c| <= hb (entries with c
  * outside [0, n) are ignored).  *info_fail = 1 on a zero or non-finite pivot. */
 int gf_bandsystem(int n, int hb, const double *ab_rows, const double *b, double *x, int *info_fail);
+/* Block-tridiagonal solve with the cyclic-reduction solver the narrow-band
+ * models use (no counterpart in the reference; a test entry point): nblk block
+ * rows of m x m row-major blocks, A[i] coupling row i to i-1 (A[0] ignored),
+ * B[i] the diagonal, C[i] coupling to i+1 (C[nblk-1] ignored), 1 <= m <= 32.
+ * info_fail = 1 on a zero or non-finite pivot. */
+int gf_btdsystem(int m, int nblk, const double *A, const double *B, const double *C, const double *b, double *x,
+                 int *info_fail);
 
 /* ---- element-level parity entry (Shell_1::Mount on the device) ------------
  * x[18] reference coordinates of nodes 1..6, prop[GF_SHELL_NPROP], d[36]
