@@ -87,9 +87,13 @@ def cpu_baseline(m, p, kind, gpu_iterations, budget_s=30.0):
             "iterations_timed": iters}
 
 
-def measure(m, p, precision, steps, warmup, timing):
-    """Warmup + `steps` timed full solves of one resident model; the state is
-    reset on the device before each solve."""
+def measure(m, p, precision, steps, warmup, probe_steps):
+    """Warmup + `steps` timed full solves of one resident model (the state is
+    reset on the device before each solve), with no instrumentation in the timed
+    region.  Then `probe_steps` further solves with per-phase HIP events and HIP
+    event pairs around every launch of the roofline candidates: the
+    attribution of device time (recording events between the launches of the
+    timed region would cost about a quarter of its throughput)."""
     import torch
     from giraffe_amd import runtime
 
@@ -102,11 +106,6 @@ def measure(m, p, precision, steps, warmup, timing):
         ts = time.perf_counter()
         dm.static_solve(p)
         log(f"[{precision}] warmup solve {w} {1e3 * (time.perf_counter() - ts):.1f} ms")
-    if timing:
-        dm.timing(True)
-        dm.timing_read(reset=True)
-        dm.kernel_probe(PROBES_BTD if info["block"] else PROBES_BAND)
-        dm.kernel_probe_read(reset=True)
     st0 = dm.stats()
     lat, reps = [], []
     torch.cuda.synchronize()
@@ -120,11 +119,23 @@ def measure(m, p, precision, steps, warmup, timing):
         log(f"[{precision}] timed solve {len(lat) - 1}: {1e3 * lat[-1]:.1f} ms, {rep.total_iterations} iterations")
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    ph = dm.timing_read(reset=True) if timing else {}
-    kp = dm.kernel_probe_read(reset=True) if timing else {}
-    if timing:
-        dm.kernel_probe([])
     st1 = dm.stats()
+    ph, kp = {}, {}
+    if probe_steps > 0:
+        dm.timing(True)
+        dm.timing_read(reset=True)
+        dm.kernel_probe(PROBES_BTD if info["block"] else PROBES_BAND)
+        dm.kernel_probe_read(reset=True)
+        for k in range(probe_steps):
+            dm.reset()
+            ts = time.perf_counter()
+            dm.static_solve(p)
+            log(f"[{precision}] attribution solve {k}: {1e3 * (time.perf_counter() - ts):.1f} ms (with events)")
+        torch.cuda.synchronize()
+        ph = dm.timing_read(reset=True)
+        kp = dm.kernel_probe_read(reset=True)
+        dm.kernel_probe([])
+        dm.timing(False)
     lat_ms = np.array(lat) * 1e3
     return {"info": info, "wall": wall, "lat_ms": lat_ms, "reps": reps, "phases": ph, "kernels": kp,
             "refine_sweeps": st1["refine_sweeps"] - st0["refine_sweeps"],
@@ -336,7 +347,7 @@ def serve_cpu_baseline(reqs, budget_s):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="catenary", choices=["catenary", "plate", "rollup", "serve", "serve-mixed"])
     ap.add_argument("--requests", type=int, default=2000, help="serve: requests in the stream")
@@ -350,7 +361,9 @@ def main():
                     help="solves of the other precision, reported beside the main figure (0: skip)")
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "pardiso", "band", "none"])
     ap.add_argument("--cpu-budget", type=float, default=30.0, help="seconds of CPU baseline sample")
-    ap.add_argument("--no-timing", action="store_true", help="do not record per-phase HIP events")
+    ap.add_argument("--no-timing", action="store_true", help="skip the attribution solves (phases, roofline)")
+    ap.add_argument("--probe-steps", type=int, default=2,
+                    help="solves after the timed region with per-phase and per-launch HIP events")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="before the workload, copy a buffer of this many MiB once on the device (a plain copy "
                          "kernel of known size, the reference point for scaling HBM counter readings)")
@@ -381,7 +394,7 @@ def main():
     if args.workload.startswith("serve"):
         return serve(args)
     m, p, desc = workload(args.workload)
-    r = measure(m, p, args.precision, args.steps, args.warmup, not args.no_timing)
+    r = measure(m, p, args.precision, args.steps, args.warmup, 0 if args.no_timing else args.probe_steps)
     info, reps = r["info"], r["reps"]
     its, solves = reps[-1][0], reps[-1][1]
     out = {
@@ -410,12 +423,16 @@ def main():
                              "fp64_fallbacks": r["fallbacks"], "tangent_solves": r["solves"]}
     if r["phases"]:
         ph = r["phases"]
-        out["phases_ms_per_solve"] = {k: v[0] / args.steps for k, v in ph.items()}
-        out["phase_launches_per_solve"] = {k: v[1] / args.steps for k, v in ph.items()}
+        n_att = args.probe_steps
+        out["phases_ms_per_solve"] = {k: v[0] / n_att for k, v in ph.items()}
+        out["phase_launches_per_solve"] = {k: v[1] / n_att for k, v in ph.items()}
         out["roofline"] = roofline(m, r, args.workload)
+        out["roofline"]["measured"] = (f"HIP event pairs around every launch of the candidate kernels on the model's "
+                                       f"stream, over {n_att} solves run after the timed region (events between the "
+                                       f"timed launches would cost about a quarter of the throughput)")
     if args.compare_steps > 0:
         other = "mixed" if args.precision == "fp64" else "fp64"
-        c = measure(m, p, other, args.compare_steps, 1, False)
+        c = measure(m, p, other, args.compare_steps, 1, 0)
         out["compare"] = {"precision": other, "solves_per_s": args.compare_steps / c["wall"],
                           "p50_ms": float(np.percentile(c["lat_ms"], 50)),
                           "p99_ms": float(np.percentile(c["lat_ms"], 99)),
