@@ -359,7 +359,9 @@ __device__ void fac_step(const Fac &F, PairSmem<T, MT> &sm, bool active, int L, 
     if (active && f && lane < MT) {
         // z of the eliminated neighbour: Binv_e f_e (f_e staged in sm.z)
         if (doE && lane < m) {
-            for (int j = 0; j < m; j++) zl += sm.e[wv][0][lane * cfg<MT>::LD + j] * sm.z[wv][j];
+            // (the padding entries of B^-1 and z are zero: all MT terms are exact)
+#pragma unroll
+            for (int j = 0; j < MT; j++) zl += sm.e[wv][0][lane * cfg<MT>::LD + j] * sm.z[wv][j];
             if (own) zout[(size_t)e * m + lane] = (double)zl;
         }
     }
@@ -400,7 +402,8 @@ __device__ void fac_step(const Fac &F, PairSmem<T, MT> &sm, bool active, int L, 
     __syncthreads();
     if (active && f && wv == 0 && lane < m) {
         T s = T(0);
-        for (int j = 0; j < m; j++)
+#pragma unroll
+        for (int j = 0; j < MT; j++)
             s += sm.k[0][lane * cfg<MT>::LD + j] * sm.z[0][j] + sm.k[1][lane * cfg<MT>::LD + j] * sm.z[1][j];
         f[(size_t)k * m + lane] = fk - (double)s;
     }
@@ -421,7 +424,9 @@ __device__ void top_step(const Fac &F, PairSmem<T, MT> &sm, int levels, const do
         store_elim<T, MT>(col, &sm.e[0][0][0], (T *)F.Bi, (T *)F.X, (T *)F.Y, e, m, true, f == nullptr);
         if (f && lane < m) {
             T s = T(0);
-            for (int j = 0; j < m; j++) s += sm.e[0][0][lane * cfg<MT>::LD + j] * (T)f[(size_t)e * m + j];
+#pragma unroll
+            for (int j = 0; j < MT; j++)
+                if (j < m) s += sm.e[0][0][lane * cfg<MT>::LD + j] * (T)f[(size_t)e * m + (j < m ? j : 0)];
             zout[(size_t)e * m + lane] = (double)s;
         }
     }
@@ -430,7 +435,10 @@ __device__ void top_step(const Fac &F, PairSmem<T, MT> &sm, int levels, const do
 
 // Back-substitution of level L for eliminated rows [tid0 .. ) strided by nthr:
 // x_e = z_e - X_e x_l - Y_e x_r (x holds z_e on entry for eliminated rows).
-template <typename T>
+// The dot products run over MT (the padded block size) with clamped indices and
+// masked terms, in chunks of 16, so each chunk's loads are in flight together;
+// the terms are subtracted in the order j = 0 .. m-1, left block first.
+template <typename T, int MT>
 __device__ __forceinline__ void back_rows(const T *X, const T *Y, int m, int L, int nL, double *x, int tid0, int nthr)
 {
     const int ne = (nL + 1) / 2;
@@ -438,16 +446,28 @@ __device__ __forceinline__ void back_rows(const T *X, const T *Y, int m, int L, 
     for (int t = tid0; t < ne * m; t += nthr) {
         const int pe = 2 * (t / m), i = t % m;
         const int e = lvl_row(pe, L);
+        const bool hasL = pe >= 1, hasR = pe + 1 < nL;
+        const int l = hasL ? lvl_row(pe - 1, L) : e, r = hasR ? lvl_row(pe + 1, L) : e;
         double s = x[(size_t)e * m + i];
-        if (pe >= 1) {
-            const int l = lvl_row(pe - 1, L);
-            const T *Xr = X + (size_t)e * bs + (size_t)i * m;
-            for (int j = 0; j < m; j++) s -= (double)Xr[j] * x[(size_t)l * m + j];
-        }
-        if (pe + 1 < nL) {
-            const int r = lvl_row(pe + 1, L);
-            const T *Yr = Y + (size_t)e * bs + (size_t)i * m;
-            for (int j = 0; j < m; j++) s -= (double)Yr[j] * x[(size_t)r * m + j];
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            const T *Mr = (side == 0 ? X : Y) + (size_t)e * bs + (size_t)i * m;
+            const double *xn = x + (size_t)(side == 0 ? l : r) * m;
+            const bool has = side == 0 ? hasL : hasR;
+#pragma unroll
+            for (int c0 = 0; c0 < MT; c0 += 16) {
+                T mv[16];
+                double xv[16];
+#pragma unroll
+                for (int j = 0; j < 16; j++) {
+                    const int jj = c0 + j < m ? c0 + j : m - 1;
+                    mv[j] = Mr[jj];
+                    xv[j] = xn[jj];
+                }
+#pragma unroll
+                for (int j = 0; j < 16; j++)
+                    if (has && c0 + j < m) s -= (double)mv[j] * xv[j];
+            }
         }
         x[(size_t)e * m + i] = s;
     }
@@ -489,7 +509,7 @@ __global__ __launch_bounds__(128 * cfg<MT>::PAIRS) void btd_tail_kernel(Fac F, i
     if (!f) return;
     // back-substitution of levels levels-1 .. Lb
     for (int L = levels - 1; L >= Lb; L--) {
-        back_rows<T>((const T *)F.X, (const T *)F.Y, F.m, L, nblk >> L, x, threadIdx.x, blockDim.x);
+        back_rows<T, MT>((const T *)F.X, (const T *)F.Y, F.m, L, nblk >> L, x, threadIdx.x, blockDim.x);
         __syncthreads();
     }
 }
@@ -536,7 +556,7 @@ __global__ __launch_bounds__(256) void btd_fwd_kernel(const T *Bi, const T *Lsv,
     fwd_row<T>(Bi, Lsv, Rsv, m, L, nL, q, f, zout, sz[wave]);
 }
 
-template <typename T>
+template <typename T, int MT>
 __global__ __launch_bounds__(1024) void btd_fwd_tail_kernel(const T *Bi, const T *X, const T *Y, const T *Lsv,
                                                             const T *Rsv, const long long *save, int m, int L0,
                                                             int levels, int nL0, double *f, double *x)
@@ -562,15 +582,15 @@ __global__ __launch_bounds__(1024) void btd_fwd_tail_kernel(const T *Bi, const T
     for (int L = levels - 1; L >= L0; L--) {
         int n = nL0;
         for (int j = L0; j < L; j++) n /= 2;
-        back_rows<T>(X, Y, m, L, n, x, threadIdx.x, blockDim.x);
+        back_rows<T, MT>(X, Y, m, L, n, x, threadIdx.x, blockDim.x);
         __syncthreads();
     }
 }
 
-template <typename T>
+template <typename T, int MT>
 __global__ __launch_bounds__(256) void btd_back_kernel(const T *X, const T *Y, int m, int L, int nL, double *x)
 {
-    back_rows<T>(X, Y, m, L, nL, x, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+    back_rows<T, MT>(X, Y, m, L, nL, x, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
 }  // namespace
@@ -658,7 +678,7 @@ static int btd_factor_t(gf_btd &b, hipStream_t s, int *d_flag, double *f, double
         for (int L = b.back_level - 1; L >= 0; L--) {
             const int nL = b.lvl_n[L];
             const int work = (nL + 1) / 2 * b.m;
-            hipLaunchKernelGGL((btd_back_kernel<T>), dim3((work + 255) / 256), dim3(256), 0, s, (const T *)b.X,
+            hipLaunchKernelGGL((btd_back_kernel<T, MT>), dim3((work + 255) / 256), dim3(256), 0, s, (const T *)b.X,
                                (const T *)b.Y, b.m, L, nL, x);
             launches++;
         }
@@ -689,7 +709,7 @@ int gf_btd_factor_solve(gf_btd &b, hipStream_t s, int *d_flag, const double *rhs
     return btd_factor_dispatch<float>(b, s, d_flag, work, x);
 }
 
-template <typename T> static int btd_solve_t(gf_btd &b, hipStream_t s, double *f, double *x)
+template <typename T, int MT> static int btd_solve_t(gf_btd &b, hipStream_t s, double *f, double *x)
 {
     const T *Bi = (const T *)b.Bi, *X = (const T *)b.X, *Y = (const T *)b.Y;
     const T *Lsv = (const T *)b.Lsv, *Rsv = (const T *)b.Rsv;
@@ -702,13 +722,13 @@ template <typename T> static int btd_solve_t(gf_btd &b, hipStream_t s, double *f
         launches++;
     }
     const int nL0 = b.tail_level < b.levels ? b.lvl_n[b.tail_level] : 1;
-    hipLaunchKernelGGL((btd_fwd_tail_kernel<T>), dim3(1), dim3(1024), 0, s, Bi, X, Y, Lsv, Rsv,
+    hipLaunchKernelGGL((btd_fwd_tail_kernel<T, MT>), dim3(1), dim3(1024), 0, s, Bi, X, Y, Lsv, Rsv,
                        (const long long *)b.d_save, b.m, b.tail_level, b.levels, nL0, f, x);
     launches++;
     for (int L = b.tail_level - 1; L >= 0; L--) {
         const int nL = b.lvl_n[L];
         const int work = (nL + 1) / 2 * b.m;
-        hipLaunchKernelGGL((btd_back_kernel<T>), dim3((work + 255) / 256), dim3(256), 0, s, X, Y, b.m, L, nL, x);
+        hipLaunchKernelGGL((btd_back_kernel<T, MT>), dim3((work + 255) / 256), dim3(256), 0, s, X, Y, b.m, L, nL, x);
         launches++;
     }
     GF_HIP(hipGetLastError());
@@ -724,6 +744,7 @@ int gf_btd_solve(gf_btd &b, hipStream_t s, const double *rhs, double *x, int n, 
         return GF_E_ARG;
     }
     if (rhs != work) GF_HIP(hipMemcpyAsync(work, rhs, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
-    if (b.elem_size == 8) return btd_solve_t<double>(b, s, work, x);
-    return btd_solve_t<float>(b, s, work, x);
+    if (b.elem_size == 8)
+        return b.mt == 16 ? btd_solve_t<double, 16>(b, s, work, x) : btd_solve_t<double, 32>(b, s, work, x);
+    return b.mt == 16 ? btd_solve_t<float, 16>(b, s, work, x) : btd_solve_t<float, 32>(b, s, work, x);
 }
