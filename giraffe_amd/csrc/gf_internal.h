@@ -339,12 +339,19 @@ struct gf_model {
     double t_ms[GF_NPHASE] = {0, 0, 0};
     long long launches[GF_NPHASE] = {0, 0, 0};
     gf_kprobe kprobe;
+    // FP64 solve sequence replayed from a graph (GF_NO_GRAPH in the environment
+    // at model creation switches this off)
+    bool use_graph = true;
+    hipGraphExec_t solve_graph = nullptr;
+    int graph_launches = 0;
 };
 
 // ---- kernel launchers (defined in the .hip files) ----------------------------
 int gf_launch_fill(hipStream_t s, double *p, size_t n, double v);
-// zeroes three double ranges in one launch
-int gf_launch_clear3(hipStream_t s, double *a, size_t na, double *b, size_t nb, double *c, size_t nc);
+// zeroes three double ranges in one launch; entries dfirst + k dstride (k < dcount)
+// of the second range are set to 1
+int gf_launch_clear3(hipStream_t s, double *a, size_t na, double *b, size_t nb, double *c, size_t nc,
+                     size_t dfirst = 0, size_t dstride = 0, size_t dcount = 0);
 int gf_launch_fill_bytes(hipStream_t s, void *p, size_t bytes);
 // Returns the number of kernel launches issued (>= 0) or a negative status.
 int gf_beam_assemble(gf_model *m, double lf);

@@ -19,12 +19,20 @@ __global__ void fill_kernel(double *p, size_t n, double v)
     for (; i < n; i += stride) p[i] = v;
 }
 
-__global__ __launch_bounds__(256) void clear3_kernel(double *a, size_t na, double *b, size_t nb, double *c, size_t nc)
+// Zeroes three ranges; in the second, the entries dfirst + k dstride (k < dcount)
+// get 1 instead (the unit diagonal of the padding rows of the global matrix).
+__global__ __launch_bounds__(256) void clear3_kernel(double *a, size_t na, double *b, size_t nb, double *c, size_t nc,
+                                                     size_t dfirst, size_t dstride, size_t dcount)
 {
     const size_t n = na + nb + nc;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         double *p = i < na ? a + i : i < na + nb ? b + (i - na) : c + (i - na - nb);
-        *p = 0.0;
+        double v = 0.0;
+        if (i >= na + dfirst && i < na + nb) {
+            const size_t d = i - na - dfirst;
+            if (d % dstride == 0 && d / dstride < dcount) v = 1.0;
+        }
+        *p = v;
     }
 }
 
@@ -200,12 +208,6 @@ __global__ void node_commit_kernel(int nn, double *__restrict__ copy, const doub
 }
 
 // Unit diagonal for the padding rows past n_free.
-__global__ void pad_diag_kernel(gf_gmat gm, int npad, int n_free)
-{
-    int i = n_free + blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= npad) return;
-    gm_add(gm, i, i, 1.0);
-}
 
 }  // namespace
 
@@ -225,11 +227,14 @@ int gf_launch_fill(hipStream_t s, double *p, size_t n, double v)
     return GF_OK;
 }
 
-int gf_launch_clear3(hipStream_t s, double *a, size_t na, double *b, size_t nb, double *c, size_t nc)
+int gf_launch_clear3(hipStream_t s, double *a, size_t na, double *b, size_t nb, double *c, size_t nc, size_t dfirst,
+                     size_t dstride, size_t dcount)
 {
     const size_t n = na + nb + nc;
     if (n == 0) return GF_OK;
-    hipLaunchKernelGGL(clear3_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, a, na, b, nb, c, nc);
+    if (dstride == 0) dcount = 0, dstride = 1;
+    hipLaunchKernelGGL(clear3_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, a, na, b, nb, c, nc, dfirst, dstride,
+                       dcount);
     GF_HIP(hipGetLastError());
     return GF_OK;
 }
@@ -283,12 +288,3 @@ int gf_node_commit(gf_model *m)
     return GF_OK;
 }
 
-int gf_pad_diag(gf_model *m)
-{
-    int n = m->npad - m->n_free;
-    if (n <= 0) return GF_OK;
-    hipLaunchKernelGGL(pad_diag_kernel, dim3((n + 63) / 64), dim3(64), 0, m->stream, gf_model_gmat(m), m->npad,
-                       m->n_free);
-    GF_HIP(hipGetLastError());
-    return GF_OK;
-}
