@@ -51,6 +51,8 @@ class GfModelDesc(C.Structure):
         ("tab_val", _dp),
         ("gravity", C.c_double * 3),
         ("gravity_on", C.c_int),
+        ("beam_number", _ip),
+        ("shell_number", _ip),
     ]
 
 

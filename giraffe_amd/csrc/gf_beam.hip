@@ -306,6 +306,45 @@ __global__ __launch_bounds__(256) void beam_commit_kernel(int nb, size_t ld, con
     gps[(o + 14) * ld + e] = k.dz.z;
 }
 
+// Results of the committed state (Beam_1::WriteResults src/Beam_1.cpp:444-458):
+// per element the means over both Gauss points of sigma_r = D epsilon_r and of
+// epsilon_r = (eta_r, kappa_r).  With zero incremental displacements the
+// kinematics reduce to the committed Q_i, kappa_i, z'_i, so eta_r and kappa_r
+// equal the values the last Mount computed at the converged configuration.
+// out: [nb][12] in colour-sorted order.
+__global__ __launch_bounds__(256) void beam_results_kernel(int nb, size_t ld, const double *__restrict__ cst,
+                                                           const double *__restrict__ gps, double *__restrict__ out)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nb) return;
+    const double GA1 = cst[(BC_D + 0) * ld + e], GA2 = cst[(BC_D + 1) * ld + e], EA = cst[(BC_D + 2) * ld + e];
+    const double EI1 = cst[(BC_D + 3) * ld + e], EI2 = cst[(BC_D + 4) * ld + e], EI12 = cst[(BC_D + 5) * ld + e];
+    const double GJ = cst[(BC_D + 6) * ld + e];
+    double d[18];
+#pragma unroll
+    for (int i = 0; i < 18; i++) d[i] = 0.0;
+    double sg[2][6], ep[2][6];
+#pragma unroll
+    for (int gp = 0; gp < 2; gp++) {
+        BeamGP k;
+        beam_kin(cst, gps, ld, e, gp, d, k);
+        ep[gp][0] = k.eta.x; ep[gp][1] = k.eta.y; ep[gp][2] = k.eta.z;
+        ep[gp][3] = k.kap.x; ep[gp][4] = k.kap.y; ep[gp][5] = k.kap.z;
+        sg[gp][0] = GA1 * k.eta.x;
+        sg[gp][1] = GA2 * k.eta.y;
+        sg[gp][2] = EA * k.eta.z;
+        sg[gp][3] = EI1 * k.kap.x + EI12 * k.kap.y;
+        sg[gp][4] = EI12 * k.kap.x + EI2 * k.kap.y;
+        sg[gp][5] = GJ * k.kap.z;
+    }
+    double *o = out + 12 * (size_t)e;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        o[i] = (sg[0][i] + sg[1][i]) / 2;
+        o[6 + i] = (ep[0][i] + ep[1][i]) / 2;
+    }
+}
+
 }  // namespace
 
 // Gauss abscissa as written in src/Beam_1.cpp:629-632.
@@ -374,6 +413,15 @@ int gf_beam_commit(gf_model *m)
     int grid = (2 * m->nb + 255) / 256;
     hipLaunchKernelGGL(beam_commit_kernel, dim3(grid), dim3(256), 0, m->stream, m->nb, (size_t)m->nb, m->d_bconn,
                        m->d_bconst, m->d_bgp, m->d_disp);
+    GF_HIP(hipGetLastError());
+    return GF_OK;
+}
+
+int gf_beam_results(gf_model *m, double *d_out)
+{
+    if (m->nb == 0) return GF_OK;
+    hipLaunchKernelGGL(beam_results_kernel, dim3((m->nb + 255) / 256), dim3(256), 0, m->stream, m->nb, (size_t)m->nb,
+                       m->d_bconst, m->d_bgp, d_out);
     GF_HIP(hipGetLastError());
     return GF_OK;
 }

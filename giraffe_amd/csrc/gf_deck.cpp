@@ -116,6 +116,7 @@ struct gf_deck {
     // resolved descriptor arrays
     std::vector<unsigned char> fix;
     std::vector<int> beam_conn, shell_conn, load_kind, load_set_off, load_set_items, load_flags, load_tab_off;
+    std::vector<int> beam_number, shell_number;
     std::vector<double> beam_cs, beam_prop, shell_prop, load_cs, tab_time, tab_val;
     gf_model_desc desc{};
 };
@@ -411,6 +412,7 @@ int resolve(gf_deck &d)
                 GF_TRY(node(e.nodes[a], k));
                 d.beam_conn.push_back(k);
             }
+            d.beam_number.push_back(e.id);
             double cs[9];
             cs_rows(ci->second, cs);
             d.beam_cs.insert(d.beam_cs.end(), cs, cs + 9);
@@ -429,6 +431,7 @@ int resolve(gf_deck &d)
                 GF_TRY(node(e.nodes[a], k));
                 d.shell_conn.push_back(k);
             }
+            d.shell_number.push_back(e.id);
             double prop[GF_SHELL_NPROP] = {mi->second[0], mi->second[1], mi->second[2], ti->second};
             d.shell_prop.insert(d.shell_prop.end(), prop, prop + GF_SHELL_NPROP);
         }
@@ -498,6 +501,19 @@ int resolve(gf_deck &d)
     g.tab_time = d.tab_time.data();
     g.tab_val = d.tab_val.data();
     for (int k = 0; k < 3; k++) g.gravity[k] = d.G[k];
+    {
+        // element numbers for the results writer, when they number the elements 1..N
+        const size_t ne = d.beam_number.size() + d.shell_number.size();
+        std::vector<char> seen(ne + 1, 0);
+        bool perm = true;
+        for (const std::vector<int> *v : {&d.beam_number, &d.shell_number})
+            for (int id : *v) {
+                if (id < 1 || (size_t)id > ne || seen[id]) perm = false;
+                else seen[id] = 1;
+            }
+        g.beam_number = perm ? d.beam_number.data() : nullptr;
+        g.shell_number = perm ? d.shell_number.data() : nullptr;
+    }
     g.gravity_on = d.g_on;
     return GF_OK;
 }
@@ -578,8 +594,12 @@ extern "C" int gf_deck_write(const char *path, const gf_model_desc *g, const gf_
         css[k] = id;
         return id;
     };
-    std::ostringstream el;
+    // element lines in element-number order (numbers from the description or 1..nb, nb+1..)
+    auto beam_num = [&](int e) { return g->beam_number ? g->beam_number[e] : e + 1; };
+    auto shell_num = [&](int e) { return g->shell_number ? g->shell_number[e] : nb + e + 1; };
+    std::vector<std::string> lines((size_t)nb + ns);
     for (int e = 0; e < nb; e++) {
+        std::ostringstream el;
         const double *pr = g->beam_prop + GF_BEAM_NPROP * (size_t)e;
         double A = pr[3], I = pr[4];
         if (pr[4] != pr[5] || pr[6] != 0.0 || std::fabs(pr[7] - 2.0 * I) > 1e-12 * I)
@@ -590,22 +610,31 @@ extern "C" int gf_deck_write(const char *path, const gf_model_desc *g, const gf_
         auto it = tubes.find(tk);
         if (it != tubes.end()) sec = it->second;
         else { sec = (int)tubes.size() + 1; tubes[tk] = sec; }
-        el << "Beam_1\t" << e + 1 << "\tMat\t" << mat_id(pr[0], pr[1], pr[2]) << "\tSec\t" << sec << "\tCS\t"
+        el << "Beam_1\t" << beam_num(e) << "\tMat\t" << mat_id(pr[0], pr[1], pr[2]) << "\tSec\t" << sec << "\tCS\t"
            << cs_id(g->beam_cs + 9 * (size_t)e) << "\tNodes";
         for (int a = 0; a < 3; a++) el << "\t" << g->beam_conn[3 * e + a] + 1;
         if (pr[8] != 0.0) el << "\tPreTension\t" << num(pr[8]);
         el << "\n";
+        const int k = beam_num(e) - 1;
+        if (k < 0 || k >= nb + ns || !lines[k].empty()) return fail("element numbers are not a permutation of 1..N");
+        lines[k] = el.str();
     }
     for (int e = 0; e < ns; e++) {
+        std::ostringstream el;
         const double *pr = g->shell_prop + GF_SHELL_NPROP * (size_t)e;
         int sec;
         auto it = thick.find(pr[3]);
         if (it != thick.end()) sec = it->second;
         else { sec = (int)thick.size() + 1; thick[pr[3]] = sec; }
-        el << "Shell_1\t" << nb + e + 1 << "\tMat\t" << mat_id(pr[0], pr[1], pr[2]) << "\tSec\t" << sec << "\tNodes";
+        el << "Shell_1\t" << shell_num(e) << "\tMat\t" << mat_id(pr[0], pr[1], pr[2]) << "\tSec\t" << sec << "\tNodes";
         for (int a = 0; a < 6; a++) el << "\t" << g->shell_conn[6 * e + a] + 1;
         el << "\n";
+        const int k = shell_num(e) - 1;
+        if (k < 0 || k >= nb + ns || !lines[k].empty()) return fail("element numbers are not a permutation of 1..N");
+        lines[k] = el.str();
     }
+    std::ostringstream el;
+    for (const std::string &ln : lines) el << ln;
     // node and element sets for loads and constraints
     std::ostringstream nsets, esets, lo, co;
     int n_nsets = 0, n_esets = 0;
@@ -615,7 +644,7 @@ extern "C" int gf_deck_write(const char *path, const gf_model_desc *g, const gf_
         int r0 = g->load_tab_off[l], r1 = g->load_tab_off[l + 1];
         if (kind == GF_LOAD_SHELL_PRESSURE) {
             esets << "ElementSet\t" << ++n_esets << "\tElements\t" << b - a << "\tList";
-            for (int q = a; q < b; q++) esets << "\t" << nb + g->load_set_items[q] + 1;
+            for (int q = a; q < b; q++) esets << "\t" << shell_num(g->load_set_items[q]);
             esets << "\n";
             lo << "ShellLoad\t" << l + 1 << "\tElementSet\t" << n_esets << "\tAreaUpdate\t"
                << (g->load_flags ? g->load_flags[l] : 0) << "\tNTimes\t" << r1 - r0 << "\n";

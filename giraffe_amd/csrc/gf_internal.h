@@ -268,7 +268,9 @@ struct gf_model {
     std::vector<unsigned char> active, fix;
     std::vector<int> beam_order;    // colour-sorted position -> element id
     std::vector<int> beam_color_off;
+    std::vector<int> beam_number, shell_number;  // deck element numbers (results writer)
     std::vector<double> h_nodes_ref;
+    std::vector<int> h_beam_conn, h_shell_conn;  // description order
     double sol_start = 0, sol_end = 1;
     double t_last = 0, dt = 0;
     double gravity[3] = {0, 0, 0};
@@ -361,6 +363,11 @@ int gf_shell_pressure(gf_model *m, double t);
 int gf_shell_eval_device(int count, const double *x, const double *prop, const double *d, const double *gp,
                          double *K_out, double *f_out);
 int gf_shell_commit(gf_model *m);
+// Gauss-point means of the committed state for the results writer, colour-sorted:
+// Beam_1 [nb][12] (sigma_r, epsilon_r), Shell_1 [ns][24] (n_r1 m_r1 n_r2 m_r2
+// eta_r1 kappa_r1 eta_r2 kappa_r2).
+int gf_beam_results(gf_model *m, double *d_out);
+int gf_shell_results(gf_model *m, double *d_out);
 void gf_shell_host_const(const double *x18, const double *prop, double *rec /*[SC_COUNT]*/,
                          double *prec /*[SP_COUNT]*/, double *gp /*[SG_COUNT]*/);
 gf_gmat gf_model_gmat(const gf_model *m);

@@ -342,6 +342,24 @@ static int create(const gf_model_desc *d, const gf_options *opt, gf_model *m)
     for (int i = 0; i < nn; i++)
         for (int k = 0; k < 3; k++) copy[6 * (size_t)i + k] = d->node_xyz[3 * (size_t)i + k];
     m->h_nodes_ref.assign(d->node_xyz, d->node_xyz + 3 * (size_t)nn);
+    m->h_beam_conn.assign(d->beam_conn, d->beam_conn + 3 * (size_t)nb);
+    m->h_shell_conn.assign(d->shell_conn, d->shell_conn + 6 * (size_t)ns);
+    {
+        // element numbers: a permutation of 1..nb+ns (the reference stores element i as number i+1)
+        m->beam_number.resize(nb);
+        m->shell_number.resize(ns);
+        std::vector<char> seen((size_t)nb + ns + 1, 0);
+        for (int e = 0; e < nb + ns; e++) {
+            const int num = e < nb ? (d->beam_number ? d->beam_number[e] : e + 1)
+                                   : (d->shell_number ? d->shell_number[e - nb] : e + 1);
+            if (num < 1 || num > nb + ns || seen[num]) {
+                gf_set_error("element numbers must be a permutation of 1..n_beams+n_shells");
+                return GF_E_ARG;
+            }
+            seen[num] = 1;
+            (e < nb ? m->beam_number[e] : m->shell_number[e - nb]) = num;
+        }
+    }
     GF_TRY(dalloc(&m->d_disp, 6 * (size_t)nn));
     GF_TRY(dalloc(&m->d_copy, 6 * (size_t)nn));
     GF_TRY(dalloc(&m->d_euler, 3 * (size_t)nn));

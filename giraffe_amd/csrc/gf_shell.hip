@@ -89,11 +89,13 @@ struct ShellKin {
     double g;
 };
 
+// ZERO: the incremental displacements are taken as zero (disp and conn unused).
+template <bool ZERO = false>
 __device__ void shell_kin(const double *__restrict__ cst, const double *__restrict__ gps, size_t ld, int e, int gp,
                           const double *__restrict__ disp, const int *__restrict__ conn, ShellKin &k)
 {
     double u[3] = {0, 0, 0}, u1[3] = {0, 0, 0}, u2[3] = {0, 0, 0};
-    for (int a = 0; a < 6; a++) {
+    for (int a = 0; a < (ZERO ? 0 : 6); a++) {
         const double *p = disp + 6 * (size_t)conn[a * ld + e];
         double c0 = cst[(SC_NU + 6 * gp + a) * ld + e], c1 = cst[(SC_NU1 + 6 * gp + a) * ld + e],
                c2 = cst[(SC_NU2 + 6 * gp + a) * ld + e];
@@ -106,7 +108,7 @@ __device__ void shell_kin(const double *__restrict__ cst, const double *__restri
         }
     }
     double al[3] = {0, 0, 0}, al1[3] = {0, 0, 0}, al2[3] = {0, 0, 0};
-    for (int b = 0; b < 3; b++) {
+    for (int b = 0; b < (ZERO ? 0 : 3); b++) {
         const double *p = disp + 6 * (size_t)conn[(3 + b) * ld + e] + 3;
         double c0 = cst[(SC_NA + 3 * gp + b) * ld + e], c1 = cst[(SC_NA1 + 3 * gp + b) * ld + e],
                c2 = cst[(SC_NA2 + 3 * gp + b) * ld + e];
@@ -152,6 +154,34 @@ __device__ void shell_kin(const double *__restrict__ cst, const double *__restri
     k.eta2.y -= 1.0;
     k.k1 = add3(mtv(Qi, mtv(k.Xi, k.a1)), kr1);
     k.k2 = add3(mtv(Qi, mtv(k.Xi, k.a2)), kr2);
+}
+
+// One point of the through-thickness rule (src/Shell_1.cpp:1083-1145): the
+// strains gamma_a = eta_a + zeta kappa_a x e3 (e3 = (0,0,1)), v and dv of the
+// hyperelastic law at jbar = det of the in-plane stretch, and the stress vectors
+// tau_1, tau_2.
+struct ShellPt {
+    double g11, g12, g13, g21, g22, g23, v, dv;
+    v3<double> tau1, tau2;
+};
+__device__ __forceinline__ ShellPt shell_point(const ShellKin &k, double mu, double la, double zeta)
+{
+    ShellPt p;
+    p.g11 = k.eta1.x + zeta * k.k1.y;
+    p.g12 = k.eta1.y - zeta * k.k1.x;
+    p.g13 = k.eta1.z;
+    p.g21 = k.eta2.x + zeta * k.k2.y;
+    p.g22 = k.eta2.y - zeta * k.k2.x;
+    p.g23 = k.eta2.z;
+    const double jb = (1.0 + p.g11) * (1.0 + p.g22) - p.g12 * p.g21;
+    p.v = (la * (jb * jb * jb - 1.0) + 2.0 * mu * (jb - 1.0)) / (la * jb * jb * jb + 2.0 * mu * jb);
+    p.dv = ((la + 2.0 * mu) * (3.0 * la * jb * jb + 2.0 * mu)) /
+           (jb * jb * (la * jb * jb + 2.0 * mu) * (la * jb * jb + 2.0 * mu));
+    p.tau1 = mk3(mu * p.v * (1.0 + p.g22) + mu * (p.g11 - p.g22), mu * p.v * (-p.g21) + mu * (p.g12 + p.g21),
+                 mu * p.g13);
+    p.tau2 = mk3(mu * p.v * (-p.g12) + mu * (p.g12 + p.g21), mu * p.v * (1.0 + p.g11) + mu * (p.g22 - p.g11),
+                 mu * p.g23);
+    return p;
 }
 
 // Column block I of dN: rows (R) and their coefficients at Gauss point gp.
@@ -206,15 +236,9 @@ __global__ __launch_bounds__(SWG) void shell_assemble_kernel(
             const double J = th / 2.0;
             for (int q = 0; q < 3; q++) {
                 double zeta = th * xi[q] / 2.0;
-                // gamma = eta + zeta kappa x e3 (e3 = (0,0,1))
-                double g11 = k.eta1.x + zeta * k.k1.y, g12 = k.eta1.y - zeta * k.k1.x, g13 = k.eta1.z;
-                double g21 = k.eta2.x + zeta * k.k2.y, g22 = k.eta2.y - zeta * k.k2.x, g23 = k.eta2.z;
-                double jb = (1.0 + g11) * (1.0 + g22) - g12 * g21;
-                double v = (la * (jb * jb * jb - 1.0) + 2.0 * mu * (jb - 1.0)) / (la * jb * jb * jb + 2.0 * mu * jb);
-                double dv = ((la + 2.0 * mu) * (3.0 * la * jb * jb + 2.0 * mu)) /
-                            (jb * jb * (la * jb * jb + 2.0 * mu) * (la * jb * jb + 2.0 * mu));
-                v3<double> tau1 = mk3(mu * v * (1.0 + g22) + mu * (g11 - g22), mu * v * (-g21) + mu * (g12 + g21), mu * g13);
-                v3<double> tau2 = mk3(mu * v * (-g12) + mu * (g12 + g21), mu * v * (1.0 + g11) + mu * (g22 - g11), mu * g23);
+                const ShellPt pt = shell_point(k, mu, la, zeta);
+                const double g11 = pt.g11, g12 = pt.g12, g21 = pt.g21, g22 = pt.g22, v = pt.v, dv = pt.dv;
+                const v3<double> tau1 = pt.tau1, tau2 = pt.tau2;
                 m3<double> C11, C22, C12, C21;
 #pragma unroll
                 for (int i = 0; i < 9; i++) C11.a[i] = C22.a[i] = C12.a[i] = 0.0;
@@ -549,6 +573,53 @@ __global__ __launch_bounds__(256) void shell_commit_kernel(int ns, size_t ld, co
     gps[(o + 20) * ld + e] += k.u2.z;
 }
 
+// Results of the committed state (Shell_1::WriteResults src/Shell_1.cpp:624-666):
+// per element the means over the three Gauss points of n_r1, m_r1, n_r2, m_r2,
+// eta_r1, kappa_r1, eta_r2, kappa_r2.  With zero incremental displacements the
+// kinematics reduce to the committed Q_i, kappa_i, z_x1, z_x2, which hold the
+// converged configuration of the last Mount.  out: [ns][24], colour-sorted.
+__global__ __launch_bounds__(256) void shell_results_kernel(int ns, size_t ld, const double *__restrict__ cst,
+                                                            const double *__restrict__ gps, double *__restrict__ out)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ns) return;
+    const double mu = cst[SC_MU * ld + e], la = cst[SC_LAMBDA * ld + e], th = cst[SC_THICK * ld + e];
+    const double drill = cst[SC_DRILL * ld + e];
+    const double xi[3] = {-0.77459666924148337703585307995648, 0.0, 0.77459666924148337703585307995648};
+    const double wt[3] = {0.55555555555555555555555555555556, 0.88888888888888888888888888888889,
+                          0.55555555555555555555555555555556};
+    const double J = th / 2.0;
+    v3<double> avg[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) avg[i] = mk3(0.0, 0.0, 0.0);
+    for (int gp = 0; gp < 3; gp++) {
+        ShellKin k;
+        shell_kin<true>(cst, gps, ld, e, gp, nullptr, nullptr, k);
+        v3<double> n1 = mk3(0.0, 0.0, 0.0), n2 = n1, m1 = n1, m2 = n1;
+        for (int q = 0; q < 3; q++) {
+            const double zeta = th * xi[q] / 2.0;
+            const ShellPt pt = shell_point(k, mu, la, zeta);
+            const double w = wt[q] * J;
+            n1 = add3(n1, scl3(w, pt.tau1));
+            n2 = add3(n2, scl3(w, pt.tau2));
+            m1 = add3(m1, scl3(w * zeta, mk3(-pt.tau1.y, pt.tau1.x, 0.0)));
+            m2 = add3(m2, scl3(w * zeta, mk3(-pt.tau2.y, pt.tau2.x, 0.0)));
+        }
+        m1.z = drill * k.k1.z;
+        m2.z = drill * k.k2.z;
+        const v3<double> r[8] = {n1, m1, n2, m2, k.eta1, k.k1, k.eta2, k.k2};
+#pragma unroll
+        for (int i = 0; i < 8; i++) avg[i] = add3(avg[i], scl3(1.0 / 3.0, r[i]));
+    }
+    double *o = out + 24 * (size_t)e;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        o[3 * i] = avg[i].x;
+        o[3 * i + 1] = avg[i].y;
+        o[3 * i + 2] = avg[i].z;
+    }
+}
+
 }  // namespace
 
 // ---- host: constants (Shell_1::PreCalc restated for the device layout) ---------
@@ -674,6 +745,15 @@ int gf_shell_commit(gf_model *m)
     if (m->ns == 0) return GF_OK;
     hipLaunchKernelGGL(shell_commit_kernel, dim3((3 * m->ns + 255) / 256), dim3(256), 0, m->stream, m->ns,
                        (size_t)m->ns, m->d_sconn, m->d_sconst, m->d_sgp, m->d_disp);
+    GF_HIP(hipGetLastError());
+    return GF_OK;
+}
+
+int gf_shell_results(gf_model *m, double *d_out)
+{
+    if (m->ns == 0) return GF_OK;
+    hipLaunchKernelGGL(shell_results_kernel, dim3((m->ns + 255) / 256), dim3(256), 0, m->stream, m->ns, (size_t)m->ns,
+                       m->d_sconst, m->d_sgp, d_out);
     GF_HIP(hipGetLastError());
     return GF_OK;
 }

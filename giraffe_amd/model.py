@@ -111,6 +111,9 @@ class Model:
         self.loads = []  # (kind, items, cs9, flags, times, vals[n][6])
         self.gravity = (0.0, 0.0, 0.0)
         self.gravity_on = 0
+        # optional deck element numbers (a permutation of 1..n_beams+n_shells)
+        self.beam_number = None
+        self.shell_number = None
         self._frozen = None
 
     # -- building ---------------------------------------------------------
@@ -249,6 +252,12 @@ class Model:
         d.tab_val = abi.ptr(f["tab_val"], C.c_double)
         d.gravity = (C.c_double * 3)(*self.gravity)
         d.gravity_on = int(self.gravity_on)
+        if self.beam_number is not None:
+            f["beam_number"] = np.ascontiguousarray(self.beam_number, np.int32)
+            d.beam_number = abi.ptr(f["beam_number"], C.c_int)
+        if self.shell_number is not None:
+            f["shell_number"] = np.ascontiguousarray(self.shell_number, np.int32)
+            d.shell_number = abi.ptr(f["shell_number"], C.c_int)
         d._keep = f  # keep arrays alive with the struct
         return d
 

@@ -59,6 +59,7 @@ def load():
         getattr(L, f).argtypes = [vp, dp]
     L.gf_read_tangent_dense.argtypes = [vp, dp, C.c_longlong]
     L.gf_write_results.argtypes = [vp, C.c_char_p]
+    L.gf_read_element_results.argtypes = [vp, dp, dp]
     L.gf_beam_eval.argtypes = [C.c_int, dp, dp, dp, dp, dp, dp, dp]
     L.gf_shell_eval.argtypes = [C.c_int, dp, dp, dp, dp, dp, dp]
     L.gf_bandsystem.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.POINTER(C.c_int)]
@@ -117,6 +118,8 @@ class DeviceModel:
         check(L.gf_model_create(C.byref(self._desc), C.byref(self._opt), C.byref(h)), "gf_model_create")
         self.h = h
         self.nn = model.n_nodes
+        self.nb = model.n_beams
+        self.ns = model.n_shells
 
     def close(self):
         if getattr(self, "h", None):
@@ -176,8 +179,16 @@ class DeviceModel:
         return self._read("gf_read_nodes", (self.nn, 6))
 
     def write_results(self, path):
-        """Configuration results file (node section) of the last converged configuration."""
+        """Configuration results file (nodes and elements) of the last converged configuration."""
         check(load().gf_write_results(self.h, os.fsencode(path)), "gf_write_results")
+
+    def element_results(self):
+        """Gauss-point means of the committed state: Beam_1 [nb, 12] (sigma_r, epsilon_r) and
+        Shell_1 [ns, 24] (n_r1 m_r1 n_r2 m_r2 eta_r1 kappa_r1 eta_r2 kappa_r2)."""
+        b = np.zeros((self.nb, 12))
+        sh = np.zeros((self.ns, 24))
+        check(load().gf_read_element_results(self.h, _dp(b), _dp(sh)), "gf_read_element_results")
+        return b, sh
 
     def disp(self):
         return self._read("gf_read_disp", (self.nn, 6))

@@ -88,6 +88,12 @@ typedef struct gf_model_desc {
 
     double gravity[3];               /* GravityData G                                 */
     int gravity_on;                  /* 1: BoolTable ramp 0->1 over the solution     */
+
+    /* Element numbers as in the deck (Element::number; together a permutation of
+     * 1..n_beams+n_shells), used by the results writer.  NULL: Beam_1 numbered
+     * 1..n_beams, then Shell_1 n_beams+1..n_beams+n_shells. */
+    const int *beam_number;          /* [n_beams] or NULL                             */
+    const int *shell_number;         /* [n_shells] or NULL                            */
 } gf_model_desc;
 
 /* Static solution-step and ConvergenceCriteria parameters, in the order of
@@ -207,10 +213,22 @@ int gf_kernel_probe_read(gf_model *m, double *ms_out, long long *launches_out, d
 /* Per node: X Y Z of the last converged configuration and the Euler rotation
  * vector (Node::WriteResults src/Node.cpp:208-219). */
 int gf_read_nodes(gf_model *m, double *out /* [n_nodes][6] */);
+/* Per element, Gauss-point averages of the back-rotated generalised stresses and
+ * strains of the last converged configuration, in description order:
+ *   Beam_1  [n_beams][12]:  sigma_r (n_r, m_r), epsilon_r (eta_r, kappa_r), each
+ *           the mean of the two Gauss points (Beam_1::WriteResults
+ *           src/Beam_1.cpp:444-458);
+ *   Shell_1 [n_shells][24]: n_r1, m_r1, n_r2, m_r2, eta_r1, kappa_r1, eta_r2,
+ *           kappa_r2, each the mean of the three Gauss points (Shell_1::WriteResults
+ *           src/Shell_1.cpp:624-666).
+ * The reference keeps the values of its last Mount, which runs at the converged
+ * configuration; here they are evaluated from the committed Gauss-point state,
+ * which holds the same configuration.  Either pointer may be NULL. */
+int gf_read_element_results(gf_model *m, double *beam_out, double *shell_out);
 /* Configuration results text file of the last converged configuration, as
  * PostFiles::WriteConfigurationResults writes it (src/PostFiles.cpp:1429-1464):
- * the node section with one Node::WriteResults line per node
- * (src/Node.cpp:208-219); the element section header only (no element lines). */
+ * one Node::WriteResults line per node (src/Node.cpp:208-219), then one
+ * Beam_1 / Shell_1 WriteResults line per element in element-number order. */
 int gf_write_results(gf_model *m, const char *path);
 /* Per node and DOF: incremental displacement d, or residual r (0 where not free). */
 int gf_read_disp(gf_model *m, double *out /* [n_nodes][6] */);

@@ -201,9 +201,9 @@ static void put3(double *M, int ld, int r0, int c0, const double *a)
         for (int j = 0; j < 3; j++) M[(r0 + i) * ld + c0 + j] = a[3 * i + j];
 }
 
-void gfo_beam_mount(const gfo_beam_const *c, const double *gp, const double *d, double *K, double *f)
+void gfo_beam_mount(const gfo_beam_const *c, const double *gp, const double *d, double *K, double *f, double *res)
 {
-    double Kl[18 * 18], fl[18];
+    double Kl[18 * 18], fl[18], sg[2][6], ep[2][6];
     memset(Kl, 0, sizeof(Kl));
     memset(fl, 0, sizeof(fl));
     for (int g = 0; g < 2; g++) {
@@ -242,6 +242,7 @@ void gfo_beam_mount(const gfo_beam_const *c, const double *gp, const double *d, 
         double eps[6], sig[6];
         for (int i = 0; i < 3; i++) { eps[i] = k.eta[i]; eps[3 + i] = k.kap[i]; }
         mm(6, 6, 1, c->D, eps, sig);
+        for (int i = 0; i < 6; i++) { sg[g][i] = sig[i]; ep[g][i] = eps[i]; }
         double n[3], m[3];
         m3_vec(k.QQ, sig, n);
         m3_vec(k.QQ, sig + 3, m);
@@ -294,6 +295,12 @@ void gfo_beam_mount(const gfo_beam_const *c, const double *gp, const double *d, 
     mtm(18, 18, 18, T, Kl, TK);
     mm(18, 18, 18, TK, T, K);
     mtm(18, 18, 1, T, fl, f);
+    /* Gauss-point means of sigma_r and epsilon_r (Beam_1::WriteResults src/Beam_1.cpp:444-458) */
+    if (res)
+        for (int i = 0; i < 6; i++) {
+            res[i] = (sg[0][i] + sg[1][i]) / 2;
+            res[6 + i] = (ep[0][i] + ep[1][i]) / 2;
+        }
 }
 
 void gfo_beam_save(const gfo_beam_const *c, double *gp, const double *d)

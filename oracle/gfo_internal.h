@@ -78,9 +78,10 @@ typedef struct gfo_beam_const {
 
 void gfo_beam_precalc(const double *x /*[9]*/, const double *cs, const double *prop,
                       gfo_beam_const *c, double *gp /*[2][GFO_BEAM_GPS]*/);
-/* Beam_1::Mount: K (18x18 row-major, global) and f_int (18, global). */
+/* Beam_1::Mount: K (18x18 row-major, global) and f_int (18, global); res (or NULL)
+ * receives the Gauss-point means of sigma_r (6) and epsilon_r (6). */
 void gfo_beam_mount(const gfo_beam_const *c, const double *gp, const double *d /*[18]*/,
-                    double *K, double *f);
+                    double *K, double *f, double *res /*[12]*/);
 /* Beam_1::SaveLagrange with the kinematics of the current d. */
 void gfo_beam_save(const gfo_beam_const *c, double *gp, const double *d);
 

@@ -59,12 +59,14 @@ typedef struct gfo_model {
     gfo_beam_const *bc;
     double *bgp;              /* [nb][2][GFO_BEAM_GPS] */
     double *bK, *bP;          /* [nb][324], [nb][18] */
+    double *bres;             /* [nb][12] last Mount's sigma_r, epsilon_r means */
     /* shells */
     int ns;
     int *sconn;
     gfo_shell_const *sc;
     double *sgp;              /* [ns][3][GFO_SHELL_GPS] */
     double *sK, *sP;          /* [ns][729], [ns][27] */
+    double *sres;             /* [ns][24] last Mount's resultant and strain means */
     /* loads */
     int nl;
     int *lkind, *lset_off, *lset, *lflags, *ltab_off;
@@ -137,6 +139,7 @@ gfo_model *gfo_create(const gf_model_desc *desc, int solver)
     m->bgp = (double *)calloc((size_t)m->nb * 2 * GFO_BEAM_GPS + 1, sizeof(double));
     m->bK = (double *)calloc((size_t)m->nb * 324 + 1, sizeof(double));
     m->bP = (double *)calloc((size_t)m->nb * 18 + 1, sizeof(double));
+    m->bres = (double *)calloc((size_t)m->nb * 12 + 1, sizeof(double));
     for (int e = 0; e < m->nb; e++) {
         double x[9];
         for (int a = 0; a < 3; a++)
@@ -151,6 +154,7 @@ gfo_model *gfo_create(const gf_model_desc *desc, int solver)
     m->sgp = (double *)calloc((size_t)m->ns * 3 * GFO_SHELL_GPS + 1, sizeof(double));
     m->sK = (double *)calloc((size_t)m->ns * 729 + 1, sizeof(double));
     m->sP = (double *)calloc((size_t)m->ns * 27 + 1, sizeof(double));
+    m->sres = (double *)calloc((size_t)m->ns * 24 + 1, sizeof(double));
     for (int e = 0; e < m->ns; e++) {
         double x[18];
         for (int a = 0; a < 6; a++)
@@ -188,8 +192,8 @@ void gfo_destroy(gfo_model *m)
     if (!m) return;
     free(m->xyz_ref); free(m->copy); free(m->euler); free(m->d); free(m->xinc);
     free(m->GL); free(m->active); free(m->fix);
-    free(m->bconn); free(m->bc); free(m->bgp); free(m->bK); free(m->bP);
-    free(m->sconn); free(m->sc); free(m->sgp); free(m->sK); free(m->sP);
+    free(m->bconn); free(m->bc); free(m->bgp); free(m->bK); free(m->bP); free(m->bres);
+    free(m->sconn); free(m->sc); free(m->sgp); free(m->sK); free(m->sP); free(m->sres);
     free(m->lkind); free(m->lset_off); free(m->lset); free(m->lcs); free(m->lflags);
     free(m->ltab_off); free(m->tab_t); free(m->tab_v); free(m->mult);
     free(m->ab); free(m->ipiv); free(m->P);
@@ -459,7 +463,7 @@ int gfo_assemble(gfo_model *m, double *res)
         double d[18];
         gather_d(m, m->bconn + 3 * e, 3, d);
         double *K = m->bK + 324 * (size_t)e, *P = m->bP + 18 * (size_t)e;
-        gfo_beam_mount(&m->bc[e], m->bgp + (size_t)e * 2 * GFO_BEAM_GPS, d, K, P);
+        gfo_beam_mount(&m->bc[e], m->bgp + (size_t)e * 2 * GFO_BEAM_GPS, d, K, P, m->bres + 12 * (size_t)e);
         if (m->grav_on) {
             const gfo_beam_const *c = &m->bc[e];
             double mult = lf * 1.0 * 1.0 * c->jac * c->rhoA;
@@ -476,7 +480,7 @@ int gfo_assemble(gfo_model *m, double *res)
         double d[36];
         gather_d(m, m->sconn + 6 * e, 6, d);
         gfo_shell_mount(gfo_shell_at(m->sc, e), m->sgp + (size_t)e * 3 * GFO_SHELL_GPS, d,
-                        m->sK + 729 * (size_t)e, m->sP + 27 * (size_t)e);
+                        m->sK + 729 * (size_t)e, m->sP + 27 * (size_t)e, m->sres + 24 * (size_t)e);
         if (m->grav_on) gfo_shell_gravity(gfo_shell_at(m->sc, e), lf, m->grav, m->sP + 27 * (size_t)e);
     }
     /* MountLoads: nodal loads write the global system; ShellLoad edits element buffers */
@@ -777,6 +781,12 @@ int gfo_read_tangent_dense(const gfo_model *m, double *out)
 }
 void gfo_read_gp_beam(const gfo_model *m, double *out) { memcpy(out, m->bgp, sizeof(double) * (size_t)m->nb * 2 * GFO_BEAM_GPS); }
 void gfo_read_gp_shell(const gfo_model *m, double *out) { memcpy(out, m->sgp, sizeof(double) * (size_t)m->ns * 3 * GFO_SHELL_GPS); }
+/* The element results of the last Mount (the values Beam_1 / Shell_1 ::WriteResults print). */
+void gfo_read_element_results(const gfo_model *m, double *beam_out, double *shell_out)
+{
+    if (beam_out) memcpy(beam_out, m->bres, sizeof(double) * 12 * (size_t)m->nb);
+    if (shell_out) memcpy(shell_out, m->sres, sizeof(double) * 24 * (size_t)m->ns);
+}
 void gfo_read_element_beam(const gfo_model *m, int e, double *K, double *P)
 {
     memcpy(K, m->bK + 324 * (size_t)e, sizeof(double) * 324);

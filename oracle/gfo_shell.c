@@ -294,17 +294,24 @@ static void thickness_integration(const gfo_shell_const *c, const shell_kin *k, 
     for (int i = 0; i < 3; i++) { sig[i] = n1[i]; sig[3 + i] = m1[i]; sig[6 + i] = n2[i]; sig[9 + i] = m2[i]; }
 }
 
-void gfo_shell_mount(const gfo_shell_const *c, const double *gp, const double *d, double *K, double *P)
+void gfo_shell_mount(const gfo_shell_const *c, const double *gp, const double *d, double *K, double *P, double *res)
 {
     double Kl[27 * 27], fl[27];
     memset(Kl, 0, sizeof(Kl));
     memset(fl, 0, sizeof(fl));
+    if (res) memset(res, 0, 24 * sizeof(double));
     for (int g = 0; g < 3; g++) {
         const double *st = gp + g * GFO_SHELL_GPS;
         shell_kin k;
         shell_kinematics(c, st, d, g, &k);
         double D[144], sig[12];
         thickness_integration(c, &k, D, sig);
+        /* Shell_1::WriteResults src/Shell_1.cpp:636-646: x_avg = x_avg + (1/3) x_gp */
+        if (res) {
+            const double *src[8] = {sig, sig + 3, sig + 6, sig + 9, k.eta1, k.k1, k.eta2, k.k2};
+            for (int q = 0; q < 8; q++)
+                for (int i = 0; i < 3; i++) res[3 * q + i] = res[3 * q + i] + (1.0 / 3.0) * src[q][i];
+        }
         /* deltaN (15 x 27), src/Shell_1.cpp:2114-2161 */
         double dN[15 * 27];
         memset(dN, 0, sizeof(dN));

@@ -54,6 +54,7 @@ def lib():
         L.gfo_read_tangent_dense.argtypes = [vp, dp]
         L.gfo_read_element_beam.argtypes = [vp, C.c_int, dp, dp]
         L.gfo_read_element_shell.argtypes = [vp, C.c_int, dp, dp]
+        L.gfo_read_element_results.argtypes = [vp, dp, dp]
         L.gfo_static_solve.argtypes = [vp, C.POINTER(abi.GfStaticParams), C.POINTER(abi.GfStaticReport)]
         L.gfo_n_free.argtypes = [vp]
         L.gfo_bandwidth.argtypes = [vp]
@@ -66,7 +67,7 @@ def lib():
         L.gfo_dV.argtypes = [dp, dp, dp, dp]
         L.gfo_compose.argtypes = [dp, dp, dp]
         L.gfo_beam_precalc.argtypes = [dp, dp, dp, C.c_void_p, dp]
-        L.gfo_beam_mount.argtypes = [C.c_void_p, dp, dp, dp, dp]
+        L.gfo_beam_mount.argtypes = [C.c_void_p, dp, dp, dp, dp, dp]
         L.gfo_beam_save.argtypes = [C.c_void_p, dp, dp]
         _lib = L
     return _lib
@@ -167,6 +168,13 @@ class OracleModel:
         lib().gfo_read_element_beam(self.h, e, _dp(K), _dp(P))
         return K, P
 
+    def element_results(self):
+        """Last Mount's Gauss-point means: Beam_1 [nb, 12] and Shell_1 [ns, 24]."""
+        b = np.zeros((self.model.n_beams, 12))
+        sh = np.zeros((self.model.n_shells, 24))
+        lib().gfo_read_element_results(self.h, _dp(b), _dp(sh))
+        return b, sh
+
     def gp_beam(self):
         return self._read("gfo_read_gp_beam", (self.model.n_beams, 2, 21))
 
@@ -211,13 +219,16 @@ def beam_element(x9, cs9, prop9):
     return c, gp
 
 
-def beam_mount(c, gp, d18):
+def beam_mount(c, gp, d18, results=False):
+    """K, f_int of one Beam_1; with results, also the Gauss-point means of sigma_r and
+    epsilon_r (12 values, as Beam_1::WriteResults prints them)."""
     K = np.zeros((18, 18))
     f = np.zeros(18)
+    r = np.zeros(12)
     d18 = np.ascontiguousarray(d18, float)
     gp = np.ascontiguousarray(gp, float)
-    lib().gfo_beam_mount(C.byref(c), _dp(gp), _dp(d18), _dp(K), _dp(f))
-    return K, f
+    lib().gfo_beam_mount(C.byref(c), _dp(gp), _dp(d18), _dp(K), _dp(f), _dp(r))
+    return (K, f, r) if results else (K, f)
 
 
 def beam_save(c, gp, d18):
@@ -282,7 +293,7 @@ def _shell_sigs():
     L.gfo_shell_new.restype = C.c_void_p
     L.gfo_shell_new.argtypes = [dp, dp, dp]
     L.gfo_shell_free.argtypes = [C.c_void_p]
-    L.gfo_shell_mount.argtypes = [C.c_void_p, dp, dp, dp, dp]
+    L.gfo_shell_mount.argtypes = [C.c_void_p, dp, dp, dp, dp, dp]
     L.gfo_shell_pressure.argtypes = [C.c_void_p, dp, dp, C.c_double, C.c_int, dp, dp]
     L.gfo_shell_gravity.argtypes = [C.c_void_p, C.c_double, dp, dp]
     L.gfo_shell_save.argtypes = [C.c_void_p, dp, dp]
@@ -308,12 +319,15 @@ class ShellElement:
         except Exception:
             pass
 
-    def mount(self, d36, gp=None):
+    def mount(self, d36, gp=None, results=False):
+        """K, f_int; with results, also the Gauss-point means n_r1 m_r1 n_r2 m_r2 eta_r1
+        kappa_r1 eta_r2 kappa_r2 (24 values, as Shell_1::WriteResults prints them)."""
         gp = self.gp if gp is None else np.ascontiguousarray(gp, float)
         K = np.zeros((27, 27))
         f = np.zeros(27)
-        _shell_sigs().gfo_shell_mount(self.h, _dp(gp), _dp(np.ascontiguousarray(d36, float)), _dp(K), _dp(f))
-        return K, f
+        r = np.zeros(24)
+        _shell_sigs().gfo_shell_mount(self.h, _dp(gp), _dp(np.ascontiguousarray(d36, float)), _dp(K), _dp(f), _dp(r))
+        return (K, f, r) if results else (K, f)
 
     def pressure(self, utot18, d36, p, area_update, K=None, P=None):
         K = np.zeros((27, 27)) if K is None else np.ascontiguousarray(K, float).copy()
