@@ -21,6 +21,8 @@ GF_LOAD_NODAL_FOLLOWER = 1
 GF_LOAD_SHELL_PRESSURE = 2
 
 GF_BEAM_NPROP = 9
+GF_BEAM_KIND_BEAM1 = 0
+GF_BEAM_KIND_PIPE1 = 1
 GF_SHELL_NPROP = 4
 
 _dp = C.POINTER(C.c_double)
@@ -53,6 +55,11 @@ class GfModelDesc(C.Structure):
         ("gravity_on", C.c_int),
         ("beam_number", _ip),
         ("shell_number", _ip),
+        ("beam_kind", _ip),
+        ("beam_buoy", _dp),
+        ("ocean_on", C.c_int),
+        ("ocean_surface", C.c_double * 3),
+        ("ocean_rho_fluid", C.c_double),
     ]
 
 

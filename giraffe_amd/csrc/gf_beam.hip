@@ -27,7 +27,7 @@ namespace {
 
 constexpr int EPB = 7;       // elements per workgroup
 constexpr int WG = 64;
-constexpr int LDS_GP = 72;   // 7 blocks * 9 + 9 (B^T sigma)
+constexpr int LDS_GP = 73;   // 7 blocks * 9 + 9 (B^T sigma) + the Gauss point's mass per length
 
 struct BeamGP {
     v3<double> a, da, dz, eta, kap;
@@ -87,12 +87,16 @@ __device__ __forceinline__ void gather_disp(const double *__restrict__ disp, con
 }
 
 // MODE 0: scatter into (A,B,C) and r.  MODE 1: write dense element outputs.
+// buoy (null: none): Pipe_1 buoyant mass per length, applied at Gauss points
+// below the ocean surface through `surf` (a point of it), with the converged
+// coordinates `copy`.
 template <int MODE>
 __global__ __launch_bounds__(WG) void beam_assemble_kernel(
     int e0, int e1, size_t ld, const int *__restrict__ conn, const double *__restrict__ cst,
     const double *__restrict__ gps, const double *__restrict__ disp, const int *__restrict__ dof,
     gf_gmat gm, double *__restrict__ r, double lf, double gx, double gy, double gz,
-    double *__restrict__ Kout, double *__restrict__ fout)
+    double *__restrict__ Kout, double *__restrict__ fout, const double *__restrict__ buoy,
+    const double *__restrict__ copy, double sx, double sy, double sz)
 {
     __shared__ double sm[EPB * 2 * LDS_GP];
     const int tid = threadIdx.x;
@@ -166,6 +170,26 @@ __global__ __launch_bounds__(WG) void beam_assemble_kernel(
             o[63] = s0.x; o[64] = s0.y; o[65] = s0.z;
             o[66] = s1.x; o[67] = s1.y; o[68] = s1.z;
             o[69] = s2.x; o[70] = s2.y; o[71] = s2.z;
+            if (buoy) {
+                // Pipe_1::MountFieldLoading src/Pipe_1.cpp:1289-1306: Gauss-point position
+                // (converged + incremental), depth along G, Rho - rho_f Aext below the surface
+                double z[3];
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    double c = 0.0, u = 0.0;
+#pragma unroll
+                    for (int a = 0; a < 3; a++) {
+                        const double Na = cst[(BC_N + 3 * gp + a) * ld + e];
+                        c += copy[6 * (size_t)conn[a * ld + e] + k] * Na;
+                        u += d[6 * a + k] * Na;
+                    }
+                    z[k] = c + u;
+                }
+                const double depth =
+                    ((z[0] - sx) * gx + (z[1] - sy) * gy + (z[2] - sz) * gz) * (1.0 / sqrt(gx * gx + gy * gy + gz * gz));
+                const double rhoA = cst[BC_RHOA * ld + e];
+                o[72] = depth > 0 ? rhoA - buoy[e] : rhoA;
+            }
         }
     }
     __syncthreads();
@@ -261,6 +285,13 @@ __global__ __launch_bounds__(WG) void beam_assemble_kernel(
             double P[6] = {fu.x, fu.y, fu.z, fa.x, fa.y, fa.z};
             if (MODE == 0) {
                 double mult = lf * w * cst[BC_RHOA * ld + e] * Nsum;
+                if (buoy) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int gp = 0; gp < 2; gp++)
+                        acc += cst[(BC_N + 3 * gp + a) * ld + e] * sm[(le * 2 + gp) * LDS_GP + 72];
+                    mult = lf * w * acc;
+                }
                 P[0] -= mult * gx; P[1] -= mult * gy; P[2] -= mult * gz;
                 int na = conn[a * ld + e];
                 int R6[6];
@@ -350,21 +381,36 @@ __global__ __launch_bounds__(256) void beam_results_kernel(int nb, size_t ld, co
 // Gauss abscissa as written in src/Beam_1.cpp:629-632.
 static const double kBeamCsi = 0.577350269189626;
 
-void gf_beam_host_const(const double *x, const double *cs, const double *prop, double *rec, double *gp)
+void gf_beam_host_const(const double *x, const double *cs, const double *prop, int kind, double *rec, double *gp)
 {
-    double E = prop[0], nu = prop[1], rho = prop[2], A = prop[3];
-    double I1 = prop[4], I2 = prop[5], I12 = prop[6], It = prop[7], T0 = prop[8];
-    double G = E / (2 * (1 + nu));
-    double sf = 1.0;
+    double T0 = 0.0;
     for (int i = 0; i < 9; i++) rec[BC_T3 + i] = cs[i];
-    rec[BC_D + 0] = sf * G * A;
-    rec[BC_D + 1] = sf * G * A;
-    rec[BC_D + 2] = E * A;
-    rec[BC_D + 3] = E * I1;
-    rec[BC_D + 4] = E * I2;
-    rec[BC_D + 5] = E * I12;
-    rec[BC_D + 6] = G * It;
-    rec[BC_RHOA] = rho * A;
+    if (kind == GF_BEAM_KIND_PIPE1) {
+        // Pipe_1::PreCalc src/Pipe_1.cpp:1108-1135: D from the pipe section, Rho per length
+        const double EA = prop[0], EI = prop[1], GJ = prop[2], GA = prop[3], Rho = prop[4];
+        rec[BC_D + 0] = GA;
+        rec[BC_D + 1] = GA;
+        rec[BC_D + 2] = EA;
+        rec[BC_D + 3] = EI;
+        rec[BC_D + 4] = EI;
+        rec[BC_D + 5] = 0.0;
+        rec[BC_D + 6] = GJ;
+        rec[BC_RHOA] = Rho;
+    } else {
+        double E = prop[0], nu = prop[1], rho = prop[2], A = prop[3];
+        double I1 = prop[4], I2 = prop[5], I12 = prop[6], It = prop[7];
+        T0 = prop[8];
+        double G = E / (2 * (1 + nu));
+        double sf = 1.0;
+        rec[BC_D + 0] = sf * G * A;
+        rec[BC_D + 1] = sf * G * A;
+        rec[BC_D + 2] = E * A;
+        rec[BC_D + 3] = E * I1;
+        rec[BC_D + 4] = E * I2;
+        rec[BC_D + 5] = E * I12;
+        rec[BC_D + 6] = G * It;
+        rec[BC_RHOA] = rho * A;
+    }
     double e3[3] = {x[6] - x[0], x[7] - x[1], x[8] - x[2]};
     double L0 = std::sqrt(e3[0] * e3[0] + e3[1] * e3[1] + e3[2] * e3[2]);
     for (int i = 0; i < 3; i++) e3[i] *= 1.0 / L0;
@@ -400,7 +446,9 @@ int gf_beam_assemble(gf_model *m, double lf)
         const int pr = gf_probe_begin(m->stream, GF_KC_BEAM_ASSEMBLE);
         hipLaunchKernelGGL(beam_assemble_kernel<0>, dim3(grid), dim3(WG), 0, m->stream, e0, e1, ld, m->d_bconn,
                            m->d_bconst, m->d_bgp, m->d_disp, m->d_dof, gf_model_gmat(m), m->d_r, lf, m->gravity[0],
-                           m->gravity[1], m->gravity[2], (double *)nullptr, (double *)nullptr);
+                           m->gravity[1], m->gravity[2], (double *)nullptr, (double *)nullptr,
+                           (const double *)m->d_bbuoy, (const double *)m->d_copy, m->ocean_surface[0],
+                           m->ocean_surface[1], m->ocean_surface[2]);
         gf_probe_end(m->stream, pr, e1 - e0);
     }
     GF_HIP(hipGetLastError());
@@ -436,7 +484,8 @@ int gf_beam_eval_device(int count, const double *x, const double *cs, const doub
     std::vector<double> cst(BC_COUNT * ld), gps(BG_COUNT * ld), disp(18 * ld);
     std::vector<double> rec(BC_COUNT), g0(BG_COUNT);
     for (int e = 0; e < count; e++) {
-        gf_beam_host_const(x + 9 * (size_t)e, cs + 9 * (size_t)e, prop + GF_BEAM_NPROP * (size_t)e, rec.data(), g0.data());
+        gf_beam_host_const(x + 9 * (size_t)e, cs + 9 * (size_t)e, prop + GF_BEAM_NPROP * (size_t)e, GF_BEAM_KIND_BEAM1,
+                           rec.data(), g0.data());
         for (int k = 0; k < BC_COUNT; k++) cst[k * ld + e] = rec[k];
         for (int k = 0; k < BG_COUNT; k++) gps[k * ld + e] = gp[(size_t)e * BG_COUNT + k];
         for (int a = 0; a < 3; a++) conn[a * ld + e] = 3 * e + a;
@@ -459,7 +508,8 @@ int gf_beam_eval_device(int count, const double *x, const double *cs, const doub
     int grid = (count + EPB - 1) / EPB;
     gf_gmat none = {};
     hipLaunchKernelGGL(beam_assemble_kernel<1>, dim3(grid), dim3(WG), 0, s, 0, count, ld, d_conn, d_cst, d_gps,
-                       d_disp, (const int *)nullptr, none, (double *)nullptr, 0.0, 0.0, 0.0, 0.0, d_K, d_f);
+                       d_disp, (const int *)nullptr, none, (double *)nullptr, 0.0, 0.0, 0.0, 0.0, d_K, d_f,
+                       (const double *)nullptr, (const double *)nullptr, 0.0, 0.0, 0.0);
     GF_HIP(hipGetLastError());
     GF_HIP(hipMemcpyAsync(K_out, d_K, sizeof(double) * 324 * ld, hipMemcpyDeviceToHost, s));
     GF_HIP(hipMemcpyAsync(f_out, d_f, sizeof(double) * 18 * ld, hipMemcpyDeviceToHost, s));

@@ -1,9 +1,9 @@
 // Reader and writer of GIRAFFE input decks, restricted to the static hot-path
-// subset: Nodes, NodeSets, ElementSets, Elements (Beam_1, Shell_1), Materials
-// (Hooke), Sections (Tube), ShellSections (Homogeneous), CoordinateSystems,
-// Loads (NodalLoad, NodalFollowerLoad, ShellLoad), Constraints
-// (NodalConstraint), Environment (GravityData), SolutionSteps (one Static),
-// ConvergenceCriteria.  Output-only blocks (Monitor, PostFiles, SolverOptions,
+// subset: Nodes, NodeSets, ElementSets, Elements (Beam_1, Shell_1, Pipe_1),
+// Materials (Hooke), Sections (Tube), PipeSections, ShellSections (Homogeneous),
+// CoordinateSystems, Loads (NodalLoad, NodalFollowerLoad, ShellLoad), Constraints
+// (NodalConstraint), Environment (GravityData, OceanData without an active sea
+// current), SolutionSteps (one Static), ConvergenceCriteria.  Output-only blocks (Monitor, PostFiles, SolverOptions,
 // ExecutionData, ConfigurationSave) are skipped; any other top-level block
 // changes the physics and is refused with a message naming it.
 //
@@ -15,6 +15,8 @@
 //   ElementSet          src/ElementSet.cpp:30-95
 //   Beam_1              src/Beam_1.cpp:370-429 (Mat Sec CS Nodes [PreTension])
 //   Shell_1             src/Shell_1.cpp:537-593 (Mat Sec [CS] Nodes n1..n6)
+//   Pipe_1              src/Pipe_1.cpp:535-571 (PipeSec CS Nodes)
+//   PS (PipeSections)   src/PipeSection.cpp:19-131 (EA EI GJ GA Rho CDt CDn CAt CAn De Di)
 //   Hooke               src/Hooke.cpp:16-49 (E Nu Rho)
 //   Tube                src/SecTube.cpp:22-44 (De Di), properties :86-106
 //   Homogeneous         src/ShellSectionHomogeneous.cpp:17-30 (Thickness)
@@ -25,6 +27,8 @@
 //   NodalConstraint     src/NodalConstraint.cpp:25-140 (NodeSet, DOF BoolTable ...)
 //   BoolTable           src/BoolTable.cpp:23-45 (values while the token starts with a digit)
 //   GravityData         src/Environment.cpp:355-384 (G gx gy gz BoolTable ...)
+//   OceanData           src/Environment.cpp:384-470 (RhoFluid SurfacePosition SeaCurrent N
+//                       BoolTable, N x Depth Speed Angle)
 //   Static              src/Static.cpp:43-130 (EndTime ... Sample, fixed order)
 //   ConvergenceCriteria src/ConvergenceCriteria.cpp:64-165 (12 keys, fixed order)
 #include "gf_internal.h"
@@ -84,7 +88,7 @@ const std::set<std::string> &top_keywords()
 }
 
 struct Elem {
-    int type;  // 0 Beam_1, 1 Shell_1
+    int type;  // 0 Beam_1, 1 Shell_1, 2 Pipe_1
     int id, mat, sec, cs;
     int nodes[6];
     double T0;
@@ -105,6 +109,9 @@ struct gf_deck {
     std::vector<Elem> elems;
     std::map<int, std::array<double, 3>> materials;
     std::map<int, std::array<double, 2>> tubes;
+    std::map<int, std::array<double, 7>> pipes;  // PS: EA EI GJ GA Rho De Di
+    int ocean_on = 0;
+    double rho_fluid = 0.0, surface[3] = {0, 0, 0};
     std::map<int, double> shell_t;
     std::map<int, std::array<double, 6>> css;
     std::vector<Load> loads;
@@ -116,7 +123,8 @@ struct gf_deck {
     // resolved descriptor arrays
     std::vector<unsigned char> fix;
     std::vector<int> beam_conn, shell_conn, load_kind, load_set_off, load_set_items, load_flags, load_tab_off;
-    std::vector<int> beam_number, shell_number;
+    std::vector<int> beam_number, shell_number, beam_kind;
+    std::vector<double> beam_buoy;
     std::vector<double> beam_cs, beam_prop, shell_prop, load_cs, tab_time, tab_val;
     gf_model_desc desc{};
 };
@@ -195,8 +203,20 @@ int parse(Tokens &tk, gf_deck &d)
                 Elem e{};
                 if (type == "Beam_1") e.type = 0;
                 else if (type == "Shell_1") e.type = 1;
+                else if (type == "Pipe_1") e.type = 2;
                 else return fail("element type " + type + " is not on the static hot path");
                 e.id = std::atoi(tk.next().c_str());
+                if (e.type == 2) {
+                    // Pipe_1::Read src/Pipe_1.cpp:535-571: PipeSec s CS c Nodes a b c
+                    if (!expect(tk, "PipeSec")) return fail("Pipe_1: expected PipeSec");
+                    e.sec = std::atoi(tk.next().c_str());
+                    if (!expect(tk, "CS")) return fail("Pipe_1: expected CS");
+                    e.cs = std::atoi(tk.next().c_str());
+                    if (!expect(tk, "Nodes")) return fail("Pipe_1: expected Nodes");
+                    for (int a = 0; a < 3; a++) e.nodes[a] = std::atoi(tk.next().c_str());
+                    d.elems.push_back(e);
+                    continue;
+                }
                 if (!expect(tk, "Mat")) return fail(type + ": expected Mat");
                 e.mat = std::atoi(tk.next().c_str());
                 if (!expect(tk, "Sec")) return fail(type + ": expected Sec");
@@ -245,6 +265,20 @@ int parse(Tokens &tk, gf_deck &d)
                 double Di = std::atof(tk.next().c_str());
                 if (tk.peek() == "AD") return fail("Tube: aerodynamic data is not on the static hot path");
                 d.tubes[id] = {De, Di};
+            }
+        } else if (kw == "PipeSections") {
+            // PipeSection::Read src/PipeSection.cpp:19-131
+            static const char *keys[11] = {"EA", "EI", "GJ", "GA", "Rho", "CDt", "CDn", "CAt", "CAn", "De", "Di"};
+            int n = std::atoi(tk.next().c_str());
+            for (int i = 0; i < n; i++) {
+                if (!expect(tk, "PS")) return fail("PipeSections: expected PS");
+                int id = std::atoi(tk.next().c_str());
+                double v[11];
+                for (int k = 0; k < 11; k++) {
+                    if (!expect(tk, keys[k])) return fail(std::string("PS: expected ") + keys[k]);
+                    v[k] = std::atof(tk.next().c_str());
+                }
+                d.pipes[id] = {v[0], v[1], v[2], v[3], v[4], v[9], v[10]};
             }
         } else if (kw == "ShellSections") {
             int n = std::atoi(tk.next().c_str());
@@ -324,6 +358,32 @@ int parse(Tokens &tk, gf_deck &d)
         } else if (kw == "Environment") {
             while (!tk.done() && !top.count(tk.peek())) {
                 std::string e = tk.next();
+                if (e == "OceanData") {
+                    // Environment::Read src/Environment.cpp:384-470
+                    if (!expect(tk, "RhoFluid")) return fail("OceanData: expected RhoFluid");
+                    d.rho_fluid = std::atof(tk.next().c_str());
+                    if (!expect(tk, "SurfacePosition")) return fail("OceanData: expected SurfacePosition");
+                    for (int k = 0; k < 3; k++) d.surface[k] = std::atof(tk.next().c_str());
+                    if (!expect(tk, "SeaCurrent")) return fail("OceanData: expected SeaCurrent");
+                    if (!expect(tk, "N")) return fail("SeaCurrent: expected N");
+                    const int np = std::atoi(tk.next().c_str());
+                    if (!expect(tk, "BoolTable")) return fail("SeaCurrent: expected BoolTable");
+                    std::vector<int> b = read_booltable(tk);
+                    bool moving = false;
+                    for (int i = 0; i < np; i++) {
+                        static const char *pk[3] = {"Depth", "Speed", "Angle"};
+                        double v[3];
+                        for (int k = 0; k < 3; k++) {
+                            if (!expect(tk, pk[k])) return fail(std::string("SeaCurrent: expected ") + pk[k]);
+                            v[k] = std::atof(tk.next().c_str());
+                        }
+                        moving = moving || v[1] != 0.0;
+                    }
+                    if (moving && !b.empty() && b[0] != 0)
+                        return fail("SeaCurrent: Morison loads of a sea current are not on the static hot path");
+                    d.ocean_on = 1;
+                    continue;
+                }
                 if (e != "GravityData") return fail("environment data " + e + " is not on the static hot path");
                 if (!expect(tk, "G")) return fail("GravityData: expected G");
                 for (int k = 0; k < 3; k++) d.G[k] = std::atof(tk.next().c_str());
@@ -400,9 +460,34 @@ int resolve(gf_deck &d)
     };
     std::map<int, int> shell_of;  // element id -> shell index
     for (const Elem &e : d.elems) {
+        if (e.type == 2) {
+            // Pipe_1::PreCalc src/Pipe_1.cpp:1106-1160: D and Rho from the pipe section,
+            // Aext = pi De^2 / 4 for the buoyancy below the surface
+            auto pi = d.pipes.find(e.sec);
+            if (pi == d.pipes.end()) return fail("pipe section " + std::to_string(e.sec) + " is not defined");
+            auto ci = d.css.find(e.cs);
+            if (ci == d.css.end()) return fail("coordinate system " + std::to_string(e.cs) + " is not defined");
+            for (int a = 0; a < 3; a++) {
+                int k;
+                GF_TRY(node(e.nodes[a], k));
+                d.beam_conn.push_back(k);
+            }
+            d.beam_number.push_back(e.id);
+            double cs[9];
+            cs_rows(ci->second, cs);
+            d.beam_cs.insert(d.beam_cs.end(), cs, cs + 9);
+            const std::array<double, 7> &ps = pi->second;
+            double prop[GF_BEAM_NPROP] = {ps[0], ps[1], ps[2], ps[3], ps[4], 0.0, 0.0, 0.0, 0.0};
+            d.beam_prop.insert(d.beam_prop.end(), prop, prop + GF_BEAM_NPROP);
+            d.beam_kind.push_back(GF_BEAM_KIND_PIPE1);
+            d.beam_buoy.push_back(d.ocean_on ? d.rho_fluid * (PI * ps[5] * ps[5] / 4.0) : 0.0);
+            continue;
+        }
         auto mi = d.materials.find(e.mat);
         if (mi == d.materials.end()) return fail("material " + std::to_string(e.mat) + " is not defined");
         if (e.type == 0) {
+            d.beam_kind.push_back(GF_BEAM_KIND_BEAM1);
+            d.beam_buoy.push_back(0.0);
             auto si = d.tubes.find(e.sec);
             if (si == d.tubes.end()) return fail("section " + std::to_string(e.sec) + " is not defined");
             auto ci = d.css.find(e.cs);
@@ -514,6 +599,11 @@ int resolve(gf_deck &d)
         g.beam_number = perm ? d.beam_number.data() : nullptr;
         g.shell_number = perm ? d.shell_number.data() : nullptr;
     }
+    g.beam_kind = d.beam_kind.data();
+    g.beam_buoy = d.beam_buoy.data();
+    g.ocean_on = d.ocean_on;
+    for (int k = 0; k < 3; k++) g.ocean_surface[k] = d.surface[k];
+    g.ocean_rho_fluid = d.rho_fluid;
     g.gravity_on = d.g_on;
     return GF_OK;
 }
@@ -598,9 +688,28 @@ extern "C" int gf_deck_write(const char *path, const gf_model_desc *g, const gf_
     auto beam_num = [&](int e) { return g->beam_number ? g->beam_number[e] : e + 1; };
     auto shell_num = [&](int e) { return g->shell_number ? g->shell_number[e] : nb + e + 1; };
     std::vector<std::string> lines((size_t)nb + ns);
+    std::map<std::array<double, 6>, int> pipes;  // EA EI GJ GA Rho De -> PS id
     for (int e = 0; e < nb; e++) {
         std::ostringstream el;
         const double *pr = g->beam_prop + GF_BEAM_NPROP * (size_t)e;
+        if (g->beam_kind && g->beam_kind[e] == GF_BEAM_KIND_PIPE1) {
+            // De back from the buoyant mass per length rho_f pi De^2 / 4 (0 without OceanData)
+            const double buoy = g->beam_buoy ? g->beam_buoy[e] : 0.0;
+            const double De = g->ocean_on && g->ocean_rho_fluid > 0 ? std::sqrt(4.0 * buoy / (PI * g->ocean_rho_fluid)) : 0.0;
+            std::array<double, 6> key{pr[0], pr[1], pr[2], pr[3], pr[4], De};
+            int sec;
+            auto it = pipes.find(key);
+            if (it != pipes.end()) sec = it->second;
+            else { sec = (int)pipes.size() + 1; pipes[key] = sec; }
+            el << "Pipe_1\t" << beam_num(e) << "\tPipeSec\t" << sec << "\tCS\t" << cs_id(g->beam_cs + 9 * (size_t)e)
+               << "\tNodes";
+            for (int a = 0; a < 3; a++) el << "\t" << g->beam_conn[3 * e + a] + 1;
+            el << "\n";
+            const int k = beam_num(e) - 1;
+            if (k < 0 || k >= nb + ns || !lines[k].empty()) return fail("element numbers are not a permutation of 1..N");
+            lines[k] = el.str();
+            continue;
+        }
         double A = pr[3], I = pr[4];
         if (pr[4] != pr[5] || pr[6] != 0.0 || std::fabs(pr[7] - 2.0 * I) > 1e-12 * I)
             return fail("beam " + std::to_string(e) + " is not a tube section");
@@ -697,6 +806,16 @@ extern "C" int gf_deck_write(const char *path, const gf_model_desc *g, const gf_
         std::sort(tv.begin(), tv.end());
         for (auto &t : tv) o << "Tube\t" << t.first << "\tDe\t" << num(t.second[0]) << "\tDi\t" << num(t.second[1]) << "\n";
     }
+    if (!pipes.empty()) {
+        o << "PipeSections\t" << pipes.size() << "\n";
+        std::vector<std::pair<int, std::array<double, 6>>> pv;
+        for (auto &t : pipes) pv.push_back({t.second, t.first});
+        std::sort(pv.begin(), pv.end());
+        for (auto &t : pv)
+            o << "PS\t" << t.first << "\tEA\t" << num(t.second[0]) << "\tEI\t" << num(t.second[1]) << "\tGJ\t"
+              << num(t.second[2]) << "\tGA\t" << num(t.second[3]) << "\tRho\t" << num(t.second[4])
+              << "\tCDt\t0\tCDn\t0\tCAt\t0\tCAn\t0\tDe\t" << num(t.second[5]) << "\tDi\t0\n";
+    }
     if (!thick.empty()) {
         o << "ShellSections\t" << thick.size() << "\n";
         std::vector<std::pair<int, double>> hv;
@@ -719,9 +838,14 @@ extern "C" int gf_deck_write(const char *path, const gf_model_desc *g, const gf_
     }
     if (nl) o << "Loads\t" << nl << "\n" << lo.str();
     if (nc) o << "Constraints\t" << nc << "\n" << co.str();
-    if (g->gravity_on || g->gravity[0] != 0.0 || g->gravity[1] != 0.0 || g->gravity[2] != 0.0)
-        o << "Environment\nGravityData\tG\t" << num(g->gravity[0]) << "\t" << num(g->gravity[1]) << "\t"
-          << num(g->gravity[2]) << "\tBoolTable\t" << (g->gravity_on ? 1 : 0) << "\n";
+    const bool grav = g->gravity_on || g->gravity[0] != 0.0 || g->gravity[1] != 0.0 || g->gravity[2] != 0.0;
+    if (grav || g->ocean_on) o << "Environment\n";
+    if (grav)
+        o << "GravityData\tG\t" << num(g->gravity[0]) << "\t" << num(g->gravity[1]) << "\t" << num(g->gravity[2])
+          << "\tBoolTable\t" << (g->gravity_on ? 1 : 0) << "\n";
+    if (g->ocean_on)
+        o << "OceanData\nRhoFluid\t" << num(g->ocean_rho_fluid) << "\nSurfacePosition\t" << num(g->ocean_surface[0])
+          << "\t" << num(g->ocean_surface[1]) << "\t" << num(g->ocean_surface[2]) << "\nSeaCurrent\tN\t0\tBoolTable\t0\n";
     o << "SolutionSteps\t1\nStatic\t1\nEndTime\t" << num(p->end_time) << "\nTimeStep\t" << num(p->time_step)
       << "\nMaxTimeStep\t" << num(p->max_time_step) << "\nMinTimeStep\t" << num(p->min_time_step) << "\nMaxIt\t"
       << p->max_it << "\nMinIt\t" << p->min_it << "\nConvIncrease\t" << p->conv_increase << "\nIncFactor\t"

@@ -288,6 +288,9 @@ struct gf_model {
     int *d_bconn = nullptr;         // [3][nb]
     double *d_bconst = nullptr;     // [BC_COUNT][nb]
     double *d_bgp = nullptr;        // [BG_COUNT][nb]
+    double *d_bbuoy = nullptr;      // [nb] Pipe_1 buoyant mass per length (null: no buoyancy)
+    std::vector<int> beam_kind;     // description order, GF_BEAM_KIND_*
+    double ocean_surface[3] = {0, 0, 0};
 
     // device: shells (colour-sorted)
     int *d_sconn = nullptr;         // [6][ns]
@@ -429,5 +432,5 @@ int gf_band_solve(const void *ab, int elem_size, int n, int hb, hipStream_t s, d
 
 int gf_beam_eval_device(int count, const double *x, const double *cs, const double *prop,
                         const double *d, const double *gp, double *K_out, double *f_out);
-void gf_beam_host_const(const double *x9, const double *cs, const double *prop, double *rec /*[BC_COUNT]*/,
-                        double *gp /*[BG_COUNT]*/);
+void gf_beam_host_const(const double *x9, const double *cs, const double *prop, int kind /*GF_BEAM_KIND_**/,
+                        double *rec /*[BC_COUNT]*/, double *gp /*[BG_COUNT]*/);

@@ -171,7 +171,7 @@ static void release(gf_model *m)
     void *bufs[] = {m->d_disp, m->d_copy, m->d_euler, m->d_dof, m->d_row_nc, m->d_bconn, m->d_bconst, m->d_bgp,
                     m->d_ln_node, m->d_ln_off, m->d_ln_ids, m->d_lrec, m->d_tab_t, m->d_tab_v, m->d_r, m->d_x,
                     m->d_scal, m->d_copy0, m->d_bgp0, m->d_sconn, m->d_sconst, m->d_spconst, m->d_sgp,
-                    m->d_sgp0, m->d_spload, m->d_spflag, m->d_band, m->d_swork, m->d_band_inv};
+                    m->d_sgp0, m->d_spload, m->d_spflag, m->d_band, m->d_swork, m->d_band_inv, m->d_bbuoy};
     for (void *b : bufs) gf_pool_free(b);
     if (m->h_scal) gf_pinned_free(m->h_scal);
     gf_btd_free(m->btd);
@@ -298,7 +298,7 @@ static size_t footprint(const gf_model_desc *d, const gf_model *m, int precision
     const size_t npad = (size_t)m->npad;
     size_t b = 0;
     b += nn * (6 + 6 + 6 + 3) * 8 + nn * 6 * 4 + nf * 4;                       // node state, DOF map
-    b += nb * (3 * 4 + BC_COUNT * 8 + 2 * BG_COUNT * 8);                      // Beam_1 records
+    b += nb * (3 * 4 + BC_COUNT * 8 + 2 * BG_COUNT * 8 + 8);                  // Beam_1 / Pipe_1 records
     b += ns * (6 * 4 + SC_COUNT * 8 + SP_COUNT * 8 + 2 * SG_COUNT * 8 + 8);   // Shell_1 records
     size_t items = d->n_loads ? (size_t)d->load_set_off[d->n_loads] : 0;
     size_t rows = d->n_loads ? (size_t)d->load_tab_off[d->n_loads] : 0;
@@ -343,6 +343,14 @@ static int create(const gf_model_desc *d, const gf_options *opt, gf_model *m)
         for (int k = 0; k < 3; k++) copy[6 * (size_t)i + k] = d->node_xyz[3 * (size_t)i + k];
     m->h_nodes_ref.assign(d->node_xyz, d->node_xyz + 3 * (size_t)nn);
     m->h_beam_conn.assign(d->beam_conn, d->beam_conn + 3 * (size_t)nb);
+    m->beam_kind.assign(nb, GF_BEAM_KIND_BEAM1);
+    for (int e = 0; e < nb && d->beam_kind; e++) {
+        if (d->beam_kind[e] != GF_BEAM_KIND_BEAM1 && d->beam_kind[e] != GF_BEAM_KIND_PIPE1) {
+            gf_set_error("unknown beam kind");
+            return GF_E_ARG;
+        }
+        m->beam_kind[e] = d->beam_kind[e];
+    }
     m->h_shell_conn.assign(d->shell_conn, d->shell_conn + 6 * (size_t)ns);
     {
         // element numbers: a permutation of 1..nb+ns (the reference stores element i as number i+1)
@@ -382,8 +390,8 @@ static int create(const gf_model_desc *d, const gf_options *opt, gf_model *m)
             double x[9];
             for (int a = 0; a < 3; a++)
                 for (int k = 0; k < 3; k++) x[3 * a + k] = d->node_xyz[3 * (size_t)d->beam_conn[3 * e + a] + k];
-            gf_beam_host_const(x, d->beam_cs + 9 * (size_t)e, d->beam_prop + GF_BEAM_NPROP * (size_t)e, rec.data(),
-                               g0.data());
+            gf_beam_host_const(x, d->beam_cs + 9 * (size_t)e, d->beam_prop + GF_BEAM_NPROP * (size_t)e,
+                               m->beam_kind[e], rec.data(), g0.data());
             for (int k = 0; k < BC_COUNT; k++) cst[k * ld + s] = rec[k];
             for (int k = 0; k < BG_COUNT; k++) gps[k * ld + s] = g0[k];
             for (int a = 0; a < 3; a++) conn[a * ld + s] = d->beam_conn[3 * e + a];
@@ -396,6 +404,19 @@ static int create(const gf_model_desc *d, const gf_options *opt, gf_model *m)
         GF_TRY(upload(m, m->d_bgp, gps.data(), gps.size()));
         GF_TRY(dalloc(&m->d_bgp0, gps.size()));
         GF_TRY(upload(m, m->d_bgp0, gps.data(), gps.size()));
+        // Pipe_1 buoyancy below the ocean surface (colour-sorted like the records)
+        bool buoyant = false;
+        std::vector<double> buoy(ld, 0.0);
+        if (d->ocean_on && d->beam_buoy)
+            for (int s = 0; s < nb; s++) {
+                buoy[s] = d->beam_buoy[m->beam_order[s]];
+                buoyant = buoyant || buoy[s] != 0.0;
+            }
+        if (buoyant) {
+            GF_TRY(dalloc(&m->d_bbuoy, buoy.size()));
+            GF_TRY(upload(m, m->d_bbuoy, buoy.data(), buoy.size()));
+            for (int k = 0; k < 3; k++) m->ocean_surface[k] = d->ocean_surface[k];
+        }
     }
     // shells, colour-sorted SoA
     if (ns > 0) {

@@ -86,10 +86,11 @@ extern "C" int gf_write_results(gf_model *m, const char *path)
             const double *r = &bres[12 * (size_t)e];
             const int c = m->h_beam_conn[3 * (size_t)e + 1];  // node 2 of Beam_1
             const double *x = &nodes[6 * (size_t)c], *u = &disp[6 * (size_t)c];
+            // Pipe_1::WriteResults (src/Pipe_1.cpp:586-599) prints the same columns
             fprintf(f,
-                    "Beam_1\t%d\tStress_r\t%.6e\t%.6e\t%.6e\t%.6e\t%.6e\t%.6e\tStrain_r\t%.6e\t%.6e\t%.6e\t%.6e\t%.6e\t%.6e"
+                    "%s\t%d\tStress_r\t%.6e\t%.6e\t%.6e\t%.6e\t%.6e\t%.6e\tStrain_r\t%.6e\t%.6e\t%.6e\t%.6e\t%.6e\t%.6e"
                     "\tRef_Coor\t%.6e\t%.6e\t%.6e\tCoor\t%.6e\t%.6e\t%.6e\n",
-                    num, r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8], r[9], r[10], r[11], xr[3 * c],
+                    m->beam_kind[e] == GF_BEAM_KIND_PIPE1 ? "Pipe_1" : "Beam_1", num, r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8], r[9], r[10], r[11], xr[3 * c],
                     xr[3 * c + 1], xr[3 * c + 2], x[0] + u[0], x[1] + u[1], x[2] + u[2]);
         } else {
             const int s = e - nb;

@@ -107,10 +107,12 @@ class Model:
         self.xyz = []
         self.fix = []
         self.beam_conn, self.beam_cs, self.beam_prop = [], [], []
+        self.beam_kind, self.beam_De = [], []   # GF_BEAM_KIND_*, Pipe_1 outer diameter
         self.shell_conn, self.shell_prop = [], []
         self.loads = []  # (kind, items, cs9, flags, times, vals[n][6])
         self.gravity = (0.0, 0.0, 0.0)
         self.gravity_on = 0
+        self.ocean_on, self.ocean_rho_fluid, self.ocean_surface = 0, 0.0, [0.0, 0.0, 0.0]
         # optional deck element numbers (a permutation of 1..n_beams+n_shells)
         self.beam_number = None
         self.shell_number = None
@@ -134,6 +136,25 @@ class Model:
         self.beam_conn.append([n1, n2, n3])
         self.beam_cs.append(list(cs9))
         self.beam_prop.append([E, nu, rho, A, I11, I22, I12, It, T0])
+        self.beam_kind.append(abi.GF_BEAM_KIND_BEAM1)
+        self.beam_De.append(0.0)
+        self._frozen = None
+
+    def add_pipe(self, n1, n2, n3, cs9, EA, EI, GJ, GA, rho_line, De=0.0):
+        """Pipe_1 with a PipeSection's EA, EI, GJ, GA, Rho (mass per length) and outer
+        diameter De (buoyancy below the ocean surface, set_ocean)."""
+        self.beam_conn.append([n1, n2, n3])
+        self.beam_cs.append(list(cs9))
+        self.beam_prop.append([EA, EI, GJ, GA, rho_line, 0.0, 0.0, 0.0, 0.0])
+        self.beam_kind.append(abi.GF_BEAM_KIND_PIPE1)
+        self.beam_De.append(float(De))
+        self._frozen = None
+
+    def set_ocean(self, rho_fluid, surface):
+        """OceanData: fluid density and a point of the free surface (no sea current)."""
+        self.ocean_on = 1
+        self.ocean_rho_fluid = float(rho_fluid)
+        self.ocean_surface = [float(v) for v in surface]
         self._frozen = None
 
     def add_shell(self, nodes6, E, nu, rho, t):
@@ -168,6 +189,9 @@ class Model:
         self.beam_conn.extend(np.asarray(conn, np.int64).reshape(-1, 3).tolist())
         self.beam_cs.extend(np.asarray(cs, float).reshape(-1, 9).tolist())
         self.beam_prop.extend(np.asarray(prop, float).reshape(-1, abi.GF_BEAM_NPROP).tolist())
+        n = len(self.beam_conn) - len(self.beam_kind)
+        self.beam_kind.extend([abi.GF_BEAM_KIND_BEAM1] * n)
+        self.beam_De.extend([0.0] * n)
         self._frozen = None
 
     # -- sizes ------------------------------------------------------------
@@ -252,6 +276,17 @@ class Model:
         d.tab_val = abi.ptr(f["tab_val"], C.c_double)
         d.gravity = (C.c_double * 3)(*self.gravity)
         d.gravity_on = int(self.gravity_on)
+        if any(k != abi.GF_BEAM_KIND_BEAM1 for k in self.beam_kind):
+            f["beam_kind"] = np.ascontiguousarray(self.beam_kind, np.int32)
+            d.beam_kind = abi.ptr(f["beam_kind"], C.c_int)
+            # Pipe_1 buoyant mass per length rho_fluid pi De^2 / 4 (only with OceanData)
+            De = np.asarray(self.beam_De, float)
+            f["beam_buoy"] = np.ascontiguousarray(self.ocean_rho_fluid * (np.pi * De * De / 4.0) if self.ocean_on
+                                                  else np.zeros_like(De))
+            d.beam_buoy = abi.ptr(f["beam_buoy"], C.c_double)
+        d.ocean_on = int(self.ocean_on)
+        d.ocean_surface = (C.c_double * 3)(*self.ocean_surface)
+        d.ocean_rho_fluid = self.ocean_rho_fluid
         if self.beam_number is not None:
             f["beam_number"] = np.ascontiguousarray(self.beam_number, np.int32)
             d.beam_number = abi.ptr(f["beam_number"], C.c_int)

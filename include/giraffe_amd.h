@@ -56,7 +56,11 @@ extern "C" {
 #define GF_LOAD_SHELL_PRESSURE  2  /* follower pressure on Shell_1 (ShellLoad)    */
 
 /* Per-element property records. */
-#define GF_BEAM_NPROP   9   /* E, nu, rho, A, I11, I22, I12, It, T0 (pre-tension) */
+#define GF_BEAM_NPROP   9   /* Beam_1: E, nu, rho, A, I11, I22, I12, It, T0 (pre-tension);
+                               Pipe_1: EA, EI, GJ, GA, Rho (mass per length), 0, 0, 0, 0 */
+#define GF_BEAM_KIND_BEAM1 0 /* Beam_1 (src/Beam_1.cpp)                         */
+#define GF_BEAM_KIND_PIPE1 1 /* Pipe_1 (src/Pipe_1.cpp): Beam_1 kinematics, the pipe
+                               section's D = diag(GA, GA, EA, EI, EI, GJ), buoyancy */
 #define GF_SHELL_NPROP  4   /* E, nu, rho, thickness                              */
 
 /* Model description: the hot-path subset of a GIRAFFE deck, already resolved
@@ -94,6 +98,18 @@ typedef struct gf_model_desc {
      * 1..n_beams, then Shell_1 n_beams+1..n_beams+n_shells. */
     const int *beam_number;          /* [n_beams] or NULL                             */
     const int *shell_number;         /* [n_shells] or NULL                            */
+
+    /* Pipe_1 and OceanData (src/Pipe_1.cpp:1278-1340, src/Environment.cpp:384-470).
+     * beam_kind[e]: GF_BEAM_KIND_* (NULL: all Beam_1).  beam_buoy[e]: buoyant mass
+     * per length rho_fluid * pi De^2 / 4 of a Pipe_1 (NULL: none); at a Gauss point
+     * below the surface (depth = (x - surface) . G / |G| > 0) the self-weight uses
+     * Rho - beam_buoy instead of Rho.  ocean_on: OceanData present; sea current
+     * (Morison) loads are not on this path. */
+    const int *beam_kind;            /* [n_beams] or NULL                             */
+    const double *beam_buoy;         /* [n_beams] or NULL                             */
+    int ocean_on;
+    double ocean_surface[3];         /* SurfacePosition                               */
+    double ocean_rho_fluid;          /* RhoFluid (kept for the deck writer)           */
 } gf_model_desc;
 
 /* Static solution-step and ConvergenceCriteria parameters, in the order of
@@ -275,10 +291,11 @@ int gf_shell_eval(int count, const double *x, const double *prop, const double *
 
 /* ---- input decks (src/IO.cpp:193-770 dispatch, hot-path subset) -------------
  * gf_deck_read parses a GIRAFFE .inp deck restricted to what this path runs:
- * Nodes, NodeSets, ElementSets, Elements (Beam_1, Shell_1), Materials (Hooke),
- * Sections (Tube), ShellSections (Homogeneous), CoordinateSystems, Loads
- * (NodalLoad, NodalFollowerLoad, ShellLoad with NTimes tables), Constraints
- * (NodalConstraint), Environment (GravityData), SolutionSteps (one Static) and
+ * Nodes, NodeSets, ElementSets, Elements (Beam_1, Shell_1, Pipe_1), Materials
+ * (Hooke), Sections (Tube), PipeSections, ShellSections (Homogeneous),
+ * CoordinateSystems, Loads (NodalLoad, NodalFollowerLoad, ShellLoad with NTimes
+ * tables), Constraints (NodalConstraint), Environment (GravityData; OceanData
+ * whose sea current is off or still), SolutionSteps (one Static) and
  * ConvergenceCriteria; Monitor, PostFiles, SolverOptions, ExecutionData and
  * ConfigurationSave are skipped; anything else is refused with a message.
  * gf_deck_desc returns the resolved description (arrays owned by the deck) and

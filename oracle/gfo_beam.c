@@ -6,6 +6,8 @@
  *   Beam_1::TransformMatrix    src/Beam_1.cpp:1385-1427
  *   Beam_1::Mount              src/Beam_1.cpp:695-835
  *   Beam_1::SaveLagrange       src/Beam_1.cpp:1494-1506
+ *   Pipe_1::PreCalc / Mount    src/Pipe_1.cpp:1106-1135, 836-973 (the Beam_1 kinematics
+ *                              with the pipe section's D)
  *   LagrangeSave initial state src/LagrangeSave.cpp:41-49
  *   V, d_V, skew               src/Matrix.cpp:1812-1833, 1999-2033
  *   Node Rodrigues composition src/Node.cpp:342
@@ -94,23 +96,37 @@ void gfo_compose(const double *a2, const double *a1, double *a3)
 /* Gauss abscissa exactly as written in src/Beam_1.cpp:629-632. */
 #define BEAM_CSI 0.577350269189626
 
-void gfo_beam_precalc(const double *x, const double *cs, const double *prop,
+void gfo_beam_precalc(const double *x, const double *cs, const double *prop, int kind,
                       gfo_beam_const *c, double *gp)
 {
-    double E = prop[0], nu = prop[1], rho = prop[2], A = prop[3];
-    double I1 = prop[4], I2 = prop[5], I12 = prop[6], It = prop[7], T0 = prop[8];
-    double G = E / (2 * (1 + nu));
-    double sf = 1.0;
+    double T0 = 0.0;
     memset(c, 0, sizeof(*c));
-    c->D[0 * 6 + 0] = sf * G * A;
-    c->D[1 * 6 + 1] = sf * G * A;
-    c->D[2 * 6 + 2] = E * A;
-    c->D[3 * 6 + 3] = E * I1;
-    c->D[4 * 6 + 4] = E * I2;
-    c->D[3 * 6 + 4] = E * I12;
-    c->D[4 * 6 + 3] = E * I12;
-    c->D[5 * 6 + 5] = G * It;
-    c->rhoA = rho * A;
+    if (kind == GF_BEAM_KIND_PIPE1) {
+        /* Pipe_1::PreCalc src/Pipe_1.cpp:1108-1135: D = diag(GA, GA, EA, EI, EI, GJ),
+         * mass per length Rho, no pretension */
+        c->D[0 * 6 + 0] = prop[3];
+        c->D[1 * 6 + 1] = prop[3];
+        c->D[2 * 6 + 2] = prop[0];
+        c->D[3 * 6 + 3] = prop[1];
+        c->D[4 * 6 + 4] = prop[1];
+        c->D[5 * 6 + 5] = prop[2];
+        c->rhoA = prop[4];
+    } else {
+        double E = prop[0], nu = prop[1], rho = prop[2], A = prop[3];
+        double I1 = prop[4], I2 = prop[5], I12 = prop[6], It = prop[7];
+        T0 = prop[8];
+        double G = E / (2 * (1 + nu));
+        double sf = 1.0;
+        c->D[0 * 6 + 0] = sf * G * A;
+        c->D[1 * 6 + 1] = sf * G * A;
+        c->D[2 * 6 + 2] = E * A;
+        c->D[3 * 6 + 3] = E * I1;
+        c->D[4 * 6 + 4] = E * I2;
+        c->D[3 * 6 + 4] = E * I12;
+        c->D[4 * 6 + 3] = E * I12;
+        c->D[5 * 6 + 5] = G * It;
+        c->rhoA = rho * A;
+    }
     /* transform3 = [E1; E2; E3] (dot products with the global basis) */
     for (int i = 0; i < 9; i++) c->T3[i] = cs[i];
     double e3[3] = {x[6] - x[0], x[7] - x[1], x[8] - x[2]};

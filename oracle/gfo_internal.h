@@ -76,7 +76,7 @@ typedef struct gfo_beam_const {
     double rhoA;           /* rho * A for self-weight                       */
 } gfo_beam_const;
 
-void gfo_beam_precalc(const double *x /*[9]*/, const double *cs, const double *prop,
+void gfo_beam_precalc(const double *x /*[9]*/, const double *cs, const double *prop, int kind,
                       gfo_beam_const *c, double *gp /*[2][GFO_BEAM_GPS]*/);
 /* Beam_1::Mount: K (18x18 row-major, global) and f_int (18, global); res (or NULL)
  * receives the Gauss-point means of sigma_r (6) and epsilon_r (6). */

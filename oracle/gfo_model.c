@@ -74,6 +74,9 @@ typedef struct gfo_model {
     double *mult;             /* [nl][6] */
     double grav[3];
     int grav_on;
+    double *bbuoy;            /* [nb] Pipe_1 buoyant mass per length (with OceanData) */
+    int ocean_on;
+    double surf[3];
     /* global system */
     int kl, ku, ldab;
     double *ab;
@@ -144,7 +147,8 @@ gfo_model *gfo_create(const gf_model_desc *desc, int solver)
         double x[9];
         for (int a = 0; a < 3; a++)
             for (int k = 0; k < 3; k++) x[3 * a + k] = m->xyz_ref[3 * m->bconn[3 * e + a] + k];
-        gfo_beam_precalc(x, desc->beam_cs + 9 * e, desc->beam_prop + GF_BEAM_NPROP * e, &m->bc[e],
+        gfo_beam_precalc(x, desc->beam_cs + 9 * e, desc->beam_prop + GF_BEAM_NPROP * e,
+                         desc->beam_kind ? desc->beam_kind[e] : GF_BEAM_KIND_BEAM1, &m->bc[e],
                          m->bgp + (size_t)e * 2 * GFO_BEAM_GPS);
     }
     /* shells */
@@ -178,6 +182,10 @@ gfo_model *gfo_create(const gf_model_desc *desc, int solver)
     m->mult = (double *)calloc(6 * (size_t)(nl ? nl : 1), sizeof(double));
     for (int k = 0; k < 3; k++) m->grav[k] = desc->gravity[k];
     m->grav_on = desc->gravity_on;
+    m->ocean_on = desc->ocean_on;
+    for (int k = 0; k < 3; k++) m->surf[k] = desc->ocean_surface[k];
+    m->bbuoy = (double *)calloc((size_t)m->nb + 1, sizeof(double));
+    if (desc->ocean_on && desc->beam_buoy) memcpy(m->bbuoy, desc->beam_buoy, sizeof(double) * (size_t)m->nb);
     m->solver = solver;
 #ifdef _OPENMP
     m->threads = omp_get_max_threads();
@@ -192,7 +200,7 @@ void gfo_destroy(gfo_model *m)
     if (!m) return;
     free(m->xyz_ref); free(m->copy); free(m->euler); free(m->d); free(m->xinc);
     free(m->GL); free(m->active); free(m->fix);
-    free(m->bconn); free(m->bc); free(m->bgp); free(m->bK); free(m->bP); free(m->bres);
+    free(m->bconn); free(m->bc); free(m->bgp); free(m->bK); free(m->bP); free(m->bres); free(m->bbuoy);
     free(m->sconn); free(m->sc); free(m->sgp); free(m->sK); free(m->sP); free(m->sres);
     free(m->lkind); free(m->lset_off); free(m->lset); free(m->lcs); free(m->lflags);
     free(m->ltab_off); free(m->tab_t); free(m->tab_v); free(m->mult);
@@ -466,12 +474,26 @@ int gfo_assemble(gfo_model *m, double *res)
         gfo_beam_mount(&m->bc[e], m->bgp + (size_t)e * 2 * GFO_BEAM_GPS, d, K, P, m->bres + 12 * (size_t)e);
         if (m->grav_on) {
             const gfo_beam_const *c = &m->bc[e];
-            double mult = lf * 1.0 * 1.0 * c->jac * c->rhoA;
             double el[18];
             memset(el, 0, sizeof(el));
-            for (int g = 0; g < 2; g++)
+            for (int g = 0; g < 2; g++) {
+                double mult = lf * 1.0 * 1.0 * c->jac * c->rhoA;
+                if (m->ocean_on && m->bbuoy[e] != 0.0) {
+                    /* Pipe_1::MountFieldLoading src/Pipe_1.cpp:1289-1306: below the surface
+                     * (depth along G > 0) the weight per length is Rho - rho_f Aext */
+                    double z[3], depth = 0.0, gn = sqrt(v3_dot(m->grav, m->grav));
+                    for (int k = 0; k < 3; k++) {
+                        z[k] = 0.0;
+                        for (int a = 0; a < 3; a++) z[k] += m->copy[6 * (size_t)m->bconn[3 * e + a] + k] * c->N[g][a];
+                        for (int a = 0; a < 3; a++) z[k] += d[6 * a + k] * c->N[g][a];
+                        depth += (z[k] - m->surf[k]) * m->grav[k];
+                    }
+                    depth = depth * (1.0 / gn);
+                    if (depth > 0) mult = lf * 1.0 * 1.0 * c->jac * (c->rhoA - m->bbuoy[e]);
+                }
                 for (int a = 0; a < 3; a++)
                     for (int k = 0; k < 3; k++) el[6 * a + k] += mult * c->N[g][a] * m->grav[k];
+            }
             for (int i = 0; i < 18; i++) P[i] = P[i] - el[i];
         }
     }

@@ -66,7 +66,7 @@ def lib():
         L.gfo_V.argtypes = [dp, dp, dp]
         L.gfo_dV.argtypes = [dp, dp, dp, dp]
         L.gfo_compose.argtypes = [dp, dp, dp]
-        L.gfo_beam_precalc.argtypes = [dp, dp, dp, C.c_void_p, dp]
+        L.gfo_beam_precalc.argtypes = [dp, dp, dp, C.c_int, C.c_void_p, dp]
         L.gfo_beam_mount.argtypes = [C.c_void_p, dp, dp, dp, dp, dp]
         L.gfo_beam_save.argtypes = [C.c_void_p, dp, dp]
         _lib = L
@@ -208,14 +208,14 @@ class _BeamConst(C.Structure):
                 ("jac", C.c_double), ("N", C.c_double * 6), ("dN", C.c_double * 6), ("rhoA", C.c_double)]
 
 
-def beam_element(x9, cs9, prop9):
-    """Returns (const struct, gp state [2][21]) for one Beam_1."""
+def beam_element(x9, cs9, prop9, kind=0):
+    """Returns (const struct, gp state [2][21]) for one Beam_1 (kind 0) or Pipe_1 (kind 1)."""
     c = _BeamConst()
     gp = np.zeros((2, 21))
     x9 = np.ascontiguousarray(x9, float)
     cs9 = np.ascontiguousarray(cs9, float)
     prop9 = np.ascontiguousarray(prop9, float)
-    lib().gfo_beam_precalc(_dp(x9), _dp(cs9), _dp(prop9), C.byref(c), _dp(gp))
+    lib().gfo_beam_precalc(_dp(x9), _dp(cs9), _dp(prop9), int(kind), C.byref(c), _dp(gp))
     return c, gp
 
 

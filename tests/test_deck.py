@@ -98,6 +98,80 @@ def test_element_numbers_round_trip(tmp_path, oracle_lib):
     assert np.abs(om.nodes() - od.nodes()).max() <= 1e-12 * np.abs(om.nodes()).max()
 
 
+PIPE_DECK = """\
+Nodes\t5
+Node\t1\t0\t0\t0
+Node\t2\t0\t0\t-5
+Node\t3\t0\t0\t-10
+Node\t4\t0\t0\t-15
+Node\t5\t0\t0\t-20
+NodeSets\t1
+NodeSet\t1\tNodes\t1\tList\t1
+Elements\t2
+Pipe_1\t1\tPipeSec\t1\tCS\t1\tNodes\t1\t2\t3
+Pipe_1\t2\tPipeSec\t1\tCS\t1\tNodes\t3\t4\t5
+PipeSections\t1
+PS\t1\tEA\t2e9\tEI\t3e6\tGJ\t2.4e6\tGA\t8e8\tRho\t120\tCDt\t0.1\tCDn\t1.2\tCAt\t0\tCAn\t1\tDe\t0.3\tDi\t0.2
+CoordinateSystems\t1
+CS\t1\tE1\t1\t0\t0\tE3\t0\t0\t-1
+Constraints\t1
+NodalConstraint\t1\tNodeSet\t1
+\tUX\t\tBoolTable\t1
+\tUY\t\tBoolTable\t1
+\tUZ\t\tBoolTable\t1
+\tROTX\tBoolTable\t1
+\tROTY\tBoolTable\t1
+\tROTZ\tBoolTable\t1
+Environment
+GravityData\tG\t0\t0\t-9.81\tBoolTable\t1
+OceanData
+RhoFluid\t1025
+SurfacePosition\t0\t0\t-10
+SeaCurrent\tN\t2\tBoolTable\t0
+Depth\t0\tSpeed\t1.5\tAngle\t0
+Depth\t100\tSpeed\t0.5\tAngle\t0
+SolutionSteps\t1
+Static\t1
+EndTime\t1
+TimeStep\t0.25
+MaxTimeStep\t0.25
+MinTimeStep\t0.01
+MaxIt\t20
+MinIt\t3
+ConvIncrease\t4
+IncFactor\t1.0
+Sample\t1
+"""
+
+
+def test_pipe_deck(tmp_path, oracle_lib):
+    """Pipe_1 / PipeSections / OceanData (an inactive sea current): the description
+    carries the pipe section's stiffness and the buoyant mass per length; the deck
+    writes back and solves alike."""
+    path = tmp_path / "pipe.inp"
+    path.write_text(PIPE_DECK)
+    deck = runtime.Deck(str(path))
+    d = deck.desc()
+    assert deck.n_beams == 2 and d.ocean_on == 1 and list(d.ocean_surface) == [0, 0, -10]
+    assert list(_arr(d.beam_kind, 2, np.int32)) == [abi.GF_BEAM_KIND_PIPE1] * 2
+    np.testing.assert_allclose(_arr(d.beam_buoy, 2, np.float64), 1025 * np.pi * 0.09 / 4, rtol=1e-15)
+    a = _desc_arrays(d)
+    assert list(a["beam_prop"][:5]) == [2e9, 3e6, 2.4e6, 8e8, 120.0]
+    p2 = str(tmp_path / "again.inp")
+    runtime.write_deck(p2, deck, deck.params)
+    again = runtime.Deck(p2)
+    b = _desc_arrays(again.desc())
+    for k in a:
+        assert np.allclose(a[k], b[k], rtol=1e-14, atol=0), k
+    ra, ia = oracle_lib.OracleModel(deck).static_solve(deck.params)
+    rb, ib = oracle_lib.OracleModel(again).static_solve(again.params)
+    assert ra.aborted == 0 and list(ia) == list(ib)
+    moving = PIPE_DECK.replace("SeaCurrent\tN\t2\tBoolTable\t0", "SeaCurrent\tN\t2\tBoolTable\t1")
+    path.write_text(moving)
+    with pytest.raises(RuntimeError, match="Morison"):
+        runtime.Deck(str(path))
+
+
 TUTORIAL_STYLE = """\
 /////////////////////////////////
 // cantilever, reference deck layout
