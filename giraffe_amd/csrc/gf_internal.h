@@ -45,7 +45,8 @@ enum { BG_PER_GP = 15, BG_COUNT = 30 };
 
 // ---- global tangent storage -------------------------------------------------
 // GM_BTD : block-tridiagonal blocks A (sub), B (diag), C (super), [nblk][m][m]
-//          row-major; used when the half-bandwidth fits a block of m <= 32.
+//          row-major, consecutive in one allocation (B = A + nblk m m,
+//          C = B + nblk m m); used when the half-bandwidth fits a block of m <= 32.
 // GM_BAND: skewed dense band, entry (i, c) with |i - c| <= hb at
 //          ab[i * 2hb + c + hb] (each row is a contiguous window of 2hb+1);
 //          sub-matrices inside the band are plain strided matrices with
@@ -58,19 +59,6 @@ struct gf_gmat {
     double *ab;
     int hb;
 };
-
-__device__ __forceinline__ void gm_add(const gf_gmat &g, int R, int Cc, double v)
-{
-    if (R < 0 || Cc < 0) return;
-    if (g.layout == GM_BTD) {
-        int bi = R / g.m, bj = Cc / g.m;
-        size_t off = ((size_t)bi * g.m + (R - bi * g.m)) * g.m + (Cc - bj * g.m);
-        double *base = bj == bi ? g.B : (bj < bi ? g.A : g.C);
-        base[off] += v;
-    } else {
-        g.ab[(size_t)R * (size_t)(2 * g.hb) + (size_t)(Cc + g.hb)] += v;
-    }
-}
 
 // Reciprocal of a pivot that is finite, non-zero and not subnormal (the solvers
 // check this first): the hardware estimate refined by Newton-Raphson steps
@@ -91,32 +79,48 @@ __device__ __forceinline__ float pivot_rcp(float p)
     return fmaf(r, e, r);
 }
 
-// Address of entry (R, Cc); for a constrained DOF (R or Cc negative) a valid
-// stand-in address whose value is never stored back.  Lets a batch of
-// read-modify-writes issue all its loads before any store (a load inside the
-// branch of gm_add waits for its own round trip).
-__device__ __forceinline__ double *gm_ptr(const gf_gmat &g, int R, int Cc)
+// Block index q = floor(x / m) for 0 <= x < 2^24 and m <= 32, through a float
+// product (one rounding, then one correction step each way): no integer divide.
+__device__ __forceinline__ int gm_block_of(int x, int m, float inv_m)
 {
-    const bool ok = R >= 0 && Cc >= 0;
-    R = ok ? R : 0;
-    Cc = ok ? Cc : 0;
-    if (g.layout == GM_BTD) {
-        const int bi = R / g.m, bj = Cc / g.m;
-        const size_t off = ((size_t)bi * g.m + (R - bi * g.m)) * g.m + (Cc - bj * g.m);
-        double *base = bj == bi ? g.B : (bj < bi ? g.A : g.C);
-        return base + off;
-    }
-    return g.ab + (size_t)R * (size_t)(2 * g.hb) + (size_t)(Cc + g.hb);
+    int q = (int)(((float)x + 0.5f) * inv_m);
+    q -= q * m > x ? 1 : 0;
+    q += (q + 1) * m <= x ? 1 : 0;
+    return q;
 }
 
-// N entries added at once: all loads, then the stores of the unconstrained ones.
+// N entries added at once: every address first (the layout test is one uniform
+// branch around the whole batch, so the loads below are straight-line code and
+// all in flight together), all loads, then the stores of the unconstrained ones.
 template <int N>
 __device__ __forceinline__ void gm_add_n(const gf_gmat &g, const int (&R)[N], const int (&Cc)[N], const double (&v)[N])
 {
     double *p[N];
     double old[N];
+    if (g.layout == GM_BTD) {
+        const int m = g.m;
+        const float inv_m = 1.0f / (float)m;
+        const ptrdiff_t stride = g.B - g.A;
 #pragma unroll
-    for (int k = 0; k < N; k++) p[k] = gm_ptr(g, R[k], Cc[k]);
+        for (int k = 0; k < N; k++) {
+            const bool ok = R[k] >= 0 && Cc[k] >= 0;
+            const int r = ok ? R[k] : 0, c = ok ? Cc[k] : 0;
+            const int bi = gm_block_of(r, m, inv_m), bj = gm_block_of(c, m, inv_m);
+            const size_t off = ((size_t)bi * m + (r - bi * m)) * m + (c - bj * m);
+            // A, B and C are one allocation in that order (gf_btd_alloc): the block
+            // kind selects an offset; a selected pointer would be re-read from the
+            // kernel arguments, one dependent load per entry
+            const ptrdiff_t kind = bj < bi ? 0 : (bj == bi ? 1 : 2);
+            p[k] = g.A + kind * stride + off;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+            const bool ok = R[k] >= 0 && Cc[k] >= 0;
+            const int r = ok ? R[k] : 0, c = ok ? Cc[k] : 0;
+            p[k] = g.ab + (size_t)r * (size_t)(2 * g.hb) + (size_t)(c + g.hb);
+        }
+    }
 #pragma unroll
     for (int k = 0; k < N; k++) old[k] = *p[k];
 #pragma unroll
