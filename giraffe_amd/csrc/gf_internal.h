@@ -89,14 +89,13 @@ __device__ __forceinline__ int gm_block_of(int x, int m, float inv_m)
     return q;
 }
 
-// N entries added at once: every address first (the layout test is one uniform
-// branch around the whole batch, so the loads below are straight-line code and
-// all in flight together), all loads, then the stores of the unconstrained ones.
+// Addresses of N entries (R, Cc); a constrained entry (R or Cc negative) gets a
+// valid stand-in address whose value is never stored back.  The layout test is
+// one uniform branch around the whole batch, so loads through the addresses are
+// straight-line code and all in flight together.
 template <int N>
-__device__ __forceinline__ void gm_add_n(const gf_gmat &g, const int (&R)[N], const int (&Cc)[N], const double (&v)[N])
+__device__ __forceinline__ void gm_addr_n(const gf_gmat &g, const int (&R)[N], const int (&Cc)[N], double *(&p)[N])
 {
-    double *p[N];
-    double old[N];
     if (g.layout == GM_BTD) {
         const int m = g.m;
         const float inv_m = 1.0f / (float)m;
@@ -121,6 +120,16 @@ __device__ __forceinline__ void gm_add_n(const gf_gmat &g, const int (&R)[N], co
             p[k] = g.ab + (size_t)r * (size_t)(2 * g.hb) + (size_t)(c + g.hb);
         }
     }
+}
+
+// N entries added at once: every address, all loads, then the stores of the
+// unconstrained ones.
+template <int N>
+__device__ __forceinline__ void gm_add_n(const gf_gmat &g, const int (&R)[N], const int (&Cc)[N], const double (&v)[N])
+{
+    double *p[N];
+    double old[N];
+    gm_addr_n<N>(g, R, Cc, p);
 #pragma unroll
     for (int k = 0; k < N; k++) old[k] = *p[k];
 #pragma unroll
