@@ -6,6 +6,10 @@
 #include <vector>
 #include "../../include/giraffe_amd.h"
 
+// Mixed precision: FP64 factorisations in a row after two FP32 fallbacks in a
+// row (the first back-off; later ones double while the retries keep failing).
+constexpr int GF_MIXED_BACKOFF = 16;
+
 // ---- error handling --------------------------------------------------------
 void gf_set_error(const std::string &msg);
 #define GF_HIP(call)                                                                  \
@@ -337,7 +341,9 @@ struct gf_model {
     gf_btd btd32;
     float *d_band32 = nullptr;
     double *d_scale = nullptr, *d_res = nullptr, *d_w = nullptr;
-    long long n_solves = 0, n_sweeps = 0, n_fallbacks = 0;
+    long long n_solves = 0, n_sweeps = 0, n_fallbacks = 0, n_direct = 0;
+    int fallback_streak = 0, fp64_direct_left = 0;  // mixed-precision back-off
+    int backoff_len = GF_MIXED_BACKOFF;
     // enqueue/result pairs (asynchronous drivers)
     bool ev_pending[GF_NPHASE] = {};
     long long pending_launches[GF_NPHASE] = {};

@@ -669,7 +669,7 @@ extern "C" int gf_model_stats(const gf_model *m, long long *out)
     out[0] = m->n_solves;
     out[1] = m->n_sweeps;
     out[2] = m->n_fallbacks;
-    out[3] = 0;
+    out[3] = m->n_direct;
     return GF_OK;
 }
 
@@ -873,6 +873,24 @@ static int solve_fp64_enqueue(gf_model *m)
     return GF_OK;
 }
 
+// Consecutive fallbacks of the mixed path; two in a row start a back-off of
+// backoff_len FP64 solves.  The first FP32 attempt after a back-off counts as
+// the second of a row, and a back-off that ends in another fallback doubles the
+// next one (up to 64 times the first); an FP32 solve that converges resets both.
+static void note_fallback(gf_model *m, bool fell_back)
+{
+    if (!fell_back) {
+        m->fallback_streak = 0;
+        m->backoff_len = GF_MIXED_BACKOFF;
+        return;
+    }
+    if (++m->fallback_streak >= 2) {
+        m->fp64_direct_left = m->backoff_len;
+        m->backoff_len = std::min(2 * m->backoff_len, 64 * GF_MIXED_BACKOFF);
+        m->fallback_streak = 1;
+    }
+}
+
 extern "C" int gf_solve(gf_model *m, int *info_fail)
 {
     if (!m || !info_fail) return GF_E_ARG;
@@ -881,7 +899,14 @@ extern "C" int gf_solve(gf_model *m, int *info_fail)
     *info_fail = 0;
     GF_TRY(gf_launch_fill_bytes(m->stream, m->d_flag, 4));
     m->n_solves++;
-    if (m->precision == GF_PREC_MIXED) {
+    // Mixed precision backs off after two fallbacks in a row: the next
+    // GF_MIXED_BACKOFF solves factor in FP64 directly, then FP32 is tried again.
+    const bool direct = m->precision == GF_PREC_MIXED && m->fp64_direct_left > 0;
+    if (direct) {
+        m->fp64_direct_left--;
+        m->n_direct++;
+    }
+    if (m->precision == GF_PREC_MIXED && !direct) {
         tick(m, GF_PH_FACTOR, true);
         GF_TRYN(nl_f32, gf_mixed_factor(m));
         tick(m, GF_PH_FACTOR, false);
@@ -898,6 +923,7 @@ extern "C" int gf_solve(gf_model *m, int *info_fail)
             GF_TRY(tock(m, GF_PH_SOLVE, nl_ref));
             m->n_sweeps += sw;
             m->n_fallbacks += fb;
+            note_fallback(m, fb != 0);
             m->factored = true;
             int flag = 0;
             GF_HIP(hipMemcpyAsync(&m->h_scal[GF_SCAL_FLAG], m->d_flag, 4, hipMemcpyDeviceToHost, m->stream));
@@ -908,6 +934,7 @@ extern "C" int gf_solve(gf_model *m, int *info_fail)
         }
         // an unusable FP32 pivot: FP64 factorisation below
         m->n_fallbacks++;
+        note_fallback(m, true);
         GF_TRY(gf_launch_fill_bytes(m->stream, m->d_flag, 4));
     }
     GF_TRY(solve_fp64_enqueue(m));

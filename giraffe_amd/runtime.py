@@ -141,7 +141,7 @@ class DeviceModel:
     def stats(self):
         o = (C.c_longlong * 4)()
         check(load().gf_model_stats(self.h, o), "gf_model_stats")
-        return {"solves": o[0], "refine_sweeps": o[1], "fallbacks": o[2]}
+        return {"solves": o[0], "refine_sweeps": o[1], "fallbacks": o[2], "fp64_direct": o[3]}
 
     def set_window(self, start, end):
         check(load().gf_set_solution_window(self.h, start, end), "gf_set_solution_window")

@@ -166,7 +166,10 @@ int gf_model_destroy(gf_model *m);
  * padded order, colours, bytes of the tangent storage, layout (0 BTD, 1 band), precision. */
 int gf_model_info(const gf_model *m, long long *info_out /* [8] */);
 /* Tangent solves so far, FP64 refinement sweeps (mixed precision), solves that
- * fell back to the FP64 factorisation, reserved. */
+ * fell back to the FP64 factorisation, solves factored in FP64 directly by the
+ * mixed path's back-off (after two fallbacks in a row the next 16 solves; a retry
+ * that falls back again doubles the next back-off, up to 1,024; a converged FP32
+ * solve resets it). */
 int gf_model_stats(const gf_model *m, long long *stats_out /* [4] */);
 /* Device bytes gf_model_create would allocate for this description (host-side
  * planning only; no device call).  Used for admission by the request server. */

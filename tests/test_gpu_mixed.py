@@ -70,7 +70,9 @@ def test_mixed_plate_static_matches_oracle(oracle_lib):
 def test_mixed_catenary_falls_back_to_fp64(oracle_lib):
     """A long slender riser has cond(D K D) ~ 1e13 (500 elements), far beyond
     1/eps32: the FP32 factor cannot drive the refinement and every solve must
-    end on the FP64 factorisation, with results identical to the FP64 path."""
+    end on the FP64 factorisation, with results identical to the FP64 path.  After
+    two fallbacks in a row the path factors in FP64 directly, for 16 solves and
+    then for doubling stretches while each retry falls back again."""
     m, p, _ = M.catenary(n_elem=500)
     d64 = runtime.DeviceModel(m)
     r64, i64 = d64.static_solve(p)
@@ -81,7 +83,11 @@ def test_mixed_catenary_falls_back_to_fp64(oracle_lib):
     n64, nmx = d64.nodes(), dmx.nodes()
     assert np.abs(nmx - n64).max() <= 1e-8 * np.abs(n64).max()
     st = dmx.stats()
-    assert st["fallbacks"] > 0
+    assert st["fallbacks"] > 0 and st["fp64_direct"] > 0
+    assert st["fallbacks"] + st["fp64_direct"] == st["solves"]
+    # the back-off: two fallbacks, 16 direct FP64 solves, then one retry per
+    # back-off while the back-offs double (16, 32, 64, ...)
+    assert st["fallbacks"] <= 2 + int(np.log2(st["solves"] / 16 + 1)) + 1
 
 
 def test_mixed_singular_tangent_reports_info_fail():
