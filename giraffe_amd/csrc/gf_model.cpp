@@ -769,12 +769,17 @@ int gf_solve_update_enqueue(gf_model *m, bool then_assemble)
 {
     gf_probe_scope ps(probe_of(m));
     GF_HIP(hipSetDevice(m->device));
-    if (m->precision == GF_PREC_MIXED) {
+    if (m->precision == GF_PREC_MIXED && m->fp64_direct_left == 0) {
         int info_fail = 0;
         GF_TRY(gf_solve(m, &info_fail));
         m->pending_info_fail = info_fail;
         GF_TRY(gf_launch_fill_bytes(m->stream, m->d_scal, GF_SCAL_FLAG * sizeof(double)));
     } else {
+        // FP64, or a mixed model backed off to FP64: queued without a host wait
+        if (m->precision == GF_PREC_MIXED) {
+            m->fp64_direct_left--;
+            m->n_direct++;
+        }
         GF_TRY(gf_launch_fill_bytes(m->stream, m->d_scal, (GF_SCAL_FLAG + 1) * sizeof(double)));
         m->n_solves++;
         GF_TRY(solve_fp64_enqueue(m));
