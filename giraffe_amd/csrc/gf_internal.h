@@ -293,6 +293,7 @@ struct gf_model {
     double gravity[3] = {0, 0, 0};
     int gravity_on = 0;
     int half_band = 0;
+    bool node_order_rcm = false;    // DOFs numbered in reverse Cuthill-McKee node order
 
     // device: node state
     double *d_disp = nullptr;       // [nn][6] incremental displacements

@@ -209,6 +209,86 @@ static int colour_elements(const int *conn, int ne, int nper, int nn, std::vecto
     return GF_OK;
 }
 
+// Reverse Cuthill-McKee order of the nodes (Cuthill & McKee 1969; George & Liu,
+// "Computer Solution of Large Sparse Positive Definite Systems", 1981): nodes are
+// adjacent when an element holds both.  Each connected component starts at a
+// pseudo-peripheral node (repeated breadth-first searches from a node of least
+// degree in the last level, while the eccentricity grows), is numbered level by
+// level with the neighbours of each node in increasing degree, and the whole
+// order is reversed.  Nodes no element holds come last.
+static std::vector<int> rcm_order(const gf_model_desc *d, int nn)
+{
+    std::vector<std::vector<int>> adj(nn);
+    auto link = [&](const int *cn, int nper) {
+        for (int a = 0; a < nper; a++)
+            for (int b = 0; b < nper; b++)
+                if (a != b) adj[cn[a]].push_back(cn[b]);
+    };
+    for (int e = 0; e < d->n_beams; e++) link(d->beam_conn + 3 * (size_t)e, 3);
+    for (int e = 0; e < d->n_shells; e++) link(d->shell_conn + 6 * (size_t)e, 6);
+    for (auto &v : adj) {
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+    }
+    std::vector<int> deg(nn);
+    for (int i = 0; i < nn; i++) deg[i] = (int)adj[i].size();
+    for (auto &v : adj)
+        std::stable_sort(v.begin(), v.end(), [&](int a, int b) { return deg[a] < deg[b]; });
+    std::vector<int> level(nn, -1), order, stamp(nn, -1);
+    order.reserve(nn);
+    // breadth-first levels from s over the component (stamp marks the visit round)
+    int round = 0;
+    auto bfs = [&](int s, std::vector<int> &seq) {
+        seq.clear();
+        seq.push_back(s);
+        stamp[s] = round;
+        level[s] = 0;
+        for (size_t h = 0; h < seq.size(); h++)
+            for (int v : adj[seq[h]])
+                if (stamp[v] != round) {
+                    stamp[v] = round;
+                    level[v] = level[seq[h]] + 1;
+                    seq.push_back(v);
+                }
+        round++;
+        return level[seq.back()];
+    };
+    std::vector<char> done(nn, 0);
+    std::vector<int> seq;
+    std::vector<int> by_degree(nn);
+    for (int i = 0; i < nn; i++) by_degree[i] = i;
+    std::stable_sort(by_degree.begin(), by_degree.end(), [&](int a, int b) { return deg[a] < deg[b]; });
+    for (int s0 : by_degree) {
+        if (done[s0] || deg[s0] == 0) continue;
+        int s = s0, ecc = bfs(s, seq);
+        for (int tries = 0; tries < 16; tries++) {
+            int best = -1;
+            for (size_t k = seq.size(); k-- > 0 && level[seq[k]] == ecc;)
+                if (best < 0 || deg[seq[k]] < deg[best]) best = seq[k];
+            std::vector<int> seq2;
+            const int e2 = bfs(best, seq2);
+            if (e2 <= ecc) break;
+            s = best;
+            ecc = e2;
+            seq.swap(seq2);
+        }
+        // Cuthill-McKee from s: neighbours in increasing degree (adj is sorted so)
+        const size_t first = order.size();
+        order.push_back(s);
+        done[s] = 1;
+        for (size_t h = first; h < order.size(); h++)
+            for (int v : adj[order[h]])
+                if (!done[v]) {
+                    done[v] = 1;
+                    order.push_back(v);
+                }
+    }
+    std::reverse(order.begin(), order.end());
+    for (int i = 0; i < nn; i++)
+        if (!done[i]) order.push_back(i);
+    return order;
+}
+
 // Host-side planning shared by create() and gf_model_footprint: validation,
 // DOFsActive + SetGlobalDOFs, half-bandwidth, storage layout, colourings.
 static int plan(const gf_model_desc *d, gf_model *m, int &bm, int &nblk)
@@ -241,34 +321,56 @@ static int plan(const gf_model_desc *d, gf_model *m, int &bm, int &nblk)
             for (int k = 0; k < (a < 3 ? 3 : 6); k++) m->active[6 * (size_t)d->shell_conn[6 * e + a] + k] = 1;
     if (d->node_fix)
         for (size_t i = 0; i < 6 * (size_t)nn; i++) m->fix[i] = d->node_fix[i] ? 1 : 0;
-    m->dof.assign(6 * (size_t)nn, -1);
-    m->row_nodecomp.clear();
-    int nf = 0;
-    for (int i = 0; i < nn; i++)
-        for (int k = 0; k < 6; k++) {
-            size_t id = 6 * (size_t)i + k;
-            if (m->active[id] && !m->fix[id]) {
-                m->dof[id] = nf++;
-                m->row_nodecomp.push_back((int)id);
-            }
-        }
-    m->n_free = nf;
-    if (nf == 0) { gf_set_error("model has no free DOFs"); return GF_E_ARG; }
-    // half-bandwidth; row span of every element
+    // Node order of the DOF numbering: the caller's, or reverse Cuthill-McKee
+    // when that gives the narrower band (the reference leaves the ordering to
+    // PARDISO's nested dissection, src/SparseMatrix.cpp:115-117; a band solver
+    // needs a bandwidth-reducing order of its own).
     std::vector<int> lo(nb + ns), hi(nb + ns);
-    int hb = 0;
-    for (int e = 0; e < nb + ns; e++) {
-        const int *cn = e < nb ? d->beam_conn + 3 * (size_t)e : d->shell_conn + 6 * (size_t)(e - nb);
-        int nper = e < nb ? 3 : 6;
-        int l = 1 << 30, h = -1;
-        for (int a = 0; a < nper; a++)
+    auto number = [&](const std::vector<int> &order) {
+        m->dof.assign(6 * (size_t)nn, -1);
+        m->row_nodecomp.clear();
+        int nf = 0;
+        for (int i : order)
             for (int k = 0; k < 6; k++) {
-                int r = m->dof[6 * (size_t)cn[a] + k];
-                if (r >= 0) { l = std::min(l, r); h = std::max(h, r); }
+                size_t id = 6 * (size_t)i + k;
+                if (m->active[id] && !m->fix[id]) {
+                    m->dof[id] = nf++;
+                    m->row_nodecomp.push_back((int)id);
+                }
             }
-        lo[e] = l; hi[e] = h;
-        if (h >= 0) hb = std::max(hb, h - l);
+        m->n_free = nf;
+        // half-bandwidth; row span of every element
+        int hb = 0;
+        for (int e = 0; e < nb + ns; e++) {
+            const int *cn = e < nb ? d->beam_conn + 3 * (size_t)e : d->shell_conn + 6 * (size_t)(e - nb);
+            int nper = e < nb ? 3 : 6;
+            int l = 1 << 30, h = -1;
+            for (int a = 0; a < nper; a++)
+                for (int k = 0; k < 6; k++) {
+                    int r = m->dof[6 * (size_t)cn[a] + k];
+                    if (r >= 0) { l = std::min(l, r); h = std::max(h, r); }
+                }
+            lo[e] = l; hi[e] = h;
+            if (h >= 0) hb = std::max(hb, h - l);
+        }
+        return hb;
+    };
+    std::vector<int> natural(nn);
+    for (int i = 0; i < nn; i++) natural[i] = i;
+    int hb = number(natural);
+    if (m->n_free == 0) { gf_set_error("model has no free DOFs"); return GF_E_ARG; }
+    m->node_order_rcm = false;
+    if (hb > 0 && getenv("GF_NO_RCM") == nullptr) {
+        const std::vector<int> rcm = rcm_order(d, nn);
+        const int hb_rcm = number(rcm);
+        if (hb_rcm < hb) {
+            hb = hb_rcm;
+            m->node_order_rcm = true;
+        } else {
+            number(natural);
+        }
     }
+    const int nf = m->n_free;
     m->half_band = hb;
     // narrow bands: block-tridiagonal form with the smallest block size m <= 32
     bm = -1;
@@ -278,6 +380,9 @@ static int plan(const gf_model_desc *d, gf_model *m, int &bm, int &nblk)
             if (hi[e] >= 0 && hi[e] / cand - lo[e] / cand > 1) ok = false;
         if (ok) bm = cand;
     }
+    // the block-tridiagonal scatter finds a row's block through a float product
+    // (gm_block_of), exact for rows below 2^24: larger orders take the band layout
+    if (bm > 0 && (long long)(nf + bm - 1) / bm * bm >= (1LL << 24)) bm = -1;
     m->layout = bm > 0 ? GM_BTD : GM_BAND;
     nblk = 0;
     if (m->layout == GM_BTD) {
@@ -567,6 +672,11 @@ extern "C" int gf_reset_state(gf_model *m)
     GF_TRY(gf_launch_fill(m->stream, m->d_euler, 3 * (size_t)m->nn, 0.0));
     GF_HIP(hipStreamSynchronize(m->stream));
     m->factored = false;
+    // the mixed-precision back-off starts over: a solve after a reset is
+    // independent of the ones before it
+    m->fallback_streak = 0;
+    m->fp64_direct_left = 0;
+    m->backoff_len = GF_MIXED_BACKOFF;
     return GF_OK;
 }
 
@@ -607,6 +717,19 @@ extern "C" int gf_timing_read(gf_model *m, double *ms_out, long long *launches_o
         if (launches_out) launches_out[i] = m->launches[i];
         if (reset) { m->t_ms[i] = 0.0; m->launches[i] = 0; }
     }
+    return GF_OK;
+}
+
+extern "C" int gf_model_plan(const gf_model_desc *desc, long long *o)
+{
+    if (!desc || !o) return GF_E_ARG;
+    gf_model tmp;
+    int bm = -1, nblk = 0;
+    GF_TRY(plan(desc, &tmp, bm, nblk));
+    o[0] = tmp.n_free;
+    o[1] = tmp.half_band;
+    o[2] = tmp.layout == GM_BTD ? bm : 0;
+    o[3] = tmp.node_order_rcm ? 1 : 0;
     return GF_OK;
 }
 

@@ -31,11 +31,14 @@ double ms_between(clk::time_point a, clk::time_point b) { return std::chrono::du
 
 // Owned copy of a gf_model_desc.
 struct DescCopy {
-    std::vector<double> xyz, beam_cs, beam_prop, shell_prop, load_cs, tab_time, tab_val;
+    std::vector<double> xyz, beam_cs, beam_prop, shell_prop, load_cs, tab_time, tab_val, beam_buoy;
     std::vector<unsigned char> fix;
     std::vector<int> beam_conn, shell_conn, load_kind, load_set_off, load_set_items, load_flags, load_tab_off;
+    std::vector<int> beam_kind, beam_number, shell_number;
     gf_model_desc d{};
 
+    // Every array the description points at is copied: the model is created
+    // later, when the request is admitted, after the caller's arrays may be gone.
     void copy_from(const gf_model_desc &s)
     {
         const size_t nn = (size_t)s.n_nodes, nb = (size_t)s.n_beams, ns = (size_t)s.n_shells, nl = (size_t)s.n_loads;
@@ -45,10 +48,14 @@ struct DescCopy {
             beam_conn.assign(s.beam_conn, s.beam_conn + 3 * nb);
             beam_cs.assign(s.beam_cs, s.beam_cs + 9 * nb);
             beam_prop.assign(s.beam_prop, s.beam_prop + GF_BEAM_NPROP * nb);
+            if (s.beam_kind) beam_kind.assign(s.beam_kind, s.beam_kind + nb);
+            if (s.beam_buoy) beam_buoy.assign(s.beam_buoy, s.beam_buoy + nb);
+            if (s.beam_number) beam_number.assign(s.beam_number, s.beam_number + nb);
         }
         if (ns) {
             shell_conn.assign(s.shell_conn, s.shell_conn + 6 * ns);
             shell_prop.assign(s.shell_prop, s.shell_prop + GF_SHELL_NPROP * ns);
+            if (s.shell_number) shell_number.assign(s.shell_number, s.shell_number + ns);
         }
         if (nl) {
             load_kind.assign(s.load_kind, s.load_kind + nl);
@@ -77,6 +84,10 @@ struct DescCopy {
         d.load_tab_off = load_tab_off.empty() ? nullptr : load_tab_off.data();
         d.tab_time = tab_time.data();
         d.tab_val = tab_val.data();
+        d.beam_kind = beam_kind.empty() ? nullptr : beam_kind.data();
+        d.beam_buoy = beam_buoy.empty() ? nullptr : beam_buoy.data();
+        d.beam_number = beam_number.empty() ? nullptr : beam_number.data();
+        d.shell_number = shell_number.empty() ? nullptr : shell_number.data();
     }
 };
 

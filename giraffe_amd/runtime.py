@@ -70,6 +70,7 @@ def load():
     L.gf_kernel_probe.argtypes = [vp, C.c_uint]
     L.gf_kernel_probe_read.argtypes = [vp, dp, C.POINTER(C.c_longlong), dp, C.c_int]
     L.gf_model_footprint.argtypes = [C.POINTER(abi.GfModelDesc), C.POINTER(abi.GfOptions), C.POINTER(C.c_ulonglong)]
+    L.gf_model_plan.argtypes = [C.POINTER(abi.GfModelDesc), C.POINTER(C.c_longlong)]
     L.gf_default_server_options.argtypes = [C.POINTER(abi.GfServerOptions)]
     L.gf_server_create.argtypes = [C.POINTER(abi.GfServerOptions), C.POINTER(vp)]
     L.gf_server_destroy.argtypes = [vp]
@@ -311,6 +312,15 @@ def footprint(model, precision=abi.GF_PREC_FP64):
     b = C.c_ulonglong()
     check(load().gf_model_footprint(C.byref(desc), C.byref(o), C.byref(b)), "gf_model_footprint")
     return int(b.value)
+
+
+def plan(model):
+    """Host-side planning of `model` (gf_model_plan): n_free, half-band, BTD block
+    size (0: band layout) and whether the DOFs follow reverse Cuthill-McKee order."""
+    desc = model.desc()
+    o = (C.c_longlong * 4)()
+    check(load().gf_model_plan(C.byref(desc), o), "gf_model_plan")
+    return {"n_free": o[0], "half_band": o[1], "block": o[2], "rcm": bool(o[3])}
 
 
 class Server:

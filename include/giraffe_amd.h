@@ -174,6 +174,11 @@ int gf_model_stats(const gf_model *m, long long *stats_out /* [4] */);
 /* Device bytes gf_model_create would allocate for this description (host-side
  * planning only; no device call).  Used for admission by the request server. */
 int gf_model_footprint(const gf_model_desc *desc, const gf_options *opt, unsigned long long *bytes);
+/* Host-side planning only (no device call; no counterpart in the reference, whose
+ * ordering is PARDISO's): info_out = n_free, half-bandwidth, BTD block size (0 for
+ * the band layout), 1 when the DOFs are numbered in reverse Cuthill-McKee node
+ * order (chosen when it narrows the band of the caller's node order). */
+int gf_model_plan(const gf_model_desc *desc, long long *info_out /* [4] */);
 
 /* ---- Newton step pieces --------------------------------------------------- */
 /* Time window of the current solution step; the gravity BoolTable ramp
@@ -198,8 +203,8 @@ int gf_rollback(gf_model *m);
 int gf_static_solve(gf_model *m, const gf_static_params *p, gf_static_report *rep);
 
 /* Return the model to the state right after gf_model_create (reference
- * configuration, identity Gauss-point rotations), keeping the device buffers,
- * so the same resident model can be solved again. */
+ * configuration, identity Gauss-point rotations, no mixed-precision back-off),
+ * keeping the device buffers, so the same resident model can be solved again. */
 int gf_reset_state(gf_model *m);
 
 /* ---- device timing ------------------------------------------------------------

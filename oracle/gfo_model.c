@@ -826,6 +826,20 @@ typedef struct conv_state {
     int n_conv_evaluated;
 } conv_state;
 
+/* EstablishResidualCriteria src/ConvergenceCriteria.cpp:254-287: the criteria of
+ * a load step's first iteration (tol x max(max |r|, min_ref)), averaged with the
+ * criteria of all earlier steps of the solution (n_conv_evaluated of them).
+ * crit[0..1] = force / moment criterion in and out; *n counts the evaluations. */
+void gfo_residual_criteria(double *crit, int *n, double max_force, double max_moment, const gf_static_params *p)
+{
+    double pf = crit[0], pm = crit[1];
+    double fc = (max_force * p->force_tol > p->force_min * p->force_tol) ? max_force * p->force_tol : p->force_min * p->force_tol;
+    double mc = (max_moment * p->moment_tol > p->moment_min * p->moment_tol) ? max_moment * p->moment_tol : p->moment_min * p->moment_tol;
+    crit[0] = (*n * pf + fc) / (*n + 1);
+    crit[1] = (*n * pm + mc) / (*n + 1);
+    (*n)++;
+}
+
 static double now_ms(void)
 {
     struct timespec ts;
@@ -867,12 +881,10 @@ int gfo_static_solve(gfo_model *m, const gf_static_params *p, gf_static_report *
             r.total_iterations++;
             if (it == 1) {
                 /* EstablishResidualCriteria src/ConvergenceCriteria.cpp:187-288 */
-                double pf = cs.force_crit, pm = cs.moment_crit;
-                double fc = (res[0] * p->force_tol > p->force_min * p->force_tol) ? res[0] * p->force_tol : p->force_min * p->force_tol;
-                double mc = (res[1] * p->moment_tol > p->moment_min * p->moment_tol) ? res[1] * p->moment_tol : p->moment_min * p->moment_tol;
-                cs.force_crit = (cs.n_conv_evaluated * pf + fc) / (cs.n_conv_evaluated + 1);
-                cs.moment_crit = (cs.n_conv_evaluated * pm + mc) / (cs.n_conv_evaluated + 1);
-                cs.n_conv_evaluated++;
+                double crit[2] = {cs.force_crit, cs.moment_crit};
+                gfo_residual_criteria(crit, &cs.n_conv_evaluated, res[0], res[1], p);
+                cs.force_crit = crit[0];
+                cs.moment_crit = crit[1];
             }
             /* CheckResidualConvergence src/ConvergenceCriteria.cpp:460-598 */
             int has_f = 0, has_m = 0;

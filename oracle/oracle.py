@@ -60,6 +60,9 @@ def lib():
         L.gfo_bandwidth.argtypes = [vp]
         L.gfo_dof_map.argtypes = [vp, C.POINTER(C.c_int)]
         L.gfo_pardiso_available.restype = C.c_int
+        L.gfo_residual_criteria.argtypes = [dp, C.POINTER(C.c_int), C.c_double, C.c_double,
+                                            C.POINTER(abi.GfStaticParams)]
+        L.gfo_residual_criteria.restype = None
         L.gfo_gbtrf.argtypes = [C.c_int, C.c_int, C.c_int, dp, C.c_int, C.POINTER(C.c_int)]
         L.gfo_gbtrs.argtypes = [C.c_int, C.c_int, C.c_int, dp, C.c_int, C.POINTER(C.c_int), dp]
         L.gfo_rodrigues.argtypes = [dp, dp, dp, dp]
@@ -75,6 +78,15 @@ def lib():
 
 def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def residual_criteria(crit, n, max_force, max_moment, params):
+    """One EstablishResidualCriteria update: returns (force_crit, moment_crit, n)."""
+    c = np.array(crit, float)
+    k = C.c_int(n)
+    pc = params.c()
+    lib().gfo_residual_criteria(_dp(c), C.byref(k), max_force, max_moment, C.byref(pc))
+    return c[0], c[1], k.value
 
 
 def pardiso_available() -> bool:

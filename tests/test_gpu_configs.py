@@ -1,0 +1,137 @@
+"""The benchmark configurations at their stated sizes on the device, each result
+checked against the oracle with the reference's own solver (MKL PARDISO, called
+as sparsesystem does, where libmkl_rt is present):
+
+* the clamped Shell_1 plate of 64 x 64 quads (8,192 elements, 84,867 free DOFs);
+* a sample of the riser serving stream covering E in [50, 5,000] with one
+  5,000-element riser, through the request server;
+* a sample of the mixed riser / plate stream with a 48 x 48 plate, through the
+  server, FP64 and mixed precision;
+* the mixed-precision server: each result the same alone and in the mix.
+
+Bar: same converged steps, Newton iterations per step within 1, node
+displacements and rotations within 1e-8 of the oracle (relative to their
+scales), final relative residual <= 1e-10."""
+import time
+
+import numpy as np
+import pytest
+
+from giraffe_amd import abi, model as M, runtime
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_result(oracle_lib, m, p):
+    solver = oracle_lib.SOLVER_PARDISO if oracle_lib.pardiso_available() else oracle_lib.SOLVER_BAND
+    om = oracle_lib.OracleModel(m, solver)
+    ro, io = om.static_solve(p)
+    return ro, np.asarray(io), om.nodes()
+
+
+def check_against_oracle(m, rep, iters, nodes, ref):
+    ro, io, no = ref
+    iters = np.asarray(iters)
+    assert rep.aborted == ro.aborted == 0
+    assert rep.converged_steps == ro.converged_steps
+    assert len(iters) == len(io) and np.abs(iters - io).max() <= 1, (iters, io)
+    x0 = m.freeze()["xyz"]
+    scale_u = max(np.abs(no[:, :3] - x0).max(), 1e-12)
+    scale_r = max(np.abs(no[:, 3:]).max(), 1e-12)
+    err_u = np.abs(nodes[:, :3] - no[:, :3]).max() / scale_u
+    err_r = np.abs(nodes[:, 3:] - no[:, 3:]).max() / scale_r
+    assert err_u <= 1e-8 and err_r <= 1e-8, (err_u, err_r)
+    assert rep.max_rel_residual <= 1e-10
+    return err_u, err_r
+
+
+@pytest.fixture(scope="module")
+def plate64(oracle_lib):
+    m, p, info = M.shell_plate(nq=64)
+    print("oracle: config 3 plate 64x64", flush=True)
+    return m, p, oracle_result(oracle_lib, m, p)
+
+
+@pytest.mark.parametrize("precision", [abi.GF_PREC_FP64, abi.GF_PREC_MIXED])
+def test_config3_plate_full_size(plate64, precision):
+    m, p, ref = plate64
+    print(f"config 3 plate 64x64, precision {precision}", flush=True)
+    dm = runtime.DeviceModel(m, precision=precision)
+    assert dm.info()["n_free"] == 84867
+    rep, iters = dm.static_solve(p)
+    err = check_against_oracle(m, rep, iters, dm.nodes(), ref)
+    print(f"errors {err}, iterations {list(iters)}", flush=True)
+
+
+def _drain(srv, ids, limit_s=110.0):
+    out = {}
+    t0 = time.monotonic()
+    while len(out) < len(ids):
+        assert time.monotonic() - t0 < limit_s, f"requests not finished: {sorted(set(ids) - set(out))}"
+        srv.pump(2000)
+        for rid in ids:
+            if rid not in out:
+                try:
+                    out[rid] = srv.take(rid)
+                except runtime.GiraffeError:
+                    pass
+    return out
+
+
+def _riser_sample(rng, sizes):
+    return [M.riser_request(rng, n_elem=ne)[:2] for ne in sizes]
+
+
+def test_config4_riser_stream_sample(oracle_lib):
+    """Risers of 50 ... 5,000 elements (the stream's range) through the server."""
+    rng = np.random.default_rng(1004)
+    reqs = _riser_sample(rng, [50, 137, 420, 1300, 5000, 2600])
+    srv = runtime.Server(max_inflight=4)
+    ids = [srv.submit(m, p) for m, p in reqs]
+    res = _drain(srv, ids)
+    srv.close()
+    for (m, p), rid in zip(reqs, ids):
+        rep, it, nodes, _ = res[rid]
+        check_against_oracle(m, rep, it, nodes, oracle_result(oracle_lib, m, p))
+
+
+@pytest.mark.parametrize("precision", [abi.GF_PREC_FP64, abi.GF_PREC_MIXED])
+def test_config5_mixed_stream_sample(oracle_lib, precision):
+    """Risers with a 48 x 48 and a 16 x 16 plate (the stream's plates) through the server."""
+    rng = np.random.default_rng(1005)
+    reqs = _riser_sample(rng, [80, 700, 3000])
+    for nq, t, k in [(48, 0.006, 12.0), (16, 0.003, 20.0)]:
+        p_max = 1.7e7 * (k / 20.0) ** 3 * (t / 0.005) ** 4
+        m, p, _ = M.shell_plate(nq=nq, t=t, p_max=p_max, seed=int(rng.integers(1 << 30)))
+        reqs.insert(1, (m, p))
+    srv = runtime.Server(precision=precision, max_inflight=4)
+    ids = [srv.submit(m, p) for m, p in reqs]
+    res = _drain(srv, ids)
+    srv.close()
+    for (m, p), rid in zip(reqs, ids):
+        rep, it, nodes, _ = res[rid]
+        check_against_oracle(m, rep, it, nodes, oracle_result(oracle_lib, m, p))
+
+
+def test_mixed_server_alone_and_in_the_mix():
+    """Mixed precision: the refinement and its back-off belong to each request's
+    model, so a request's result does not depend on what else is resident."""
+    rng = np.random.default_rng(77)
+    reqs = _riser_sample(rng, [60, 240, 900])
+    m, p, _ = M.shell_plate(nq=12, n_steps=5, p_max=4e6)
+    reqs.append((m, p))
+    alone = []
+    for m, p in reqs:
+        dm = runtime.DeviceModel(m, precision=abi.GF_PREC_MIXED)
+        rep, it = dm.static_solve(p)
+        alone.append((list(it), dm.nodes()))
+        dm.close()
+    srv = runtime.Server(precision=abi.GF_PREC_MIXED, max_inflight=3)
+    ids = [srv.submit(m, p) for m, p in reqs]
+    res = _drain(srv, ids)
+    srv.close()
+    for (it_a, nodes_a), rid in zip(alone, ids):
+        rep, it, nodes, _ = res[rid]
+        assert rep.aborted == 0
+        assert list(it) == it_a
+        assert np.array_equal(nodes, nodes_a)

@@ -30,9 +30,12 @@ def _alone(m, p):
     return rep, iters, dm.nodes()
 
 
-def _drain(srv, ids, watch=None):
+def _drain(srv, ids, watch=None, limit_s=100.0):
+    import time
     results = {}
+    t0 = time.monotonic()
     while len(results) < len(ids):
+        assert time.monotonic() - t0 < limit_s, f"requests not finished: {sorted(set(ids) - set(results))}"
         srv.pump(2000)
         c = srv.counts()
         if watch:
@@ -86,4 +89,32 @@ def test_request_larger_than_cap_is_refused():
     srv = runtime.Server(max_inflight=2, mem_cap=runtime.footprint(m) // 2)
     with pytest.raises(runtime.GiraffeError):
         srv.submit(m, p)
+    srv.close()
+
+
+def test_submit_copies_every_array():
+    """gf_server_submit copies the whole description: Pipe_1 kinds, buoyancy and
+    deck element numbers included.  The requests wait in the queue behind a
+    blocker while the caller's arrays are overwritten; their results must still
+    equal the standalone solves."""
+    from test_gpu_pipe import _hanging
+    pipe, pp = _hanging()
+    pipe.beam_number = list(range(pipe.n_beams, 0, -1))
+    plate, ps, _ = M.shell_plate(nq=4, n_steps=3, p_max=2e5)
+    plate.shell_number = list(range(plate.n_shells, 0, -1))
+    alone = [_alone(pipe, pp), _alone(plate, ps)]
+    blocker, pb, _ = M.catenary(n_elem=300)
+    srv = runtime.Server(max_inflight=1)
+    ids = [srv.submit(blocker, pb), srv.submit(pipe, pp), srv.submit(plate, ps)]
+    f = pipe.freeze()
+    f["beam_kind"][:] = 7
+    f["beam_buoy"][:] = 1e6
+    f["beam_number"][:] = 0
+    plate.freeze()["shell_number"][:] = 0
+    res = _drain(srv, ids)
+    for (rep_a, it_a, nodes_a), rid in zip(alone, ids[1:]):
+        rep, it, nodes, _ = res[rid]
+        assert rep.aborted == rep_a.aborted == 0
+        assert list(it) == list(it_a)
+        assert np.array_equal(nodes, nodes_a)
     srv.close()
