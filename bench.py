@@ -147,8 +147,9 @@ def measure(m, p, precision, steps, warmup, probe_steps):
 PROBES_BTD = ("btd_level", "btd_tail", "beam_assemble", "shell_assemble")
 PROBES_BAND = ("band_panel", "band_update", "band_sweep", "beam_assemble", "shell_assemble")
 
-KERNEL_NAMES = {"btd_level": "btd_level_kernel", "btd_tail": "btd_tail_kernel", "band_panel": "band_panel_kernel",
-                "band_update": "band_update_kernel", "band_sweep": "band_sweep_kernel",
+KERNEL_NAMES = {"btd_level": "btd_level_kernel", "btd_tail": "btd_tail_kernel",
+                "band_panel": "band_batch_inv_kernel", "band_update": "band_step_kernel",
+                "band_sweep": "band_batch_apply_kernel + band_couple_kernel",
                 "beam_assemble": "beam_assemble_kernel<0>", "shell_assemble": "shell_assemble_kernel<0>"}
 
 
@@ -163,9 +164,9 @@ def kernel_work(cls, info):
     if cls == "band_update":
         return "mfma", lu["flops"] / n, "flop", "band column eliminated: 2 b^2 flops of the band LU"
     if cls == "band_panel":
-        return "hbm", 4.0 * b * s, "B", "band column eliminated: read + write one column of L21 and one row of U12"
+        return "mfma", 1.0, "flop", "flop of the batch inverses of the diagonal blocks (64^3 products)"
     if cls == "band_sweep":
-        return "hbm", FM.band_trsv(n, b, s)["bytes"] / n / 2, "B", "row swept: half of the triangular-solve bytes"
+        return "hbm", FM.band_trsv(n, b, s)["bytes"] / n / 2, "B", "row solved: half of the triangular-solve bytes"
     if cls == "beam_assemble":
         return "hbm", FM.BEAM_BYTES_FP64, "B", "element (FP64 tangent)"
     return "hbm", FM.SHELL_BYTES_FP64, "B", "element (FP64 tangent)"
@@ -350,7 +351,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="catenary", choices=["catenary", "plate", "rollup", "serve", "serve-mixed"])
+    ap.add_argument("--workload", default="plate", choices=["catenary", "plate", "rollup", "serve", "serve-mixed"])
     ap.add_argument("--requests", type=int, default=2000, help="serve: requests in the stream")
     ap.add_argument("--max-inflight", type=int, default=16, help="serve: resident requests")
     ap.add_argument("--load", type=float, default=0.8, help="serve: arrival rate / closed-loop capacity")

@@ -1,38 +1,53 @@
-// Blocked band LU on gfx950 for wide bands (shell meshes): the tangent solve
-// that replaces sparsesystem -> MKL PARDISO (src/SparseMatrix.cpp:74-166)
-// when the half-bandwidth does not fit the block-tridiagonal solver.
+// Wide-band tangent solve on gfx950 (shell meshes): the solve that replaces
+// sparsesystem -> MKL PARDISO (src/SparseMatrix.cpp:74-166) when the
+// half-bandwidth does not fit the block-tridiagonal solver.
 //
-// Storage (GM_BAND): entry (i, c), |i - c| <= hb, at ab[i*2hb + c + hb], so any
-// sub-matrix inside the band is a strided matrix with leading dimension 2hb.
-// Entries outside the band alias neighbouring rows and are never touched:
-// every load masks them to zero and every store skips them.
+// Storage (GM_BAND): entry (r, c) with -kl <= c - r <= ku at ab[r*ld + c + kl],
+// ld = kl + ku, so any sub-matrix inside the band is a strided matrix with
+// leading dimension ld.  The assembled tangent has kl = ku = hb; the stored
+// band has ku = hb + 63, the fill of the block factorisation's upper strips.
+// Entries outside the band alias neighbouring rows: every load masks them to
+// zero and every store skips them.
 //
-// Right-looking, no row interchanges (PARDISO's mtype 11 default is static
-// pivoting), panel width NB = 64 over the padded order n (multiple of NB).
-// Per panel p (columns k0 = 64 p ..):
-//   panel  L21 = A21 U11^-1 and U12 = L11^-1 A12 as MFMA products with the
-//          panel's stored inverses (no sequential substitution);
-//   update A22 -= L21 U12 on the band window, MFMA 16x16x4 tiles of 64 x 64;
-//          the workgroup owning the next diagonal block factors it in LDS
-//          (register-blocked, one barrier per pivot) and writes its triangular
-//          inverses, which the next panel and both triangular solves use.
-// Zero or non-finite pivots raise the failure flag (info_fail).
+// Factorisation: block LU over 64 x 64 blocks, no interchanges between blocks
+// (PARDISO's mtype 11 default is static pivoting with perturbation),
+//   A = L U,  L block lower with L(p,p) = D_p,  U block unit upper,
+//   E_p = D_p^-1 (Gauss-Jordan in one workgroup),  U(p,j) = E_p A'(p,j),
+//   A(i,j) -= A(i,p) U(p,j)                      (MFMA 64 x 64 tiles),
+// so L(i,p) is the updated A(i,p) itself (no triangular solve on the panel).
+// The band stays closed under the update: U(p, .) reaches at most 63 columns
+// past the band of its first row (hence ku), and L U adds nothing outside
+// -hb..hb.  One launch per block column p with look-ahead:
+//   T workgroups  A(i,j) -= A(i,p) U(p,j)  for i >= p+2, j >= p+1;
+//   S workgroups  row p+1: D = A(p+1,p+1) - A(p+1,p) U(p,p+1), E = D^-1 and
+//                 U(p+1,j) = E (A(p+1,j) - A(p+1,p) U(p,j)), every S workgroup
+//                 inverting D itself (no workgroup waits on another);
+// the two groups touch disjoint blocks, so stream order is the only
+// synchronisation.  A pivot below 1e-13 of its row's original diagonal entry
+// is perturbed to that size (flag bit 2); a non-finite one fails (bit 1).
 //
-// Triangular solves: one single-workgroup launch per sweep over 64-row blocks,
-// each block finished by its stored inverse (no other workgroup takes part).
+// Triangular solves: the diagonal blocks are grouped in batches of SB = 16;
+// after the factorisation each batch's diagonal part of L is inverted
+// (G = L_bb^-1, lower, E_p on its diagonal) and of U (H = U_bb^-1, unit
+// upper), one launch per block distance.  A solve then takes, per batch, one
+// launch that applies the batch inverse (all rows in parallel) and one that
+// subtracts the batch's couplings from the rows within reach.
 #include "gf_internal.h"
 
 namespace {
 
-constexpr int NB = 64;
-constexpr int KT = 32;  // k-chunk of the MFMA tiles
+constexpr int NB = 64;   // block size
+constexpr int SB = 16;   // blocks per solve batch
+constexpr int BW = NB * SB;
+constexpr int KT = 32;   // k-chunk of the staged MFMA products
+constexpr int LDT = NB + 1;
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-// Operands of both: A(i, k) in lane i + 16 k, B(k, j) in lane j + 16 k.  The
-// 16 x 16 result differs: item v of lane l holds row 4 v + l / 16 for f64 and
-// row 4 (l / 16) + v for f32, column l % 16 in both (checked on gfx950).
+// 16x16x4 MFMA.  Operands: A(i, k) in lane i + 16 k, B(k, j) in lane j + 16 k.
+// Result item v of lane l: row 4 v + l / 16 (f64) or 4 (l / 16) + v (f32),
+// column l % 16 (checked on gfx950).
 template <typename T> struct mfma;
 template <> struct mfma<double> {
     typedef v4d acc_t;
@@ -51,271 +66,66 @@ template <> struct mfma<float> {
     static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
-__device__ __forceinline__ size_t bidx(int i, int c, int hb) { return (size_t)i * (size_t)(2 * hb) + (size_t)(c + hb); }
-__device__ __forceinline__ bool inband(int i, int c, int hb) { return i - c <= hb && c - i <= hb; }
-
-// Masked band load without a branch around it: the address is clamped into the
-// stored band of row min(i, n - 1) (always inside the n (2hb+1) allocation) and
-// the value is selected afterwards, so a thread's loads are all in flight
-// together instead of each waiting in its own branch.  i, c >= 0.
-template <typename T> __device__ __forceinline__ T band_ld(const T *ab, int i, int c, int hb, int n, bool ok)
-{
-    const int ii = min(i, n - 1);
-    const int cc = min(max(c, ii - hb), ii + hb);
-    const T v = ab[bidx(ii, cc, hb)];
-    return ok && i < n && inband(i, c, hb) ? v : T(0);
-}
-
-template <typename T> __device__ __forceinline__ bool bad_pivot(T v)
-{
-    const double a = fabs((double)v);
-    return !(v == v) || a >= 1e300 || a < (sizeof(T) == 4 ? 1e-37 : 1e-300);
-}
-
-// Per-panel inverses: inv + p * 2 * NB * NB holds Uinv (row-major), then Linv (unit lower).
-template <typename T> __device__ __forceinline__ T *uinv_of(T *inv, int p) { return inv + (size_t)p * 2 * NB * NB; }
-template <typename T> __device__ __forceinline__ T *linv_of(T *inv, int p) { return inv + (size_t)p * 2 * NB * NB + NB * NB; }
-
-// ---- factorisation of one 64 x 64 diagonal block --------------------------------
-// Blocked over 16 x 16 blocks D(a, b) of the LDS tile s (leading dimension ld),
-// by a workgroup of 4 wavefronts:
-//   for kb = 0..3:  wave 0 factors D(kb, kb) in registers (wave_lu16: L\U in
-//                   place, its triangular inverses into li[kb] / ui[kb]);
-//                   L(i, kb) = D(i, kb) Uinv(kb), U(kb, j) = Linv(kb) D(kb, j);
-//                   D(i, j) -= L(i, kb) U(kb, j)            (MFMA, all waves)
-//   then wave j assembles block column j of the 64 x 64 Linv and Uinv by block
-//   substitution (MFMA) and writes both to global memory.
-// Barriers: 3 per kb; the sequential pivot steps run inside one wavefront.
-constexpr int LB = 16, LLD = LB + 1;  // 16 x 16 blocks in LDS with leading dimension 17
-
-template <typename T> __device__ __forceinline__ T rdl(T v, int l);
-template <> __device__ __forceinline__ double rdl<double>(double v, int l)
-{
-    int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
-    int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
-    return __hiloint2double(hi, lo);
-}
-template <> __device__ __forceinline__ float rdl<float>(float v, int l)
-{
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
-
-// One wavefront: LU (no pivoting) of the 16 x 16 block at s (leading dimension
-// ld), with Linv (unit lower) and Uinv written row-major to li / ui (leading
-// dimension LLD).  Lane c of the first 16 holds column c of the block, lanes
-// 16..31 the identity that the forward elimination turns into Linv, lanes
-// 32..47 the identity that the backward substitution turns into Uinv.  On
-// return s holds U on and above the diagonal; below it the values are not the
-// multipliers (nothing reads them).
-template <typename T> __device__ __forceinline__ void wave_lu16(T *s, int ld, T *li, T *ui, int *flag)
-{
-    // Branch-free: the masks are applied arithmetically and a bad pivot is only
-    // recorded, so the unrolled sweep is one basic block and the scheduler can
-    // start the next pivot's reciprocal while the current updates issue.
-    const int lane = threadIdx.x & 63, part = lane >> 4, c = lane & 15;
-    T col[LB];
-#pragma unroll
-    for (int i = 0; i < LB; i++) {
-        const T d = s[i * ld + c];
-        col[i] = part == 0 ? d : (part == 1 && i == c) ? T(1) : T(0);
-    }
-    T rp[LB];
-    bool any_bad = false;
-#pragma unroll
-    for (int k = 0; k < LB; k++) {
-        T p = rdl<T>(col[k], k);
-        const bool bad = bad_pivot(p);
-        any_bad |= bad;
-        p = bad ? T(1) : p;
-        rp[k] = pivot_rcp(p);
-        T a[LB];
-#pragma unroll
-        for (int i = k + 1; i < LB; i++) a[i] = rdl<T>(col[i], k);
-        // row i -= (a_ik / a_kk) row k, applied to the columns right of k and to
-        // every column of the identity part (elsewhere the scale is zero); the
-        // multipliers themselves are not kept: only the inverses of the
-        // diagonal blocks are used downstream.
-        const bool act = (part == 0 && c > k) || part == 1;
-        const T ck = act ? col[k] * rp[k] : T(0);
-#pragma unroll
-        for (int i = k + 1; i < LB; i++) col[i] -= a[i] * ck;
-    }
-    const bool inv_u = part == 2;
-#pragma unroll
-    for (int i = 0; i < LB; i++) col[i] = inv_u ? (i == c ? T(1) : T(0)) : col[i];
-#pragma unroll
-    for (int k = LB - 1; k >= 0; k--) {
-        T u[LB];
-#pragma unroll
-        for (int i = 0; i < k; i++) u[i] = rdl<T>(col[i], k);
-        const T xk = col[k] * rp[k];
-        col[k] = inv_u ? xk : col[k];
-        const T xm = inv_u ? xk : T(0);
-#pragma unroll
-        for (int i = 0; i < k; i++) col[i] -= u[i] * xm;
-    }
-    if (any_bad && lane == 0) atomicExch(flag, 1);
-    if (part < 3) {
-        T *dst = part == 0 ? s : part == 1 ? li : ui;
-        const int ldd = part == 0 ? ld : LLD;
-#pragma unroll
-        for (int i = 0; i < LB; i++) dst[i * ldd + c] = col[i];
-    }
-}
-
-// acc += P Q for 16 x 16 blocks in LDS (one wavefront, 4 MFMA 16x16x4).
-template <typename T>
-__device__ __forceinline__ void mm16(typename mfma<T>::acc_t &acc, const T *P, int ldp, const T *Q, int ldq)
-{
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int k0 = 0; k0 < LB; k0 += 4) {
-        const T a = P[(lane & 15) * ldp + k0 + (lane >> 4)];
-        const T b = Q[(k0 + (lane >> 4)) * ldq + (lane & 15)];
-        acc = mfma<T>::run(a, b, acc);
-    }
-}
-
-// Stores sign * acc (result layout) as a row-major 16 x 16 block.
-template <typename T>
-__device__ __forceinline__ void st16(const typename mfma<T>::acc_t &acc, T *dst, int ld, T sign)
-{
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int v = 0; v < 4; v++) dst[mfma<T>::row(lane, v) * ld + (lane & 15)] = sign * acc[v];
-}
-
-template <typename T> __device__ __forceinline__ void wave_sync_lds()
-{
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-}
-
-template <typename T> struct LuSmem {
-    T li[4][LB * LLD], ui[4][LB * LLD];  // inverses of the 16 x 16 diagonal blocks
-    T ws[4][LB * LLD];                   // per-wave staging block
+struct Geo {
+    int n;       // padded order (multiple of NB)
+    int kl, ku;  // stored band
+    int ld;      // kl + ku
+    int nbk;     // n / NB
+    int R;       // reach in blocks of L below and U right of the diagonal block
 };
 
-// s: the 64 x 64 tile (ld) in LDS; uinv / linv: 64 x 64 row-major outputs.
-// All 256 threads call it.
-template <typename T>
-__device__ __forceinline__ void tile_lu_inv(T *s, int ld, LuSmem<T> &w, T *uinv, T *linv, int *flag)
+__device__ __forceinline__ bool inband(int r, int c, const Geo &g) { return r - c <= g.kl && c - r <= g.ku; }
+
+// Masked band load without a branch around it: the address is clamped into
+// the stored band of row min(max(r, 0), n - 1) (inside the n (ld + 1)
+// allocation) and the value selected afterwards, so a thread's loads are all
+// in flight together.
+template <typename T> __device__ __forceinline__ T band_ld(const T *ab, int r, int c, const Geo &g)
 {
-    typedef typename mfma<T>::acc_t acc_t;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    auto D = [&](int a, int b) { return s + (LB * a) * ld + LB * b; };
-    for (int kb = 0; kb < 4; kb++) {
-        if (wave == 0) wave_lu16<T>(D(kb, kb), ld, w.li[kb], w.ui[kb], flag);
-        __syncthreads();
-        // L(i, kb) = D(i, kb) Uinv(kb) for i > kb; U(kb, j) = Linv(kb) D(kb, j) for j > kb
-        const int nr = 3 - kb;
-        for (int t = wave; t < 2 * nr; t += 4) {
-            acc_t acc = acc_t{};
-            T *dst;
-            if (t < nr) {
-                dst = D(kb + 1 + t, kb);
-                mm16<T>(acc, dst, ld, w.ui[kb], LLD);
-            } else {
-                dst = D(kb, kb + 1 + (t - nr));
-                mm16<T>(acc, w.li[kb], LLD, dst, ld);
-            }
-            wave_sync_lds<T>();
-            st16<T>(acc, dst, ld, T(1));
-        }
-        __syncthreads();
-        // D(i, j) -= L(i, kb) U(kb, j)
-        for (int t = wave; t < nr * nr; t += 4) {
-            const int i = kb + 1 + t / nr, j = kb + 1 + t % nr;
-            acc_t acc = acc_t{};
-            mm16<T>(acc, D(i, kb), ld, D(kb, j), ld);
-            T *dst = D(i, j);
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                T &e = dst[mfma<T>::row(lane, v) * ld + (lane & 15)];
-                e -= acc[v];
-            }
-        }
-        __syncthreads();
-    }
-    // Block column j = wave of Linv (forward) and Uinv (backward) substitution.
-    const int j = wave;
-    T *ws = w.ws[j];
-    auto put = [&](T *out, int bi, const acc_t &acc, T sign) {
-#pragma unroll
-        for (int v = 0; v < 4; v++)
-            out[(LB * bi + mfma<T>::row(lane, v)) * NB + LB * j + (lane & 15)] = sign * acc[v];
-    };
-    auto put_lds = [&](T *out, int bi, const T *src) {  // copy a 16 x 16 LDS block (LLD) to the output
-        for (int t = lane; t < LB * LB; t += 64)
-            out[(LB * bi + t / LB) * NB + LB * j + t % LB] = src[(t / LB) * LLD + t % LB];
-    };
-    auto zero = [&](T *out, int bi) {
-        for (int t = lane; t < LB * LB; t += 64) out[(LB * bi + t / LB) * NB + LB * j + t % LB] = T(0);
-    };
-    // Linv(:, j): X(j) = Linv(j); X(i) = -Linv(i) sum_{k=j}^{i-1} L(i, k) X(k), i > j
-    // (loops unrolled over the block indices; j is wave-uniform)
-    {
-        for (int i = 0; i < j; i++) zero(linv, i);
-        put_lds(linv, j, w.li[j]);
-        acc_t acc[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) acc[i] = acc_t{};
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            if (k < j) continue;
-            const T *xk = k == j ? w.li[j] : ws;  // X(k) as a product operand
-#pragma unroll
-            for (int i = k + 1; i < 4; i++) mm16<T>(acc[i], D(i, k), ld, xk, LLD);
-            // X(k+1) = -Linv(k+1) acc[k+1]
-            wave_sync_lds<T>();
-            st16<T>(acc[k + 1], ws, LLD, T(1));
-            wave_sync_lds<T>();
-            acc_t x = acc_t{};
-            mm16<T>(x, w.li[k + 1], LLD, ws, LLD);
-            put(linv, k + 1, x, T(-1));
-            wave_sync_lds<T>();
-            st16<T>(x, ws, LLD, T(-1));
-            wave_sync_lds<T>();
-        }
-    }
-    wave_sync_lds<T>();
-    // Uinv(:, j): X(j) = Uinv(j); X(i) = -Uinv(i) sum_{k=i+1}^{j} U(i, k) X(k), i < j
-    {
-        for (int i = j + 1; i < 4; i++) zero(uinv, i);
-        put_lds(uinv, j, w.ui[j]);
-        acc_t acc[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) acc[i] = acc_t{};
-#pragma unroll
-        for (int k = 3; k > 0; k--) {
-            if (k > j) continue;
-            const T *xk = k == j ? w.ui[j] : ws;
-#pragma unroll
-            for (int i = 0; i < k; i++) mm16<T>(acc[i], D(i, k), ld, xk, LLD);
-            wave_sync_lds<T>();
-            st16<T>(acc[k - 1], ws, LLD, T(1));
-            wave_sync_lds<T>();
-            acc_t x = acc_t{};
-            mm16<T>(x, w.ui[k - 1], LLD, ws, LLD);
-            put(uinv, k - 1, x, T(-1));
-            wave_sync_lds<T>();
-            st16<T>(x, ws, LLD, T(-1));
-            wave_sync_lds<T>();
-        }
-    }
-    __syncthreads();
+    const int rr = min(max(r, 0), g.n - 1);
+    const int cc = min(max(c, rr - g.kl), rr + g.ku);
+    const T v = ab[(size_t)rr * (size_t)g.ld + (size_t)(cc + g.kl)];
+    return r >= 0 && r < g.n && c >= 0 && c < g.n && inband(r, c, g) ? v : T(0);
+}
+template <typename T> __device__ __forceinline__ void band_st(T *ab, int r, int c, const Geo &g, T v)
+{
+    if (r >= 0 && r < g.n && c >= 0 && c < g.n && inband(r, c, g)) ab[(size_t)r * (size_t)g.ld + (size_t)(c + g.kl)] = v;
 }
 
-// 64 x 64 x 64 product on 256 threads (4 waves, 2 x 2 of 32 x 32):
-// acc = P Q with P and Q staged chunk by chunk through sa [64][KT+1], sb [KT][64].
+// Per-wave part of a 64 x 64 result: rows wr + 16a + row(lane, v), columns wc + 16b + lane % 16.
+template <typename T> struct Acc {
+    typename mfma<T>::acc_t a[2][2];
+    __device__ __forceinline__ void zero()
+    {
+#pragma unroll
+        for (int x = 0; x < 2; x++)
+#pragma unroll
+            for (int y = 0; y < 2; y++) a[x][y] = typename mfma<T>::acc_t{};
+    }
+};
+__device__ __forceinline__ int acc_row(int wave) { return (wave >> 1) * 32; }
+__device__ __forceinline__ int acc_col(int wave) { return (wave & 1) * 32; }
+template <typename T, typename F> __device__ __forceinline__ void acc_each(F f)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++)
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                f(x, y, v, acc_row(wave) + 16 * x + mfma<T>::row(lane, v), acc_col(wave) + 16 * y + (lane & 15));
+}
+
+// acc += P Q (64 x 64 x 64, 256 threads).  P and Q are read through loaders in
+// chunks of KT columns / rows, all loads of a chunk issued before its LDS
+// stores, then the four waves run their 2 x 2 MFMA tiles from LDS.
 template <typename T, typename LP, typename LQ>
-__device__ __forceinline__ void tile_mm(typename mfma<T>::acc_t (&acc)[2][2], T *sa, T *sb, LP loadP, LQ loadQ)
+__device__ __forceinline__ void tile_mm(Acc<T> &acc, T *sa, T *sb, LP loadP, LQ loadQ)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    const int wr = acc_row(wave), wc = acc_col(wave);
     for (int kc = 0; kc < NB; kc += KT) {
-        // all loads of the chunk first (in flight together), then the LDS stores
         constexpr int PER = NB * KT / 256;
         T pv[PER], qv[PER];
 #pragma unroll
@@ -335,287 +145,400 @@ __device__ __forceinline__ void tile_mm(typename mfma<T>::acc_t (&acc)[2][2], T 
         for (int kk = 0; kk < KT; kk += 4) {
             T av[2], bv[2];
 #pragma unroll
-            for (int a = 0; a < 2; a++) av[a] = sa[(wr + 16 * a + (lane & 15)) * (KT + 1) + kk + (lane >> 4)];
+            for (int x = 0; x < 2; x++) av[x] = sa[(wr + 16 * x + (lane & 15)) * (KT + 1) + kk + (lane >> 4)];
 #pragma unroll
-            for (int b = 0; b < 2; b++) bv[b] = sb[(kk + (lane >> 4)) * NB + wc + 16 * b + (lane & 15)];
+            for (int y = 0; y < 2; y++) bv[y] = sb[(kk + (lane >> 4)) * NB + wc + 16 * y + (lane & 15)];
 #pragma unroll
-            for (int a = 0; a < 2; a++)
+            for (int x = 0; x < 2; x++)
 #pragma unroll
-                for (int b = 0; b < 2; b++) acc[a][b] = mfma<T>::run(av[a], bv[b], acc[a][b]);
+                for (int y = 0; y < 2; y++) acc.a[x][y] = mfma<T>::run(av[x], bv[y], acc.a[x][y]);
         }
         __syncthreads();
     }
 }
 
-template <typename T> struct TileSmem {
-    T ab[NB * (KT + 1) + KT * NB];  // staging of tile_mm; after it, the LU tile [64][65]
-};
-template <typename T> struct DiagSmem {
-    T ab[NB * (KT + 1) + KT * NB];
-    LuSmem<T> lu;
-};
-
-// First diagonal block: LU in place and its inverses (panel 0).
-template <typename T>
-__global__ __launch_bounds__(256) void band_diag_kernel(T *ab, int hb, int n, T *inv, int *flag)
+// acc += P Q with both operands in LDS ([64][LDT]); no barrier inside.
+template <typename T> __device__ __forceinline__ void lds_mm(Acc<T> &acc, const T *P, const T *Q)
 {
-    __shared__ DiagSmem<T> sm;
-    T *s = sm.ab;
-    T v[NB * NB / 256];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = acc_row(wave), wc = acc_col(wave);
+#pragma unroll 4
+    for (int k = 0; k < NB; k += 4) {
+        T av[2], bv[2];
 #pragma unroll
-    for (int r = 0; r < NB * NB / 256; r++) {
-        const int t = threadIdx.x + 256 * r;
-        v[r] = band_ld(ab, t / NB, t % NB, hb, n, true);
-    }
+        for (int x = 0; x < 2; x++) av[x] = P[(wr + 16 * x + (lane & 15)) * LDT + k + (lane >> 4)];
 #pragma unroll
-    for (int r = 0; r < NB * NB / 256; r++) {
-        const int t = threadIdx.x + 256 * r;
-        s[(t / NB) * (NB + 1) + t % NB] = v[r];
-    }
-    __syncthreads();
-    tile_lu_inv<T>(s, NB + 1, sm.lu, uinv_of(inv, 0), linv_of(inv, 0), flag);
-    __syncthreads();
-    for (int t = threadIdx.x; t < NB * NB; t += 256) {
-        int i = t / NB, c = t % NB;
-        if (inband(i, c, hb)) ab[bidx(i, c, hb)] = s[i * (NB + 1) + c];
+        for (int y = 0; y < 2; y++) bv[y] = Q[(k + (lane >> 4)) * LDT + wc + 16 * y + (lane & 15)];
+#pragma unroll
+        for (int x = 0; x < 2; x++)
+#pragma unroll
+            for (int y = 0; y < 2; y++) acc.a[x][y] = mfma<T>::run(av[x], bv[y], acc.a[x][y]);
     }
 }
 
-// Panel p: blockIdx < nt: L21 tile = A21 Uinv; otherwise U12 tile = Linv A12.
-template <typename T> __global__ __launch_bounds__(256) void band_panel_kernel(T *ab, int hb, int p, int n, int nt,
-                                                                               const T *inv)
-{
-    typedef typename mfma<T>::acc_t acc_t;
-    __shared__ TileSmem<T> sm;
-    T *sa = sm.ab, *sb = sm.ab + NB * (KT + 1);
-    const int k0 = p * NB, lim = min(n, k0 + NB + hb);
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-    acc_t acc[2][2];
-#pragma unroll
-    for (int a = 0; a < 2; a++)
-#pragma unroll
-        for (int b = 0; b < 2; b++) acc[a][b] = acc_t{};
-    const T *U = uinv_of(const_cast<T *>(inv), p), *L = linv_of(const_cast<T *>(inv), p);
-    const bool lower = (int)blockIdx.x < nt;
-    const int o0 = k0 + NB + (lower ? blockIdx.x : blockIdx.x - nt) * NB;  // first row (lower) or column (upper)
-    if (lower) {
-        tile_mm<T>(acc, sa, sb,
-                   [&](int i, int k) { int gi = o0 + i; return band_ld(ab, gi, k0 + k, hb, n, gi < lim); },
-                   [&](int k, int j) { return U[k * NB + j]; });
-    } else {
-        tile_mm<T>(acc, sa, sb, [&](int i, int k) { return L[i * NB + k]; },
-                   [&](int k, int j) { int gj = o0 + j; return band_ld(ab, k0 + k, gj, hb, n, gj < lim); });
-    }
-#pragma unroll
-    for (int a = 0; a < 2; a++)
-#pragma unroll
-        for (int b = 0; b < 2; b++)
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                int i = wr + 16 * a + mfma<T>::row(lane, v), j = wc + 16 * b + (lane & 15);
-                int gi = lower ? o0 + i : k0 + i, gj = lower ? k0 + j : o0 + j;
-                bool ok = lower ? gi < lim : gj < lim;
-                if (ok && inband(gi, gj, hb)) ab[bidx(gi, gj, hb)] = acc[a][b][v];
-            }
-}
+template <typename T> __device__ __forceinline__ T rcp_of(T p);
+template <> __device__ __forceinline__ double rcp_of<double>(double p) { return pivot_rcp(p); }
+template <> __device__ __forceinline__ float rcp_of<float>(float p) { return pivot_rcp(p); }
 
-// A22 -= L21 U12 on 64 x 64 tiles of the window [k0+NB, lim)^2; the tile that
-// holds the next diagonal block then factors it and writes panel p+1's inverses.
-template <typename T> __global__ __launch_bounds__(256) void band_update_kernel(T *ab, int hb, int p, int n, int nt,
-                                                                                T *inv, int *flag)
+// In-place Gauss-Jordan inverse of the 64 x 64 matrix s ([64][LDT]) by 256
+// threads: thread t holds row t / 4, columns 16 (t % 4) .. +15 in registers;
+// per pivot the pivot row and column go through LDS (double-buffered, one
+// barrier per pivot).  No interchanges: a pivot smaller than 1e-13 (FP32: 1e-6)
+// of its row's original diagonal entry (of the block's largest entry where
+// that is zero) -- a relative threshold that does not depend on how the rows
+// and columns are scaled -- is replaced by that size with its sign (bit 2 of
+// *flag); a non-finite pivot, an all-zero block, or in FP32 a perturbed pivot
+// sets bit 1.
+template <typename T> __device__ __forceinline__ void gj_inverse64(T *s, T *xbuf /*[2][128] + 72*/, int *flag)
 {
-    typedef typename mfma<T>::acc_t acc_t;
-    __shared__ DiagSmem<T> sm;
-    T *sa = sm.ab, *sb = sm.ab + NB * (KT + 1), *sc = sm.ab;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int k0 = p * NB, base = k0 + NB, lim = min(n, k0 + NB + hb);
-    const int ti = blockIdx.x / nt, tj = blockIdx.x % nt;
-    const int R0 = base + ti * NB, C0 = base + tj * NB;
-    if (R0 - (C0 + NB - 1) > hb || C0 - (R0 + NB - 1) > hb) return;
-    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-    acc_t acc[2][2];
-#pragma unroll
-    for (int a = 0; a < 2; a++)
-#pragma unroll
-        for (int b = 0; b < 2; b++) acc[a][b] = acc_t{};
-    // the old values of this thread's result entries, read before the product
-    // so their latency hides behind it
-    T old[2][2][4];
-#pragma unroll
-    for (int a = 0; a < 2; a++)
-#pragma unroll
-        for (int b = 0; b < 2; b++)
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                int i = wr + 16 * a + mfma<T>::row(lane, v), j = wc + 16 * b + (lane & 15);
-                old[a][b][v] = band_ld(ab, R0 + i, C0 + j, hb, n, C0 + j < n);
-            }
-    tile_mm<T>(acc, sa, sb,
-               [&](int i, int k) { int gi = R0 + i; return band_ld(ab, gi, k0 + k, hb, n, gi < lim); },
-               [&](int k, int j) { int gj = C0 + j; return band_ld(ab, k0 + k, gj, hb, n, gj < lim); });
-    const bool fuse = (ti == 0 && tj == 0 && base + NB <= n);
-#pragma unroll
-    for (int a = 0; a < 2; a++)
-#pragma unroll
-        for (int b = 0; b < 2; b++)
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                int i = wr + 16 * a + mfma<T>::row(lane, v), j = wc + 16 * b + (lane & 15);
-                int gi = R0 + i, gj = C0 + j;
-                // rows/columns past the window carry a zero update (masked operands)
-                const bool in = gi < n && gj < n && inband(gi, gj, hb);
-                const T nv = old[a][b][v] - acc[a][b][v];
-                if (fuse) sc[i * (NB + 1) + j] = in ? nv : T(0);
-                else if (in) ab[bidx(gi, gj, hb)] = nv;
-            }
-    if (!fuse) return;
-    __syncthreads();
-    tile_lu_inv<T>(sc, NB + 1, sm.lu, uinv_of(inv, p + 1), linv_of(inv, p + 1), flag);
-    __syncthreads();
-    for (int t = tid; t < NB * NB; t += 256) {
-        int i = t / NB, j = t % NB;
-        int gi = R0 + i, gj = C0 + j;
-        if (inband(gi, gj, hb)) ab[bidx(gi, gj, hb)] = sc[i * (NB + 1) + j];
-    }
-}
-
-// ---- triangular solves ------------------------------------------------------------
-// Right-looking block sweeps, one launch per batch of K consecutive 64-row
-// blocks (forward: L y = b with unit L, blocks ascending; backward: U x = y,
-// blocks descending).  The right-hand side f is updated in place: when a batch
-// starts, every contribution of the blocks before it has been subtracted.
-// Every workgroup of the launch solves the batch itself from f and the stored
-// inverses (K inverse and K(K-1)/2 coupling products, the same data for all
-// workgroups, so it comes from L2); workgroup 0 writes the batch's solution and
-// workgroup g >= 1 subtracts the batch's couplings from block g past it.  No
-// workgroup reads what another workgroup of the same launch writes, so the
-// stream order of the launches is the only synchronisation, and the coupling
-// strips are streamed by as many CUs as the band reaches blocks.
-template <typename T, typename V>
-__device__ __forceinline__ V mv_part(const T *row, int c0, const V *v)  // 16 products of one row quarter
-{
+    const int tid = threadIdx.x, r = tid >> 2, q = tid & 3, c0 = 16 * q;
     T a[16];
 #pragma unroll
-    for (int j = 0; j < 16; j++) a[j] = row[c0 + j];
-    V s = V(0);
+    for (int j = 0; j < 16; j++) a[j] = s[r * LDT + c0 + j];
+    // largest magnitude in the block, and the original diagonal
+    T mx = T(0);
 #pragma unroll
-    for (int j = 0; j < 16; j++) s += (V)a[j] * v[c0 + j];
-    return s;
+    for (int j = 0; j < 16; j++) mx = fmax(mx, fabs(a[j]));
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) mx = fmax(mx, __shfl_xor(mx, o));
+    T *red = xbuf + 256, *diag0 = xbuf + 264;
+    if ((tid & 63) == 0) red[tid >> 6] = mx;
+    if (q == (r >> 4)) diag0[r] = fabs(s[r * LDT + r]);
+    __syncthreads();
+    mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    const T rel = sizeof(T) == 8 ? T(1e-13) : T(1e-6);
+    const T huge = sizeof(T) == 8 ? T(1e300) : T(1e37);
+    bool bad = !(mx > T(0)) || !(mx < huge), pert = false;
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+        T *rowb = xbuf + (k & 1) * 128, *colb = rowb + 64;
+        if (r == k) {
+#pragma unroll
+            for (int j = 0; j < 16; j++) rowb[c0 + j] = a[j];
+        }
+        if (q == (k >> 4)) colb[r] = a[k & 15];
+        __syncthreads();
+        T p = rowb[k];
+        const T ap = fabs(p), dk = diag0[k];
+        const T eps = rel * (dk > T(0) ? dk : mx);
+        const bool nonfinite = !(ap < huge);
+        bad |= nonfinite;
+        const bool small = !nonfinite && ap < eps;
+        pert |= small;
+        p = nonfinite ? T(1) : small ? (p >= T(0) ? eps : -eps) : p;
+        const T ip = rcp_of<T>(p);
+        const T f = colb[r];
+        T rk[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) rk[j] = rowb[c0 + j] * ip;
+        const bool isk = r == k, colk = q == (k >> 4);
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const T upd = isk ? rk[j] : a[j] - f * rk[j];
+            a[j] = (colk && j == (k & 15)) ? (isk ? ip : -f * ip) : upd;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 16; j++) s[r * LDT + c0 + j] = a[j];
+    if (tid == 0) {
+        const int bits = (bad || (sizeof(T) == 4 && pert) ? 1 : 0) | (pert ? 2 : 0);
+        if (bits) atomicOr(flag, bits);
+    }
+    __syncthreads();
 }
 
-template <typename V> __device__ __forceinline__ V quad_sum(V s)
+// LDS of the step and batch-inverse kernels: the staging of tile_mm, reused
+// as X (a [64][LDT] operand) once the products that stage through it are done.
+template <typename T> struct StepSmem {
+    union {
+        T stage[NB * (KT + 1) + KT * NB];
+        T X[NB * LDT];
+    };
+    T D[NB * LDT];
+    T xbuf[2 * 128 + 8 + NB];
+};
+static_assert(NB * (KT + 1) + KT * NB == NB * LDT, "staging and X share the LDS");
+
+// Batch store: per batch b, a BW x BW row-major matrix; block (r, c) holds G(r, c)
+// for c <= r (G = L_bb^-1, E on the diagonal) and H(r, c) for c > r (H = U_bb^-1).
+template <typename T> __device__ __forceinline__ T *batch_blk(T *gh, int b, int r, int c)
+{
+    return gh + (size_t)b * BW * BW + (size_t)(r * NB) * BW + c * NB;
+}
+
+// One block column of the factorisation (see the file header).  p = -1: row 0
+// only (no update).  Workgroups [0, nS): S group for row block p+1, workgroup
+// s handles U(p+1, p+1+s) (s = 0: only stores E_{p+1}); the rest: T tiles
+// (i, j), i = p+2 + t / nc, j = p+1 + t % nc.
+template <typename T>
+__global__ __launch_bounds__(256) void band_step_kernel(T *ab, Geo g, int p, int nS, int nc, T *gh, int *flag)
+{
+    __shared__ StepSmem<T> sm;
+    T *sa = sm.stage, *sb = sm.stage + NB * (KT + 1);
+    const int tid = threadIdx.x;
+    const int bid = blockIdx.x;
+    if (bid >= nS) {
+        // ---- T: A(i, j) -= A(i, p) U(p, j)
+        const int t = bid - nS, bi = p + 2 + t / nc, bj = p + 1 + t % nc;
+        const int R0 = bi * NB, C0 = bj * NB, K0 = p * NB;
+        T old[2][2][4];
+        acc_each<T>([&](int x, int y, int v, int i, int j) { old[x][y][v] = band_ld(ab, R0 + i, C0 + j, g); });
+        Acc<T> acc;
+        acc.zero();
+        tile_mm<T>(acc, sa, sb, [&](int i, int k) { return band_ld(ab, R0 + i, K0 + k, g); },
+                   [&](int k, int j) { return band_ld(ab, K0 + k, C0 + j, g); });
+        acc_each<T>([&](int x, int y, int v, int i, int j) {
+            band_st(ab, R0 + i, C0 + j, g, old[x][y][v] - acc.a[x][y][v]);
+        });
+        return;
+    }
+    // ---- S: row block p+1
+    const int s = bid, bi = p + 1, bj = p + 1 + s;
+    const int R0 = bi * NB, C0 = bj * NB, K0 = p * NB;
+    const bool upd = p >= 0;
+    // D = A(p+1, p+1) - A(p+1, p) U(p, p+1)
+    {
+        T old[2][2][4];
+        acc_each<T>([&](int x, int y, int v, int i, int j) { old[x][y][v] = band_ld(ab, R0 + i, R0 + j, g); });
+        Acc<T> acc;
+        acc.zero();
+        if (upd)
+            tile_mm<T>(acc, sa, sb, [&](int i, int k) { return band_ld(ab, R0 + i, K0 + k, g); },
+                       [&](int k, int j) { return band_ld(ab, K0 + k, R0 + j, g); });
+        acc_each<T>([&](int x, int y, int v, int i, int j) { sm.D[i * LDT + j] = old[x][y][v] - acc.a[x][y][v]; });
+    }
+    // A' = A(p+1, bj) - A(p+1, p) U(p, bj)   (U(p, bj) is zero past the reach)
+    if (s > 0) {
+        T old[2][2][4];
+        acc_each<T>([&](int x, int y, int v, int i, int j) { old[x][y][v] = band_ld(ab, R0 + i, C0 + j, g); });
+        Acc<T> acc;
+        acc.zero();
+        if (upd && bj <= p + g.R)
+            tile_mm<T>(acc, sa, sb, [&](int i, int k) { return band_ld(ab, R0 + i, K0 + k, g); },
+                       [&](int k, int j) { return band_ld(ab, K0 + k, C0 + j, g); });
+        acc_each<T>([&](int x, int y, int v, int i, int j) { sm.X[i * LDT + j] = old[x][y][v] - acc.a[x][y][v]; });
+    }
+    __syncthreads();
+    gj_inverse64<T>(sm.D, sm.xbuf, flag);  // D := E_{p+1}
+    if (s == 0) {
+        T *E = batch_blk(gh, bi / SB, bi % SB, bi % SB);
+        for (int t = tid; t < NB * NB; t += 256) E[(t / NB) * BW + t % NB] = sm.D[(t / NB) * LDT + t % NB];
+        return;
+    }
+    Acc<T> acc;
+    acc.zero();
+    lds_mm<T>(acc, sm.D, sm.X);
+    acc_each<T>([&](int x, int y, int v, int i, int j) { band_st(ab, R0 + i, C0 + j, g, acc.a[x][y][v]); });
+}
+
+// Batch inverses for block distance d (1 <= d < SB).  Workgroup (b, c, kind):
+//   kind 0:  G(c+d, c) = -E_{c+d} sum_{q=c}^{c+d-1} L(c+d, q) G(q, c)
+//   kind 1:  H(c, c+d) = -(U(c, c+d) + sum_{q=c+1}^{c+d-1} U(c, q) H(q, c+d))
+// (block indices inside batch b; L and U from the factored band).
+template <typename T>
+__global__ __launch_bounds__(256) void band_batch_inv_kernel(const T *ab, Geo g, int d, T *gh)
+{
+    __shared__ StepSmem<T> sm;
+    T *sa = sm.stage, *sb = sm.stage + NB * (KT + 1);
+    const int per = SB - d;
+    const int w = blockIdx.x % (2 * per), b = blockIdx.x / (2 * per);
+    const int kind = w / per, c = w % per;
+    const int kc = min(SB, g.nbk - b * SB);
+    if (c + d >= kc) return;
+    const int B0 = b * SB * NB;  // first row / column of the batch
+    Acc<T> acc;
+    acc.zero();
+    if (kind == 0) {
+        const int r = c + d;
+        for (int q = c; q < r; q++) {
+            const T *Gq = batch_blk(gh, b, q, c);
+            tile_mm<T>(acc, sa, sb, [&](int i, int k) { return band_ld(ab, B0 + r * NB + i, B0 + q * NB + k, g); },
+                       [&](int k, int j) { return Gq[(size_t)k * BW + j]; });
+        }
+        acc_each<T>([&](int x, int y, int v, int i, int j) { sm.X[i * LDT + j] = acc.a[x][y][v]; });
+        const T *Er = batch_blk(gh, b, r, r);
+        for (int t = threadIdx.x; t < NB * NB; t += 256) sm.D[(t / NB) * LDT + t % NB] = Er[(size_t)(t / NB) * BW + t % NB];
+        __syncthreads();
+        Acc<T> out;
+        out.zero();
+        lds_mm<T>(out, sm.D, sm.X);
+        T *Gr = batch_blk(gh, b, r, c);
+        acc_each<T>([&](int x, int y, int v, int i, int j) { Gr[(size_t)i * BW + j] = -out.a[x][y][v]; });
+    } else {
+        const int r = c, cc = c + d;
+        for (int q = r + 1; q < cc; q++) {
+            const T *Hq = batch_blk(gh, b, q, cc);
+            tile_mm<T>(acc, sa, sb, [&](int i, int k) { return band_ld(ab, B0 + r * NB + i, B0 + q * NB + k, g); },
+                       [&](int k, int j) { return Hq[(size_t)k * BW + j]; });
+        }
+        T *Hr = batch_blk(gh, b, r, cc);
+        acc_each<T>([&](int x, int y, int v, int i, int j) {
+            Hr[(size_t)i * BW + j] = -(acc.a[x][y][v] + band_ld(ab, B0 + r * NB + i, B0 + cc * NB + j, g));
+        });
+    }
+}
+
+// ---- triangular solves -----------------------------------------------------------
+// 16 rows per workgroup, 16 threads per row; each thread takes 4 consecutive
+// columns of every 64-column chunk (coalesced 512-byte row segments).
+template <typename V> __device__ __forceinline__ V sum16(V s)
 {
     s += __shfl_xor(s, 1);
     s += __shfl_xor(s, 2);
+    s += __shfl_xor(s, 4);
+    s += __shfl_xor(s, 8);
     return s;
 }
 
-// Row i = tid / 4 of the 64 x 64 block coupling band rows R0.. to band columns
-// C0.., times v (LDS): the sum over the quad (masked to the band).
-template <typename T, typename V>
-__device__ __forceinline__ V coupling_row(const T *ab, int hb, int R0, int C0, const V *v)
+// FWD: y = G f over batch b (rows of the batch; out = z); BWD: x = H g, H unit
+// upper (out = x).  Workgroup w: rows 16 w ..; the batch's right-hand side
+// segment is staged in LDS first.
+template <typename T, typename V, bool FWD>
+__global__ __launch_bounds__(256) void band_batch_apply_kernel(const T *gh, Geo g, int b, const V *rhs, V *out)
 {
-    const int tid = threadIdx.x, i = tid >> 2, c0 = 16 * (tid & 3);
-    const T *rowp = ab + (size_t)(R0 + i) * (size_t)(2 * hb) + hb + C0;  // entry (R0+i, C0+c) at rowp[c]
-    T a[16];
-#pragma unroll
-    for (int j = 0; j < 16; j++) a[j] = rowp[c0 + j];
+    __shared__ V f[BW];
+    const int kc = min(SB, g.nbk - b * SB), B0 = b * BW;
+    for (int t = threadIdx.x; t < kc * NB; t += 256) f[t] = rhs[B0 + t];
+    __syncthreads();
+    const int row = blockIdx.x * 16 + (threadIdx.x >> 4), l = threadIdx.x & 15;
+    if (row >= kc * NB) return;
+    const int rb = row / NB;
+    const T *M = gh + (size_t)b * BW * BW + (size_t)row * BW;
+    const int c_lo = FWD ? 0 : rb + 1, c_hi = FWD ? rb + 1 : kc;  // block columns
     V s = V(0);
+    for (int cb = c_lo; cb < c_hi; cb++) {
+        const int c = cb * NB + 4 * l;
+        T m4[4];
 #pragma unroll
-    for (int j = 0; j < 16; j++) s += inband(R0 + i, C0 + c0 + j, hb) ? (V)a[j] * v[c0 + j] : V(0);
-    return quad_sum(s);
+        for (int e = 0; e < 4; e++) m4[e] = M[c + e];
+#pragma unroll
+        for (int e = 0; e < 4; e++) s += (V)m4[e] * f[c + e];
+    }
+    s = sum16(s);
+    if (l == 0) out[B0 + row] = FWD ? s : f[row] + s;
 }
 
-template <typename T, typename V, bool FWD, int K>
-__global__ __launch_bounds__(256) void band_sweep_kernel(const T *ab, int hb, int n, const T *inv, int r0, int kc,
-                                                         V *f, V *out)
+// FWD: f_q -= L(q, batch) y_batch for the block rows q after batch b within
+// reach; BWD: g_q -= U(q, batch) x_batch for the block rows before it.
+// Workgroup w: rows 16 w of the target range.
+template <typename T, typename V, bool FWD>
+__global__ __launch_bounds__(256) void band_couple_kernel(const T *ab, Geo g, int b, const V *sol, V *f, int q0)
 {
-    __shared__ V fb[K][NB], ys[K][NB];
-    const int tid = threadIdx.x, i = tid >> 2, c0 = 16 * (tid & 3);
-    auto blk = [&](int k) { return FWD ? r0 + k : r0 - k; };
-    for (int t = tid; t < kc * NB; t += 256) fb[t / NB][t % NB] = f[(size_t)blk(t / NB) * NB + t % NB];
+    __shared__ V y[BW];
+    const int kc = min(SB, g.nbk - b * SB), B0 = b * BW;
+    for (int t = threadIdx.x; t < kc * NB; t += 256) y[t] = sol[B0 + t];
     __syncthreads();
-    for (int k = 0; k < kc; k++) {
-        const int b = blk(k);
-        const T *Ti = FWD ? linv_of(const_cast<T *>(inv), b) : uinv_of(const_cast<T *>(inv), b);
-        const V yv = quad_sum(mv_part<T, V>(Ti + i * NB, c0, fb[k]));
-        if ((tid & 3) == 0) ys[k][i] = yv;
-        __syncthreads();
-        for (int k2 = k + 1; k2 < kc; k2++) {
-            const V cv = coupling_row<T, V>(ab, hb, blk(k2) * NB, b * NB, ys[k]);
-            if ((tid & 3) == 0) fb[k2][i] -= cv;
-        }
-        __syncthreads();
+    const int row = q0 * NB + blockIdx.x * 16 + (threadIdx.x >> 4), l = threadIdx.x & 15;
+    V s = V(0);
+    for (int cb = 0; cb < kc; cb++) {
+        const int c = B0 + cb * NB + 4 * l;
+        T m4[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) m4[e] = band_ld(ab, row, c + e, g);
+#pragma unroll
+        for (int e = 0; e < 4; e++) s += (V)m4[e] * y[c - B0 + e];
     }
-    if (blockIdx.x == 0) {
-        for (int t = tid; t < kc * NB; t += 256) out[(size_t)blk(t / NB) * NB + t % NB] = ys[t / NB][t % NB];
-        return;
-    }
-    const int q = FWD ? r0 + kc - 1 + (int)blockIdx.x : r0 - kc + 1 - (int)blockIdx.x;  // target block
-    V acc = V(0);
-    for (int k = 0; k < kc; k++) acc += coupling_row<T, V>(ab, hb, q * NB, blk(k) * NB, ys[k]);
-    if ((tid & 3) == 0) f[(size_t)q * NB + i] -= acc;
+    s = sum16(s);
+    if (l == 0 && row < g.n) f[row] -= s;
 }
 
 }  // namespace
 
 // ---- host -----------------------------------------------------------------------
-template <typename T> static int band_factor_t(T *ab, int n, int hb, hipStream_t s, int *flag, T *inv)
+int gf_band_ku(int hb) { return hb + NB - 1; }
+
+static Geo make_geo(int n, int hb)
 {
+    Geo g;
+    g.n = n;
+    g.kl = hb;
+    g.ku = gf_band_ku(hb);
+    g.ld = g.kl + g.ku;
+    g.nbk = n / NB;
+    g.R = (NB - 1 + hb) / NB;
+    return g;
+}
+
+template <typename T> static int band_factor_t(T *ab, int n, int hb, hipStream_t s, int *flag, T *gh)
+{
+    const Geo g = make_geo(n, hb);
     int launches = 0;
-    hipLaunchKernelGGL(band_diag_kernel<T>, dim3(1), dim3(256), 0, s, ab, hb, n, inv, flag);
-    launches++;
-    for (int p = 0; p * NB < n; p++) {
-        int k0 = p * NB;
-        int lim = std::min(n, k0 + NB + hb);
-        int rem = lim - (k0 + NB);
-        if (rem <= 0) break;
-        int nt = (rem + NB - 1) / NB;
-        int pr = gf_probe_begin(s, GF_KC_BAND_PANEL);
-        hipLaunchKernelGGL(band_panel_kernel<T>, dim3(2 * nt), dim3(256), 0, s, ab, hb, p, n, nt, (const T *)inv);
+    for (int p = -1; p + 1 < g.nbk; p++) {
+        const int last = g.nbk - 1;
+        const int nS = std::min(last, p + 1 + g.R) - p;
+        const int nr = p >= 0 ? std::min(last, p + g.R) - (p + 2) + 1 : 0;
+        const int nc = std::min(last, p + g.R) - p;
+        const int grid = nS + std::max(0, nr) * std::max(0, nc);
+        const int pr = gf_probe_begin(s, GF_KC_BAND_UPDATE);
+        hipLaunchKernelGGL(band_step_kernel<T>, dim3(grid), dim3(256), 0, s, ab, g, p, nS, std::max(1, nc), gh, flag);
         gf_probe_end(s, pr, NB);
-        pr = gf_probe_begin(s, GF_KC_BAND_UPDATE);
-        hipLaunchKernelGGL(band_update_kernel<T>, dim3(nt * nt), dim3(256), 0, s, ab, hb, p, n, nt, inv, flag);
-        gf_probe_end(s, pr, NB);
-        launches += 2;
+        launches++;
+    }
+    const int nbatch = (g.nbk + SB - 1) / SB;
+    for (int d = 1; d < std::min(SB, g.nbk); d++) {
+        const int pr = gf_probe_begin(s, GF_KC_BAND_PANEL);
+        hipLaunchKernelGGL(band_batch_inv_kernel<T>, dim3(nbatch * 2 * (SB - d)), dim3(256), 0, s, (const T *)ab, g,
+                           d, gh);
+        // flops of the launch: a G block at distance d takes d + 1 products of
+        // 64 x 64 x 64, an H block d of them
+        double blocks = 0.0;
+        for (int bb = 0; bb < nbatch; bb++) blocks += std::max(0, std::min(SB, g.nbk - bb * SB) - d);
+        gf_probe_end(s, pr, blocks * (2.0 * d + 1.0) * 2.0 * NB * NB * NB);
+        launches++;
     }
     GF_HIP(hipGetLastError());
     return launches;
 }
-
-constexpr int SWEEP_K = 2;  // blocks per sweep launch (4 measured slower: longer in-batch chain)
 
 template <typename T, typename V>
-static int band_solve_t(const T *ab, int n, int hb, hipStream_t s, V *x, V *z, const T *inv)
+static int band_solve_t(const T *ab, int n, int hb, hipStream_t s, V *x, V *z, const T *gh)
 {
-    const int nbk = n / NB, reach = (hb + NB - 1) / NB;
+    const Geo g = make_geo(n, hb);
+    const int nbatch = (g.nbk + SB - 1) / SB;
     int launches = 0;
-    // forward: L z = x, x updated in place as the right-hand side
-    for (int r0 = 0; r0 < nbk; r0 += SWEEP_K) {
-        const int kc = std::min(SWEEP_K, nbk - r0), last = r0 + kc - 1;
-        const int g = 1 + std::min(reach, nbk - 1 - last);
-        const int pr = gf_probe_begin(s, GF_KC_BAND_SWEEP);
-        hipLaunchKernelGGL((band_sweep_kernel<T, V, true, SWEEP_K>), dim3(g), dim3(256), 0, s, ab, hb, n, inv, r0, kc,
-                           x, z);
+    // forward: L z = x (x updated in place as the right-hand side)
+    for (int b = 0; b < nbatch; b++) {
+        const int kc = std::min(SB, g.nbk - b * SB), q0 = b * SB + kc;
+        const int q1 = std::min(g.nbk - 1, b * SB + kc - 1 + g.R);
+        int pr = gf_probe_begin(s, GF_KC_BAND_SWEEP);
+        hipLaunchKernelGGL((band_batch_apply_kernel<T, V, true>), dim3(kc * NB / 16), dim3(256), 0, s, gh, g, b,
+                           (const V *)x, z);
         gf_probe_end(s, pr, kc * NB);
         launches++;
+        if (q1 >= q0) {
+            pr = gf_probe_begin(s, GF_KC_BAND_SWEEP);
+            hipLaunchKernelGGL((band_couple_kernel<T, V, true>), dim3((q1 - q0 + 1) * NB / 16), dim3(256), 0, s, ab, g,
+                               b, (const V *)z, x, q0);
+            gf_probe_end(s, pr, 0.0);  // rows are counted on the apply launches
+            launches++;
+        }
     }
-    // backward: U x = z, z updated in place
-    for (int r0 = nbk - 1; r0 >= 0; r0 -= SWEEP_K) {
-        const int kc = std::min(SWEEP_K, r0 + 1), last = r0 - kc + 1;
-        const int g = 1 + std::min(reach, last);
-        const int pr = gf_probe_begin(s, GF_KC_BAND_SWEEP);
-        hipLaunchKernelGGL((band_sweep_kernel<T, V, false, SWEEP_K>), dim3(g), dim3(256), 0, s, ab, hb, n, inv, r0,
-                           kc, z, x);
+    // backward: U x = z (z updated in place)
+    for (int b = nbatch - 1; b >= 0; b--) {
+        const int kc = std::min(SB, g.nbk - b * SB);
+        const int q1 = b * SB - 1, q0 = std::max(0, b * SB - g.R);
+        int pr = gf_probe_begin(s, GF_KC_BAND_SWEEP);
+        hipLaunchKernelGGL((band_batch_apply_kernel<T, V, false>), dim3(kc * NB / 16), dim3(256), 0, s, gh, g, b,
+                           (const V *)z, x);
         gf_probe_end(s, pr, kc * NB);
         launches++;
+        if (q1 >= q0) {
+            pr = gf_probe_begin(s, GF_KC_BAND_SWEEP);
+            hipLaunchKernelGGL((band_couple_kernel<T, V, false>), dim3((q1 - q0 + 1) * NB / 16), dim3(256), 0, s, ab,
+                               g, b, (const V *)x, z, q0);
+            gf_probe_end(s, pr, 0.0);  // rows are counted on the apply launches
+            launches++;
+        }
     }
     GF_HIP(hipGetLastError());
     return launches;
 }
 
-size_t gf_band_inv_elems(int n) { return (size_t)(n / NB) * 2 * NB * NB; }
+size_t gf_band_inv_elems(int n)
+{
+    const int nbk = n / NB;
+    return (size_t)((nbk + SB - 1) / SB) * BW * BW;
+}
 
 int gf_band_factor(void *ab, int elem_size, int n, int hb, hipStream_t s, int *flag, void *inv)
 {

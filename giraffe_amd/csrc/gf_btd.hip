@@ -595,7 +595,9 @@ __global__ __launch_bounds__(256) void btd_back_kernel(const T *X, const T *Y, i
 
 }  // namespace
 
-int gf_btd_alloc(gf_btd &b, int m, int nblk, int elem_size)
+// Level structure of the cyclic reduction (no device call).  Returns the number
+// of saved coupling blocks (the Lsv / Rsv extent).
+int gf_btd_plan(gf_btd &b, int m, int nblk, int elem_size)
 {
     if (m < 1 || m > 32) {
         gf_set_error("block-tridiagonal solver: block size out of range");
@@ -625,6 +627,14 @@ int gf_btd_alloc(gf_btd &b, int m, int nblk, int elem_size)
     // the back-substitution of the levels above the tail runs as one wide launch
     // per level: inside the single tail workgroup it took longer than the launches
     b.back_level = b.tail_level;
+    b.n_save = (long long)save;
+    return GF_OK;
+}
+
+int gf_btd_alloc(gf_btd &b, int m, int nblk, int elem_size)
+{
+    GF_TRY(gf_btd_plan(b, m, nblk, elem_size));
+    const size_t save = (size_t)b.n_save;
     size_t blocks = (size_t)nblk * b.bsz * elem_size;
     size_t sv = (save ? save : 1) * b.bsz * elem_size;
     // A, B, C contiguous: the assembly clears them with one fill

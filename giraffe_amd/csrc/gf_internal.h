@@ -10,6 +10,11 @@
 // row (the first back-off; later ones double while the retries keep failing).
 constexpr int GF_MIXED_BACKOFF = 16;
 
+// Bits of a factorisation's flag word: a zero or non-finite pivot (the solve
+// fails, sparsesystem's info_fail), and a tiny pivot replaced by a perturbation
+// of the block's scale (PARDISO's static-pivoting default; the solve goes on).
+constexpr int GF_FLAG_FAIL = 1, GF_FLAG_PERTURBED = 2;
+
 // ---- error handling --------------------------------------------------------
 void gf_set_error(const std::string &msg);
 #define GF_HIP(call)                                                                  \
@@ -51,10 +56,11 @@ enum { BG_PER_GP = 15, BG_COUNT = 30 };
 // GM_BTD : block-tridiagonal blocks A (sub), B (diag), C (super), [nblk][m][m]
 //          row-major, consecutive in one allocation (B = A + nblk m m,
 //          C = B + nblk m m); used when the half-bandwidth fits a block of m <= 32.
-// GM_BAND: skewed dense band, entry (i, c) with |i - c| <= hb at
-//          ab[i * 2hb + c + hb] (each row is a contiguous window of 2hb+1);
-//          sub-matrices inside the band are plain strided matrices with
-//          leading dimension 2hb, which is what the band LU tiles use.
+// GM_BAND: skewed dense band, entry (i, c) with -hb <= c - i <= ku at
+//          ab[i * ld + c + hb], ld = hb + ku (each row a contiguous window of
+//          ld + 1); ku = gf_band_ku(hb) leaves room for the factorisation's
+//          fill.  Sub-matrices inside the band are plain strided matrices with
+//          leading dimension ld, which is what the band LU tiles use.
 enum { GM_BTD = 0, GM_BAND = 1 };
 struct gf_gmat {
     int layout;
@@ -62,7 +68,9 @@ struct gf_gmat {
     double *A, *B, *C;
     double *ab;
     int hb;
+    int ld;
 };
+int gf_band_ku(int hb);
 
 // Reciprocal of a pivot that is finite, non-zero and not subnormal (the solvers
 // check this first): the hardware estimate refined by Newton-Raphson steps
@@ -121,7 +129,7 @@ __device__ __forceinline__ void gm_addr_n(const gf_gmat &g, const int (&R)[N], c
         for (int k = 0; k < N; k++) {
             const bool ok = R[k] >= 0 && Cc[k] >= 0;
             const int r = ok ? R[k] : 0, c = ok ? Cc[k] : 0;
-            p[k] = g.ab + (size_t)r * (size_t)(2 * g.hb) + (size_t)(c + g.hb);
+            p[k] = g.ab + (size_t)r * (size_t)g.ld + (size_t)(c + g.hb);
         }
     }
 }
@@ -226,8 +234,10 @@ struct gf_btd {
     long long *d_save = nullptr;    // lvl_save on the device
     std::vector<int> lvl_n;         // rows active at each level
     std::vector<long long> lvl_save; // offsets (in blocks) into Lsv/Rsv per level
+    long long n_save = 0;           // saved coupling blocks over all levels
     int elem_size = 8;
 };
+int gf_btd_plan(gf_btd &b, int m, int nblk, int elem_size);  // level structure only, no device call
 
 // ---- per-kernel timing --------------------------------------------------------
 // While a model's timing is on and its probe mask selects a kernel class, every
@@ -293,6 +303,7 @@ struct gf_model {
     double gravity[3] = {0, 0, 0};
     int gravity_on = 0;
     int half_band = 0;
+    int band_ld = 0;                // GM_BAND row stride: half_band + gf_band_ku(half_band)
     bool node_order_rcm = false;    // DOFs numbered in reverse Cuthill-McKee node order
 
     // device: node state

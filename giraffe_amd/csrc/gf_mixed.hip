@@ -24,7 +24,7 @@ __global__ void diag_scale_kernel(gf_gmat g, int n, int npad, double *D)
         int bi = i / g.m, lr = i - bi * g.m;
         v = g.B[((size_t)bi * g.m + lr) * g.m + lr];
     } else {
-        v = g.ab[(size_t)i * (size_t)(2 * g.hb) + (size_t)(i + g.hb)];
+        v = g.ab[(size_t)i * (size_t)g.ld + (size_t)(i + g.hb)];
     }
     double a = fabs(v);
     D[i] = (i < n && a > 0.0 && a < 1e300) ? 1.0 / sqrt(a) : 1.0;
@@ -48,17 +48,17 @@ __global__ void convert_btd_kernel(const double *A, const double *B, const doubl
     }
 }
 
-// Band storage -> scaled FP32 copy (same skewed layout).
-__global__ void convert_band_kernel(const double *ab, float *ab32, int npad, int hb, const double *D)
+// Band storage -> scaled FP32 copy (same skewed layout, row stride ld).
+__global__ void convert_band_kernel(const double *ab, float *ab32, int npad, int hb, int ld, const double *D)
 {
-    size_t w = (size_t)(2 * hb + 1);
+    size_t w = (size_t)(ld + 1);
     size_t total = (size_t)npad * w;
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
         int i = (int)(t / w);
         int c = i - hb + (int)(t % w);
         double v = 0.0;
-        if (c >= 0 && c < npad) v = D[i] * ab[(size_t)i * (size_t)(2 * hb) + (size_t)(c + hb)] * D[c];
-        ab32[(size_t)i * (size_t)(2 * hb) + (size_t)(c + hb)] = (float)v;
+        if (c >= 0 && c < npad) v = D[i] * ab[t] * D[c];
+        ab32[t] = (float)v;
     }
 }
 
@@ -84,7 +84,7 @@ __global__ void residual_kernel(gf_gmat g, int npad, const double *r, const doub
         for (int k = lane; k <= 2 * hb; k += 64) {
             int c = row - hb + k;
             if (c < 0 || c >= npad) continue;
-            acc += g.ab[(size_t)row * (size_t)(2 * hb) + (size_t)(c + hb)] * x[c];
+            acc += g.ab[(size_t)row * (size_t)g.ld + (size_t)(c + hb)] * x[c];
         }
     }
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
@@ -170,7 +170,7 @@ int gf_mixed_factor(gf_model *m)
         return l < 0 ? l : l + 2;
     }
     hipLaunchKernelGGL(convert_band_kernel, dim3(grid_for(m->band_elems, 256)), dim3(256), 0, m->stream, m->d_band,
-                       m->d_band32, npad, m->half_band, m->d_scale);
+                       m->d_band32, npad, m->half_band, m->band_ld, m->d_scale);
     GF_HIP(hipGetLastError());
     int l = gf_band_factor(m->d_band32, 4, npad, m->half_band, m->stream, m->d_flag, m->d_band32_inv);
     return l < 0 ? l : l + 2;

@@ -156,6 +156,7 @@ gf_gmat gf_model_gmat(const gf_model *m)
     g.C = (double *)m->btd.C;
     g.ab = m->d_band;
     g.hb = m->half_band;
+    g.ld = m->band_ld;
     return g;
 }
 
@@ -414,7 +415,8 @@ static size_t footprint(const gf_model_desc *d, const gf_model *m, int precision
         size_t blocks = 6 * (size_t)nblk + 2 * (size_t)nblk;                  // A B C Bi X Y + saved couplings
         tangent = blocks * (size_t)bm * bm;
     } else {
-        tangent = npad * (size_t)(2 * m->half_band + 1) + gf_band_inv_elems(m->npad);
+        const int hb = m->half_band;
+        tangent = npad * (size_t)(hb + gf_band_ku(hb) + 1) + gf_band_inv_elems(m->npad);
     }
     b += tangent * 8;
     if (precision == GF_PREC_MIXED) b += tangent * 4 + 3 * npad * 8;
@@ -640,7 +642,8 @@ static int create(const gf_model_desc *d, const gf_options *opt, gf_model *m)
     if (m->layout == GM_BTD) {
         GF_TRY(gf_btd_alloc(m->btd, bm, nblk, 8));
     } else {
-        m->band_elems = npad * (size_t)(2 * hb + 1);
+        m->band_ld = hb + gf_band_ku(hb);
+        m->band_elems = npad * (size_t)(m->band_ld + 1);
         GF_TRY(dalloc(&m->d_band, m->band_elems));
         GF_TRY(dalloc(&m->d_band_inv, gf_band_inv_elems(m->npad)));
     }
@@ -834,8 +837,8 @@ int gf_assemble_enqueue(gf_model *m, bool copy_norms)
         dfirst = (size_t)m->btd.nblk * m->btd.bsz + last * m->btd.bsz + i0 * mb + i0;  // inside B
         dstride = mb + 1;
     } else {
-        dfirst = nf * (size_t)(2 * m->half_band + 1) + (size_t)m->half_band;
-        dstride = (size_t)(2 * m->half_band + 1);
+        dfirst = nf * (size_t)(m->band_ld + 1) + (size_t)m->half_band;
+        dstride = (size_t)(m->band_ld + 1);
     }
     GF_TRY(gf_launch_clear3(m->stream, m->d_r, npad, tan, ntan, m->d_scal + GF_SCAL_RES, 8, dfirst, dstride, dcount));
     double t = m->t_last + m->dt;
@@ -923,7 +926,7 @@ int gf_solve_update_result(gf_model *m, int *info_fail, double *inc_out)
     } else {
         int flag = 0;
         memcpy(&flag, &m->h_scal[GF_SCAL_FLAG], 4);
-        *info_fail = flag ? 1 : 0;
+        *info_fail = (flag & GF_FLAG_FAIL) ? 1 : 0;
     }
     unsigned long long *u = (unsigned long long *)m->h_scal;
     for (int i = 0; i < 4; i++) memcpy(&inc_out[i], &u[i], 8);
@@ -1043,7 +1046,7 @@ extern "C" int gf_solve(gf_model *m, int *info_fail)
         GF_HIP(hipStreamSynchronize(m->stream));
         memcpy(&flag32, &m->h_scal[GF_SCAL_FLAG], 4);
         GF_TRY(tock(m, GF_PH_FACTOR, nl_f32));
-        if (!flag32) {
+        if (!(flag32 & GF_FLAG_FAIL)) {
             int fb = 0, sw = 0;
             tick(m, GF_PH_SOLVE, true);
             GF_TRYN(nl_ref, gf_mixed_solve(m, &fb, &sw));
@@ -1057,7 +1060,7 @@ extern "C" int gf_solve(gf_model *m, int *info_fail)
             GF_HIP(hipMemcpyAsync(&m->h_scal[GF_SCAL_FLAG], m->d_flag, 4, hipMemcpyDeviceToHost, m->stream));
             GF_HIP(hipStreamSynchronize(m->stream));
             memcpy(&flag, &m->h_scal[GF_SCAL_FLAG], 4);
-            if (flag) *info_fail = 1;
+            if (flag & GF_FLAG_FAIL) *info_fail = 1;
             return GF_OK;
         }
         // an unusable FP32 pivot: FP64 factorisation below
@@ -1071,7 +1074,7 @@ extern "C" int gf_solve(gf_model *m, int *info_fail)
     GF_HIP(hipMemcpyAsync(&m->h_scal[GF_SCAL_FLAG], m->d_flag, 4, hipMemcpyDeviceToHost, m->stream));
     GF_HIP(hipStreamSynchronize(m->stream));
     memcpy(&flag, &m->h_scal[GF_SCAL_FLAG], 4);
-    if (flag) *info_fail = 1;
+    if (flag & GF_FLAG_FAIL) *info_fail = 1;
     return GF_OK;
 }
 
@@ -1187,7 +1190,7 @@ extern "C" int gf_read_tangent_dense(gf_model *m, double *out, long long cap)
         for (int R = 0; R < m->n_free; R++)
             for (int Cc = std::max(0, R - hb); Cc <= std::min(m->n_free - 1, R + hb); Cc++)
                 out[(size_t)m->row_nodecomp[R] * n6 + m->row_nodecomp[Cc]] =
-                    ab[(size_t)R * (size_t)(2 * hb) + (size_t)(Cc + hb)];
+                    ab[(size_t)R * (size_t)m->band_ld + (size_t)(Cc + hb)];
         return GF_OK;
     }
     size_t nbl = (size_t)m->btd.nblk * m->btd.bsz;
@@ -1213,17 +1216,17 @@ extern "C" int gf_bandsystem(int n, int hb, const double *ab_rows, const double 
     if (n <= 0 || hb < 0 || !ab_rows || !b || !x || !info_fail) return GF_E_ARG;
     *info_fail = 0;
     int npad = gf_band_pad(n);
-    size_t w = (size_t)(2 * hb + 1);
-    size_t elems = (size_t)npad * w;
+    const size_t w = (size_t)(2 * hb + 1);                  // caller's row length
+    const size_t ws = (size_t)(hb + gf_band_ku(hb) + 1);    // stored row length (room for the fill)
+    size_t elems = (size_t)npad * ws;
     // rows of the band: row i holds columns i-hb .. i+hb; columns outside [0, n) must be zero
-    std::vector<double> rows((size_t)n * w);
+    std::vector<double> rows((size_t)npad * ws, 0.0);
     for (int i = 0; i < n; i++)
         for (int k = 0; k < (int)w; k++) {
             int c = i - hb + k;
-            rows[(size_t)i * w + k] = (c >= 0 && c < n) ? ab_rows[(size_t)i * w + k] : 0.0;
+            rows[(size_t)i * ws + k] = (c >= 0 && c < n) ? ab_rows[(size_t)i * w + k] : 0.0;
         }
-    std::vector<double> diag((size_t)(npad - n) * w, 0.0);
-    for (int i = n; i < npad; i++) diag[(size_t)(i - n) * w + hb] = 1.0;
+    for (int i = n; i < npad; i++) rows[(size_t)i * ws + hb] = 1.0;
     double *d_ab = nullptr, *d_x = nullptr, *d_z = nullptr, *d_inv = nullptr;
     int *d_flag = nullptr;
     hipStream_t s = nullptr;
@@ -1240,8 +1243,6 @@ extern "C" int gf_bandsystem(int n, int hb, const double *ab_rows, const double 
         GF_TRY(gf_launch_fill(s, d_x, (size_t)npad, 0.0));
         GF_TRY(gf_launch_fill_bytes(s, d_flag, 4));
         GF_HIP(hipMemcpyAsync(d_ab, rows.data(), rows.size() * 8, hipMemcpyHostToDevice, s));
-        if (npad > n)
-            GF_HIP(hipMemcpyAsync(d_ab + (size_t)n * w, diag.data(), diag.size() * 8, hipMemcpyHostToDevice, s));
         GF_HIP(hipMemcpyAsync(d_x, b, (size_t)n * 8, hipMemcpyHostToDevice, s));
         int l = gf_band_factor(d_ab, 8, npad, hb, s, d_flag, d_inv);
         if (l < 0) return l;
@@ -1258,7 +1259,7 @@ extern "C" int gf_bandsystem(int n, int hb, const double *ab_rows, const double 
         if (p) (void)hipFree(p);
     if (s) (void)hipStreamDestroy(s);
     if (st) return st;
-    if (flag) *info_fail = 1;
+    if (flag & GF_FLAG_FAIL) *info_fail = 1;
     return GF_OK;
 }
 

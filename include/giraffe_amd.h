@@ -223,9 +223,11 @@ int gf_timing_read(gf_model *m, double *ms_out, long long *launches_out, int res
  * accumulated milliseconds, launches, and work units (GF_KC_* comments). */
 #define GF_KC_BTD_LEVEL      0  /* btd_level_kernel; unit: block row eliminated     */
 #define GF_KC_BTD_TAIL       1  /* btd_tail_kernel; unit: block row eliminated      */
-#define GF_KC_BAND_PANEL     2  /* band_panel_kernel; unit: band column eliminated  */
-#define GF_KC_BAND_UPDATE    3  /* band_update_kernel; unit: band column eliminated */
-#define GF_KC_BAND_SWEEP     4  /* band_sweep_kernel; unit: row swept               */
+#define GF_KC_BAND_PANEL     2  /* band_batch_inv_kernel; unit: flop of the batch inverses */
+#define GF_KC_BAND_UPDATE    3  /* band_step_kernel; unit: band column eliminated   */
+#define GF_KC_BAND_SWEEP     4  /* band_batch_apply_kernel, band_couple_kernel; unit: row
+                                   solved (forward or backward; counted on the apply
+                                   launches)                                        */
 #define GF_KC_BEAM_ASSEMBLE  5  /* beam_assemble_kernel; unit: element              */
 #define GF_KC_SHELL_ASSEMBLE 6  /* shell_assemble_kernel; unit: element             */
 #define GF_NKC               7
@@ -275,9 +277,13 @@ int gf_beam_eval(int count, const double *x, const double *cs, const double *pro
 
 /* ---- standalone band solve -------------------------------------------------
  * Replaces bandsystem (src/BandMatrix.cpp:257-287, LAPACK dgbtrf/dgbtrs) with
- * the device band LU (no row interchanges) for one right-hand side.
+ * the device band solver (block LU over 64 x 64 blocks, each diagonal block
+ * inverted without interchanges) for one right-hand side.
  * ab_rows[i*(2hb+1) + (c - i + hb)] = A(i, c) for |i - c| <= hb (entries with c
- * outside [0, n) are ignored).  *info_fail = 1 on a zero or non-finite pivot. */
+ * outside [0, n) are ignored).  A pivot smaller than 1e-13 of its diagonal
+ * block's largest entry is replaced by that size with its sign and the solve
+ * goes on (PARDISO's static pivoting, as sparsesystem's mtype 11 defaults);
+ * *info_fail = 1 on a non-finite pivot or an all-zero diagonal block. */
 int gf_bandsystem(int n, int hb, const double *ab_rows, const double *b, double *x, int *info_fail);
 /* Block-tridiagonal solve with the cyclic-reduction solver the narrow-band
  * models use (no counterpart in the reference; a test entry point): nblk block

@@ -34,7 +34,12 @@ def check_against_oracle(m, rep, iters, nodes, ref):
     iters = np.asarray(iters)
     assert rep.aborted == ro.aborted == 0
     assert rep.converged_steps == ro.converged_steps
-    assert len(iters) == len(io) and np.abs(iters - io).max() <= 1, (iters, io)
+    # the same attempts converge and diverge; converged steps agree to one
+    # iteration (how long a diverging attempt runs before it is stopped is not
+    # a parity figure: both bisect it)
+    assert len(iters) == len(io) and np.array_equal(np.sign(iters), np.sign(io)), (iters, io)
+    conv = io > 0
+    assert np.abs(iters[conv] - io[conv]).max() <= 1, (iters, io)
     x0 = m.freeze()["xyz"]
     scale_u = max(np.abs(no[:, :3] - x0).max(), 1e-12)
     scale_r = max(np.abs(no[:, 3:]).max(), 1e-12)

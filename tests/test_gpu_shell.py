@@ -46,7 +46,8 @@ def test_shell_eval_matches_oracle(oracle_lib):
     assert errf < 1e-9, errf
 
 
-@pytest.mark.parametrize("n,hb", [(50, 7), (300, 40), (1000, 100), (4000, 300)])
+@pytest.mark.parametrize("n,hb", [(50, 7), (300, 40), (1000, 100), (4000, 300), (1500, 63), (1500, 64),
+                                  (3000, 1346), (2100, 1500)])
 def test_bandsystem_matches_dense(n, hb):
     rng = np.random.default_rng(n + hb)
     w = 2 * hb + 1
@@ -66,12 +67,72 @@ def test_bandsystem_matches_dense(n, hb):
     assert np.abs(x - xr).max() <= 1e-12 * np.abs(xr).max() * n
 
 
-def test_bandsystem_reports_zero_pivot():
-    rows = np.zeros((70, 3))
+def test_bandsystem_reports_singular_block():
+    """A diagonal block that is entirely zero, or a non-finite entry, fails the solve."""
+    rows = np.zeros((200, 3))
     rows[:, 1] = 1.0
-    rows[10, 1] = 0.0
-    x, fail = runtime.bandsystem(rows, 1, np.ones(70))
+    rows[64:128, :] = 0.0
+    x, fail = runtime.bandsystem(rows, 1, np.ones(200))
     assert fail == 1
+    rows[64:128, 1] = 1.0
+    rows[150, 1] = np.nan
+    x, fail = runtime.bandsystem(rows, 1, np.ones(200))
+    assert fail == 1
+
+
+def test_bandsystem_perturbs_tiny_pivot():
+    """A pivot below 1e-13 of its row's original diagonal (of the block's largest
+    entry where that diagonal is zero) is replaced by that size with its sign and
+    the solve goes on (PARDISO's static-pivoting default for mtype 11).  A zero
+    diagonal entry in a block whose largest entry is 2 becomes 2e-13; a tiny
+    but non-zero diagonal entry is a true pivot and stays."""
+    n = 130
+    rows = np.zeros((n, 3))
+    rows[:, 1] = 2.0
+    rows[10, 1] = 0.0
+    rows[40, 1] = 1e-20
+    # rows 100, 101: [[1, 1], [1, 1 + 1e-15]], whose second pivot cancels to ~1e-15
+    rows[100] = [0.0, 1.0, 1.0]
+    rows[101] = [1.0, 1.0 + 1e-15, 0.0]
+    b = np.ones(n)
+    x, fail = runtime.bandsystem(rows, 1, b)
+    assert fail == 0 and np.isfinite(x).all()
+    exp = np.full(n, 0.5)
+    exp[10] = 1.0 / 2e-13
+    exp[40] = 1e20
+    np.testing.assert_allclose(np.delete(x, [100, 101]), np.delete(exp, [100, 101]), rtol=1e-12)
+    # the perturbed pivot (1e-13 instead of ~1e-15) still satisfies the first row
+    assert abs(x[100] + x[101] - 1.0) <= 1e-12 * max(1.0, abs(x[100]))
+
+
+def test_bandsystem_with_interchange_free_blocks_of_mixed_scale():
+    """Rows scaled over 12 orders of magnitude (translations and rotations of a
+    stiffness matrix): the block inverses keep the solve accurate."""
+    rng = np.random.default_rng(5)
+    n, hb = 700, 90
+    w = 2 * hb + 1
+    rows = rng.normal(size=(n, w))
+    for i in range(n):
+        rows[i, hb] = 4.0 * w
+    A = np.zeros((n, n))
+    for i in range(n):
+        for k in range(w):
+            c = i - hb + k
+            if 0 <= c < n:
+                A[i, c] = rows[i, k]
+    s = 10.0 ** rng.uniform(-6, 6, n)
+    A = s[:, None] * A * s[None, :]
+    rows2 = np.array([[A[i, i - hb + k] if 0 <= i - hb + k < n else 0.0 for k in range(w)] for i in range(n)])
+    b = rng.normal(size=n)
+    x, fail = runtime.bandsystem(rows2, hb, b)
+    assert fail == 0
+    xr = np.linalg.solve(A, b)
+
+    def backward_error(v):  # componentwise: |A v - b| / (|A| |v| + |b|), worst row
+        return (np.abs(A @ v - b) / (np.abs(A) @ np.abs(v) + np.abs(b))).max()
+
+    # as accurate as LAPACK's partially pivoted solve of the same matrix
+    assert backward_error(x) <= max(1e-14, 100.0 * backward_error(xr)), (backward_error(x), backward_error(xr))
 
 
 def test_plate_assembly_matches_oracle(oracle_lib):
