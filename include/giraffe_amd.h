@@ -292,6 +292,17 @@ int gf_bandsystem(int n, int hb, const double *ab_rows, const double *b, double 
  * info_fail = 1 on a zero or non-finite pivot. */
 int gf_btdsystem(int m, int nblk, const double *A, const double *B, const double *C, const double *b, double *x,
                  int *info_fail);
+/* gf_btdsystem on the caller's DEVICE buffers, allocating nothing.  No
+ * counterpart in the reference; it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  A, B, C, b, x
+ * and info_fail are device pointers (info_fail: one int, written 0 or 1); the
+ * work is enqueued on `stream` (a hipStream_t, NULL for the null stream) and
+ * the call returns without waiting.  The factor workspace lives in a fixed
+ * device array of the library (9.4 MB on each GPU it runs on, held until the
+ * process ends): nblk * m * m <= 131072, else GF_E_CAPACITY before any device
+ * work.  Calls that overlap on one GPU share that array and race. */
+int gf_btdsystem_noalloc(int m, int nblk, const double *A, const double *B, const double *C, const double *b,
+                         double *x, int *info_fail, void *stream);
 
 /* ---- element-level parity entry (Shell_1::Mount on the device) ------------
  * x[18] reference coordinates of nodes 1..6, prop[GF_SHELL_NPROP], d[36]
