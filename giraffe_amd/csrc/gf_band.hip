@@ -26,7 +26,7 @@
 // synchronisation.  A pivot below 1e-13 of its row's original diagonal entry
 // is perturbed to that size (flag bit 2); a non-finite one fails (bit 1).
 //
-// Triangular solves: the diagonal blocks are grouped in batches of SB = 16;
+// Triangular solves: the diagonal blocks are grouped in batches of SB = 8;
 // after the factorisation each batch's diagonal part of L is inverted
 // (G = L_bb^-1, lower, E_p on its diagonal) and of U (H = U_bb^-1, unit
 // upper), one launch per block distance.  A solve then takes, per batch, one
@@ -34,13 +34,21 @@
 // subtracts the batch's couplings from the rows within reach.
 #include "gf_internal.h"
 
+#include <algorithm>
+
 namespace {
 
 constexpr int NB = 64;   // block size
-constexpr int SB = 16;   // blocks per solve batch
+constexpr int SB = 8;    // blocks per solve batch
 constexpr int BW = NB * SB;
 constexpr int KT = 32;   // k-chunk of the staged MFMA products
-constexpr int LDT = NB + 1;
+// LDS strides (elements) chosen for the MFMA operand patterns in FP64 (64
+// banks of 4 bytes, 32 lanes of 8-byte reads per cycle): an A operand reads
+// rows i = 0..15 at column offsets c = 0..1, a B operand rows c at columns
+// j = 0..15.  Row stride 66 (2 mod 32) and 34 spread A reads over all banks,
+// 80 (16 mod 32) does the same for B reads, 18 for the saved 16 x 16 blocks.
+constexpr int LDT = NB + 2;
+constexpr int SA_LD = KT + 2, SB_LD = NB + 16, TMP_LD = 18;
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -117,37 +125,41 @@ template <typename T, typename F> __device__ __forceinline__ void acc_each(F f)
                 f(x, y, v, acc_row(wave) + 16 * x + mfma<T>::row(lane, v), acc_col(wave) + 16 * y + (lane & 15));
 }
 
-// acc += P Q (64 x 64 x 64, 256 threads).  P and Q are read through loaders in
-// chunks of KT columns / rows, all loads of a chunk issued before its LDS
-// stores, then the four waves run their 2 x 2 MFMA tiles from LDS.
+// acc += P Q (64 x 64 x 64, 256 threads).  P and Q are read through loaders,
+// every load of both K-chunks issued before the first LDS store (one memory
+// round trip per product), then per chunk of KT the four waves run their
+// 2 x 2 MFMA tiles from LDS.
 template <typename T, typename LP, typename LQ>
 __device__ __forceinline__ void tile_mm(Acc<T> &acc, T *sa, T *sb, LP loadP, LQ loadQ)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = acc_row(wave), wc = acc_col(wave);
-    for (int kc = 0; kc < NB; kc += KT) {
-        constexpr int PER = NB * KT / 256;
-        T pv[PER], qv[PER];
+    constexpr int PER = NB * KT / 256, NCH = NB / KT;
+    T pv[NCH][PER], qv[NCH][PER];
+#pragma unroll
+    for (int c = 0; c < NCH; c++)
 #pragma unroll
         for (int r = 0; r < PER; r++) {
             const int t = tid + 256 * r;
-            pv[r] = loadP(t / KT, kc + t % KT);
-            qv[r] = loadQ(kc + t / NB, t % NB);
+            pv[c][r] = loadP(t / KT, c * KT + t % KT);
+            qv[c][r] = loadQ(c * KT + t / NB, t % NB);
         }
 #pragma unroll
+    for (int c = 0; c < NCH; c++) {
+#pragma unroll
         for (int r = 0; r < PER; r++) {
             const int t = tid + 256 * r;
-            sa[(t / KT) * (KT + 1) + t % KT] = pv[r];
-            sb[(t / NB) * NB + t % NB] = qv[r];
+            sa[(t / KT) * SA_LD + t % KT] = pv[c][r];
+            sb[(t / NB) * SB_LD + t % NB] = qv[c][r];
         }
         __syncthreads();
 #pragma unroll
         for (int kk = 0; kk < KT; kk += 4) {
             T av[2], bv[2];
 #pragma unroll
-            for (int x = 0; x < 2; x++) av[x] = sa[(wr + 16 * x + (lane & 15)) * (KT + 1) + kk + (lane >> 4)];
+            for (int x = 0; x < 2; x++) av[x] = sa[(wr + 16 * x + (lane & 15)) * SA_LD + kk + (lane >> 4)];
 #pragma unroll
-            for (int y = 0; y < 2; y++) bv[y] = sb[(kk + (lane >> 4)) * NB + wc + 16 * y + (lane & 15)];
+            for (int y = 0; y < 2; y++) bv[y] = sb[(kk + (lane >> 4)) * SB_LD + wc + 16 * y + (lane & 15)];
 #pragma unroll
             for (int x = 0; x < 2; x++)
 #pragma unroll
@@ -180,84 +192,147 @@ template <typename T> __device__ __forceinline__ T rcp_of(T p);
 template <> __device__ __forceinline__ double rcp_of<double>(double p) { return pivot_rcp(p); }
 template <> __device__ __forceinline__ float rcp_of<float>(float p) { return pivot_rcp(p); }
 
-// In-place Gauss-Jordan inverse of the 64 x 64 matrix s ([64][LDT]) by 256
-// threads: thread t holds row t / 4, columns 16 (t % 4) .. +15 in registers;
-// per pivot the pivot row and column go through LDS (double-buffered, one
-// barrier per pivot).  No interchanges: a pivot smaller than 1e-13 (FP32: 1e-6)
-// of its row's original diagonal entry (of the block's largest entry where
-// that is zero) -- a relative threshold that does not depend on how the rows
-// and columns are scaled -- is replaced by that size with its sign (bit 2 of
-// *flag); a non-finite pivot, an all-zero block, or in FP32 a perturbed pivot
-// sets bit 1.
-template <typename T> __device__ __forceinline__ void gj_inverse64(T *s, T *xbuf /*[2][128] + 72*/, int *flag)
+template <typename T> struct PivotGuard {
+    T rel, huge;
+    __device__ __forceinline__ PivotGuard() : rel(sizeof(T) == 8 ? T(1e-13) : T(1e-6)), huge(sizeof(T) == 8 ? T(1e300) : T(1e37)) {}
+};
+
+// One wavefront: in-place Gauss-Jordan inverse (no interchanges) of the 16 x 16
+// block at b (stride LDT).  Lane l holds row l / 4, columns 4 (l % 4) .. +3; per
+// pivot the pivot row, the column entry of the lane's row and the pivot itself
+// come from other lanes by shuffles, so the 16 pivots need no barrier.  eps[k]:
+// the perturbation size of pivot k.
+template <typename T>
+__device__ __forceinline__ void inv16(T *b, const T (&eps)[16], T huge, bool &bad, bool &pert)
 {
-    const int tid = threadIdx.x, r = tid >> 2, q = tid & 3, c0 = 16 * q;
-    T a[16];
+    const int lane = threadIdx.x & 63, i = lane >> 2, q = lane & 3;
+    T a[4];
 #pragma unroll
-    for (int j = 0; j < 16; j++) a[j] = s[r * LDT + c0 + j];
-    // largest magnitude in the block, and the original diagonal
-    T mx = T(0);
+    for (int e = 0; e < 4; e++) a[e] = b[i * LDT + 4 * q + e];
 #pragma unroll
-    for (int j = 0; j < 16; j++) mx = fmax(mx, fabs(a[j]));
+    for (int k = 0; k < 16; k++) {
+        T rk[4];
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) mx = fmax(mx, __shfl_xor(mx, o));
-    T *red = xbuf + 256, *diag0 = xbuf + 264;
-    if ((tid & 63) == 0) red[tid >> 6] = mx;
-    if (q == (r >> 4)) diag0[r] = fabs(s[r * LDT + r]);
-    __syncthreads();
-    mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-    const T rel = sizeof(T) == 8 ? T(1e-13) : T(1e-6);
-    const T huge = sizeof(T) == 8 ? T(1e300) : T(1e37);
-    bool bad = !(mx > T(0)) || !(mx < huge), pert = false;
-#pragma unroll
-    for (int k = 0; k < NB; k++) {
-        T *rowb = xbuf + (k & 1) * 128, *colb = rowb + 64;
-        if (r == k) {
-#pragma unroll
-            for (int j = 0; j < 16; j++) rowb[c0 + j] = a[j];
-        }
-        if (q == (k >> 4)) colb[r] = a[k & 15];
-        __syncthreads();
-        T p = rowb[k];
-        const T ap = fabs(p), dk = diag0[k];
-        const T eps = rel * (dk > T(0) ? dk : mx);
+        for (int e = 0; e < 4; e++) rk[e] = __shfl(a[e], 4 * k + q);
+        const T f = __shfl(a[k & 3], 4 * i + (k >> 2));
+        T p = __shfl(a[k & 3], 4 * k + (k >> 2));
+        const T ap = fabs(p);
         const bool nonfinite = !(ap < huge);
+        const bool small = !nonfinite && ap < eps[k];
         bad |= nonfinite;
-        const bool small = !nonfinite && ap < eps;
         pert |= small;
-        p = nonfinite ? T(1) : small ? (p >= T(0) ? eps : -eps) : p;
+        p = nonfinite ? T(1) : small ? (p >= T(0) ? eps[k] : -eps[k]) : p;
         const T ip = rcp_of<T>(p);
-        const T f = colb[r];
-        T rk[16];
+        const bool isk = i == k;
 #pragma unroll
-        for (int j = 0; j < 16; j++) rk[j] = rowb[c0 + j] * ip;
-        const bool isk = r == k, colk = q == (k >> 4);
-#pragma unroll
-        for (int j = 0; j < 16; j++) {
-            const T upd = isk ? rk[j] : a[j] - f * rk[j];
-            a[j] = (colk && j == (k & 15)) ? (isk ? ip : -f * ip) : upd;
+        for (int e = 0; e < 4; e++) {
+            const T r = rk[e] * ip;
+            const T upd = isk ? r : a[e] - f * r;
+            a[e] = (4 * q + e == k) ? (isk ? ip : -f * ip) : upd;
         }
     }
 #pragma unroll
-    for (int j = 0; j < 16; j++) s[r * LDT + c0 + j] = a[j];
+    for (int e = 0; e < 4; e++) b[i * LDT + 4 * q + e] = a[e];
+}
+
+// acc = P Q for 16 x 16 blocks in LDS (strides lp, lq); one wavefront.
+template <typename T>
+__device__ __forceinline__ typename mfma<T>::acc_t mm16(const T *P, int lp, const T *Q, int lq)
+{
+    const int lane = threadIdx.x & 63;
+    typename mfma<T>::acc_t acc{};
+#pragma unroll
+    for (int k0 = 0; k0 < 16; k0 += 4)
+        acc = mfma<T>::run(P[(lane & 15) * lp + k0 + (lane >> 4)], Q[(k0 + (lane >> 4)) * lq + (lane & 15)], acc);
+    return acc;
+}
+
+// In-place inverse of the 64 x 64 matrix M ([64][LDT]) by 256 threads: block
+// Gauss-Jordan over 16 x 16 blocks.  For each diagonal block kb: wave 0 inverts
+// the pivot block P (inv16) while waves 1-3 save the column blocks C_i; waves
+// 0-2 form the row blocks R_j = P^-1 M(kb, j); then the four waves update
+// M(i, j) -= C_i R_j and set M(i, kb) = -C_i P^-1 (MFMA 16x16x4), three
+// barriers per block.  No interchanges: a pivot smaller than 1e-13 (FP32:
+// 1e-6) of its row's original diagonal entry (of the block's largest entry
+// where that is zero) -- a threshold that does not depend on how rows and
+// columns are scaled -- is replaced by that size with its sign (bit 2 of
+// *flag); a non-finite pivot, an all-zero block, or in FP32 a perturbed pivot
+// sets bit 1.  tmp: 3 x 16 x TMP_LD entries; aux: 4 + 64 entries.
+template <typename T> __device__ __forceinline__ void inverse64(T *M, T *tmp, T *aux, int *flag)
+{
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const PivotGuard<T> pg;
+    // largest magnitude in the block, and the original diagonal
+    {
+        const int r = tid >> 2, c0 = 16 * (tid & 3);
+        T mx = T(0);
+#pragma unroll
+        for (int j = 0; j < 16; j++) mx = fmax(mx, fabs(M[r * LDT + c0 + j]));
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) mx = fmax(mx, __shfl_xor(mx, o));
+        if (lane == 0) aux[wave] = mx;
+        if (tid < NB) aux[4 + tid] = fabs(M[tid * LDT + tid]);
+    }
+    __syncthreads();
+    const T mx = fmax(fmax(aux[0], aux[1]), fmax(aux[2], aux[3]));
+    bool bad = !(mx > T(0)) || !(mx < pg.huge), pert = false;
+    for (int kb = 0; kb < 4; kb++) {
+        T *P = M + 16 * kb * LDT + 16 * kb;
+        if (wave == 0) {
+            T eps[16];
+#pragma unroll
+            for (int k = 0; k < 16; k++) {
+                const T dk = aux[4 + 16 * kb + k];
+                eps[k] = pg.rel * (dk > T(0) ? dk : mx);
+            }
+            inv16<T>(P, eps, pg.huge, bad, pert);
+        } else {
+            const int ri = wave - 1, i = ri < kb ? ri : ri + 1;
+            const T *src = M + 16 * i * LDT + 16 * kb;
+            for (int t = lane; t < 256; t += 64) tmp[ri * 16 * TMP_LD + (t >> 4) * TMP_LD + (t & 15)] = src[(t >> 4) * LDT + (t & 15)];
+        }
+        __syncthreads();
+        if (wave < 3) {
+            T *R = M + 16 * kb * LDT + 16 * (wave < kb ? wave : wave + 1);
+            const typename mfma<T>::acc_t acc = mm16<T>(P, LDT, R, LDT);
+#pragma unroll
+            for (int v = 0; v < 4; v++) R[mfma<T>::row(lane, v) * LDT + (lane & 15)] = acc[v];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < 3; s++) {
+            const int u = wave + 4 * s, ri = u >> 2, cj = u & 3;
+            const int i = ri < kb ? ri : ri + 1;
+            T *Mij = M + 16 * i * LDT + 16 * cj;
+            const T *Ci = tmp + ri * 16 * TMP_LD;
+            const bool piv = cj == kb;
+            const typename mfma<T>::acc_t acc = mm16<T>(Ci, TMP_LD, piv ? P : M + 16 * kb * LDT + 16 * cj, LDT);
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                T &e = Mij[mfma<T>::row(lane, v) * LDT + (lane & 15)];
+                e = piv ? -acc[v] : e - acc[v];
+            }
+        }
+        __syncthreads();
+    }
     if (tid == 0) {
         const int bits = (bad || (sizeof(T) == 4 && pert) ? 1 : 0) | (pert ? 2 : 0);
         if (bits) atomicOr(flag, bits);
     }
-    __syncthreads();
 }
 
 // LDS of the step and batch-inverse kernels: the staging of tile_mm, reused
 // as X (a [64][LDT] operand) once the products that stage through it are done.
 template <typename T> struct StepSmem {
     union {
-        T stage[NB * (KT + 1) + KT * NB];
+        T stage[NB * SA_LD + KT * SB_LD];
         T X[NB * LDT];
     };
     T D[NB * LDT];
-    T xbuf[2 * 128 + 8 + NB];
+    T tmp[3 * 16 * TMP_LD];
+    T aux[4 + NB];
 };
-static_assert(NB * (KT + 1) + KT * NB == NB * LDT, "staging and X share the LDS");
+static_assert(NB * SA_LD + KT * SB_LD >= NB * LDT, "X fits in the staging area");
 
 // Batch store: per batch b, a BW x BW row-major matrix; block (r, c) holds G(r, c)
 // for c <= r (G = L_bb^-1, E on the diagonal) and H(r, c) for c > r (H = U_bb^-1).
@@ -266,20 +341,44 @@ template <typename T> __device__ __forceinline__ T *batch_blk(T *gh, int b, int 
     return gh + (size_t)b * BW * BW + (size_t)(r * NB) * BW + c * NB;
 }
 
+// Tile of workgroup b in the launch of block column p: the tiles are the S row
+// p+1 (nS of them, columns p+1 ..) and the T rows p+2 .. p+1+nr (nc each,
+// columns p+1 ..).  Workgroups are dispatched round-robin over the 8 XCDs
+// (b mod 8), and row i is always given to XCD i mod 8, so a band row's tiles
+// are updated by the same XCD at every step and stay in its L2 (a row of the
+// update window is 22 tiles, ~0.7 MB in FP64; an XCD holds about 3 rows of the
+// window at a time).  Within an XCD the rows are taken in order, the S row
+// first.  Returns false for a workgroup past its XCD's share.
+__device__ __forceinline__ bool step_tile(int b, int p, int nS, int nc, int nr, int &bi, int &bj)
+{
+    const int x = b & 7;
+    int k = b >> 3;
+    for (int i = p + 1 + (((x - (p + 1)) % 8) + 8) % 8; i <= p + 1 + nr; i += 8) {
+        const int cnt = i == p + 1 ? nS : nc;
+        if (k < cnt) {
+            bi = i;
+            bj = p + 1 + k;
+            return true;
+        }
+        k -= cnt;
+    }
+    return false;
+}
+
 // One block column of the factorisation (see the file header).  p = -1: row 0
-// only (no update).  Workgroups [0, nS): S group for row block p+1, workgroup
-// s handles U(p+1, p+1+s) (s = 0: only stores E_{p+1}); the rest: T tiles
-// (i, j), i = p+2 + t / nc, j = p+1 + t % nc.
+// only (no update).  S tiles (p+1, p+1+s): s = 0 only stores E_{p+1}, s > 0
+// forms U(p+1, p+1+s); T tiles (i, j), i >= p+2: the update of step p.
 template <typename T>
-__global__ __launch_bounds__(256) void band_step_kernel(T *ab, Geo g, int p, int nS, int nc, T *gh, int *flag)
+__global__ __launch_bounds__(256) void band_step_kernel(T *ab, Geo g, int p, int nS, int nc, int nr, T *gh, int *flag)
 {
     __shared__ StepSmem<T> sm;
-    T *sa = sm.stage, *sb = sm.stage + NB * (KT + 1);
+    T *sa = sm.stage, *sb = sm.stage + NB * SA_LD;
     const int tid = threadIdx.x;
-    const int bid = blockIdx.x;
-    if (bid >= nS) {
+    int ti, tj;
+    if (!step_tile(blockIdx.x, p, nS, nc, nr, ti, tj)) return;
+    if (ti > p + 1) {
         // ---- T: A(i, j) -= A(i, p) U(p, j)
-        const int t = bid - nS, bi = p + 2 + t / nc, bj = p + 1 + t % nc;
+        const int bi = ti, bj = tj;
         const int R0 = bi * NB, C0 = bj * NB, K0 = p * NB;
         T old[2][2][4];
         acc_each<T>([&](int x, int y, int v, int i, int j) { old[x][y][v] = band_ld(ab, R0 + i, C0 + j, g); });
@@ -293,7 +392,7 @@ __global__ __launch_bounds__(256) void band_step_kernel(T *ab, Geo g, int p, int
         return;
     }
     // ---- S: row block p+1
-    const int s = bid, bi = p + 1, bj = p + 1 + s;
+    const int bi = p + 1, bj = tj, s = tj - (p + 1);
     const int R0 = bi * NB, C0 = bj * NB, K0 = p * NB;
     const bool upd = p >= 0;
     // D = A(p+1, p+1) - A(p+1, p) U(p, p+1)
@@ -319,7 +418,7 @@ __global__ __launch_bounds__(256) void band_step_kernel(T *ab, Geo g, int p, int
         acc_each<T>([&](int x, int y, int v, int i, int j) { sm.X[i * LDT + j] = old[x][y][v] - acc.a[x][y][v]; });
     }
     __syncthreads();
-    gj_inverse64<T>(sm.D, sm.xbuf, flag);  // D := E_{p+1}
+    inverse64<T>(sm.D, sm.tmp, sm.aux, flag);  // D := E_{p+1}
     if (s == 0) {
         T *E = batch_blk(gh, bi / SB, bi % SB, bi % SB);
         for (int t = tid; t < NB * NB; t += 256) E[(t / NB) * BW + t % NB] = sm.D[(t / NB) * LDT + t % NB];
@@ -339,7 +438,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void band_batch_inv_kernel(const T *ab, Geo g, int d, T *gh)
 {
     __shared__ StepSmem<T> sm;
-    T *sa = sm.stage, *sb = sm.stage + NB * (KT + 1);
+    T *sa = sm.stage, *sb = sm.stage + NB * SA_LD;
     const int per = SB - d;
     const int w = blockIdx.x % (2 * per), b = blockIdx.x / (2 * per);
     const int kind = w / per, c = w % per;
@@ -398,46 +497,57 @@ __global__ __launch_bounds__(256) void band_batch_apply_kernel(const T *gh, Geo 
 {
     __shared__ V f[BW];
     const int kc = min(SB, g.nbk - b * SB), B0 = b * BW;
-    for (int t = threadIdx.x; t < kc * NB; t += 256) f[t] = rhs[B0 + t];
-    __syncthreads();
+    // grid = kc * NB / 16 workgroups: row < kc * NB; the 16 rows of a workgroup
+    // lie in one block row, so the block-column range is uniform
     const int row = blockIdx.x * 16 + (threadIdx.x >> 4), l = threadIdx.x & 15;
-    if (row >= kc * NB) return;
     const int rb = row / NB;
     const T *M = gh + (size_t)b * BW * BW + (size_t)row * BW;
     const int c_lo = FWD ? 0 : rb + 1, c_hi = FWD ? rb + 1 : kc;  // block columns
+    // every matrix load of the row issued before the right-hand side is staged
+    T m[SB][4];
+#pragma unroll
+    for (int cb = 0; cb < SB; cb++)
+        if (cb >= c_lo && cb < c_hi)
+#pragma unroll
+            for (int e = 0; e < 4; e++) m[cb][e] = M[cb * NB + 4 * l + e];
+    for (int t = threadIdx.x; t < kc * NB; t += 256) f[t] = rhs[B0 + t];
+    __syncthreads();
     V s = V(0);
-    for (int cb = c_lo; cb < c_hi; cb++) {
-        const int c = cb * NB + 4 * l;
-        T m4[4];
 #pragma unroll
-        for (int e = 0; e < 4; e++) m4[e] = M[c + e];
+    for (int cb = 0; cb < SB; cb++)
+        if (cb >= c_lo && cb < c_hi)
 #pragma unroll
-        for (int e = 0; e < 4; e++) s += (V)m4[e] * f[c + e];
-    }
+            for (int e = 0; e < 4; e++) s += (V)m[cb][e] * f[cb * NB + 4 * l + e];
     s = sum16(s);
     if (l == 0) out[B0 + row] = FWD ? s : f[row] + s;
 }
 
 // FWD: f_q -= L(q, batch) y_batch for the block rows q after batch b within
 // reach; BWD: g_q -= U(q, batch) x_batch for the block rows before it.
-// Workgroup w: rows 16 w of the target range.
+// Workgroups of CPL_ROWS rows (16 threads each): the target strip is short
+// (the reach past one batch), so small workgroups put it on more CUs.
+constexpr int CPL_ROWS = 4;
 template <typename T, typename V, bool FWD>
-__global__ __launch_bounds__(256) void band_couple_kernel(const T *ab, Geo g, int b, const V *sol, V *f, int q0)
+__global__ __launch_bounds__(16 * CPL_ROWS) void band_couple_kernel(const T *ab, Geo g, int b, const V *sol, V *f, int q0)
 {
     __shared__ V y[BW];
     const int kc = min(SB, g.nbk - b * SB), B0 = b * BW;
-    for (int t = threadIdx.x; t < kc * NB; t += 256) y[t] = sol[B0 + t];
+    const int row = q0 * NB + blockIdx.x * CPL_ROWS + (threadIdx.x >> 4), l = threadIdx.x & 15;
+    // every coupling load issued before the solution segment is staged
+    T m[SB][4];
+#pragma unroll
+    for (int cb = 0; cb < SB; cb++)
+        if (cb < kc)
+#pragma unroll
+            for (int e = 0; e < 4; e++) m[cb][e] = band_ld(ab, row, B0 + cb * NB + 4 * l + e, g);
+    for (int t = threadIdx.x; t < kc * NB; t += 16 * CPL_ROWS) y[t] = sol[B0 + t];
     __syncthreads();
-    const int row = q0 * NB + blockIdx.x * 16 + (threadIdx.x >> 4), l = threadIdx.x & 15;
     V s = V(0);
-    for (int cb = 0; cb < kc; cb++) {
-        const int c = B0 + cb * NB + 4 * l;
-        T m4[4];
 #pragma unroll
-        for (int e = 0; e < 4; e++) m4[e] = band_ld(ab, row, c + e, g);
+    for (int cb = 0; cb < SB; cb++)
+        if (cb < kc)
 #pragma unroll
-        for (int e = 0; e < 4; e++) s += (V)m4[e] * y[c - B0 + e];
-    }
+            for (int e = 0; e < 4; e++) s += (V)m[cb][e] * y[cb * NB + 4 * l + e];
     s = sum16(s);
     if (l == 0 && row < g.n) f[row] -= s;
 }
@@ -468,9 +578,13 @@ template <typename T> static int band_factor_t(T *ab, int n, int hb, hipStream_t
         const int nS = std::min(last, p + 1 + g.R) - p;
         const int nr = p >= 0 ? std::min(last, p + g.R) - (p + 2) + 1 : 0;
         const int nc = std::min(last, p + g.R) - p;
-        const int grid = nS + std::max(0, nr) * std::max(0, nc);
+        // tiles per XCD (row i on XCD i mod 8): the grid covers the largest share
+        int share[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int i = p + 1; i <= p + 1 + std::max(0, nr); i++) share[i & 7] += i == p + 1 ? nS : nc;
+        const int grid = 8 * *std::max_element(share, share + 8);
         const int pr = gf_probe_begin(s, GF_KC_BAND_UPDATE);
-        hipLaunchKernelGGL(band_step_kernel<T>, dim3(grid), dim3(256), 0, s, ab, g, p, nS, std::max(1, nc), gh, flag);
+        hipLaunchKernelGGL(band_step_kernel<T>, dim3(grid), dim3(256), 0, s, ab, g, p, nS, nc, std::max(0, nr), gh,
+                           flag);
         gf_probe_end(s, pr, NB);
         launches++;
     }
@@ -507,7 +621,7 @@ static int band_solve_t(const T *ab, int n, int hb, hipStream_t s, V *x, V *z, c
         launches++;
         if (q1 >= q0) {
             pr = gf_probe_begin(s, GF_KC_BAND_SWEEP);
-            hipLaunchKernelGGL((band_couple_kernel<T, V, true>), dim3((q1 - q0 + 1) * NB / 16), dim3(256), 0, s, ab, g,
+            hipLaunchKernelGGL((band_couple_kernel<T, V, true>), dim3((q1 - q0 + 1) * NB / CPL_ROWS), dim3(16 * CPL_ROWS), 0, s, ab, g,
                                b, (const V *)z, x, q0);
             gf_probe_end(s, pr, 0.0);  // rows are counted on the apply launches
             launches++;
@@ -524,7 +638,7 @@ static int band_solve_t(const T *ab, int n, int hb, hipStream_t s, V *x, V *z, c
         launches++;
         if (q1 >= q0) {
             pr = gf_probe_begin(s, GF_KC_BAND_SWEEP);
-            hipLaunchKernelGGL((band_couple_kernel<T, V, false>), dim3((q1 - q0 + 1) * NB / 16), dim3(256), 0, s, ab,
+            hipLaunchKernelGGL((band_couple_kernel<T, V, false>), dim3((q1 - q0 + 1) * NB / CPL_ROWS), dim3(16 * CPL_ROWS), 0, s, ab,
                                g, b, (const V *)x, z, q0);
             gf_probe_end(s, pr, 0.0);  // rows are counted on the apply launches
             launches++;
