@@ -94,13 +94,15 @@ template <int MODE>
 __global__ __launch_bounds__(WG) void beam_assemble_kernel(
     int e0, int e1, size_t ld, const int *__restrict__ conn, const double *__restrict__ cst,
     const double *__restrict__ gps, const double *__restrict__ disp, const int *__restrict__ dof,
-    gf_gmat gm, double *__restrict__ r, double lf, double gx, double gy, double gz,
+    gf_gmat gm, double *__restrict__ r, const double *__restrict__ lfp, double gx, double gy, double gz,
     double *__restrict__ Kout, double *__restrict__ fout, const double *__restrict__ buoy,
     const double *__restrict__ copy, double sx, double sy, double sz)
 {
     __shared__ double sm[EPB * 2 * LDS_GP];
     const int tid = threadIdx.x;
     const int ebase = e0 + blockIdx.x * EPB;
+    // gravity ramp factor, device-resident so a captured Newton step reads the current one
+    const double lf = lfp ? *lfp : 0.0;
 
     // ---- phase 1: per (element, Gauss point) -------------------------------
     if (tid < 2 * EPB) {
@@ -433,7 +435,7 @@ void gf_beam_host_const(const double *x, const double *cs, const double *prop, i
     }
 }
 
-int gf_beam_assemble(gf_model *m, double lf)
+int gf_beam_assemble(gf_model *m, const double *lfp)
 {
     const size_t ld = (size_t)m->nb;
     int ncol = (int)m->beam_color_off.size() - 1;
@@ -445,7 +447,7 @@ int gf_beam_assemble(gf_model *m, double lf)
         int grid = (e1 - e0 + EPB - 1) / EPB;
         const int pr = gf_probe_begin(m->stream, GF_KC_BEAM_ASSEMBLE);
         hipLaunchKernelGGL(beam_assemble_kernel<0>, dim3(grid), dim3(WG), 0, m->stream, e0, e1, ld, m->d_bconn,
-                           m->d_bconst, m->d_bgp, m->d_disp, m->d_dof, gf_model_gmat(m), m->d_r, lf, m->gravity[0],
+                           m->d_bconst, m->d_bgp, m->d_disp, m->d_dof, gf_model_gmat(m), m->d_r, lfp, m->gravity[0],
                            m->gravity[1], m->gravity[2], (double *)nullptr, (double *)nullptr,
                            (const double *)m->d_bbuoy, (const double *)m->d_copy, m->ocean_surface[0],
                            m->ocean_surface[1], m->ocean_surface[2]);
@@ -508,7 +510,8 @@ int gf_beam_eval_device(int count, const double *x, const double *cs, const doub
     int grid = (count + EPB - 1) / EPB;
     gf_gmat none = {};
     hipLaunchKernelGGL(beam_assemble_kernel<1>, dim3(grid), dim3(WG), 0, s, 0, count, ld, d_conn, d_cst, d_gps,
-                       d_disp, (const int *)nullptr, none, (double *)nullptr, 0.0, 0.0, 0.0, 0.0, d_K, d_f,
+                       d_disp, (const int *)nullptr, none, (double *)nullptr, (const double *)nullptr, 0.0, 0.0, 0.0, d_K,
+                       d_f,
                        (const double *)nullptr, (const double *)nullptr, 0.0, 0.0, 0.0);
     GF_HIP(hipGetLastError());
     GF_HIP(hipMemcpyAsync(K_out, d_K, sizeof(double) * 324 * ld, hipMemcpyDeviceToHost, s));

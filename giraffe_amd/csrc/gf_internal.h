@@ -276,7 +276,11 @@ inline void gf_probe_end(hipStream_t s, int slot, double units)
 
 // scalar block of a model (doubles): increment norms, failure flag, residual
 // norms, refinement norms
-enum { GF_SCAL_INC = 0, GF_SCAL_FLAG = 8, GF_SCAL_RES = 16, GF_SCAL_REF = 24, GF_SCAL_N = 32 };
+// refinement norms, and the load time and gravity ramp factor of the current
+// step (read by the assembly kernels, so a captured Newton step replays with
+// the step's values)
+enum { GF_SCAL_INC = 0, GF_SCAL_FLAG = 8, GF_SCAL_RES = 16, GF_SCAL_REF = 24, GF_SCAL_TIME = 28, GF_SCAL_LF = 29,
+       GF_SCAL_N = 32 };
 
 struct gf_model {
     int device = 0;
@@ -360,6 +364,10 @@ struct gf_model {
     bool ev_pending[GF_NPHASE] = {};
     long long pending_launches[GF_NPHASE] = {};
     int pending_info_fail = 0;
+    // asynchronous refinement of the current tangent solve (mixed precision)
+    bool refining = false, refine_then_assemble = false;
+    int refine_it = 0;
+    double refine_prev = 0.0;
     int npad = 0;                   // padded system size
     double *d_scal = nullptr;       // small reduction results
     double *h_scal = nullptr;       // pinned mirror
@@ -379,6 +387,8 @@ struct gf_model {
     // at model creation switches this off)
     bool use_graph = true;
     hipGraphExec_t solve_graph = nullptr;
+    // whole FP64 Newton iteration (solve, update, next assembly, scalar copy)
+    hipGraphExec_t iter_graph = nullptr;
     int graph_launches = 0;
 };
 
@@ -389,11 +399,13 @@ int gf_launch_fill(hipStream_t s, double *p, size_t n, double v);
 int gf_launch_clear3(hipStream_t s, double *a, size_t na, double *b, size_t nb, double *c, size_t nc,
                      size_t dfirst = 0, size_t dstride = 0, size_t dcount = 0);
 int gf_launch_fill_bytes(hipStream_t s, void *p, size_t bytes);
+// p[0] = a, p[1] = b (a kernel, so it can sit in a captured graph)
+int gf_launch_set2(hipStream_t s, double *p, double a, double b);
 // Returns the number of kernel launches issued (>= 0) or a negative status.
-int gf_beam_assemble(gf_model *m, double lf);
+int gf_beam_assemble(gf_model *m, const double *lfp);
 int gf_beam_commit(gf_model *m);
-int gf_shell_assemble(gf_model *m, double lf);
-int gf_shell_pressure(gf_model *m, double t);
+int gf_shell_assemble(gf_model *m, const double *lfp);
+int gf_shell_pressure(gf_model *m, const double *tp);
 int gf_shell_eval_device(int count, const double *x, const double *prop, const double *d, const double *gp,
                          double *K_out, double *f_out);
 int gf_shell_commit(gf_model *m);
@@ -405,7 +417,7 @@ int gf_shell_results(gf_model *m, double *d_out);
 void gf_shell_host_const(const double *x18, const double *prop, double *rec /*[SC_COUNT]*/,
                          double *prec /*[SP_COUNT]*/, double *gp /*[SG_COUNT]*/);
 gf_gmat gf_model_gmat(const gf_model *m);
-int gf_nodal_loads(gf_model *m, double t);
+int gf_nodal_loads(gf_model *m, const double *tp);
 int gf_scan_residual(gf_model *m);
 int gf_update_disp(gf_model *m, bool zero_norms);
 int gf_node_commit(gf_model *m);
@@ -430,7 +442,7 @@ int gf_commit_enqueue(gf_model *m);
 
 // The Static loop as a resumable state machine (gf_static.cpp).
 struct gf_static_run {
-    enum Phase { PH_STEP, PH_ASSEMBLED, PH_SOLVED, PH_ITER_END, PH_DONE };
+    enum Phase { PH_STEP, PH_ASSEMBLED, PH_REFINE, PH_SOLVED, PH_ITER_END, PH_DONE };
     gf_model *m = nullptr;
     gf_static_params p{};
     gf_static_report r{};
@@ -453,7 +465,13 @@ struct gf_static_run {
 int gf_mixed_alloc(gf_model *m);
 void gf_mixed_free(gf_model *m);
 int gf_mixed_factor(gf_model *m);
-int gf_mixed_solve(gf_model *m, int *fallback, int *sweeps);
+int gf_mixed_refine_begin(gf_model *m);
+int gf_mixed_sweep(gf_model *m);      // enqueue one refinement sweep (norms copied to h_scal)
+int gf_mixed_fallback(gf_model *m);   // enqueue the FP64 factorisation + solve
+// Mixed-precision refinement in progress (gf_solve_update_enqueue started it):
+// after the last enqueued sweep's event, decides and enqueues the next step.
+// *more = 1: another sweep is enqueued; 0: the update (and assembly) are.
+int gf_refine_step(gf_model *m, int *more);
 
 int gf_band_pad(int n);
 size_t gf_band_inv_elems(int n);   // per-panel triangular inverses of a factor of order n

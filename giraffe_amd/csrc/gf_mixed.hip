@@ -1,5 +1,8 @@
 // Mixed-precision tangent solve: FP32 factor of the equilibrated tangent,
-// FP64 iterative refinement against the FP64 tangent.
+// FP64 iterative refinement against the FP64 tangent.  The refinement is a
+// sequence of enqueued sweeps; the host reads each sweep's norms when its
+// event has completed and enqueues the next sweep, the update, or the FP64
+// fallback (gf_refine_step), so a server's host thread never waits on it.
 //
 //   D = diag(|K_ii|)^-1/2,  K32 = fl32(D K D)  (factored in FP32)
 //   x = 0, s = r
@@ -176,64 +179,54 @@ int gf_mixed_factor(gf_model *m)
     return l < 0 ? l : l + 2;
 }
 
-// Refined solve of K x = r (r = d_r, x = d_x).  Returns launches, or a negative
-// status; *fallback = 1 when the FP32 factor did not converge and the FP64
-// factorisation was used instead.
-int gf_mixed_solve(gf_model *m, int *fallback, int *sweeps)
+// Refinement of K x = r (r = d_r, x = d_x), enqueue-only pieces (the caller
+// decides between sweeps from the norms each sweep copies back, without
+// blocking the host: gf_refine_step).
+
+// x = 0, s = r.
+int gf_mixed_refine_begin(gf_model *m)
 {
-    int npad = m->npad, launches = 0;
+    GF_HIP(hipMemcpyAsync(m->d_res, m->d_r, sizeof(double) * m->npad, hipMemcpyDeviceToDevice, m->stream));
+    return gf_launch_fill(m->stream, m->d_x, (size_t)m->npad, 0.0);
+}
+
+// One sweep: w = K32^-1 (D s), x += D w, s = r - K x; max|D w| and max|x| to
+// d_scal[GF_SCAL_REF..+1] and, with the factorisation flag, to the host mirror.
+// Returns launches or a negative status.
+int gf_mixed_sweep(gf_model *m)
+{
+    const int npad = m->npad;
     gf_gmat g = gf_model_gmat(m);
-    *fallback = 0;
-    *sweeps = 0;
-    GF_HIP(hipMemcpyAsync(m->d_res, m->d_r, sizeof(double) * npad, hipMemcpyDeviceToDevice, m->stream));
-    int st = gf_launch_fill(m->stream, m->d_x, (size_t)npad, 0.0);
-    if (st) return st;
-    double prev = 1e308;
-    bool ok = false;
-    for (int it = 0; it < std::max(1, m->refine_max); it++) {
-        hipLaunchKernelGGL(scale_kernel, dim3((npad + 255) / 256), dim3(256), 0, m->stream, npad, m->d_scale, m->d_res,
-                           m->d_w);
-        // BTD: solution in d_swork (d_w is reduced in place); band: in place in d_w
-        int l = m->layout == GM_BTD ? gf_btd_solve(m->btd32, m->stream, m->d_w, m->d_swork, npad, m->d_w)
-                                    : gf_band_solve(m->d_band32, 4, npad, m->half_band, m->stream, m->d_w, m->d_swork,
-                                                    m->d_band32_inv);
-        if (l < 0) return l;
-        const double *sol = m->layout == GM_BTD ? m->d_swork : m->d_w;
-        st = gf_launch_fill_bytes(m->stream, m->d_scal + GF_SCAL_REF, 2 * sizeof(double));
-        if (st) return st;
-        hipLaunchKernelGGL(correct_kernel, dim3(grid_for(npad, 256, 512)), dim3(256), 0, m->stream, npad, m->d_scale,
-                           sol, m->d_x, (unsigned long long *)(m->d_scal + GF_SCAL_REF));
-        hipLaunchKernelGGL(residual_kernel, dim3((npad * 64 + 255) / 256), dim3(256), 0, m->stream, g, npad, m->d_r,
-                           m->d_x, m->d_res);
-        GF_HIP(hipGetLastError());
-        launches += l + 3;
-        GF_HIP(hipMemcpyAsync(m->h_scal + GF_SCAL_REF, m->d_scal + GF_SCAL_REF, 2 * sizeof(double), hipMemcpyDeviceToHost, m->stream));
-        GF_HIP(hipStreamSynchronize(m->stream));
-        double dn, xn;
-        memcpy(&dn, m->h_scal + GF_SCAL_REF, 8);
-        memcpy(&xn, m->h_scal + GF_SCAL_REF + 1, 8);
-        (*sweeps)++;
-        if (!(dn == dn) || !(xn == xn)) break;
-        if (dn <= m->refine_tol * xn) { ok = true; break; }
-        if (it > 0 && dn > 0.5 * prev) {
-            // stagnation at the FP64 rounding floor counts as converged once the
-            // correction is already tiny; otherwise the FP32 factor is too poor
-            if (dn <= 1e-10 * xn) ok = true;
-            break;
-        }
-        prev = dn;
-    }
-    if (ok) return launches;
-    // fall back to the FP64 factorisation of the FP64 tangent
-    *fallback = 1;
-    if (m->layout == GM_BTD) {
-        int l = gf_btd_factor_solve(m->btd, m->stream, m->d_flag, m->d_r, m->d_x, npad, m->d_res);
-        return l < 0 ? l : launches + l;
-    }
-    int l = gf_band_factor(m->d_band, 8, npad, m->half_band, m->stream, m->d_flag, m->d_band_inv);
+    hipLaunchKernelGGL(scale_kernel, dim3((npad + 255) / 256), dim3(256), 0, m->stream, npad, m->d_scale, m->d_res,
+                       m->d_w);
+    // BTD: solution in d_swork (d_w is reduced in place); band: in place in d_w
+    const int l = m->layout == GM_BTD ? gf_btd_solve(m->btd32, m->stream, m->d_w, m->d_swork, npad, m->d_w)
+                                      : gf_band_solve(m->d_band32, 4, npad, m->half_band, m->stream, m->d_w,
+                                                      m->d_swork, m->d_band32_inv);
+    if (l < 0) return l;
+    const double *sol = m->layout == GM_BTD ? m->d_swork : m->d_w;
+    GF_TRY(gf_launch_fill_bytes(m->stream, m->d_scal + GF_SCAL_REF, 2 * sizeof(double)));
+    hipLaunchKernelGGL(correct_kernel, dim3(grid_for(npad, 256, 512)), dim3(256), 0, m->stream, npad, m->d_scale, sol,
+                       m->d_x, (unsigned long long *)(m->d_scal + GF_SCAL_REF));
+    hipLaunchKernelGGL(residual_kernel, dim3((npad * 64 + 255) / 256), dim3(256), 0, m->stream, g, npad, m->d_r,
+                       m->d_x, m->d_res);
+    GF_HIP(hipGetLastError());
+    GF_HIP(hipMemcpyAsync(m->h_scal + GF_SCAL_FLAG, m->d_scal + GF_SCAL_FLAG, sizeof(double), hipMemcpyDeviceToHost,
+                          m->stream));
+    GF_HIP(hipMemcpyAsync(m->h_scal + GF_SCAL_REF, m->d_scal + GF_SCAL_REF, 2 * sizeof(double),
+                          hipMemcpyDeviceToHost, m->stream));
+    return l + 3;
+}
+
+// The FP64 factorisation and solve of the FP64 tangent (x = d_x).
+int gf_mixed_fallback(gf_model *m)
+{
+    const int npad = m->npad;
+    GF_TRY(gf_launch_fill_bytes(m->stream, m->d_flag, 4));
+    if (m->layout == GM_BTD) return gf_btd_factor_solve(m->btd, m->stream, m->d_flag, m->d_r, m->d_x, npad, m->d_res);
+    const int l = gf_band_factor(m->d_band, 8, npad, m->half_band, m->stream, m->d_flag, m->d_band_inv);
     if (l < 0) return l;
     GF_HIP(hipMemcpyAsync(m->d_x, m->d_r, sizeof(double) * npad, hipMemcpyDeviceToDevice, m->stream));
-    int l2 = gf_band_solve(m->d_band, 8, npad, m->half_band, m->stream, m->d_x, m->d_swork, m->d_band_inv);
-    if (l2 < 0) return l2;
-    return launches + l + l2;
+    const int l2 = gf_band_solve(m->d_band, 8, npad, m->half_band, m->stream, m->d_x, m->d_swork, m->d_band_inv);
+    return l2 < 0 ? l2 : l + l2;
 }

@@ -63,18 +63,6 @@ static void tick(gf_model *m, int ph, bool start)
     (void)hipEventRecord(m->ev[2 * ph + (start ? 0 : 1)], m->stream);
     if (!start) m->ev_pending[ph] = true;
 }
-// Call after the stream work of the phase was enqueued; waits for its end event.
-static int tock(gf_model *m, int ph, int launches)
-{
-    if (!m->timing) return GF_OK;
-    GF_HIP(hipEventSynchronize(m->ev[2 * ph + 1]));
-    float ms = 0.f;
-    GF_HIP(hipEventElapsedTime(&ms, m->ev[2 * ph], m->ev[2 * ph + 1]));
-    m->t_ms[ph] += ms;
-    m->launches[ph] += launches;
-    m->ev_pending[ph] = false;
-    return m->kprobe.flush();
-}
 // Accumulates the phases whose end events the stream has passed (enqueue/result pairs).
 static int flush_timing(gf_model *m)
 {
@@ -166,6 +154,7 @@ static void release(gf_model *m)
     // the cache hands freed blocks to the next model: nothing may still run on them
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     if (m->solve_graph) (void)hipGraphExecDestroy(m->solve_graph);
+    if (m->iter_graph) (void)hipGraphExecDestroy(m->iter_graph);
     m->solve_graph = nullptr;
     for (int i = 0; i < 2 * GF_NPHASE; i++)
         if (m->ev[i]) (void)hipEventDestroy(m->ev[i]);
@@ -799,6 +788,16 @@ extern "C" int gf_model_stats(const gf_model *m, long long *out)
     return GF_OK;
 }
 
+// The step's load time t = t_last + dt and gravity ramp factor (the BoolTable
+// ramp over the solution window) into the scalar block, where the assembly
+// kernels read them.
+static int set_step_scalars(gf_model *m)
+{
+    const double t = m->t_last + m->dt;
+    const double lf = m->gravity_on ? (t - m->sol_start) / (m->sol_end - m->sol_start) : 0.0;
+    return gf_launch_set2(m->stream, m->d_scal + GF_SCAL_TIME, t, lf);
+}
+
 extern "C" int gf_begin_step(gf_model *m, double t_last, double dt)
 {
     if (!m) return GF_E_ARG;
@@ -806,6 +805,7 @@ extern "C" int gf_begin_step(gf_model *m, double t_last, double dt)
     m->t_last = t_last;
     m->dt = dt;
     m->factored = false;
+    GF_TRY(set_step_scalars(m));
     return gf_launch_fill(m->stream, m->d_disp, 6 * (size_t)m->nn, 0.0);
 }
 
@@ -815,7 +815,8 @@ extern "C" int gf_set_solution_window(gf_model *m, double start, double end)
     if (!m) return GF_E_ARG;
     m->sol_start = start;
     m->sol_end = end;
-    return GF_OK;
+    GF_HIP(hipSetDevice(m->device));
+    return set_step_scalars(m);
 }
 
 // Enqueues Clear + MountLocal/ElementLoads/Loads + MountGlobal and the residual
@@ -841,15 +842,15 @@ int gf_assemble_enqueue(gf_model *m, bool copy_norms)
         dstride = (size_t)(m->band_ld + 1);
     }
     GF_TRY(gf_launch_clear3(m->stream, m->d_r, npad, tan, ntan, m->d_scal + GF_SCAL_RES, 8, dfirst, dstride, dcount));
-    double t = m->t_last + m->dt;
-    GF_TRY(gf_nodal_loads(m, t));
-    double lf = m->gravity_on ? (t - m->sol_start) / (m->sol_end - m->sol_start) : 0.0;
+    // load time and gravity ramp factor from the scalar block (set by gf_begin_step)
+    const double *tp = m->d_scal + GF_SCAL_TIME, *lfp = m->d_scal + GF_SCAL_LF;
+    GF_TRY(gf_nodal_loads(m, tp));
     tick(m, GF_PH_ASSEMBLE, true);
-    GF_TRYN(nl_beam, gf_beam_assemble(m, lf));
-    GF_TRYN(nl_shell, gf_shell_assemble(m, lf));
+    GF_TRYN(nl_beam, gf_beam_assemble(m, lfp));
+    GF_TRYN(nl_shell, gf_shell_assemble(m, lfp));
     tick(m, GF_PH_ASSEMBLE, false);
     m->pending_launches[GF_PH_ASSEMBLE] += nl_beam + nl_shell;
-    GF_TRY(gf_shell_pressure(m, t));
+    GF_TRY(gf_shell_pressure(m, tp));
     m->factored = false;
     GF_TRY(gf_scan_residual(m));
     if (copy_norms)
@@ -883,39 +884,158 @@ extern "C" int gf_assemble(gf_model *m, double *res_out)
     return GF_OK;
 }
 
-// FP64: enqueues the tangent solve, UpdateDisps and the increment norms
-// (d_scal[0..4], failure flag at d_scal[GF_SCAL_FLAG]).  Mixed precision: the
-// solve runs to completion here (its refinement is host-steered), the update is
-// enqueued after it.  then_assemble: the next iteration's assembly and residual
-// norms follow in the same batch (the Newton loop assembles after every solve
-// that does not end the load step; assembly changes no state, so a batch whose
-// step ends on the increments simply does not use it).  One copy brings the
-// scalar block back.
-int gf_solve_update_enqueue(gf_model *m, bool then_assemble)
+// UpdateDisps, the next iteration's assembly (then_assemble) and one copy of
+// the scalar block, after a solve whose increment is in d_x.
+static int update_tail_enqueue(gf_model *m, bool then_assemble)
 {
-    gf_probe_scope ps(probe_of(m));
-    GF_HIP(hipSetDevice(m->device));
-    if (m->precision == GF_PREC_MIXED && m->fp64_direct_left == 0) {
-        int info_fail = 0;
-        GF_TRY(gf_solve(m, &info_fail));
-        m->pending_info_fail = info_fail;
-        GF_TRY(gf_launch_fill_bytes(m->stream, m->d_scal, GF_SCAL_FLAG * sizeof(double)));
-    } else {
-        // FP64, or a mixed model backed off to FP64: queued without a host wait
-        if (m->precision == GF_PREC_MIXED) {
-            m->fp64_direct_left--;
-            m->n_direct++;
-        }
-        GF_TRY(gf_launch_fill_bytes(m->stream, m->d_scal, (GF_SCAL_FLAG + 1) * sizeof(double)));
-        m->n_solves++;
-        GF_TRY(solve_fp64_enqueue(m));
-        m->pending_info_fail = -1;
-    }
     GF_TRY(gf_update_disp(m, false));
     if (then_assemble) GF_TRY(gf_assemble_enqueue(m, false));
     const int nscal = then_assemble ? GF_SCAL_RES + 3 : GF_SCAL_FLAG + 1;
     GF_HIP(hipMemcpyAsync(m->h_scal, m->d_scal, nscal * sizeof(double), hipMemcpyDeviceToHost, m->stream));
     return GF_OK;
+}
+
+static void note_fallback(gf_model *m, bool fell_back);
+
+// Mixed precision: the FP32 factorisation of the equilibrated tangent and the
+// first refinement sweep (the model is then `refining` until gf_refine_step
+// has enqueued the update).
+static int mixed_start_enqueue(gf_model *m)
+{
+    tick(m, GF_PH_FACTOR, true);
+    GF_TRYN(nl_f32, gf_mixed_factor(m));
+    tick(m, GF_PH_FACTOR, false);
+    m->pending_launches[GF_PH_FACTOR] += nl_f32;
+    GF_TRY(gf_mixed_refine_begin(m));
+    tick(m, GF_PH_SOLVE, true);
+    GF_TRYN(nl_sw, gf_mixed_sweep(m));
+    tick(m, GF_PH_SOLVE, false);
+    m->pending_launches[GF_PH_SOLVE] += nl_sw;
+    m->n_sweeps++;
+    m->refining = true;
+    m->refine_it = 1;
+    m->refine_prev = 1e308;
+    return GF_OK;
+}
+
+// After the last sweep's copy has reached the host: converged (the correction is
+// below refine_tol of x, or stagnates at the FP64 rounding floor), another sweep,
+// or the FP64 factorisation (an unusable FP32 pivot, a non-finite or stagnating
+// correction, refine_max sweeps).  *next: 0 converged, 1 another sweep enqueued,
+// 2 FP64 fallback enqueued.
+static int refine_decide(gf_model *m, int *next)
+{
+    GF_TRY(flush_timing(m));
+    int flag = 0;
+    double dn, xn;
+    memcpy(&flag, &m->h_scal[GF_SCAL_FLAG], 4);
+    memcpy(&dn, m->h_scal + GF_SCAL_REF, 8);
+    memcpy(&xn, m->h_scal + GF_SCAL_REF + 1, 8);
+    bool ok = false, fall = (flag & GF_FLAG_FAIL) != 0 || !(dn == dn) || !(xn == xn);
+    if (!fall) {
+        if (dn <= m->refine_tol * xn) ok = true;
+        else if (m->refine_it > 1 && dn > 0.5 * m->refine_prev) {
+            if (dn <= 1e-10 * xn) ok = true;
+            else fall = true;
+        } else if (m->refine_it >= std::max(1, m->refine_max)) fall = true;
+    }
+    if (!ok && !fall) {
+        m->refine_prev = dn;
+        m->refine_it++;
+        m->n_sweeps++;
+        tick(m, GF_PH_SOLVE, true);
+        GF_TRYN(l, gf_mixed_sweep(m));
+        tick(m, GF_PH_SOLVE, false);
+        m->pending_launches[GF_PH_SOLVE] += l;
+        *next = 1;
+        return GF_OK;
+    }
+    m->refining = false;
+    m->factored = true;
+    note_fallback(m, fall);
+    *next = ok ? 0 : 2;
+    if (fall) {
+        m->n_fallbacks++;
+        tick(m, GF_PH_FACTOR, true);
+        GF_TRYN(l, gf_mixed_fallback(m));
+        tick(m, GF_PH_FACTOR, false);
+        m->pending_launches[GF_PH_FACTOR] += l;
+    }
+    return GF_OK;
+}
+
+int gf_refine_step(gf_model *m, int *more)
+{
+    gf_probe_scope ps(probe_of(m));
+    GF_HIP(hipSetDevice(m->device));
+    int next = 0;
+    GF_TRY(refine_decide(m, &next));
+    *more = next == 1 ? 1 : 0;
+    return next == 1 ? GF_OK : update_tail_enqueue(m, m->refine_then_assemble);
+}
+
+static int solve_fp64_launches(gf_model *m, int &nl_fac, int &nl_sol);
+
+// The whole FP64 Newton iteration -- clear of the increment norms and flag, the
+// factorisation and solve, UpdateDisps, the next iteration's assembly and
+// residual scan, the copy of the scalar block -- captured once per model into a
+// graph and replayed on every iteration (the launch sequence depends on the
+// model only; the load time and gravity factor are read from the scalar block).
+// The graph holds kernel and copy nodes, no memset.
+static int iteration_graph_launch(gf_model *m)
+{
+    if (!m->iter_graph) {
+        GF_HIP(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+        int nf = 0, ns = 0;
+        int st = gf_launch_fill_bytes(m->stream, m->d_scal, (GF_SCAL_FLAG + 1) * sizeof(double));
+        if (!st) st = solve_fp64_launches(m, nf, ns);
+        if (!st) st = update_tail_enqueue(m, true);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(m->stream, &g);
+        if (st || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            if (st) return st;
+            GF_HIP(e);
+        }
+        const hipError_t ei = hipGraphInstantiate(&m->iter_graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) m->iter_graph = nullptr;
+        GF_HIP(ei);
+    }
+    GF_HIP(hipGraphLaunch(m->iter_graph, m->stream));
+    m->factored = false;  // the iteration ends with the next assembly
+    return GF_OK;
+}
+
+// Enqueues the tangent solve, UpdateDisps and the increment norms (d_scal[0..4],
+// failure flag at d_scal[GF_SCAL_FLAG]).  then_assemble: the next iteration's
+// assembly and residual norms follow in the same batch (the Newton loop
+// assembles after every solve that does not end the load step; assembly changes
+// no state, so a batch whose step ends on the increments simply does not use
+// it).  One copy brings the scalar block back.  Mixed precision (not backed
+// off): only the FP32 factorisation and the first refinement sweep are enqueued
+// here; gf_refine_step continues once their event has completed, and enqueues
+// the update when the refinement has ended.
+int gf_solve_update_enqueue(gf_model *m, bool then_assemble)
+{
+    gf_probe_scope ps(probe_of(m));
+    GF_HIP(hipSetDevice(m->device));
+    m->n_solves++;
+    m->pending_info_fail = -1;
+    const bool mixed = m->precision == GF_PREC_MIXED && m->fp64_direct_left == 0;
+    // FP64, or a mixed model backed off to FP64
+    if (!mixed && m->precision == GF_PREC_MIXED) {
+        m->fp64_direct_left--;
+        m->n_direct++;
+    }
+    if (!mixed && then_assemble && !m->timing && m->use_graph) return iteration_graph_launch(m);
+    GF_TRY(gf_launch_fill_bytes(m->stream, m->d_scal, (GF_SCAL_FLAG + 1) * sizeof(double)));
+    if (mixed) {
+        m->refine_then_assemble = then_assemble;
+        return mixed_start_enqueue(m);
+    }
+    GF_TRY(solve_fp64_enqueue(m));
+    return update_tail_enqueue(m, then_assemble);
 }
 
 int gf_solve_update_result(gf_model *m, int *info_fail, double *inc_out)
@@ -1038,41 +1158,21 @@ extern "C" int gf_solve(gf_model *m, int *info_fail)
         m->n_direct++;
     }
     if (m->precision == GF_PREC_MIXED && !direct) {
-        tick(m, GF_PH_FACTOR, true);
-        GF_TRYN(nl_f32, gf_mixed_factor(m));
-        tick(m, GF_PH_FACTOR, false);
-        int flag32 = 0;
-        GF_HIP(hipMemcpyAsync(&m->h_scal[GF_SCAL_FLAG], m->d_flag, 4, hipMemcpyDeviceToHost, m->stream));
-        GF_HIP(hipStreamSynchronize(m->stream));
-        memcpy(&flag32, &m->h_scal[GF_SCAL_FLAG], 4);
-        GF_TRY(tock(m, GF_PH_FACTOR, nl_f32));
-        if (!(flag32 & GF_FLAG_FAIL)) {
-            int fb = 0, sw = 0;
-            tick(m, GF_PH_SOLVE, true);
-            GF_TRYN(nl_ref, gf_mixed_solve(m, &fb, &sw));
-            tick(m, GF_PH_SOLVE, false);
-            GF_TRY(tock(m, GF_PH_SOLVE, nl_ref));
-            m->n_sweeps += sw;
-            m->n_fallbacks += fb;
-            note_fallback(m, fb != 0);
-            m->factored = true;
-            int flag = 0;
-            GF_HIP(hipMemcpyAsync(&m->h_scal[GF_SCAL_FLAG], m->d_flag, 4, hipMemcpyDeviceToHost, m->stream));
+        // the refinement of gf_solve_update_enqueue, waiting for each sweep
+        GF_TRY(mixed_start_enqueue(m));
+        int next = 1;
+        while (next == 1) {
             GF_HIP(hipStreamSynchronize(m->stream));
-            memcpy(&flag, &m->h_scal[GF_SCAL_FLAG], 4);
-            if (flag & GF_FLAG_FAIL) *info_fail = 1;
-            return GF_OK;
+            GF_TRY(refine_decide(m, &next));
         }
-        // an unusable FP32 pivot: FP64 factorisation below
-        m->n_fallbacks++;
-        note_fallback(m, true);
-        GF_TRY(gf_launch_fill_bytes(m->stream, m->d_flag, 4));
+    } else {
+        GF_TRY(solve_fp64_enqueue(m));
     }
-    GF_TRY(solve_fp64_enqueue(m));
     GF_TRY(flush_timing(m));
     int flag = 0;
     GF_HIP(hipMemcpyAsync(&m->h_scal[GF_SCAL_FLAG], m->d_flag, 4, hipMemcpyDeviceToHost, m->stream));
     GF_HIP(hipStreamSynchronize(m->stream));
+    GF_TRY(flush_timing(m));
     memcpy(&flag, &m->h_scal[GF_SCAL_FLAG], 4);
     if (flag & GF_FLAG_FAIL) *info_fail = 1;
     return GF_OK;

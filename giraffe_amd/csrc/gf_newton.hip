@@ -49,8 +49,9 @@ __global__ __launch_bounds__(128) void nodal_loads_kernel(int n_ln, const int *_
                                    const int *__restrict__ ln_ids, const double *__restrict__ rec,
                                    const double *__restrict__ tab_t, const double *__restrict__ tab_v,
                                    const double *__restrict__ disp, const double *__restrict__ copy,
-                                   const int *__restrict__ dof, gf_gmat gm, double *r, double t)
+                                   const int *__restrict__ dof, gf_gmat gm, double *r, const double *__restrict__ tp)
 {
+    const double t = *tp;  // load time, device-resident (a captured Newton step reads the current one)
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n_ln) return;
     int node = ln_node[k];
@@ -239,6 +240,23 @@ int gf_launch_clear3(hipStream_t s, double *a, size_t na, double *b, size_t nb, 
     return GF_OK;
 }
 
+namespace {
+__global__ void set2_kernel(double *p, double a, double b)
+{
+    if (threadIdx.x == 0) {
+        p[0] = a;
+        p[1] = b;
+    }
+}
+}  // namespace
+
+int gf_launch_set2(hipStream_t s, double *p, double a, double b)
+{
+    hipLaunchKernelGGL(set2_kernel, dim3(1), dim3(64), 0, s, p, a, b);
+    GF_HIP(hipGetLastError());
+    return GF_OK;
+}
+
 int gf_launch_fill_bytes(hipStream_t s, void *p, size_t bytes)
 {
     size_t n = bytes / 4;
@@ -248,12 +266,12 @@ int gf_launch_fill_bytes(hipStream_t s, void *p, size_t bytes)
     return GF_OK;
 }
 
-int gf_nodal_loads(gf_model *m, double t)
+int gf_nodal_loads(gf_model *m, const double *tp)
 {
     if (m->n_load_nodes == 0) return GF_OK;
     hipLaunchKernelGGL(nodal_loads_kernel, dim3((m->n_load_nodes + 127) / 128), dim3(128), 0, m->stream,
                        m->n_load_nodes, m->d_ln_node, m->d_ln_off, m->d_ln_ids, m->d_lrec, m->d_tab_t, m->d_tab_v,
-                       m->d_disp, m->d_copy, m->d_dof, gf_model_gmat(m), m->d_r, t);
+                       m->d_disp, m->d_copy, m->d_dof, gf_model_gmat(m), m->d_r, tp);
     GF_HIP(hipGetLastError());
     return GF_OK;
 }

@@ -211,12 +211,14 @@ template <int MODE>
 __global__ __launch_bounds__(SWG) void shell_assemble_kernel(
     int e0, int e1, size_t ld, const int *__restrict__ conn, const double *__restrict__ cst,
     const double *__restrict__ gps, const double *__restrict__ disp, const int *__restrict__ dof, gf_gmat gm,
-    double *__restrict__ r, double lf, double gx, double gy, double gz, double *__restrict__ Kout,
+    double *__restrict__ r, const double *__restrict__ lfp, double gx, double gy, double gz, double *__restrict__ Kout,
     double *__restrict__ fout)
 {
     __shared__ double sm[SEPB * 3 * L_GP];
     const int tid = threadIdx.x;
     const int ebase = e0 + blockIdx.x * SEPB;
+    // gravity ramp factor, device-resident so a captured Newton step reads the current one
+    const double lf = lfp ? *lfp : 0.0;
 
     // ---- phase 1 -------------------------------------------------------------
     if (tid < 3 * SEPB) {
@@ -460,9 +462,10 @@ __global__ __launch_bounds__(256) void shell_pressure_kernel(
     int e0, int e1, size_t ld, const int *__restrict__ conn, const double *__restrict__ cst,
     const double *__restrict__ pc, const double *__restrict__ disp, const double *__restrict__ copy,
     const double *__restrict__ copy0, const int *__restrict__ pload, const int *__restrict__ pflag,
-    const double *__restrict__ lrec, const double *__restrict__ tab_t, const double *__restrict__ tab_v, double time,
+    const double *__restrict__ lrec, const double *__restrict__ tab_t, const double *__restrict__ tab_v, const double *__restrict__ tp,
     const int *__restrict__ dof, gf_gmat gm, double *__restrict__ r)
 {
+    const double time = *tp;  // load time, device-resident (a captured Newton step reads the current one)
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     int e = e0 + t / 6, a = t - 6 * (t / 6);
     if (e >= e1) return;
@@ -720,7 +723,7 @@ void gf_shell_host_const(const double *x, const double *prop, double *rec, doubl
 }
 
 // ---- launchers --------------------------------------------------------------
-int gf_shell_assemble(gf_model *m, double lf)
+int gf_shell_assemble(gf_model *m, const double *lfp)
 {
     const size_t ld = (size_t)m->ns;
     int ncol = (int)m->shell_color_off.size() - 1;
@@ -732,7 +735,7 @@ int gf_shell_assemble(gf_model *m, double lf)
         int grid = (e1 - e0 + SEPB - 1) / SEPB;
         const int pr = gf_probe_begin(m->stream, GF_KC_SHELL_ASSEMBLE);
         hipLaunchKernelGGL(shell_assemble_kernel<0>, dim3(grid), dim3(SWG), 0, m->stream, e0, e1, ld, m->d_sconn,
-                           m->d_sconst, m->d_sgp, m->d_disp, m->d_dof, gf_model_gmat(m), m->d_r, lf, m->gravity[0],
+                           m->d_sconst, m->d_sgp, m->d_disp, m->d_dof, gf_model_gmat(m), m->d_r, lfp, m->gravity[0],
                            m->gravity[1], m->gravity[2], (double *)nullptr, (double *)nullptr);
         gf_probe_end(m->stream, pr, e1 - e0);
     }
@@ -758,7 +761,7 @@ int gf_shell_results(gf_model *m, double *d_out)
     return GF_OK;
 }
 
-int gf_shell_pressure(gf_model *m, double time)
+int gf_shell_pressure(gf_model *m, const double *tp)
 {
     if (!m->any_pressure) return GF_OK;
     const size_t ld = (size_t)m->ns;
@@ -769,7 +772,7 @@ int gf_shell_pressure(gf_model *m, double time)
         int threads = 6 * (e1 - e0);
         hipLaunchKernelGGL(shell_pressure_kernel, dim3((threads + 255) / 256), dim3(256), 0, m->stream, e0, e1, ld,
                            m->d_sconn, m->d_sconst, m->d_spconst, m->d_disp, m->d_copy, m->d_copy0, m->d_spload,
-                           m->d_spflag, m->d_lrec, m->d_tab_t, m->d_tab_v, time, m->d_dof, gf_model_gmat(m),
+                           m->d_spflag, m->d_lrec, m->d_tab_t, m->d_tab_v, tp, m->d_dof, gf_model_gmat(m),
                            m->d_r);
     }
     GF_HIP(hipGetLastError());
@@ -807,8 +810,8 @@ int gf_shell_eval_device(int count, const double *x, const double *prop, const d
     GF_HIP(hipMemcpyAsync(d_disp, disp.data(), sizeof(double) * disp.size(), hipMemcpyHostToDevice, s));
     gf_gmat none = {};
     hipLaunchKernelGGL(shell_assemble_kernel<1>, dim3((count + SEPB - 1) / SEPB), dim3(SWG), 0, s, 0, count, ld,
-                       d_conn, d_cst, d_gps, d_disp, (const int *)nullptr, none, (double *)nullptr, 0.0, 0.0, 0.0,
-                       0.0, d_K, d_f);
+                       d_conn, d_cst, d_gps, d_disp, (const int *)nullptr, none, (double *)nullptr,
+                       (const double *)nullptr, 0.0, 0.0, 0.0, d_K, d_f);
     GF_HIP(hipGetLastError());
     GF_HIP(hipMemcpyAsync(K_out, d_K, sizeof(double) * 729 * ld, hipMemcpyDeviceToHost, s));
     GF_HIP(hipMemcpyAsync(f_out, d_f, sizeof(double) * 27 * ld, hipMemcpyDeviceToHost, s));

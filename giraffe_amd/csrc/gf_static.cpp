@@ -130,11 +130,19 @@ int gf_static_run::advance()
                 if ((st = record())) return st;
                 r.total_solves++;
                 res_conv = false;
-                phase = PH_SOLVED;
+                phase = m->refining ? PH_REFINE : PH_SOLVED;
                 return 1;
             }
             phase = PH_ITER_END;
             break;
+        }
+        case PH_REFINE: {
+            // mixed precision: the last refinement sweep's norms are on the host
+            int more = 0;
+            if ((st = gf_refine_step(m, &more))) return st;
+            if ((st = record())) return st;
+            if (!more) phase = PH_SOLVED;
+            return 1;
         }
         case PH_SOLVED: {
             r.t_solve_ms += ms_since(t_mark);

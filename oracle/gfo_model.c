@@ -545,7 +545,19 @@ typedef void (*pardiso_t)(void *, const int *, const int *, const int *, const i
 typedef void (*mkl_threads_t)(int);
 static pardisoinit_t p_init;
 static pardiso_t p_solve;
+static mkl_threads_t p_threads;
 static int mkl_state = 0; /* 0 unknown, 1 ok, -1 absent */
+
+/* SolverOptions::PreCalc src/SolverOptions.cpp:63-70: MKL threads = P-1 */
+static int default_mkl_threads(void)
+{
+#ifdef _OPENMP
+    int P = omp_get_max_threads();
+#else
+    int P = 1;
+#endif
+    return P != 1 ? P - 1 : P;
+}
 
 int gfo_pardiso_available(void)
 {
@@ -559,18 +571,21 @@ int gfo_pardiso_available(void)
     if (!h) { mkl_state = -1; return 0; }
     p_init = (pardisoinit_t)dlsym(h, "pardisoinit");
     p_solve = (pardiso_t)dlsym(h, "pardiso");
-    mkl_threads_t st = (mkl_threads_t)dlsym(h, "MKL_Set_Num_Threads");
+    p_threads = (mkl_threads_t)dlsym(h, "MKL_Set_Num_Threads");
     if (!p_init || !p_solve) { mkl_state = -1; return 0; }
-    if (st) {
-        /* SolverOptions::PreCalc src/SolverOptions.cpp:63-70: MKL threads = P-1 */
-#ifdef _OPENMP
-        int P = omp_get_max_threads();
-#else
-        int P = 1;
-#endif
-        st(P != 1 ? P - 1 : P);
-    }
+    if (p_threads) p_threads(default_mkl_threads());
     mkl_state = 1;
+    return 1;
+}
+
+/* PARDISO's thread count: n > 0 sets it, n <= 0 restores the reference's P-1.
+   PARDISO's rounding depends on its thread count (and on its dynamic schedule
+   when threaded); one thread makes a run reproducible.  Returns 0 when MKL is
+   absent or cannot set it. */
+int gfo_pardiso_threads(int n)
+{
+    if (!gfo_pardiso_available() || !p_threads) return 0;
+    p_threads(n > 0 ? n : default_mkl_threads());
     return 1;
 }
 

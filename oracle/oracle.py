@@ -60,6 +60,8 @@ def lib():
         L.gfo_bandwidth.argtypes = [vp]
         L.gfo_dof_map.argtypes = [vp, C.POINTER(C.c_int)]
         L.gfo_pardiso_available.restype = C.c_int
+        L.gfo_pardiso_threads.argtypes = [C.c_int]
+        L.gfo_pardiso_threads.restype = C.c_int
         L.gfo_residual_criteria.argtypes = [dp, C.POINTER(C.c_int), C.c_double, C.c_double,
                                             C.POINTER(abi.GfStaticParams)]
         L.gfo_residual_criteria.restype = None
@@ -91,6 +93,12 @@ def residual_criteria(crit, n, max_force, max_moment, params):
 
 def pardiso_available() -> bool:
     return bool(lib().gfo_pardiso_available())
+
+
+def pardiso_threads(n: int) -> bool:
+    """PARDISO's thread count (n <= 0: the reference's P-1).  PARDISO's
+    rounding depends on it; one thread makes runs reproducible."""
+    return bool(lib().gfo_pardiso_threads(int(n)))
 
 
 class OracleModel:

@@ -11,7 +11,14 @@ as sparsesystem does, where libmkl_rt is present):
 
 Bar: same converged steps, Newton iterations per step within 1, node
 displacements and rotations within 1e-8 of the oracle (relative to their
-scales), final relative residual <= 1e-10."""
+scales), final relative residual <= 1e-10.
+
+The stream samples run PARDISO on one thread.  Their 3,000-element riser
+bisects four times, and whether one of its retried steps converges (19
+iterations) or diverges depends on rounding: PARDISO itself gives 99 or 100
+converged steps from run to run at 16 threads (its threaded schedule is not
+reproducible), 100 at 1, 4 and 8 threads and with the band LU.  One thread
+makes the oracle's answer reproducible."""
 import time
 
 import numpy as np
@@ -22,10 +29,16 @@ from giraffe_amd import abi, model as M, runtime
 pytestmark = pytest.mark.gpu
 
 
-def oracle_result(oracle_lib, m, p):
+def oracle_result(oracle_lib, m, p, one_thread=False):
     solver = oracle_lib.SOLVER_PARDISO if oracle_lib.pardiso_available() else oracle_lib.SOLVER_BAND
-    om = oracle_lib.OracleModel(m, solver)
-    ro, io = om.static_solve(p)
+    if one_thread:
+        oracle_lib.pardiso_threads(1)
+    try:
+        om = oracle_lib.OracleModel(m, solver)
+        ro, io = om.static_solve(p)
+    finally:
+        if one_thread:
+            oracle_lib.pardiso_threads(0)
     return ro, np.asarray(io), om.nodes()
 
 
@@ -97,7 +110,7 @@ def test_config4_riser_stream_sample(oracle_lib):
     srv.close()
     for (m, p), rid in zip(reqs, ids):
         rep, it, nodes, _ = res[rid]
-        check_against_oracle(m, rep, it, nodes, oracle_result(oracle_lib, m, p))
+        check_against_oracle(m, rep, it, nodes, oracle_result(oracle_lib, m, p, one_thread=True))
 
 
 @pytest.mark.parametrize("precision", [abi.GF_PREC_FP64, abi.GF_PREC_MIXED])
@@ -115,7 +128,7 @@ def test_config5_mixed_stream_sample(oracle_lib, precision):
     srv.close()
     for (m, p), rid in zip(reqs, ids):
         rep, it, nodes, _ = res[rid]
-        check_against_oracle(m, rep, it, nodes, oracle_result(oracle_lib, m, p))
+        check_against_oracle(m, rep, it, nodes, oracle_result(oracle_lib, m, p, one_thread=True))
 
 
 def test_mixed_server_alone_and_in_the_mix():

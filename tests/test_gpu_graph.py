@@ -1,6 +1,8 @@
-"""The FP64 solve sequence replayed from a captured graph gives bit-identical
-Newton histories and results to direct launches (every Newton iteration replays
-the graph between the eager assembly, update and commit work)."""
+"""The FP64 Newton iteration replayed from a captured graph (factorisation,
+solve, UpdateDisps, the next assembly and residual scan, the copy of the scalar
+block; load time and gravity factor read from device memory) gives
+bit-identical Newton histories and results to direct launches, across load
+steps, bisections and a reset."""
 import os
 
 import numpy as np
