@@ -2,9 +2,11 @@
 
 One step = one complete Static::Solve (all load steps converged) of the
 workload, with the model resident in HBM (the state is reset on the device
-between solves).  Default workload: the riser catenary of 10,000 Beam_1
-elements (20,001 nodes), self-weight plus seeded N(0, 500 N) lateral nodal
-loads over 10 load steps, Newton tolerance 1e-10.
+between solves).  Default workload: the clamped Shell_1 plate of 64 x 64 quads
+(8,192 elements, 16,641 nodes, 84,867 free DOFs) under a follower pressure
+ramped over 15 load steps; the riser catenary of 10,000 Beam_1 elements and
+the serving streams are behind --workload.  The line also carries the other
+precision's figures (`compare`: mixed-precision solves/s next to FP64).
 
 Prints one JSON line (see the repository's measurement contract): value =
 solves/s, p50/p99 latency, the roofline of the dominant kernel measured live
@@ -85,6 +87,29 @@ def cpu_baseline(m, p, kind, gpu_iterations, budget_s=30.0):
             "sample": f"{what}: {iters} Newton iterations in {dt:.1f} s ({per_iter * 1e3:.0f} ms/iteration), "
                       f"scaled to the {gpu_iterations} iterations of one full solve; {label}",
             "iterations_timed": iters}
+
+
+def newton_iterations(m, p, precision, count):
+    """`count` Newton iterations of the first load step through the per-phase
+    entry points (assembly, tangent solve, update), one line each: a short run
+    of every kernel of the path for counter passes under rocprofv3."""
+    from giraffe_amd import runtime
+
+    prec = abi.GF_PREC_FP64 if precision == "fp64" else abi.GF_PREC_MIXED
+    dm = runtime.DeviceModel(m, precision=prec)
+    dm.set_window(p.start_time, p.end_time)
+    dm.begin_step(p.start_time, p.time_step)
+    out = []
+    for k in range(count):
+        ts = time.perf_counter()
+        res = dm.assemble()
+        fail = dm.solve()
+        inc = dm.update()
+        out.append({"residual": [float(v) for v in res[:2]], "increment": [float(v) for v in inc[:2]],
+                    "info_fail": int(fail), "ms": 1e3 * (time.perf_counter() - ts)})
+        log(f"[{precision}] Newton iteration {k}: {out[-1]}")
+    dm.close()
+    return out
 
 
 def measure(m, p, precision, steps, warmup, probe_steps):
@@ -349,7 +374,7 @@ def serve_cpu_baseline(reqs, budget_s):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="plate", choices=["catenary", "plate", "rollup", "serve", "serve-mixed"])
     ap.add_argument("--requests", type=int, default=2000, help="serve: requests in the stream")
@@ -369,6 +394,9 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="before the workload, copy a buffer of this many MiB once on the device (a plain copy "
                          "kernel of known size, the reference point for scaling HBM counter readings)")
+    ap.add_argument("--newton-iterations", type=int, default=0,
+                    help="instead of timed solves, run this many Newton iterations of the first load step "
+                         "(assembly, tangent solve, update) and print them: a short run for counter passes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -396,6 +424,11 @@ def main():
     if args.workload.startswith("serve"):
         return serve(args)
     m, p, desc = workload(args.workload)
+    if args.newton_iterations > 0:
+        its = newton_iterations(m, p, args.precision, args.newton_iterations)
+        print(json.dumps({"mode": "newton-iterations", "workload": desc, "precision": args.precision,
+                          "iterations": its}))
+        return 0
     r = measure(m, p, args.precision, args.steps, args.warmup, 0 if args.no_timing else args.probe_steps)
     info, reps = r["info"], r["reps"]
     its, solves = reps[-1][0], reps[-1][1]
