@@ -17,12 +17,19 @@
 // so L(i,p) is the updated A(i,p) itself (no triangular solve on the panel).
 // The band stays closed under the update: U(p, .) reaches at most 63 columns
 // past the band of its first row (hence ku), and L U adds nothing outside
-// -hb..hb.  One launch per block column p with look-ahead:
-//   T workgroups  A(i,j) -= A(i,p) U(p,j)  for i >= p+2, j >= p+1;
-//   S workgroups  row p+1: D = A(p+1,p+1) - A(p+1,p) U(p,p+1), E = D^-1 and
-//                 U(p+1,j) = E (A(p+1,j) - A(p+1,p) U(p,j)), every S workgroup
-//                 inverting D itself (no workgroup waits on another);
-// the two groups touch disjoint blocks, so stream order is the only
+// -hb..hb.  The upper strip of row p is kept as A'(p,j) (updated, before the
+// multiplication by E_p) while the factorisation needs it, and every update
+// tile forms U(p,j) = E_p A'(p,j) itself, so one launch per block column p
+// holds all of step p:
+//   update    A(i,j) -= A(i,p) (E_p A'(p,j))  for i, j in p+1 .. p+R (two
+//             64^3 MFMA products per tile, operands staged through LDS);
+//   diagonal  the same for (p+1, p+1), then E_{p+1} = D^-1 (one workgroup:
+//             the launch's critical path is one memory round trip, two
+//             products and the inverse);
+//   convert   A'(p-1,j) := E_{p-1} A'(p-1,j), the strip that step p-1 was
+//             the last to read.
+// No workgroup waits on another: the groups touch disjoint blocks and read
+// only what earlier launches wrote, so stream order is the only
 // synchronisation.  A pivot below 1e-13 of its row's original diagonal entry
 // is perturbed to that size (flag bit 2); a non-finite one fails (bit 1).
 //
@@ -47,7 +54,7 @@ constexpr int KT = 32;   // k-chunk of the staged MFMA products
 // rows i = 0..15 at column offsets c = 0..1, a B operand rows c at columns
 // j = 0..15.  Row stride 66 (2 mod 32) and 34 spread A reads over all banks,
 // 80 (16 mod 32) does the same for B reads, 18 for the saved 16 x 16 blocks.
-constexpr int LDT = NB + 2;
+constexpr int LDT = NB + 2, LDQ = NB + 16;
 constexpr int SA_LD = KT + 2, SB_LD = NB + 16, TMP_LD = 18;
 
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -169,8 +176,9 @@ __device__ __forceinline__ void tile_mm(Acc<T> &acc, T *sa, T *sb, LP loadP, LQ 
     }
 }
 
-// acc += P Q with both operands in LDS ([64][LDT]); no barrier inside.
-template <typename T> __device__ __forceinline__ void lds_mm(Acc<T> &acc, const T *P, const T *Q)
+// acc += P Q with both operands in LDS (64 x 64, row strides lp and lq); no
+// barrier inside.
+template <typename T> __device__ __forceinline__ void lds_mm(Acc<T> &acc, const T *P, int lp, const T *Q, int lq)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wr = acc_row(wave), wc = acc_col(wave);
@@ -178,9 +186,9 @@ template <typename T> __device__ __forceinline__ void lds_mm(Acc<T> &acc, const 
     for (int k = 0; k < NB; k += 4) {
         T av[2], bv[2];
 #pragma unroll
-        for (int x = 0; x < 2; x++) av[x] = P[(wr + 16 * x + (lane & 15)) * LDT + k + (lane >> 4)];
+        for (int x = 0; x < 2; x++) av[x] = P[(wr + 16 * x + (lane & 15)) * lp + k + (lane >> 4)];
 #pragma unroll
-        for (int y = 0; y < 2; y++) bv[y] = Q[(k + (lane >> 4)) * LDT + wc + 16 * y + (lane & 15)];
+        for (int y = 0; y < 2; y++) bv[y] = Q[(k + (lane >> 4)) * lq + wc + 16 * y + (lane & 15)];
 #pragma unroll
         for (int x = 0; x < 2; x++)
 #pragma unroll
@@ -197,40 +205,86 @@ template <typename T> struct PivotGuard {
     __device__ __forceinline__ PivotGuard() : rel(sizeof(T) == 8 ? T(1e-13) : T(1e-6)), huge(sizeof(T) == 8 ? T(1e300) : T(1e37)) {}
 };
 
+// Cross-lane moves of the 16 x 16 inverse.  A wavefront is four DPP rows of
+// 16 lanes.  row_newbcast:K (DPP) gives every lane the value of lane K of its
+// own row; rowcast<R> gives every lane the value at the same position of row R
+// (v_permlane16_swap: each pair of rows swaps its odd row with the even one,
+// so one of the two results holds the pair's even row twice, the other its
+// odd row; v_permlane32_swap does the same for the two halves).  Register
+// moves only: no LDS round trip on the pivot chain.
+template <int CTRL> __device__ __forceinline__ double dpp_mov(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+template <int CTRL> __device__ __forceinline__ float dpp_mov(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int R> __device__ __forceinline__ unsigned rowcast_u32(unsigned v)
+{
+    const auto h = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    const unsigned t = (R & 1) ? h[1] : h[0];
+    const auto w = __builtin_amdgcn_permlane32_swap(t, t, false, false);
+    return (R & 2) ? w[1] : w[0];
+}
+template <int R> __device__ __forceinline__ double rowcast(double v)
+{
+    const unsigned lo = rowcast_u32<R>((unsigned)__double2loint(v)), hi = rowcast_u32<R>((unsigned)__double2hiint(v));
+    return __hiloint2double((int)hi, (int)lo);
+}
+template <int R> __device__ __forceinline__ float rowcast(float v)
+{
+    return __uint_as_float(rowcast_u32<R>(__float_as_uint(v)));
+}
+// Pivot K of the in-place Gauss-Jordan inverse of a 16 x 16 block held by one
+// wavefront: lane 16 q + i holds row i, columns 4 q .. 4 q + 3 (a[0..3]).  The
+// pivot reaches every lane by two register moves (column K of the lane's row
+// from row block K / 4, then lane K of each DPP row), so the chain from one
+// pivot to the next is VALU only: no readlane, no scalar selects.  A pivot
+// below eps[K] takes the reciprocal of +-eps[K] (ieps, computed before the
+// first pivot), a non-finite one the reciprocal 1.
+template <typename T, int K>
+__device__ __forceinline__ void inv16_pivot(T (&a)[4], int i, int q, const T (&eps)[16], const T (&ieps)[16],
+                                            T huge, bool &bad, bool &pert)
+{
+    constexpr int KQ = K >> 2, KE = K & 3;
+    T rk[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) rk[e] = dpp_mov<0x150 + K>(a[e]);  // row K, this lane's columns
+    const T f = rowcast<KQ>(a[KE]);                                  // row i, column K
+    const T p = dpp_mov<0x150 + K>(f);                               // the pivot
+    const T ap = fabs(p);
+    const bool nonfinite = !(ap < huge);
+    const bool small = !nonfinite && ap < eps[K];
+    bad |= nonfinite;
+    pert |= small;
+    const T ip = nonfinite ? T(1) : small ? (p >= T(0) ? ieps[K] : -ieps[K]) : rcp_of<T>(p);
+    const bool isk = i == K;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const T r = rk[e] * ip;
+        const T upd = isk ? r : a[e] - f * r;
+        a[e] = (4 * q + e == K) ? (isk ? ip : -f * ip) : upd;
+    }
+    if constexpr (K + 1 < 16) inv16_pivot<T, K + 1>(a, i, q, eps, ieps, huge, bad, pert);
+}
+
 // One wavefront: in-place Gauss-Jordan inverse (no interchanges) of the 16 x 16
-// block at b (stride LDT).  Lane l holds row l / 4, columns 4 (l % 4) .. +3; per
-// pivot the pivot row, the column entry of the lane's row and the pivot itself
-// come from other lanes by shuffles, so the 16 pivots need no barrier.  eps[k]:
-// the perturbation size of pivot k.
+// block at b (stride LDT); the 16 pivots run in registers with DPP and
+// permlane moves, no barrier and no LDS access between the load and the store.
+// eps[k]: the perturbation size of pivot k.
 template <typename T>
 __device__ __forceinline__ void inv16(T *b, const T (&eps)[16], T huge, bool &bad, bool &pert)
 {
-    const int lane = threadIdx.x & 63, i = lane >> 2, q = lane & 3;
-    T a[4];
+    const int lane = threadIdx.x & 63, i = lane & 15, q = lane >> 4;
+    T a[4], ieps[16];
 #pragma unroll
     for (int e = 0; e < 4; e++) a[e] = b[i * LDT + 4 * q + e];
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
-        T rk[4];
-#pragma unroll
-        for (int e = 0; e < 4; e++) rk[e] = __shfl(a[e], 4 * k + q);
-        const T f = __shfl(a[k & 3], 4 * i + (k >> 2));
-        T p = __shfl(a[k & 3], 4 * k + (k >> 2));
-        const T ap = fabs(p);
-        const bool nonfinite = !(ap < huge);
-        const bool small = !nonfinite && ap < eps[k];
-        bad |= nonfinite;
-        pert |= small;
-        p = nonfinite ? T(1) : small ? (p >= T(0) ? eps[k] : -eps[k]) : p;
-        const T ip = rcp_of<T>(p);
-        const bool isk = i == k;
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const T r = rk[e] * ip;
-            const T upd = isk ? r : a[e] - f * r;
-            a[e] = (4 * q + e == k) ? (isk ? ip : -f * ip) : upd;
-        }
-    }
+    for (int k = 0; k < 16; k++) ieps[k] = rcp_of<T>(eps[k]);
+    inv16_pivot<T, 0>(a, i, q, eps, ieps, huge, bad, pert);
 #pragma unroll
     for (int e = 0; e < 4; e++) b[i * LDT + 4 * q + e] = a[e];
 }
@@ -321,9 +375,41 @@ template <typename T> __device__ __forceinline__ void inverse64(T *M, T *tmp, T 
     }
 }
 
-// LDS of the step and batch-inverse kernels: the staging of tile_mm, reused
-// as X (a [64][LDT] operand) once the products that stage through it are done.
+// Row-major 64 x 64 tile through registers: thread t holds elements t + 256 r
+// (each row of 64 read or written by one 64-lane wave, coalesced).
+template <typename T, typename F> __device__ __forceinline__ void tile_load(T (&v)[16], F load)
+{
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int t = threadIdx.x + 256 * r;
+        v[r] = load(t >> 6, t & 63);
+    }
+}
+template <typename T> __device__ __forceinline__ void tile_put(T *dst, int ld, const T (&v)[16])
+{
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int t = threadIdx.x + 256 * r;
+        dst[(t >> 6) * ld + (t & 63)] = v[r];
+    }
+}
+
+// LDS of the step kernel: P holds A operands (E_p, then L(i, p); the diagonal
+// block D for the inverse), Q B operands (A'(p, j), then U(p, j)); the
+// inverse's scratch reuses Q.  75 KB in FP64, so two workgroups share a CU and
+// a launch's tiles are all resident at once.
 template <typename T> struct StepSmem {
+    T P[NB * LDT];
+    union {
+        T Q[NB * LDQ];
+        T tmp[3 * 16 * TMP_LD];
+    };
+    T aux[4 + NB];
+};
+
+// LDS of the batch-inverse kernel: the staging of tile_mm, reused as X (a
+// [64][LDT] operand) once the products that stage through it are done.
+template <typename T> struct BatchSmem {
     union {
         T stage[NB * SA_LD + KT * SB_LD];
         T X[NB * LDT];
@@ -341,93 +427,109 @@ template <typename T> __device__ __forceinline__ T *batch_blk(T *gh, int b, int 
     return gh + (size_t)b * BW * BW + (size_t)(r * NB) * BW + c * NB;
 }
 
-// Tile of workgroup b in the launch of block column p: the tiles are the S row
-// p+1 (nS of them, columns p+1 ..) and the T rows p+2 .. p+1+nr (nc each,
-// columns p+1 ..).  Workgroups are dispatched round-robin over the 8 XCDs
-// (b mod 8), and row i is always given to XCD i mod 8, so a band row's tiles
-// are updated by the same XCD at every step and stay in its L2 (a row of the
-// update window is 22 tiles, ~0.7 MB in FP64; an XCD holds about 3 rows of the
-// window at a time).  Within an XCD the rows are taken in order, the S row
-// first.  Returns false for a workgroup past its XCD's share.
-__device__ __forceinline__ bool step_tile(int b, int p, int nS, int nc, int nr, int &bi, int &bj)
+// Tile of workgroup b in launch p: the update tiles (i, j), i, j in
+// p+1 .. p+nr (for p = -1 only the diagonal tile (0, 0)), then the nv
+// conversion tiles of row p-1 (columns p ..).  Workgroups are dispatched
+// round-robin over the 8 XCDs (b mod 8), and row i is always given to XCD
+// i mod 8, so a band row's tiles are updated by the same XCD at every step
+// (a row of the update window is 22 tiles, ~0.7 MB in FP64); within an XCD the
+// rows are taken in order, so the diagonal tile (p+1, p+1) -- the only one
+// with a dependent inverse -- is the first workgroup of its XCD.  kind: 0
+// diagonal, 1 update, 2 conversion.  Returns false past the XCD's share.
+__device__ __forceinline__ bool step_tile(int b, int p, int nr, int nv, int &kind, int &bi, int &bj)
 {
     const int x = b & 7;
     int k = b >> 3;
-    for (int i = p + 1 + (((x - (p + 1)) % 8) + 8) % 8; i <= p + 1 + nr; i += 8) {
-        const int cnt = i == p + 1 ? nS : nc;
-        if (k < cnt) {
+    const int ncol = p < 0 ? 1 : nr;
+    for (int i = p + 1 + (((x - (p + 1)) % 8) + 8) % 8; i <= p + nr; i += 8) {
+        if (k < ncol) {
             bi = i;
             bj = p + 1 + k;
+            kind = bi == p + 1 && bj == p + 1 ? 0 : 1;
             return true;
         }
-        k -= cnt;
+        k -= ncol;
+    }
+    if (nv > 0 && ((p - 1) & 7) == x && k < nv) {
+        bi = p - 1;
+        bj = p + k;
+        kind = 2;
+        return true;
     }
     return false;
 }
 
-// One block column of the factorisation (see the file header).  p = -1: row 0
-// only (no update).  S tiles (p+1, p+1+s): s = 0 only stores E_{p+1}, s > 0
-// forms U(p+1, p+1+s); T tiles (i, j), i >= p+2: the update of step p.
+// Launch p of the factorisation (see the file header):
+//   update    A(i, j) -= A(i, p) (E_p A'(p, j))            i, j in p+1 .. p+R
+//   diagonal  the same for (p+1, p+1), then E_{p+1} = D^-1 into the batch store
+//   convert   A'(p-1, j) := U(p-1, j) = E_{p-1} A'(p-1, j)  (no later launch
+//             reads A'(p-1, .), and no workgroup of this one touches row p-1)
+// Every operand is final before the launch starts, so stream order is the only
+// synchronisation.  Each workgroup issues all of its loads before the first
+// LDS store (one memory round trip), then runs the products from LDS.
 template <typename T>
-__global__ __launch_bounds__(256) void band_step_kernel(T *ab, Geo g, int p, int nS, int nc, int nr, T *gh, int *flag)
+__global__ __launch_bounds__(256) void band_step_kernel(T *ab, Geo g, int p, int nr, int nv, T *gh, int *flag)
 {
     __shared__ StepSmem<T> sm;
-    T *sa = sm.stage, *sb = sm.stage + NB * SA_LD;
+    int kind, ti, tj;
+    if (!step_tile(blockIdx.x, p, nr, nv, kind, ti, tj)) return;
+    // the diagonal tile is the launch's critical path: its waves go first when
+    // they share a CU with an update tile
+    if (kind == 0) __builtin_amdgcn_s_setprio(3);
     const int tid = threadIdx.x;
-    int ti, tj;
-    if (!step_tile(blockIdx.x, p, nS, nc, nr, ti, tj)) return;
-    if (ti > p + 1) {
-        // ---- T: A(i, j) -= A(i, p) U(p, j)
-        const int bi = ti, bj = tj;
-        const int R0 = bi * NB, C0 = bj * NB, K0 = p * NB;
-        T old[2][2][4];
+    const int R0 = ti * NB, C0 = tj * NB;
+    Acc<T> acc;
+    acc.zero();
+    if (kind == 2) {
+        T e[16], a[16];
+        const T *E = batch_blk(gh, ti / SB, ti % SB, ti % SB);
+        tile_load(e, [&](int i, int k) { return E[(size_t)i * BW + k]; });
+        tile_load(a, [&](int k, int j) { return band_ld(ab, R0 + k, C0 + j, g); });
+        tile_put(sm.P, LDT, e);
+        tile_put(sm.Q, LDQ, a);
+        __syncthreads();
+        lds_mm<T>(acc, sm.P, LDT, sm.Q, LDQ);
+        acc_each<T>([&](int x, int y, int v, int i, int j) { band_st(ab, R0 + i, C0 + j, g, acc.a[x][y][v]); });
+        return;
+    }
+    T old[2][2][4];
+    if (p >= 0) {
+        // E_p and A'(p, j) first (32 loads per thread, one round trip); L(i, p)
+        // and the old tile are issued behind the first product, which hides
+        // their latency (a wave keeps at most 63 loads in flight)
+        const int K0 = p * NB;
+        T e[16], a[16], l[16];
+        const T *E = batch_blk(gh, p / SB, p % SB, p % SB);
+        tile_load(e, [&](int i, int k) { return E[(size_t)i * BW + k]; });
+        tile_load(a, [&](int k, int j) { return band_ld(ab, K0 + k, C0 + j, g); });
+        tile_put(sm.P, LDT, e);
+        tile_put(sm.Q, LDQ, a);
+        __syncthreads();
+        tile_load(l, [&](int i, int k) { return band_ld(ab, R0 + i, K0 + k, g); });
         acc_each<T>([&](int x, int y, int v, int i, int j) { old[x][y][v] = band_ld(ab, R0 + i, C0 + j, g); });
-        Acc<T> acc;
-        acc.zero();
-        tile_mm<T>(acc, sa, sb, [&](int i, int k) { return band_ld(ab, R0 + i, K0 + k, g); },
-                   [&](int k, int j) { return band_ld(ab, K0 + k, C0 + j, g); });
+        Acc<T> w;
+        w.zero();
+        lds_mm<T>(w, sm.P, LDT, sm.Q, LDQ);  // U(p, j)
+        __syncthreads();
+        acc_each<T>([&](int x, int y, int v, int i, int j) { sm.Q[i * LDQ + j] = w.a[x][y][v]; });
+        tile_put(sm.P, LDT, l);
+        __syncthreads();
+        lds_mm<T>(acc, sm.P, LDT, sm.Q, LDQ);  // L(i, p) U(p, j)
+    } else {
+        acc_each<T>([&](int x, int y, int v, int i, int j) { old[x][y][v] = band_ld(ab, R0 + i, C0 + j, g); });
+    }
+    if (kind == 1) {
         acc_each<T>([&](int x, int y, int v, int i, int j) {
             band_st(ab, R0 + i, C0 + j, g, old[x][y][v] - acc.a[x][y][v]);
         });
         return;
     }
-    // ---- S: row block p+1
-    const int bi = p + 1, bj = tj, s = tj - (p + 1);
-    const int R0 = bi * NB, C0 = bj * NB, K0 = p * NB;
-    const bool upd = p >= 0;
-    // D = A(p+1, p+1) - A(p+1, p) U(p, p+1)
-    {
-        T old[2][2][4];
-        acc_each<T>([&](int x, int y, int v, int i, int j) { old[x][y][v] = band_ld(ab, R0 + i, R0 + j, g); });
-        Acc<T> acc;
-        acc.zero();
-        if (upd)
-            tile_mm<T>(acc, sa, sb, [&](int i, int k) { return band_ld(ab, R0 + i, K0 + k, g); },
-                       [&](int k, int j) { return band_ld(ab, K0 + k, R0 + j, g); });
-        acc_each<T>([&](int x, int y, int v, int i, int j) { sm.D[i * LDT + j] = old[x][y][v] - acc.a[x][y][v]; });
-    }
-    // A' = A(p+1, bj) - A(p+1, p) U(p, bj)   (U(p, bj) is zero past the reach)
-    if (s > 0) {
-        T old[2][2][4];
-        acc_each<T>([&](int x, int y, int v, int i, int j) { old[x][y][v] = band_ld(ab, R0 + i, C0 + j, g); });
-        Acc<T> acc;
-        acc.zero();
-        if (upd && bj <= p + g.R)
-            tile_mm<T>(acc, sa, sb, [&](int i, int k) { return band_ld(ab, R0 + i, K0 + k, g); },
-                       [&](int k, int j) { return band_ld(ab, K0 + k, C0 + j, g); });
-        acc_each<T>([&](int x, int y, int v, int i, int j) { sm.X[i * LDT + j] = old[x][y][v] - acc.a[x][y][v]; });
-    }
+    __syncthreads();  // the products' LDS reads are done
+    acc_each<T>([&](int x, int y, int v, int i, int j) { sm.P[i * LDT + j] = old[x][y][v] - acc.a[x][y][v]; });
     __syncthreads();
-    inverse64<T>(sm.D, sm.tmp, sm.aux, flag);  // D := E_{p+1}
-    if (s == 0) {
-        T *E = batch_blk(gh, bi / SB, bi % SB, bi % SB);
-        for (int t = tid; t < NB * NB; t += 256) E[(t / NB) * BW + t % NB] = sm.D[(t / NB) * LDT + t % NB];
-        return;
-    }
-    Acc<T> acc;
-    acc.zero();
-    lds_mm<T>(acc, sm.D, sm.X);
-    acc_each<T>([&](int x, int y, int v, int i, int j) { band_st(ab, R0 + i, C0 + j, g, acc.a[x][y][v]); });
+    inverse64<T>(sm.P, sm.tmp, sm.aux, flag);  // E_{p+1}
+    T *E = batch_blk(gh, ti / SB, ti % SB, ti % SB);
+    for (int t = tid; t < NB * NB; t += 256) E[(t / NB) * BW + t % NB] = sm.P[(t / NB) * LDT + t % NB];
 }
 
 // Batch inverses for block distance d (1 <= d < SB).  Workgroup (b, c, kind):
@@ -437,7 +539,7 @@ __global__ __launch_bounds__(256) void band_step_kernel(T *ab, Geo g, int p, int
 template <typename T>
 __global__ __launch_bounds__(256) void band_batch_inv_kernel(const T *ab, Geo g, int d, T *gh)
 {
-    __shared__ StepSmem<T> sm;
+    __shared__ BatchSmem<T> sm;
     T *sa = sm.stage, *sb = sm.stage + NB * SA_LD;
     const int per = SB - d;
     const int w = blockIdx.x % (2 * per), b = blockIdx.x / (2 * per);
@@ -460,7 +562,7 @@ __global__ __launch_bounds__(256) void band_batch_inv_kernel(const T *ab, Geo g,
         __syncthreads();
         Acc<T> out;
         out.zero();
-        lds_mm<T>(out, sm.D, sm.X);
+        lds_mm<T>(out, sm.D, LDT, sm.X, LDT);
         T *Gr = batch_blk(gh, b, r, c);
         acc_each<T>([&](int x, int y, int v, int i, int j) { Gr[(size_t)i * BW + j] = -out.a[x][y][v]; });
     } else {
@@ -573,19 +675,21 @@ template <typename T> static int band_factor_t(T *ab, int n, int hb, hipStream_t
 {
     const Geo g = make_geo(n, hb);
     int launches = 0;
-    for (int p = -1; p + 1 < g.nbk; p++) {
-        const int last = g.nbk - 1;
-        const int nS = std::min(last, p + 1 + g.R) - p;
-        const int nr = p >= 0 ? std::min(last, p + g.R) - (p + 2) + 1 : 0;
-        const int nc = std::min(last, p + g.R) - p;
+    const int last = g.nbk - 1;
+    // launch p = -1 .. last: E_{p+1} and the update of step p (p < last), the
+    // conversion of row p-1 to U (p >= 1)
+    for (int p = -1; p <= last; p++) {
+        const int nr = p < 0 ? 1 : p < last ? std::min(last, p + g.R) - p : 0;
+        const int nv = p >= 1 ? std::min(last, p - 1 + g.R) - p + 1 : 0;
+        if (nr == 0 && nv == 0) continue;
         // tiles per XCD (row i on XCD i mod 8): the grid covers the largest share
         int share[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        for (int i = p + 1; i <= p + 1 + std::max(0, nr); i++) share[i & 7] += i == p + 1 ? nS : nc;
+        for (int i = p + 1; i <= p + nr; i++) share[i & 7] += p < 0 ? 1 : nr;
+        if (nv > 0) share[(p - 1) & 7] += nv;
         const int grid = 8 * *std::max_element(share, share + 8);
         const int pr = gf_probe_begin(s, GF_KC_BAND_UPDATE);
-        hipLaunchKernelGGL(band_step_kernel<T>, dim3(grid), dim3(256), 0, s, ab, g, p, nS, nc, std::max(0, nr), gh,
-                           flag);
-        gf_probe_end(s, pr, NB);
+        hipLaunchKernelGGL(band_step_kernel<T>, dim3(grid), dim3(256), 0, s, ab, g, p, nr, nv, gh, flag);
+        gf_probe_end(s, pr, nr > 0 ? NB : 0);  // block column p+1 eliminated
         launches++;
     }
     const int nbatch = (g.nbk + SB - 1) / SB;
