@@ -1,6 +1,6 @@
 // Reader and writer of GIRAFFE input decks, restricted to the static hot-path
 // subset: Nodes, NodeSets, ElementSets, Elements (Beam_1, Shell_1, Pipe_1),
-// Materials (Hooke), Sections (Tube), PipeSections, ShellSections (Homogeneous),
+// Materials (Hooke), Sections (Tube, Rectangle), PipeSections, ShellSections (Homogeneous),
 // CoordinateSystems, Loads (NodalLoad, NodalFollowerLoad, ShellLoad), Constraints
 // (NodalConstraint), Environment (GravityData, OceanData without an active sea
 // current), SolutionSteps (one Static), ConvergenceCriteria.  Output-only blocks (Monitor, PostFiles, SolverOptions,
@@ -19,6 +19,7 @@
 //   PS (PipeSections)   src/PipeSection.cpp:19-131 (EA EI GJ GA Rho CDt CDn CAt CAn De Di)
 //   Hooke               src/Hooke.cpp:16-49 (E Nu Rho)
 //   Tube                src/SecTube.cpp:22-44 (De Di), properties :86-106
+//   Rectangle           src/SecRectangle.cpp:19-75 (B H [AD ...]), properties :84-97
 //   Homogeneous         src/ShellSectionHomogeneous.cpp:17-30 (Thickness)
 //   CS                  src/CoordinateSystem.cpp:23-92 (E1 ... E3 ...)
 //   NodalLoad           src/NodalLoad.cpp:36-100 (NodeSet CS NTimes + rows t FX..MZ)
@@ -48,6 +49,33 @@
 namespace {
 
 const double PI = 3.1415926535897932384626433832795;
+
+// Torsion constant of a B x H rectangle: the series of the reference
+// (src/SecRectangle.cpp:91-96), odd n up to 21.
+double rect_torsion(double b, double h)
+{
+    double sum = 0.0;
+    for (int n = 1; n < 22; n += 2) sum += std::tanh(n * PI * h / (2.0 * b)) / std::pow((double)n, 5);
+    return b * b * b * h / 3.0 * (1.0 - 192.0 * b * sum / (std::pow(PI, 5) * h));
+}
+
+// A, I11, I22, It of a section {kind, a, b}: Tube (De, Di), SecTube::PreCalc
+// src/SecTube.cpp:86-106; Rectangle (B, H), SecRectangle::PreCalc
+// src/SecRectangle.cpp:84-97.
+void section_props(const std::array<double, 3> &sec, double &A, double &I11, double &I22, double &It)
+{
+    const double a = sec[1], b = sec[2];
+    if (sec[0] == 0.0) {
+        A = PI / 4.0 * (a * a - b * b);
+        I11 = I22 = PI / 64.0 * (std::pow(a, 4) - std::pow(b, 4));
+        It = PI / 32.0 * (std::pow(a, 4) - std::pow(b, 4));
+    } else {
+        A = a * b;
+        I11 = a * b * b * b / 12.0;
+        I22 = b * a * a * a / 12.0;
+        It = rect_torsion(a, b);
+    }
+}
 
 struct Tokens {
     std::vector<std::string> t;
@@ -108,7 +136,7 @@ struct gf_deck {
     std::map<int, std::vector<int>> nodesets, elemsets;
     std::vector<Elem> elems;
     std::map<int, std::array<double, 3>> materials;
-    std::map<int, std::array<double, 2>> tubes;
+    std::map<int, std::array<double, 3>> secs;  // kind (0 Tube, 1 Rectangle), De Di | B H
     std::map<int, std::array<double, 7>> pipes;  // PS: EA EI GJ GA Rho De Di
     int ocean_on = 0;
     double rho_fluid = 0.0, surface[3] = {0, 0, 0};
@@ -257,14 +285,16 @@ int parse(Tokens &tk, gf_deck &d)
             int n = std::atoi(tk.next().c_str());
             for (int i = 0; i < n; i++) {
                 std::string type = tk.next();
-                if (type != "Tube") return fail("section " + type + " is not on the static hot path (Tube only)");
+                if (type != "Tube" && type != "Rectangle")
+                    return fail("section " + type + " is not on the static hot path (Tube and Rectangle only)");
+                const bool tube = type == "Tube";
                 int id = std::atoi(tk.next().c_str());
-                if (!expect(tk, "De")) return fail("Tube: expected De");
-                double De = std::atof(tk.next().c_str());
-                if (!expect(tk, "Di")) return fail("Tube: expected Di");
-                double Di = std::atof(tk.next().c_str());
-                if (tk.peek() == "AD") return fail("Tube: aerodynamic data is not on the static hot path");
-                d.tubes[id] = {De, Di};
+                if (!expect(tk, tube ? "De" : "B")) return fail(type + (tube ? ": expected De" : ": expected B"));
+                double a = std::atof(tk.next().c_str());
+                if (!expect(tk, tube ? "Di" : "H")) return fail(type + (tube ? ": expected Di" : ": expected H"));
+                double b = std::atof(tk.next().c_str());
+                if (tk.peek() == "AD") return fail(type + ": aerodynamic data is not on the static hot path");
+                d.secs[id] = {tube ? 0.0 : 1.0, a, b};
             }
         } else if (kw == "PipeSections") {
             // PipeSection::Read src/PipeSection.cpp:19-131
@@ -488,8 +518,8 @@ int resolve(gf_deck &d)
         if (e.type == 0) {
             d.beam_kind.push_back(GF_BEAM_KIND_BEAM1);
             d.beam_buoy.push_back(0.0);
-            auto si = d.tubes.find(e.sec);
-            if (si == d.tubes.end()) return fail("section " + std::to_string(e.sec) + " is not defined");
+            auto si = d.secs.find(e.sec);
+            if (si == d.secs.end()) return fail("section " + std::to_string(e.sec) + " is not defined");
             auto ci = d.css.find(e.cs);
             if (ci == d.css.end()) return fail("coordinate system " + std::to_string(e.cs) + " is not defined");
             for (int a = 0; a < 3; a++) {
@@ -501,11 +531,9 @@ int resolve(gf_deck &d)
             double cs[9];
             cs_rows(ci->second, cs);
             d.beam_cs.insert(d.beam_cs.end(), cs, cs + 9);
-            // SecTube::PreCalc src/SecTube.cpp:86-106
-            double De = si->second[0], Di = si->second[1];
-            double A = PI / 4.0 * (De * De - Di * Di), I = PI / 64.0 * (std::pow(De, 4) - std::pow(Di, 4));
-            double It = PI / 32.0 * (std::pow(De, 4) - std::pow(Di, 4));
-            double prop[GF_BEAM_NPROP] = {mi->second[0], mi->second[1], mi->second[2], A, I, I, 0.0, It, e.T0};
+            double A, I11, I22, It;
+            section_props(si->second, A, I11, I22, It);
+            double prop[GF_BEAM_NPROP] = {mi->second[0], mi->second[1], mi->second[2], A, I11, I22, 0.0, It, e.T0};
             d.beam_prop.insert(d.beam_prop.end(), prop, prop + GF_BEAM_NPROP);
         } else {
             auto ti = d.shell_t.find(e.sec);
@@ -650,7 +678,7 @@ extern "C" int gf_deck_destroy(gf_deck *d)
     return GF_OK;
 }
 
-// Writes the description as a deck of the same subset: one Hooke / Tube or
+// Writes the description as a deck of the same subset: one Hooke / Tube / Rectangle or
 // Homogeneous / CS per distinct combination, node sets per load and per
 // constraint pattern, element sets per shell load.
 extern "C" int gf_deck_write(const char *path, const gf_model_desc *g, const gf_static_params *p)
@@ -665,7 +693,7 @@ extern "C" int gf_deck_write(const char *path, const gf_model_desc *g, const gf_
           << num(g->node_xyz[3 * i + 2]) << "\n";
     // materials, sections, coordinate systems (deduplicated)
     std::map<std::array<double, 3>, int> mats;
-    std::map<std::array<double, 2>, int> tubes;
+    std::map<std::array<double, 3>, int> tubes;  // kind, De Di | B H
     std::map<double, int> thick;
     std::map<std::array<double, 6>, int> css;
     auto mat_id = [&](double E, double nu, double rho) {
@@ -711,10 +739,20 @@ extern "C" int gf_deck_write(const char *path, const gf_model_desc *g, const gf_
             continue;
         }
         double A = pr[3], I = pr[4];
-        if (pr[4] != pr[5] || pr[6] != 0.0 || std::fabs(pr[7] - 2.0 * I) > 1e-12 * I)
-            return fail("beam " + std::to_string(e) + " is not a tube section");
-        double s = 16.0 * I / A, dd = 4.0 * A / PI;  // De^2 + Di^2, De^2 - Di^2
-        std::array<double, 2> tk{std::sqrt(0.5 * (s + dd)), std::sqrt(std::fmax(0.0, 0.5 * (s - dd)))};
+        std::array<double, 3> tk;
+        if (pr[6] == 0.0 && pr[4] == pr[5] && std::fabs(pr[7] - 2.0 * I) <= 1e-12 * I) {
+            double s = 16.0 * I / A, dd = 4.0 * A / PI;  // De^2 + Di^2, De^2 - Di^2
+            tk = {0.0, std::sqrt(0.5 * (s + dd)), std::sqrt(std::fmax(0.0, 0.5 * (s - dd)))};
+        } else {
+            // a B x H rectangle: I11 / I22 = (H / B)^2, A = B H
+            const double r = std::sqrt(pr[4] / pr[5]), B = std::sqrt(A / r);
+            tk = {1.0, B, A / B};
+            double A2, I11, I22, It;
+            section_props(tk, A2, I11, I22, It);
+            if (pr[6] != 0.0 || std::fabs(A2 - A) > 1e-12 * A || std::fabs(I11 - pr[4]) > 1e-12 * pr[4] ||
+                std::fabs(I22 - pr[5]) > 1e-12 * pr[5] || std::fabs(It - pr[7]) > 1e-9 * pr[7])
+                return fail("beam " + std::to_string(e) + " is neither a tube nor a rectangle section");
+        }
         int sec;
         auto it = tubes.find(tk);
         if (it != tubes.end()) sec = it->second;
@@ -801,10 +839,15 @@ extern "C" int gf_deck_write(const char *path, const gf_model_desc *g, const gf_
           << num(m.second[2]) << "\n";
     if (!tubes.empty()) {
         o << "Sections\t" << tubes.size() << "\n";
-        std::vector<std::pair<int, std::array<double, 2>>> tv;
+        std::vector<std::pair<int, std::array<double, 3>>> tv;
         for (auto &t : tubes) tv.push_back({t.second, t.first});
         std::sort(tv.begin(), tv.end());
-        for (auto &t : tv) o << "Tube\t" << t.first << "\tDe\t" << num(t.second[0]) << "\tDi\t" << num(t.second[1]) << "\n";
+        for (auto &t : tv) {
+            if (t.second[0] == 0.0)
+                o << "Tube\t" << t.first << "\tDe\t" << num(t.second[1]) << "\tDi\t" << num(t.second[2]) << "\n";
+            else
+                o << "Rectangle\t" << t.first << "\tB\t" << num(t.second[1]) << "\tH\t" << num(t.second[2]) << "\n";
+        }
     }
     if (!pipes.empty()) {
         o << "PipeSections\t" << pipes.size() << "\n";

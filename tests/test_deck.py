@@ -249,7 +249,7 @@ def test_reference_layout_deck(tmp_path):
 
 
 @pytest.mark.parametrize("edit,msg", [
-    (("Tube\t1\tDe\t0.1\tDi\t0.08", "Rectangle\t1\tB\t0.1\tH\t0.1"), "Rectangle"),
+    (("Tube\t1\tDe\t0.1\tDi\t0.08", "SuperEllipse\t1\tA\t0.1\tB\t0.1\tN\t2"), "SuperEllipse"),
     (("SolverOptions", "Contacts\t0\nSolverOptions"), "Contacts"),
     (("SolutionSteps\t1", "SolutionSteps\t2"), "one solution step"),
     (("NodalLoad\t1\tNodeSet\t2\tCS\t1\tNTimes\t2", "NodalLoad\t1\tNodeSet\t2\tCS\t1\tMathCode"), "MathCode"),
@@ -259,3 +259,73 @@ def test_refusals(tmp_path, edit, msg):
     path.write_text(TUTORIAL_STYLE.replace(*edit))
     with pytest.raises(runtime.GiraffeError, match=msg):
         runtime.Deck(str(path))
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def reissner_cantilever(E, nu, b, h, L, P):
+    """Tip (x, z, rotation) of a geometrically exact planar beam (extension and
+    shear with shear factor 1, as Beam_1's D), clamped at s = 0 along z, under a
+    dead tip force P along x: theta'' = -(P / EI) z', integrated by shooting on
+    theta'(0) so that the tip moment vanishes."""
+    from scipy.integrate import solve_ivp
+    from scipy.optimize import brentq
+
+    A, I, G = b * h, b * h ** 3 / 12.0, E / (2.0 * (1.0 + nu))
+
+    def rhs(s, y):
+        th, k = y[0], y[1]
+        eps, gam = P * np.sin(th) / (E * A), P * np.cos(th) / (G * A)
+        rx = (1 + eps) * np.sin(th) + gam * np.cos(th)
+        rz = (1 + eps) * np.cos(th) - gam * np.sin(th)
+        return [k, -P * rz / (E * I), rx, rz]
+
+    def shoot(k0):
+        return solve_ivp(rhs, [0, L], [0, k0, 0, 0], rtol=1e-12, atol=1e-14).y[:, -1]
+
+    k0 = brentq(lambda k: shoot(k)[1], 1e-3, 50.0)
+    th, _, x, z = shoot(k0)
+    return x, z, th
+
+
+def test_tutorial01_deck_matches_the_exact_beam(oracle_lib):
+    """The reference's inputs/tutorial01 read unchanged (a fixture copied by
+    tests/golden/copy_tutorial_decks.py): five Beam_1 on a 1 m cantilever with a
+    0.1 x 0.1 Rectangle section, E = 1e7, a dead tip load of 1,000 N over ten
+    steps (PL^2/EI = 12, tip rotation ~84 degrees).  The section's properties
+    match the closed forms (square torsion constant 0.1406 a^4), and the oracle's
+    tip matches an independent integration of the exact beam to the five
+    elements' discretisation error (< 1e-3)."""
+    deck = runtime.Deck(os.path.join(GOLDEN, "tutorial01.inp"))
+    a = _desc_arrays(deck.desc())
+    assert deck.n_beams == 5 and deck.n_nodes == 11
+    pr = a["beam_prop"][:abi.GF_BEAM_NPROP]
+    assert pr[3] == pytest.approx(0.01, rel=1e-15)
+    assert pr[4] == pytest.approx(0.1 ** 4 / 12, rel=1e-14) and pr[5] == pr[4] and pr[6] == 0.0
+    assert pr[7] == pytest.approx(0.1406 * 0.1 ** 4, rel=1e-3)
+    assert list(a["fix"][:6]) == [1] * 6 and a["fix"][6:].sum() == 0
+    om = oracle_lib.OracleModel(deck, oracle_lib.SOLVER_BAND)
+    ro, io = om.static_solve(deck.params)
+    assert ro.aborted == 0 and ro.converged_steps == 10
+    tip = om.nodes()[-1]
+    x, z, th = reissner_cantilever(1e7, 0.3, 0.1, 0.1, 1.0, 1000.0)
+    assert abs(tip[0] - x) < 1e-3 * x and abs(tip[2] - z) < 1e-3 * z
+    assert abs(tip[4] - th) < 1e-3 * th and abs(tip[1]) < 1e-12
+
+
+def test_rectangle_section_round_trip(tmp_path):
+    """A B x H Rectangle (B != H) written back from its properties is the same section."""
+    text = open(os.path.join(GOLDEN, "tutorial01.inp")).read().replace("B\t0.1\tH\t0.1", "B\t0.05\tH\t0.2")
+    path = tmp_path / "r.inp"
+    path.write_text(text)
+    deck = runtime.Deck(str(path))
+    pr = _desc_arrays(deck.desc())["beam_prop"][:abi.GF_BEAM_NPROP]
+    assert pr[4] == pytest.approx(0.05 * 0.2 ** 3 / 12, rel=1e-14)
+    assert pr[5] == pytest.approx(0.2 * 0.05 ** 3 / 12, rel=1e-14)
+    p2 = str(tmp_path / "r2.inp")
+    runtime.write_deck(p2, deck, deck.params)
+    assert "Rectangle" in open(p2).read()
+    again = runtime.Deck(p2)
+    np.testing.assert_allclose(_desc_arrays(again.desc())["beam_prop"], _desc_arrays(deck.desc())["beam_prop"],
+                               rtol=1e-12)
