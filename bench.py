@@ -298,6 +298,7 @@ def serve(args):
     arrivals = np.cumsum(gaps)
     lat = {}
     pending = set()
+    index_of = {}
     t_start = time.perf_counter()
     for i, (m, p) in enumerate(reqs):
         while time.perf_counter() - t_start < arrivals[i]:
@@ -309,14 +310,17 @@ def serve(args):
                     continue
                 pending.discard(rid)
                 lat[rid] = (rep, times[1])
-        pending.add(srv.submit(m, p))
+        rid = srv.submit(m, p)
+        index_of[rid] = i
+        pending.add(rid)
         if (i + 1) % 200 == 0:
             log(f"submitted {i + 1} requests, {len(lat)} finished, {srv.counts()}")
     drain(srv, pending, lat, None)
     makespan = time.perf_counter() - t_start
     srv.close()
     lat_ms = np.array([v[1] for v in lat.values()])
-    aborted = sum(1 for v in lat.values() if v[0].aborted)
+    aborted = sorted(index_of[rid] for rid, v in lat.items() if v[0].aborted)
+    finished = [v[0] for v in lat.values() if not v[0].aborted]
     out = {
         "metric": METRIC,
         "value": len(lat) / makespan,
@@ -340,8 +344,9 @@ def serve(args):
                    "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))},
         "p50_ms": float(np.percentile(lat_ms, 50)),
         "p99_ms": float(np.percentile(lat_ms, 99)),
-        "aborted": aborted,
-        "max_rel_residual": max(v[0].max_rel_residual for v in lat.values()),
+        "aborted": len(aborted),
+        "aborted_requests": aborted[:20],
+        "max_rel_residual": max((r.max_rel_residual for r in finished), default=0.0),
     }
     if args.cpu_baseline != "none":
         out["cpu_baseline"] = serve_cpu_baseline(reqs, args.cpu_budget)

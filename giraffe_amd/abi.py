@@ -15,6 +15,7 @@ GF_E_CAPACITY = -7
 
 GF_PREC_FP64 = 0
 GF_PREC_MIXED = 1
+GF_GRAVITY_OFF, GF_GRAVITY_RAMP_UP, GF_GRAVITY_ON, GF_GRAVITY_RAMP_DOWN = 0, 1, 2, 3
 
 GF_LOAD_NODAL = 0
 GF_LOAD_NODAL_FOLLOWER = 1

@@ -3,8 +3,8 @@
 // Materials (Hooke), Sections (Tube, Rectangle), PipeSections, ShellSections (Homogeneous),
 // CoordinateSystems, Loads (NodalLoad, NodalFollowerLoad, ShellLoad), Constraints
 // (NodalConstraint), Environment (GravityData, OceanData without an active sea
-// current), SolutionSteps (one Static), ConvergenceCriteria.  Output-only blocks (Monitor, PostFiles, SolverOptions,
-// ExecutionData, ConfigurationSave) are skipped; any other top-level block
+// current), SolutionSteps (Static steps), ConvergenceCriteria.  Output-only blocks (Monitor, PostFiles, SolverOptions,
+// ExecutionData, ConfigurationSave, ConcomitantSolution) are skipped; any other top-level block
 // changes the physics and is refused with a message naming it.
 //
 // Grammar as the reference readers take it (token based, "//" starts a comment
@@ -143,10 +143,16 @@ struct gf_deck {
     std::map<int, double> shell_t;
     std::map<int, std::array<double, 6>> css;
     std::vector<Load> loads;
-    std::vector<std::pair<int, std::array<int, 6>>> constraints;
+    std::vector<std::pair<int, std::array<int, 6>>> constraints;  // values in every step
+    std::vector<std::pair<int, std::array<std::vector<int>, 6>>> ctables;  // BoolTables as read
     double G[3] = {0, 0, 0};
     int g_on = 0;
-    gf_static_params p{};
+    std::vector<int> g_table;  // gravity BoolTable
+    bool current_moving = false;
+    std::vector<int> current_table;  // sea-current BoolTable (when its speed is non-zero)
+    gf_static_params p{};     // ConvergenceCriteria (and the first step after resolve)
+    std::vector<gf_static_params> steps;
+    std::vector<int> g_mode;  // GF_GRAVITY_* per step
     bool have_static = false;
     // resolved descriptor arrays
     std::vector<unsigned char> fix;
@@ -166,6 +172,14 @@ int fail(const std::string &msg)
 }
 
 bool expect(Tokens &tk, const char *kw) { return tk.next() == kw; }
+
+// BoolTable::GetAt src/BoolTable.cpp:69-82: entry i, the last entry past the
+// end, false before the first (and for an empty table).
+bool bool_at(const std::vector<int> &t, int i)
+{
+    if (i < 0 || t.empty()) return false;
+    return t[std::min((size_t)i, t.size() - 1)] != 0;
+}
 
 // src/BoolTable.cpp:23-45: values while the next token starts with a digit
 std::vector<int> read_booltable(Tokens &tk)
@@ -201,8 +215,12 @@ int read_set(Tokens &tk, const char *head, const char *items, std::map<int, std:
 int parse(Tokens &tk, gf_deck &d)
 {
     const std::set<std::string> &top = top_keywords();
+    // output only: ConcomitantSolution runs a Modal analysis at sampled
+    // converged configurations and writes its eigenvalues to a file
+    // (ConcomitantSolution::UpdateConcomitantSolution src/ConcomitantSolution.cpp:106-124);
+    // it does not change the static solution, and its file is not written here
     const std::set<std::string> ignored = {"Monitor", "SolverOptions", "PostFiles", "ExecutionData",
-                                           "ConfigurationSave"};
+                                           "ConfigurationSave", "ConcomitantSolution"};
     while (!tk.done()) {
         std::string kw = tk.next();
         if (kw == "EOF") break;
@@ -371,7 +389,7 @@ int parse(Tokens &tk, gf_deck &d)
                 tk.next();  // id
                 if (!expect(tk, "NodeSet")) return fail("NodalConstraint: expected NodeSet");
                 int set = std::atoi(tk.next().c_str());
-                std::array<int, 6> f{};
+                std::array<std::vector<int>, 6> f{};
                 static const char *dofs[6] = {"UX", "UY", "UZ", "ROTX", "ROTY", "ROTZ"};
                 for (;;) {
                     int k = -1;
@@ -380,10 +398,9 @@ int parse(Tokens &tk, gf_deck &d)
                     if (k < 0) break;
                     tk.next();
                     if (!expect(tk, "BoolTable")) return fail("NodalConstraint: expected BoolTable");
-                    std::vector<int> b = read_booltable(tk);
-                    f[k] = b.empty() ? 0 : b[0];  // the table's entry for the first solution step
+                    f[k] = read_booltable(tk);
                 }
-                d.constraints.push_back({set, f});
+                d.ctables.push_back({set, f});
             }
         } else if (kw == "Environment") {
             while (!tk.done() && !top.count(tk.peek())) {
@@ -409,8 +426,8 @@ int parse(Tokens &tk, gf_deck &d)
                         }
                         moving = moving || v[1] != 0.0;
                     }
-                    if (moving && !b.empty() && b[0] != 0)
-                        return fail("SeaCurrent: Morison loads of a sea current are not on the static hot path");
+                    d.current_moving = moving;
+                    d.current_table = b;
                     d.ocean_on = 1;
                     continue;
                 }
@@ -418,31 +435,39 @@ int parse(Tokens &tk, gf_deck &d)
                 if (!expect(tk, "G")) return fail("GravityData: expected G");
                 for (int k = 0; k < 3; k++) d.G[k] = std::atof(tk.next().c_str());
                 if (!expect(tk, "BoolTable")) return fail("GravityData: expected BoolTable");
-                std::vector<int> b = read_booltable(tk);
-                d.g_on = b.empty() ? 0 : b[0];
+                d.g_table = read_booltable(tk);
             }
         } else if (kw == "SolutionSteps") {
+            // Static::Read src/Static.cpp:43-130, one per step; start times follow
+            // Database::EvaluateStartEndTimes src/Database.cpp:760-770
             int n = std::atoi(tk.next().c_str());
-            if (n != 1) return fail("exactly one solution step is supported");
-            if (!expect(tk, "Static")) return fail("only a Static solution step is on the hot path");
-            tk.next();  // id
-            static const char *keys[9] = {"EndTime", "TimeStep", "MaxTimeStep", "MinTimeStep", "MaxIt",
-                                          "MinIt", "ConvIncrease", "IncFactor", "Sample"};
-            double v[9];
-            for (int k = 0; k < 9; k++) {
-                if (!expect(tk, keys[k])) return fail(std::string("Static: expected ") + keys[k]);
-                v[k] = std::atof(tk.next().c_str());
+            if (n < 1) return fail("SolutionSteps: no solution step");
+            double start = 0.0;
+            for (int i = 0; i < n; i++) {
+                std::string type = tk.next();
+                if (type != "Static") return fail("solution step " + type + " is not on the static hot path");
+                tk.next();  // id
+                static const char *keys[9] = {"EndTime", "TimeStep", "MaxTimeStep", "MinTimeStep", "MaxIt",
+                                              "MinIt", "ConvIncrease", "IncFactor", "Sample"};
+                double v[9];
+                for (int k = 0; k < 9; k++) {
+                    if (!expect(tk, keys[k])) return fail(std::string("Static: expected ") + keys[k]);
+                    v[k] = std::atof(tk.next().c_str());
+                }
+                gf_static_params q{};
+                q.start_time = start;
+                q.end_time = v[0];
+                q.time_step = v[1];
+                q.max_time_step = v[2];
+                q.min_time_step = v[3];
+                q.max_it = (int)v[4];
+                q.min_it = (int)v[5];
+                q.conv_increase = (int)v[6];
+                q.inc_factor = v[7];
+                q.sample = (int)v[8];
+                d.steps.push_back(q);
+                start = q.end_time;
             }
-            d.p.start_time = 0.0;
-            d.p.end_time = v[0];
-            d.p.time_step = v[1];
-            d.p.max_time_step = v[2];
-            d.p.min_time_step = v[3];
-            d.p.max_it = (int)v[4];
-            d.p.min_it = (int)v[5];
-            d.p.conv_increase = (int)v[6];
-            d.p.inc_factor = v[7];
-            d.p.sample = (int)v[8];
             d.have_static = true;
         } else if (kw == "ConvergenceCriteria") {
             static const char *keys[12] = {"ForceTolerance", "MomentTolerance", "ForceMinimumReference",
@@ -462,6 +487,46 @@ int parse(Tokens &tk, gf_deck &d)
         }
     }
     if (!d.have_static) return fail("no Static solution step");
+    // per step: the criteria (one ConvergenceCriteria for all steps), the gravity
+    // factor mode from the table's entries for this and the previous step
+    // (BoolTable::GetAt src/BoolTable.cpp:69-82: past the end the last entry,
+    // before the first false), the constraints, which must not change
+    const int n = (int)d.steps.size();
+    d.g_on = 0;
+    for (int i = 0; i < n; i++) {
+        gf_static_params &q = d.steps[i];
+        const double t0 = q.start_time, t1 = q.end_time, dt = q.time_step, dtmax = q.max_time_step,
+                     dtmin = q.min_time_step, fi = q.inc_factor;
+        const int mi = q.max_it, mn = q.min_it, ci = q.conv_increase, sa = q.sample;
+        q = d.p;
+        q.start_time = t0;
+        q.end_time = t1;
+        q.time_step = dt;
+        q.max_time_step = dtmax;
+        q.min_time_step = dtmin;
+        q.max_it = mi;
+        q.min_it = mn;
+        q.conv_increase = ci;
+        q.inc_factor = fi;
+        q.sample = sa;
+        const bool now = bool_at(d.g_table, i), before = bool_at(d.g_table, i - 1);
+        d.g_mode.push_back(now ? (before ? GF_GRAVITY_ON : GF_GRAVITY_RAMP_UP)
+                               : (before ? GF_GRAVITY_RAMP_DOWN : GF_GRAVITY_OFF));
+        if (now) d.g_on = 1;
+        if (d.current_moving && bool_at(d.current_table, i))
+            return fail("SeaCurrent: Morison loads of a sea current are not on the static hot path");
+    }
+    for (auto &c : d.ctables) {
+        std::array<int, 6> f{};
+        for (int k = 0; k < 6; k++) {
+            f[k] = bool_at(c.second[k], 0) ? 1 : 0;
+            for (int i = 1; i < n; i++)
+                if (bool_at(c.second[k], i) != (f[k] != 0))
+                    return fail("constraints that change between solution steps are not supported");
+        }
+        d.constraints.push_back({c.first, f});
+    }
+    d.p = d.steps[0];
     return GF_OK;
 }
 
@@ -669,6 +734,21 @@ extern "C" int gf_deck_desc(const gf_deck *d, gf_model_desc *desc, gf_static_par
     if (!d) return GF_E_ARG;
     if (desc) *desc = d->desc;
     if (p) *p = d->p;
+    return GF_OK;
+}
+
+extern "C" int gf_deck_steps(const gf_deck *d, int *n)
+{
+    if (!d || !n) return GF_E_ARG;
+    *n = (int)d->steps.size();
+    return GF_OK;
+}
+
+extern "C" int gf_deck_step(const gf_deck *d, int i, gf_static_params *p, int *gravity_mode)
+{
+    if (!d || i < 0 || i >= (int)d->steps.size()) return GF_E_ARG;
+    if (p) *p = d->steps[i];
+    if (gravity_mode) *gravity_mode = d->g_mode[i];
     return GF_OK;
 }
 

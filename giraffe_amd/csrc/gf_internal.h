@@ -306,6 +306,7 @@ struct gf_model {
     double t_last = 0, dt = 0;
     double gravity[3] = {0, 0, 0};
     int gravity_on = 0;
+    int gravity_mode = 0;  // GF_GRAVITY_*: the BoolTable factor across solution steps
     int half_band = 0;
     int band_ld = 0;                // GM_BAND row stride: half_band + gf_band_ku(half_band)
     bool node_order_rcm = false;    // DOFs numbered in reverse Cuthill-McKee node order

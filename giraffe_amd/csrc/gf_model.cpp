@@ -623,6 +623,7 @@ static int create(const gf_model_desc *d, const gf_options *opt, gf_model *m)
     }
     for (int k = 0; k < 3; k++) m->gravity[k] = d->gravity[k];
     m->gravity_on = d->gravity_on;
+    m->gravity_mode = d->gravity_on ? GF_GRAVITY_RAMP_UP : GF_GRAVITY_OFF;
     // global system
     size_t npad = (size_t)m->npad;
     GF_TRY(dalloc(&m->d_r, npad));
@@ -788,14 +789,33 @@ extern "C" int gf_model_stats(const gf_model *m, long long *out)
     return GF_OK;
 }
 
-// The step's load time t = t_last + dt and gravity ramp factor (the BoolTable
-// ramp over the solution window) into the scalar block, where the assembly
-// kernels read them.
+// The step's load time t = t_last + dt and gravity factor into the scalar
+// block, where the assembly kernels read them.  The factor follows
+// BoolTable::GetLinearFactorAtCurrentTime (src/BoolTable.cpp:84-109): a ramp
+// 0 -> 1 across the solution window where gravity turns on, 1 while it stays
+// on, 1 -> 0 where it turns off.
 static int set_step_scalars(gf_model *m)
 {
     const double t = m->t_last + m->dt;
-    const double lf = m->gravity_on ? (t - m->sol_start) / (m->sol_end - m->sol_start) : 0.0;
+    const double r = (t - m->sol_start) / (m->sol_end - m->sol_start);
+    double lf = 0.0;
+    if (m->gravity_on) {
+        switch (m->gravity_mode) {
+        case GF_GRAVITY_RAMP_UP: lf = r; break;
+        case GF_GRAVITY_ON: lf = 1.0; break;
+        case GF_GRAVITY_RAMP_DOWN: lf = 1.0 - r; break;
+        default: lf = 0.0;
+        }
+    }
     return gf_launch_set2(m->stream, m->d_scal + GF_SCAL_TIME, t, lf);
+}
+
+extern "C" int gf_set_gravity_mode(gf_model *m, int mode)
+{
+    if (!m || mode < GF_GRAVITY_OFF || mode > GF_GRAVITY_RAMP_DOWN) return GF_E_ARG;
+    m->gravity_mode = mode;
+    GF_HIP(hipSetDevice(m->device));
+    return set_step_scalars(m);
 }
 
 extern "C" int gf_begin_step(gf_model *m, double t_last, double dt)

@@ -242,3 +242,24 @@ extern "C" int gf_static_solve(gf_model *m, const gf_static_params *p, gf_static
     if (rep) *rep = run.r;
     return GF_OK;
 }
+
+// The sequence of solution steps as the reference's main loop runs them
+// (src/main.cpp:53-62): each Static step in turn, its gravity factor mode set
+// first, stopping after a step that aborts (Static::Solve returns false).  The
+// state carries over; each step starts its own criteria history, as
+// Static::Solve does (src/Static.cpp:146).  reps[i] is filled for the steps
+// run; *n_run counts them.
+extern "C" int gf_static_solve_steps(gf_model *m, int n, const gf_static_params *steps, const int *gravity_modes,
+                                     gf_static_report *reps, int *n_run)
+{
+    if (!m || n < 0 || (n && (!steps || !gravity_modes || !reps)) || !n_run) return GF_E_ARG;
+    *n_run = 0;
+    for (int i = 0; i < n; i++) {
+        int st = gf_set_gravity_mode(m, gravity_modes[i]);
+        if (st) return st;
+        if ((st = gf_static_solve(m, &steps[i], &reps[i]))) return st;
+        *n_run = i + 1;
+        if (reps[i].aborted) break;
+    }
+    return GF_OK;
+}

@@ -55,6 +55,11 @@ def load():
     L.gf_commit.argtypes = [vp]
     L.gf_rollback.argtypes = [vp]
     L.gf_static_solve.argtypes = [vp, C.POINTER(abi.GfStaticParams), C.POINTER(abi.GfStaticReport)]
+    L.gf_set_gravity_mode.argtypes = [vp, C.c_int]
+    L.gf_static_solve_steps.argtypes = [vp, C.c_int, C.POINTER(abi.GfStaticParams), C.POINTER(C.c_int),
+                                        C.POINTER(abi.GfStaticReport), C.POINTER(C.c_int)]
+    L.gf_deck_steps.argtypes = [vp, C.POINTER(C.c_int)]
+    L.gf_deck_step.argtypes = [vp, C.c_int, C.POINTER(abi.GfStaticParams), C.POINTER(C.c_int)]
     for f in ("gf_read_nodes", "gf_read_disp", "gf_read_residual", "gf_read_increment", "gf_write_disp"):
         getattr(L, f).argtypes = [vp, dp]
     L.gf_read_tangent_dense.argtypes = [vp, dp, C.c_longlong]
@@ -252,6 +257,25 @@ class DeviceModel:
         n = min(rep.n_step_iters, cap)
         return rep, iters[:n].copy()
 
+    def set_gravity_mode(self, mode):
+        check(load().gf_set_gravity_mode(self.h, int(mode)), "gf_set_gravity_mode")
+
+    def static_solve_steps(self, steps, cap=4096):
+        """gf_static_solve_steps: steps = [(params, gravity_mode)] in sequence,
+        stopping after a step that aborts.  Returns [(report, iterations)] for
+        the steps run."""
+        n = len(steps)
+        iters = [np.zeros(cap, np.int32) for _ in range(n)]
+        ps = (abi.GfStaticParams * n)(*[q.c() for q, _ in steps])
+        modes = (C.c_int * n)(*[int(m) for _, m in steps])
+        reps = (abi.GfStaticReport * n)()
+        for r, it in zip(reps, iters):
+            r.step_iters = it.ctypes.data_as(C.POINTER(C.c_int))
+            r.step_iters_cap = cap
+        run = C.c_int(0)
+        check(load().gf_static_solve_steps(self.h, n, ps, modes, reps, C.byref(run)), "gf_static_solve_steps")
+        return [(reps[i], iters[i][:min(reps[i].n_step_iters, cap)].copy()) for i in range(run.value)]
+
 
 def beam_eval(x, cs, prop, d, gp):
     """Element-level Beam_1::Mount on the device for independent elements.
@@ -399,6 +423,14 @@ class Deck:
         check(L.gf_deck_desc(self.h, C.byref(d), C.byref(gp)), "gf_deck_desc")
         self._desc = d
         self.params = StaticParams(**{f: getattr(gp, f) for f, _ in abi.GfStaticParams._fields_})
+        n = C.c_int(0)
+        check(L.gf_deck_steps(self.h, C.byref(n)), "gf_deck_steps")
+        self.steps = []  # [(StaticParams, gravity mode)] of every solution step
+        for i in range(n.value):
+            q, mode = abi.GfStaticParams(), C.c_int(0)
+            check(L.gf_deck_step(self.h, i, C.byref(q), C.byref(mode)), "gf_deck_step")
+            self.steps.append((StaticParams(**{f: getattr(q, f) for f, _ in abi.GfStaticParams._fields_}),
+                               mode.value))
 
     @property
     def n_nodes(self):

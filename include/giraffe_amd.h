@@ -50,6 +50,14 @@ extern "C" {
 #define GF_PREC_FP64   0     /* tangent assembled and factored in FP64          */
 #define GF_PREC_MIXED  1     /* FP32 tangent + FP32 factor, FP64 refinement     */
 
+/* ---- gravity factor of a solution step (BoolTable::GetLinearFactorAtCurrentTime,
+ * src/BoolTable.cpp:84-109: the gravity BoolTable's entries for this and the
+ * previous solution step) -------------------------------------------------- */
+#define GF_GRAVITY_OFF        0  /* off now and before: factor 0                   */
+#define GF_GRAVITY_RAMP_UP    1  /* on now, off before: 0 -> 1 across the window   */
+#define GF_GRAVITY_ON         2  /* on now and before: factor 1                    */
+#define GF_GRAVITY_RAMP_DOWN  3  /* off now, on before: 1 -> 0 across the window   */
+
 /* ---- load kinds (src/NodalLoad.cpp, src/NodalFollowerLoad.cpp, src/ShellLoad.cpp) */
 #define GF_LOAD_NODAL           0  /* dead force + pseudo-moment (NodalLoad)      */
 #define GF_LOAD_NODAL_FOLLOWER  1  /* follower force / moment (NodalFollowerLoad) */
@@ -185,6 +193,10 @@ int gf_model_plan(const gf_model_desc *desc, long long *info_out /* [4] */);
  * (BoolTable::GetLinearFactorAtCurrentTime src/BoolTable.cpp:84-109) reads it.
  * gf_static_solve sets it from its parameters. */
 int gf_set_solution_window(gf_model *m, double start_time, double end_time);
+/* Gravity factor mode of the next solution step (GF_GRAVITY_*); a model starts
+ * in GF_GRAVITY_RAMP_UP when its description has gravity on, the first
+ * solution step of a deck whose gravity table is on. */
+int gf_set_gravity_mode(gf_model *m, int mode);
 /* New load step: incremental displacements := 0, load time = t_last + dt. */
 int gf_begin_step(gf_model *m, double t_last, double dt);
 /* Assemble K and r = f_ext - f_int at the current state; res_out[0..2] =
@@ -201,6 +213,11 @@ int gf_rollback(gf_model *m);
 
 /* Whole Static::Solve loop (adaptive steps, bisection, convergence criteria). */
 int gf_static_solve(gf_model *m, const gf_static_params *p, gf_static_report *rep);
+/* The solution-step sequence of the reference's main loop (src/main.cpp:53-62):
+ * steps[i] with gravity mode gravity_modes[i] in turn on the same state,
+ * stopping after a step that aborts; reps[i] for each step run, *n_run of them. */
+int gf_static_solve_steps(gf_model *m, int n, const gf_static_params *steps, const int *gravity_modes,
+                          gf_static_report *reps, int *n_run);
 
 /* Return the model to the state right after gf_model_create (reference
  * configuration, identity Gauss-point rotations, no mixed-precision back-off),
@@ -317,17 +334,24 @@ int gf_shell_eval(int count, const double *x, const double *prop, const double *
 /* ---- input decks (src/IO.cpp:193-770 dispatch, hot-path subset) -------------
  * gf_deck_read parses a GIRAFFE .inp deck restricted to what this path runs:
  * Nodes, NodeSets, ElementSets, Elements (Beam_1, Shell_1, Pipe_1), Materials
- * (Hooke), Sections (Tube), PipeSections, ShellSections (Homogeneous),
- * CoordinateSystems, Loads (NodalLoad, NodalFollowerLoad, ShellLoad with NTimes
- * tables), Constraints (NodalConstraint), Environment (GravityData; OceanData
- * whose sea current is off or still), SolutionSteps (one Static) and
+ * (Hooke), Sections (Tube, Rectangle), PipeSections, ShellSections
+ * (Homogeneous), CoordinateSystems, Loads (NodalLoad, NodalFollowerLoad,
+ * ShellLoad with NTimes tables), Constraints (NodalConstraint, the same in every
+ * solution step), Environment (GravityData; OceanData whose sea current is off
+ * or still), SolutionSteps (Static steps, run in sequence) and
  * ConvergenceCriteria; Monitor, PostFiles, SolverOptions, ExecutionData and
  * ConfigurationSave are skipped; anything else is refused with a message.
  * gf_deck_desc returns the resolved description (arrays owned by the deck) and
- * the Static parameters.  gf_deck_write emits a description as such a deck. */
+ * the first step's Static parameters; gf_deck_steps / gf_deck_step give every
+ * step (start time = the previous step's end time,
+ * Database::EvaluateStartEndTimes src/Database.cpp:760-770) with its gravity mode (GF_GRAVITY_*) for
+ * gf_static_solve_steps.  gf_deck_write emits a description and one step as
+ * such a deck. */
 typedef struct gf_deck gf_deck;
 int gf_deck_read(const char *path, gf_deck **out);
 int gf_deck_desc(const gf_deck *deck, gf_model_desc *desc_out, gf_static_params *params_out);
+int gf_deck_steps(const gf_deck *deck, int *n_steps);
+int gf_deck_step(const gf_deck *deck, int step, gf_static_params *params_out, int *gravity_mode);
 int gf_deck_destroy(gf_deck *deck);
 int gf_deck_write(const char *path, const gf_model_desc *desc, const gf_static_params *params);
 

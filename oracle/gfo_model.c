@@ -74,6 +74,7 @@ typedef struct gfo_model {
     double *mult;             /* [nl][6] */
     double grav[3];
     int grav_on;
+    int grav_mode; /* 0 off, 1 ramp up, 2 on, 3 ramp down (BoolTable across solutions) */
     double *bbuoy;            /* [nb] Pipe_1 buoyant mass per length (with OceanData) */
     int ocean_on;
     double surf[3];
@@ -182,6 +183,7 @@ gfo_model *gfo_create(const gf_model_desc *desc, int solver)
     m->mult = (double *)calloc(6 * (size_t)(nl ? nl : 1), sizeof(double));
     for (int k = 0; k < 3; k++) m->grav[k] = desc->gravity[k];
     m->grav_on = desc->gravity_on;
+    m->grav_mode = m->grav_on ? 1 : 0;
     m->ocean_on = desc->ocean_on;
     for (int k = 0; k < 3; k++) m->surf[k] = desc->ocean_surface[k];
     m->bbuoy = (double *)calloc((size_t)m->nb + 1, sizeof(double));
@@ -298,9 +300,19 @@ void gfo_begin_step(gfo_model *m, double t_last, double dt)
 
 static double gravity_factor(const gfo_model *m)
 {
-    /* BoolTable ramp for a gravity table that is on in the first solution. */
-    return (m->t_last + m->dt - m->sol_start) / (m->sol_end - m->sol_start);
+    /* BoolTable::GetLinearFactorAtCurrentTime src/BoolTable.cpp:84-109: on in this solution and
+       off in the previous one ramps up across the solution, on in both keeps 1, off now and on
+       before ramps down, off in both is 0. */
+    const double r = (m->t_last + m->dt - m->sol_start) / (m->sol_end - m->sol_start);
+    switch (m->grav_mode) {
+    case 1: return r;
+    case 2: return 1.0;
+    case 3: return 1.0 - r;
+    default: return 0.0;
+    }
 }
+
+void gfo_set_gravity_mode(gfo_model *m, int mode) { m->grav_mode = mode; }
 
 /* ------------------------------------------------------------ global adds */
 static inline void kadd(gfo_model *m, int gi, int gj, double v);

@@ -60,6 +60,8 @@ def lib():
         L.gfo_bandwidth.argtypes = [vp]
         L.gfo_dof_map.argtypes = [vp, C.POINTER(C.c_int)]
         L.gfo_pardiso_available.restype = C.c_int
+        L.gfo_set_gravity_mode.argtypes = [vp, C.c_int]
+        L.gfo_set_gravity_mode.restype = None
         L.gfo_pardiso_threads.argtypes = [C.c_int]
         L.gfo_pardiso_threads.restype = C.c_int
         L.gfo_residual_criteria.argtypes = [dp, C.POINTER(C.c_int), C.c_double, C.c_double,
@@ -220,6 +222,22 @@ class OracleModel:
             raise RuntimeError(f"gfo_static_solve: {st}")
         n = min(rep.n_step_iters, cap)
         return rep, iters[:n].copy()
+
+    def set_gravity_mode(self, mode):
+        """Gravity factor of the next solution step: 0 off, 1 ramp up, 2 on, 3 ramp down."""
+        lib().gfo_set_gravity_mode(self.h, int(mode))
+
+    def static_solve_steps(self, steps, cap=4096):
+        """The solution steps in sequence as the reference's main loop runs them
+        (src/main.cpp:53-62), stopping after a step that aborts; steps = [(params,
+        gravity_mode)].  Returns [(report, iterations)] for the steps run."""
+        out = []
+        for params, mode in steps:
+            self.set_gravity_mode(mode)
+            out.append(self.static_solve(params, cap))
+            if out[-1][0].aborted:
+                break
+        return out
 
 
 # ---- element-level helpers ----------------------------------------------------

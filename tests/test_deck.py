@@ -251,7 +251,7 @@ def test_reference_layout_deck(tmp_path):
 @pytest.mark.parametrize("edit,msg", [
     (("Tube\t1\tDe\t0.1\tDi\t0.08", "SuperEllipse\t1\tA\t0.1\tB\t0.1\tN\t2"), "SuperEllipse"),
     (("SolverOptions", "Contacts\t0\nSolverOptions"), "Contacts"),
-    (("SolutionSteps\t1", "SolutionSteps\t2"), "one solution step"),
+    (("SolutionSteps\t1\nStatic\t1", "SolutionSteps\t1\nDynamic\t1"), "Dynamic"),
     (("NodalLoad\t1\tNodeSet\t2\tCS\t1\tNTimes\t2", "NodalLoad\t1\tNodeSet\t2\tCS\t1\tMathCode"), "MathCode"),
 ])
 def test_refusals(tmp_path, edit, msg):
@@ -329,3 +329,48 @@ def test_rectangle_section_round_trip(tmp_path):
     again = runtime.Deck(p2)
     np.testing.assert_allclose(_desc_arrays(again.desc())["beam_prop"], _desc_arrays(deck.desc())["beam_prop"],
                                rtol=1e-12)
+
+
+def test_multi_step_decks_parse_with_their_gravity_modes():
+    """Several Static steps run in sequence (src/main.cpp:53-62): start times follow
+    the previous end times (Database::EvaluateStartEndTimes); the gravity factor of
+    a step follows the BoolTable's entries for it and the step before
+    (src/BoolTable.cpp:84-109): tutorial07's table 0 0 1 1 1 0 gives off, off,
+    ramp up, on, on, ramp down."""
+    d7 = runtime.Deck(os.path.join(GOLDEN, "tutorial07.inp"))
+    assert [m for _, m in d7.steps] == [abi.GF_GRAVITY_OFF, abi.GF_GRAVITY_OFF, abi.GF_GRAVITY_RAMP_UP,
+                                        abi.GF_GRAVITY_ON, abi.GF_GRAVITY_ON, abi.GF_GRAVITY_RAMP_DOWN]
+    assert [(q.start_time, q.end_time) for q, _ in d7.steps] == [(i, i + 1.0) for i in range(6)]
+    assert d7.desc().gravity_on == 1 and list(d7.desc().gravity) == [9.81, 0.0, 0.0]
+    d3 = runtime.Deck(os.path.join(GOLDEN, "tutorial03.inp"))
+    assert [(q.time_step, q.min_time_step) for q, _ in d3.steps] == [(1.0, 0.01), (0.005, 0.002)]
+    assert d3.steps[0][0].force_tol == d3.steps[1][0].force_tol  # one ConvergenceCriteria for all steps
+    for name in ("tutorial08", "tutorial09"):
+        assert len(runtime.Deck(os.path.join(GOLDEN, name + ".inp")).steps) in (2, 3)
+
+
+def test_changing_constraints_are_refused(tmp_path):
+    text = open(os.path.join(GOLDEN, "tutorial07.inp")).read().replace("UX\t\tBoolTable\t1", "UX\t\tBoolTable\t1 0")
+    path = tmp_path / "c.inp"
+    path.write_text(text)
+    with pytest.raises(runtime.GiraffeError, match="change between solution steps"):
+        runtime.Deck(str(path))
+
+
+@pytest.mark.parametrize("name", ["tutorial07", "tutorial08"])
+def test_unloaded_end_state(oracle_lib, name):
+    """tutorial07 (tip force, tip moment, then gravity switched on, the loads taken
+    off, gravity ramped off) and tutorial08 (loads on, then off) end with no load
+    at all: the oracle's cantilever returns to its undeformed configuration, to the
+    Newton tolerance (1e-3 of the 1 m length; the loads deflect the tip by about
+    half a metre on the way)."""
+    deck = runtime.Deck(os.path.join(GOLDEN, name + ".inp"))
+    om = oracle_lib.OracleModel(deck, oracle_lib.SOLVER_BAND)
+    peak = []
+    for q, mode in deck.steps:
+        om.set_gravity_mode(mode)
+        rep, _ = om.static_solve(q)
+        assert rep.aborted == 0
+        peak.append(np.abs(om.nodes()[:, :3] - np.array(deck.desc().node_xyz[:3 * deck.n_nodes]).reshape(-1, 3)).max())
+    assert max(peak) > 0.1
+    assert peak[-1] < 1e-3
