@@ -165,7 +165,8 @@ def measure(m, p, precision, steps, warmup, probe_steps):
     return {"info": info, "wall": wall, "lat_ms": lat_ms, "reps": reps, "phases": ph, "kernels": kp,
             "refine_sweeps": st1["refine_sweeps"] - st0["refine_sweeps"],
             "fallbacks": st1["fallbacks"] - st0["fallbacks"], "solves": st1["solves"] - st0["solves"],
-            "fp64_direct": st1["fp64_direct"] - st0["fp64_direct"]}
+            "fp64_direct": st1["fp64_direct"] - st0["fp64_direct"],
+            "fp32_factorizations": st1["fp32_factorizations"] - st0["fp32_factorizations"]}
 
 
 # Kernel classes bracketed by HIP events in the timed region (per layout).
@@ -461,7 +462,7 @@ def main():
     if args.precision == "mixed":
         out["refinement"] = {"sweeps_per_solve": r["refine_sweeps"] / max(1, r["solves"]),
                              "fp64_fallbacks": r["fallbacks"], "fp64_direct": r["fp64_direct"],
-                             "tangent_solves": r["solves"]}
+                             "fp32_factorizations": r["fp32_factorizations"], "tangent_solves": r["solves"]}
     if r["phases"]:
         ph = r["phases"]
         n_att = args.probe_steps
@@ -480,7 +481,8 @@ def main():
                           "newton_iterations": c["reps"][-1][0],
                           "max_rel_residual": max(x[2] for x in c["reps"]),
                           "refine_sweeps_per_solve": c["refine_sweeps"] / max(1, c["solves"]),
-                          "fp64_fallbacks": c["fallbacks"], "fp64_direct": c["fp64_direct"]}
+                          "fp64_fallbacks": c["fallbacks"], "fp64_direct": c["fp64_direct"],
+                          "fp32_factorizations": c["fp32_factorizations"], "tangent_solves": c["solves"]}
         out["compare"]["same_iterations"] = c["reps"][-1][4] == reps[-1][4]
     if args.cpu_baseline != "none":
         out["cpu_baseline"] = cpu_baseline(m, p, args.cpu_baseline, its, args.cpu_budget)

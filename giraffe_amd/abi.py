@@ -116,6 +116,7 @@ class GfOptions(C.Structure):
         ("stream", C.c_void_p),
         ("refine_max", C.c_int),
         ("refine_tol", C.c_double),
+        ("refine_reuse", C.c_int),
     ]
 
 

@@ -289,6 +289,7 @@ struct gf_model {
     int precision = GF_PREC_FP64;
     int refine_max = 8;
     double refine_tol = 1e-14;
+    bool refine_reuse = true;  // keep the FP32 factor across tangents while it converges
 
     int nn = 0, nb = 0, ns = 0, nl = 0;
     int n_free = 0;
@@ -358,8 +359,10 @@ struct gf_model {
     gf_btd btd32;
     float *d_band32 = nullptr;
     double *d_scale = nullptr, *d_res = nullptr, *d_w = nullptr;
-    long long n_solves = 0, n_sweeps = 0, n_fallbacks = 0, n_direct = 0;
+    long long n_solves = 0, n_sweeps = 0, n_fallbacks = 0, n_direct = 0, n_f32_factor = 0;
     int fallback_streak = 0, fp64_direct_left = 0;  // mixed-precision back-off
+    bool f32_have = false;     // the FP32 factor holds an earlier tangent's factorisation
+    bool f32_reused = false;   // the current refinement runs on such a factor
     int backoff_len = GF_MIXED_BACKOFF;
     // enqueue/result pairs (asynchronous drivers)
     bool ev_pending[GF_NPHASE] = {};

@@ -36,6 +36,7 @@ extern "C" int gf_default_options(gf_options *o)
     o->stream = nullptr;
     o->refine_max = 8;
     o->refine_tol = 1e-14;
+    o->refine_reuse = 1;
     return GF_OK;
 }
 
@@ -670,6 +671,7 @@ extern "C" int gf_reset_state(gf_model *m)
     m->fallback_streak = 0;
     m->fp64_direct_left = 0;
     m->backoff_len = GF_MIXED_BACKOFF;
+    m->f32_have = false;
     return GF_OK;
 }
 
@@ -744,6 +746,7 @@ extern "C" int gf_model_create(const gf_model_desc *desc, const gf_options *opt,
     if (opt) {
         m->refine_max = opt->refine_max;
         m->refine_tol = opt->refine_tol;
+        m->refine_reuse = opt->refine_reuse != 0;
     }
     int st = create(desc, opt, m);
     if (st != GF_OK) {
@@ -786,6 +789,7 @@ extern "C" int gf_model_stats(const gf_model *m, long long *out)
     out[1] = m->n_sweeps;
     out[2] = m->n_fallbacks;
     out[3] = m->n_direct;
+    out[4] = m->n_f32_factor;
     return GF_OK;
 }
 
@@ -915,17 +919,32 @@ static int update_tail_enqueue(gf_model *m, bool then_assemble)
     return GF_OK;
 }
 
-static void note_fallback(gf_model *m, bool fell_back);
+static void note_fallback(gf_model *m, bool fell_back, bool divergent = false);
 
-// Mixed precision: the FP32 factorisation of the equilibrated tangent and the
-// first refinement sweep (the model is then `refining` until gf_refine_step
-// has enqueued the update).
-static int mixed_start_enqueue(gf_model *m)
+// The FP32 factorisation of the equilibrated current tangent.
+static int mixed_factor_enqueue(gf_model *m)
 {
     tick(m, GF_PH_FACTOR, true);
     GF_TRYN(nl_f32, gf_mixed_factor(m));
     tick(m, GF_PH_FACTOR, false);
     m->pending_launches[GF_PH_FACTOR] += nl_f32;
+    m->n_f32_factor++;
+    m->f32_have = true;
+    m->f32_reused = false;
+    return GF_OK;
+}
+
+// Mixed precision: the FP32 factorisation of the equilibrated tangent and the
+// first refinement sweep (the model is then `refining` until gf_refine_step
+// has enqueued the update).  With refine_reuse the FP32 factor of an earlier
+// tangent (and its equilibration) is kept as long as the refinement against
+// the current FP64 tangent keeps converging: the refinement's residual is
+// always the current tangent's, so only the number of sweeps changes; a
+// refinement that stagnates on the kept factor refactors and continues.
+static int mixed_start_enqueue(gf_model *m)
+{
+    if (m->refine_reuse && m->f32_have) m->f32_reused = true;
+    else GF_TRY(mixed_factor_enqueue(m));
     GF_TRY(gf_mixed_refine_begin(m));
     tick(m, GF_PH_SOLVE, true);
     GF_TRYN(nl_sw, gf_mixed_sweep(m));
@@ -951,17 +970,42 @@ static int refine_decide(gf_model *m, int *next)
     memcpy(&flag, &m->h_scal[GF_SCAL_FLAG], 4);
     memcpy(&dn, m->h_scal + GF_SCAL_REF, 8);
     memcpy(&xn, m->h_scal + GF_SCAL_REF + 1, 8);
-    bool ok = false, fall = (flag & GF_FLAG_FAIL) != 0 || !(dn == dn) || !(xn == xn);
+    const bool finite = dn == dn && xn == xn;
+    bool ok = false, fall = (flag & GF_FLAG_FAIL) != 0 || !finite, divergent = false;
     if (!fall) {
         if (dn <= m->refine_tol * xn) ok = true;
         else if (m->refine_it > 1 && dn > 0.5 * m->refine_prev) {
             if (dn <= 1e-10 * xn) ok = true;
-            else fall = true;
+            else {
+                fall = true;
+                // no contraction at all: the FP32 factor does not approximate
+                // this model's tangent (spectral radius of the sweep >= 1)
+                divergent = m->refine_it == 2 && dn >= m->refine_prev;
+            }
         } else if (m->refine_it >= std::max(1, m->refine_max)) fall = true;
     }
-    if (!ok && !fall) {
-        m->refine_prev = dn;
-        m->refine_it++;
+    // A kept FP32 factor (an earlier tangent's) is replaced by the current
+    // tangent's when its refinement would fall back, or when it contracts by
+    // less than 20x per sweep (then the sweeps still needed cost more than a
+    // factorisation: at the plate's sizes one FP32 factorisation is about five
+    // sweeps and a fresh factor contracts ~1000x per sweep).
+    bool refactor = false;
+    if (m->f32_reused && (flag & GF_FLAG_FAIL) == 0) {
+        if (fall) refactor = true;
+        else if (!ok && m->refine_it > 1 && dn > 0.05 * m->refine_prev) refactor = true;
+    }
+    if ((!ok && !fall) || refactor) {
+        if (refactor) {
+            GF_TRY(mixed_factor_enqueue(m));
+            // continue from the current x (already accurate to the kept
+            // factor's sweeps), unless it is not finite
+            if (!finite) GF_TRY(gf_mixed_refine_begin(m));
+            m->refine_prev = 1e308;
+            m->refine_it = 1;
+        } else {
+            m->refine_prev = dn;
+            m->refine_it++;
+        }
         m->n_sweeps++;
         tick(m, GF_PH_SOLVE, true);
         GF_TRYN(l, gf_mixed_sweep(m));
@@ -972,9 +1016,11 @@ static int refine_decide(gf_model *m, int *next)
     }
     m->refining = false;
     m->factored = true;
-    note_fallback(m, fall);
+    note_fallback(m, fall, divergent);
     *next = ok ? 0 : 2;
     if (fall) {
+        // a fresh FP32 factor that could not drive the refinement is not kept
+        m->f32_have = false;
         m->n_fallbacks++;
         tick(m, GF_PH_FACTOR, true);
         GF_TRYN(l, gf_mixed_fallback(m));
@@ -1148,11 +1194,21 @@ static int solve_fp64_enqueue(gf_model *m)
 // backoff_len FP64 solves.  The first FP32 attempt after a back-off counts as
 // the second of a row, and a back-off that ends in another fallback doubles the
 // next one (up to 64 times the first); an FP32 solve that converges resets both.
-static void note_fallback(gf_model *m, bool fell_back)
+// A refinement whose second correction is no smaller than its first (the FP32
+// factor of this tangent does not contract the error at all, as for slender
+// beam chains whose equilibrated tangent is beyond FP32's range) starts the
+// longest back-off at once.
+static void note_fallback(gf_model *m, bool fell_back, bool divergent)
 {
     if (!fell_back) {
         m->fallback_streak = 0;
         m->backoff_len = GF_MIXED_BACKOFF;
+        return;
+    }
+    if (divergent) {
+        m->fp64_direct_left = 64 * GF_MIXED_BACKOFF;
+        m->backoff_len = 64 * GF_MIXED_BACKOFF;
+        m->fallback_streak = 1;
         return;
     }
     if (++m->fallback_streak >= 2) {

@@ -115,11 +115,12 @@ def options(precision=abi.GF_PREC_FP64, device=0, refine_max=8, refine_tol=1e-14
 class DeviceModel:
     """One model resident on the GPU; mirrors the gf_* entry points."""
 
-    def __init__(self, model, precision=abi.GF_PREC_FP64, device=0):
+    def __init__(self, model, precision=abi.GF_PREC_FP64, device=0, refine_reuse=True):
         L = load()
         self.model = model
         self._desc = model.desc()
         self._opt = options(precision, device)
+        self._opt.refine_reuse = 1 if refine_reuse else 0
         h = C.c_void_p()
         check(L.gf_model_create(C.byref(self._desc), C.byref(self._opt), C.byref(h)), "gf_model_create")
         self.h = h
@@ -145,9 +146,10 @@ class DeviceModel:
         return dict(zip(keys, list(o)))
 
     def stats(self):
-        o = (C.c_longlong * 4)()
+        o = (C.c_longlong * 5)()
         check(load().gf_model_stats(self.h, o), "gf_model_stats")
-        return {"solves": o[0], "refine_sweeps": o[1], "fallbacks": o[2], "fp64_direct": o[3]}
+        return {"solves": o[0], "refine_sweeps": o[1], "fallbacks": o[2], "fp64_direct": o[3],
+                "fp32_factorizations": o[4]}
 
     def set_window(self, start, end):
         check(load().gf_set_solution_window(self.h, start, end), "gf_set_solution_window")

@@ -157,7 +157,10 @@ typedef struct gf_options {
     int device;                  /* HIP device ordinal                            */
     void *stream;                /* hipStream_t to run on, or NULL for a new one  */
     int refine_max;              /* max FP64 refinement sweeps per solve (mixed)  */
-    double refine_tol;           /* target ||r - K x||_inf / ||r||_inf            */
+    double refine_tol;           /* refinement ends when max|correction| <= refine_tol max|x| */
+    int refine_reuse;            /* mixed: keep the FP32 factor of an earlier tangent while the
+                                    refinement against the current FP64 tangent converges fast
+                                    enough; it is refactored when it does not (default 1) */
 } gf_options;
 
 typedef struct gf_model gf_model;
@@ -177,8 +180,9 @@ int gf_model_info(const gf_model *m, long long *info_out /* [8] */);
  * fell back to the FP64 factorisation, solves factored in FP64 directly by the
  * mixed path's back-off (after two fallbacks in a row the next 16 solves; a retry
  * that falls back again doubles the next back-off, up to 1,024; a converged FP32
- * solve resets it). */
-int gf_model_stats(const gf_model *m, long long *stats_out /* [4] */);
+ * solve resets it; a refinement that does not contract at all starts the 1,024
+ * at once), FP32 factorisations. */
+int gf_model_stats(const gf_model *m, long long *stats_out /* [5] */);
 /* Device bytes gf_model_create would allocate for this description (host-side
  * planning only; no device call).  Used for admission by the request server. */
 int gf_model_footprint(const gf_model_desc *desc, const gf_options *opt, unsigned long long *bytes);

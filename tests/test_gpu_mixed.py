@@ -67,12 +67,41 @@ def test_mixed_plate_static_matches_oracle(oracle_lib):
     assert dm.stats()["fallbacks"] == 0
 
 
+@pytest.mark.parametrize("nq", [12, 24])
+def test_mixed_reused_fp32_factor(nq):
+    """With refine_reuse (the default) the FP32 factor of an earlier tangent is
+    kept while the refinement against the current FP64 tangent converges fast;
+    every increment is still refined to the same tolerance against the current
+    tangent, so the Newton history is the one of fresh factorisations and the
+    nodes agree to the refinement's accuracy, with fewer FP32 factorisations
+    than tangent solves."""
+    m, p, _ = M.shell_plate(nq=nq, n_steps=5, p_max=4e6)
+    out = {}
+    for reuse in (False, True):
+        dm = runtime.DeviceModel(m, precision=abi.GF_PREC_MIXED, refine_reuse=reuse)
+        rep, it = dm.static_solve(p)
+        assert rep.aborted == 0
+        out[reuse] = (list(it), dm.nodes(), dm.stats())
+    assert out[True][0] == out[False][0]
+    nf, nr = out[False][1], out[True][1]
+    x0 = m.freeze()["xyz"]
+    assert np.abs(nr - nf).max() <= 1e-9 * np.abs(nf[:, :3] - x0).max()
+    sf, sr = out[False][2], out[True][2]
+    assert sf["fp32_factorizations"] + sf["fp64_direct"] == sf["solves"]
+    # a kept factor is replaced when it contracts less than 20x per sweep, so
+    # reuse never factors more often; on the 12 x 12 plate it keeps most factors
+    assert sr["fp32_factorizations"] <= sr["solves"]
+    if nq == 12:
+        assert sr["fp32_factorizations"] < sr["solves"]
+
+
 def test_mixed_catenary_falls_back_to_fp64(oracle_lib):
     """A long slender riser has cond(D K D) ~ 1e13 (500 elements), far beyond
     1/eps32: the FP32 factor cannot drive the refinement and every solve must
     end on the FP64 factorisation, with results identical to the FP64 path.  After
     two fallbacks in a row the path factors in FP64 directly, for 16 solves and
-    then for doubling stretches while each retry falls back again."""
+    then for doubling stretches while each retry falls back again; a refinement
+    that does not contract at all starts the longest back-off at once."""
     m, p, _ = M.catenary(n_elem=500)
     d64 = runtime.DeviceModel(m)
     r64, i64 = d64.static_solve(p)
