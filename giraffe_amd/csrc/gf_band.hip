@@ -310,8 +310,10 @@ __device__ __forceinline__ typename mfma<T>::acc_t mm16(const T *P, int lp, cons
 // 1e-6) of its row's original diagonal entry (of the block's largest entry
 // where that is zero) -- a threshold that does not depend on how rows and
 // columns are scaled -- is replaced by that size with its sign (bit 2 of
-// *flag); a non-finite pivot, an all-zero block, or in FP32 a perturbed pivot
-// sets bit 1.  tmp: 3 x 16 x TMP_LD entries; aux: 4 + 64 entries.
+// *flag); a non-finite pivot or an all-zero block sets bit 1.  (An FP32 factor
+// with perturbed pivots is still refined against the FP64 tangent, which
+// decides whether it was good enough.)  tmp: 3 x 16 x TMP_LD entries; aux:
+// 4 + 64 entries.
 template <typename T> __device__ __forceinline__ void inverse64(T *M, T *tmp, T *aux, int *flag)
 {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -370,7 +372,7 @@ template <typename T> __device__ __forceinline__ void inverse64(T *M, T *tmp, T 
         __syncthreads();
     }
     if (tid == 0) {
-        const int bits = (bad || (sizeof(T) == 4 && pert) ? 1 : 0) | (pert ? 2 : 0);
+        const int bits = (bad ? 1 : 0) | (pert ? 2 : 0);
         if (bits) atomicOr(flag, bits);
     }
 }
@@ -580,77 +582,64 @@ __global__ __launch_bounds__(256) void band_batch_inv_kernel(const T *ab, Geo g,
 }
 
 // ---- triangular solves -----------------------------------------------------------
-// 16 rows per workgroup, 16 threads per row; each thread takes 4 consecutive
-// columns of every 64-column chunk (coalesced 512-byte row segments).
-template <typename V> __device__ __forceinline__ V sum16(V s)
+// One wavefront per row: lane l takes columns l, l + 64, ... of the batch's 512,
+// so each load instruction of a wave reads 64 consecutive entries of the row
+// (coalesced) and every lane's loads are independent; the row sum is a
+// wavefront reduction.
+template <typename V> __device__ __forceinline__ V sum64(V s)
 {
-    s += __shfl_xor(s, 1);
-    s += __shfl_xor(s, 2);
-    s += __shfl_xor(s, 4);
-    s += __shfl_xor(s, 8);
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) s += __shfl_xor(s, o);
     return s;
 }
 
-// FWD: y = G f over batch b (rows of the batch; out = z); BWD: x = H g, H unit
-// upper (out = x).  Workgroup w: rows 16 w ..; the batch's right-hand side
-// segment is staged in LDS first.
+// FWD: y = G f over batch b (out = z); BWD: x = H g, H unit upper (out = x).
+// One row per workgroup of one wavefront; rows of block row rb use block
+// columns [0, rb] (FWD) or (rb, kc) (BWD) of the batch store.
 template <typename T, typename V, bool FWD>
-__global__ __launch_bounds__(256) void band_batch_apply_kernel(const T *gh, Geo g, int b, const V *rhs, V *out)
+__global__ __launch_bounds__(64) void band_batch_apply_kernel(const T *gh, Geo g, int b, const V *rhs, V *out)
 {
-    __shared__ V f[BW];
     const int kc = min(SB, g.nbk - b * SB), B0 = b * BW;
-    // grid = kc * NB / 16 workgroups: row < kc * NB; the 16 rows of a workgroup
-    // lie in one block row, so the block-column range is uniform
-    const int row = blockIdx.x * 16 + (threadIdx.x >> 4), l = threadIdx.x & 15;
+    const int row = blockIdx.x, l = threadIdx.x;  // grid = kc * NB
     const int rb = row / NB;
     const T *M = gh + (size_t)b * BW * BW + (size_t)row * BW;
     const int c_lo = FWD ? 0 : rb + 1, c_hi = FWD ? rb + 1 : kc;  // block columns
-    // every matrix load of the row issued before the right-hand side is staged
-    T m[SB][4];
+    T m[SB];
+    V y[SB];
 #pragma unroll
-    for (int cb = 0; cb < SB; cb++)
-        if (cb >= c_lo && cb < c_hi)
-#pragma unroll
-            for (int e = 0; e < 4; e++) m[cb][e] = M[cb * NB + 4 * l + e];
-    for (int t = threadIdx.x; t < kc * NB; t += 256) f[t] = rhs[B0 + t];
-    __syncthreads();
+    for (int cb = 0; cb < SB; cb++) {
+        const bool use = cb >= c_lo && cb < c_hi;
+        m[cb] = use ? M[cb * NB + l] : T(0);
+        y[cb] = use ? rhs[B0 + cb * NB + l] : V(0);
+    }
     V s = V(0);
 #pragma unroll
-    for (int cb = 0; cb < SB; cb++)
-        if (cb >= c_lo && cb < c_hi)
-#pragma unroll
-            for (int e = 0; e < 4; e++) s += (V)m[cb][e] * f[cb * NB + 4 * l + e];
-    s = sum16(s);
-    if (l == 0) out[B0 + row] = FWD ? s : f[row] + s;
+    for (int cb = 0; cb < SB; cb++) s += (V)m[cb] * y[cb];
+    s = sum64(s);
+    if (l == 0) out[B0 + row] = FWD ? s : rhs[B0 + row] + s;
 }
 
-// FWD: f_q -= L(q, batch) y_batch for the block rows q after batch b within
-// reach; BWD: g_q -= U(q, batch) x_batch for the block rows before it.
-// Workgroups of CPL_ROWS rows (16 threads each): the target strip is short
-// (the reach past one batch), so small workgroups put it on more CUs.
+// FWD: f_q -= L(q, batch) y_batch for the rows q after batch b within reach;
+// BWD: g_q -= U(q, batch) x_batch for the rows before it.  CPL_ROWS rows per
+// workgroup, one wavefront each.
 constexpr int CPL_ROWS = 4;
 template <typename T, typename V, bool FWD>
-__global__ __launch_bounds__(16 * CPL_ROWS) void band_couple_kernel(const T *ab, Geo g, int b, const V *sol, V *f, int q0)
+__global__ __launch_bounds__(64 * CPL_ROWS) void band_couple_kernel(const T *ab, Geo g, int b, const V *sol, V *f, int q0)
 {
-    __shared__ V y[BW];
     const int kc = min(SB, g.nbk - b * SB), B0 = b * BW;
-    const int row = q0 * NB + blockIdx.x * CPL_ROWS + (threadIdx.x >> 4), l = threadIdx.x & 15;
-    // every coupling load issued before the solution segment is staged
-    T m[SB][4];
+    const int row = q0 * NB + blockIdx.x * CPL_ROWS + (threadIdx.x >> 6), l = threadIdx.x & 63;
+    T m[SB];
+    V y[SB];
 #pragma unroll
-    for (int cb = 0; cb < SB; cb++)
-        if (cb < kc)
-#pragma unroll
-            for (int e = 0; e < 4; e++) m[cb][e] = band_ld(ab, row, B0 + cb * NB + 4 * l + e, g);
-    for (int t = threadIdx.x; t < kc * NB; t += 16 * CPL_ROWS) y[t] = sol[B0 + t];
-    __syncthreads();
+    for (int cb = 0; cb < SB; cb++) {
+        const bool use = cb < kc;
+        m[cb] = use ? band_ld(ab, row, B0 + cb * NB + l, g) : T(0);
+        y[cb] = use ? sol[B0 + cb * NB + l] : V(0);
+    }
     V s = V(0);
 #pragma unroll
-    for (int cb = 0; cb < SB; cb++)
-        if (cb < kc)
-#pragma unroll
-            for (int e = 0; e < 4; e++) s += (V)m[cb][e] * y[cb * NB + 4 * l + e];
-    s = sum16(s);
+    for (int cb = 0; cb < SB; cb++) s += (V)m[cb] * y[cb];
+    s = sum64(s);
     if (l == 0 && row < g.n) f[row] -= s;
 }
 
@@ -719,13 +708,13 @@ static int band_solve_t(const T *ab, int n, int hb, hipStream_t s, V *x, V *z, c
         const int kc = std::min(SB, g.nbk - b * SB), q0 = b * SB + kc;
         const int q1 = std::min(g.nbk - 1, b * SB + kc - 1 + g.R);
         int pr = gf_probe_begin(s, GF_KC_BAND_SWEEP);
-        hipLaunchKernelGGL((band_batch_apply_kernel<T, V, true>), dim3(kc * NB / 16), dim3(256), 0, s, gh, g, b,
+        hipLaunchKernelGGL((band_batch_apply_kernel<T, V, true>), dim3(kc * NB), dim3(64), 0, s, gh, g, b,
                            (const V *)x, z);
         gf_probe_end(s, pr, kc * NB);
         launches++;
         if (q1 >= q0) {
             pr = gf_probe_begin(s, GF_KC_BAND_SWEEP);
-            hipLaunchKernelGGL((band_couple_kernel<T, V, true>), dim3((q1 - q0 + 1) * NB / CPL_ROWS), dim3(16 * CPL_ROWS), 0, s, ab, g,
+            hipLaunchKernelGGL((band_couple_kernel<T, V, true>), dim3((q1 - q0 + 1) * NB / CPL_ROWS), dim3(64 * CPL_ROWS), 0, s, ab, g,
                                b, (const V *)z, x, q0);
             gf_probe_end(s, pr, 0.0);  // rows are counted on the apply launches
             launches++;
@@ -736,13 +725,13 @@ static int band_solve_t(const T *ab, int n, int hb, hipStream_t s, V *x, V *z, c
         const int kc = std::min(SB, g.nbk - b * SB);
         const int q1 = b * SB - 1, q0 = std::max(0, b * SB - g.R);
         int pr = gf_probe_begin(s, GF_KC_BAND_SWEEP);
-        hipLaunchKernelGGL((band_batch_apply_kernel<T, V, false>), dim3(kc * NB / 16), dim3(256), 0, s, gh, g, b,
+        hipLaunchKernelGGL((band_batch_apply_kernel<T, V, false>), dim3(kc * NB), dim3(64), 0, s, gh, g, b,
                            (const V *)z, x);
         gf_probe_end(s, pr, kc * NB);
         launches++;
         if (q1 >= q0) {
             pr = gf_probe_begin(s, GF_KC_BAND_SWEEP);
-            hipLaunchKernelGGL((band_couple_kernel<T, V, false>), dim3((q1 - q0 + 1) * NB / CPL_ROWS), dim3(16 * CPL_ROWS), 0, s, ab,
+            hipLaunchKernelGGL((band_couple_kernel<T, V, false>), dim3((q1 - q0 + 1) * NB / CPL_ROWS), dim3(64 * CPL_ROWS), 0, s, ab,
                                g, b, (const V *)x, z, q0);
             gf_probe_end(s, pr, 0.0);  // rows are counted on the apply launches
             launches++;

@@ -287,8 +287,8 @@ struct gf_model {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     int precision = GF_PREC_FP64;
-    int refine_max = 8;
-    double refine_tol = 1e-14;
+    int refine_max = 12;
+    double refine_tol = 1e-12;
     bool refine_reuse = true;  // keep the FP32 factor across tangents while it converges
 
     int nn = 0, nb = 0, ns = 0, nl = 0;

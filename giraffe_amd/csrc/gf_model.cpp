@@ -34,8 +34,8 @@ extern "C" int gf_default_options(gf_options *o)
     o->precision = GF_PREC_FP64;
     o->device = 0;
     o->stream = nullptr;
-    o->refine_max = 8;
-    o->refine_tol = 1e-14;
+    o->refine_max = 12;
+    o->refine_tol = 1e-12;
     o->refine_reuse = 1;
     return GF_OK;
 }
@@ -986,13 +986,14 @@ static int refine_decide(gf_model *m, int *next)
     }
     // A kept FP32 factor (an earlier tangent's) is replaced by the current
     // tangent's when its refinement would fall back, or when it contracts by
-    // less than 20x per sweep (then the sweeps still needed cost more than a
-    // factorisation: at the plate's sizes one FP32 factorisation is about five
-    // sweeps and a fresh factor contracts ~1000x per sweep).
+    // less than 10x per sweep: then the sweeps still needed cost more than a
+    // factorisation and a fresh refinement (at the plate's sizes one FP32
+    // factorisation costs about eight sweeps, and a fresh factor contracts
+    // about 300x per sweep).
     bool refactor = false;
     if (m->f32_reused && (flag & GF_FLAG_FAIL) == 0) {
         if (fall) refactor = true;
-        else if (!ok && m->refine_it > 1 && dn > 0.05 * m->refine_prev) refactor = true;
+        else if (!ok && m->refine_it > 1 && dn > 0.1 * m->refine_prev) refactor = true;
     }
     if ((!ok && !fall) || refactor) {
         if (refactor) {
