@@ -288,7 +288,7 @@ struct gf_model {
     bool own_stream = false;
     int precision = GF_PREC_FP64;
     int refine_max = 12;
-    double refine_tol = 1e-12;
+    double refine_tol = 1e-11;
     bool refine_reuse = true;  // keep the FP32 factor across tangents while it converges
 
     int nn = 0, nb = 0, ns = 0, nl = 0;
@@ -393,6 +393,7 @@ struct gf_model {
     hipGraphExec_t solve_graph = nullptr;
     // whole FP64 Newton iteration (solve, update, next assembly, scalar copy)
     hipGraphExec_t iter_graph = nullptr;
+    hipGraphExec_t f32_factor_graph = nullptr, sweep_graph = nullptr;  // mixed precision
     int graph_launches = 0;
 };
 

@@ -102,13 +102,17 @@ def check(status, what=""):
         raise GiraffeError(f"{what} failed with status {status}: {msg}")
 
 
-def options(precision=abi.GF_PREC_FP64, device=0, refine_max=8, refine_tol=1e-14):
+def options(precision=abi.GF_PREC_FP64, device=0, refine_max=None, refine_tol=None):
+    """gf_options from the library's defaults (gf_default_options), with the
+    given fields replaced."""
     o = abi.GfOptions()
     load().gf_default_options(C.byref(o))
     o.precision = precision
     o.device = device
-    o.refine_max = refine_max
-    o.refine_tol = refine_tol
+    if refine_max is not None:
+        o.refine_max = refine_max
+    if refine_tol is not None:
+        o.refine_tol = refine_tol
     return o
 
 

@@ -157,7 +157,9 @@ typedef struct gf_options {
     int device;                  /* HIP device ordinal                            */
     void *stream;                /* hipStream_t to run on, or NULL for a new one  */
     int refine_max;              /* max FP64 refinement sweeps per solve (mixed)  */
-    double refine_tol;           /* refinement ends when max|correction| <= refine_tol max|x| */
+    double refine_tol;           /* refinement ends when max|correction| <= refine_tol max|x|
+                                    (default 1e-11: the increment is then accurate far below the
+                                    Newton loop's own tolerances) */
     int refine_reuse;            /* mixed: keep the FP32 factor of an earlier tangent while the
                                     refinement against the current FP64 tangent converges fast
                                     enough; it is refactored when it does not (default 1) */
