@@ -149,7 +149,7 @@ def measure(m, p, precision, steps, warmup, probe_steps):
     if probe_steps > 0:
         dm.timing(True)
         dm.timing_read(reset=True)
-        dm.kernel_probe(PROBES_BTD if info["block"] else PROBES_BAND)
+        dm.kernel_probe(PROBES_FRONTS if info["layout"] == 2 else PROBES_BTD if info["block"] else PROBES_BAND)
         dm.kernel_probe_read(reset=True)
         for k in range(probe_steps):
             dm.reset()
@@ -172,11 +172,14 @@ def measure(m, p, precision, steps, warmup, probe_steps):
 # Kernel classes bracketed by HIP events in the timed region (per layout).
 PROBES_BTD = ("btd_level", "btd_tail", "beam_assemble", "shell_assemble")
 PROBES_BAND = ("band_panel", "band_update", "band_sweep", "beam_assemble", "shell_assemble")
+PROBES_FRONTS = ("front_step", "front_assemble", "front_solve", "beam_assemble", "shell_assemble")
 
 KERNEL_NAMES = {"btd_level": "btd_level_kernel", "btd_tail": "btd_tail_kernel",
                 "band_panel": "band_batch_inv_kernel", "band_update": "band_step_kernel",
                 "band_sweep": "band_batch_apply_kernel + band_couple_kernel",
-                "beam_assemble": "beam_assemble_kernel<0>", "shell_assemble": "shell_assemble_kernel<0>"}
+                "beam_assemble": "beam_assemble_kernel<0>", "shell_assemble": "shell_assemble_kernel<0>",
+                "front_step": "front_step_kernel", "front_assemble": "front_assemble_kernel",
+                "front_solve": "front_fwd_own/fwd_upd/bwd_upd/bwd_own_kernel"}
 
 
 def kernel_work(cls, info):
@@ -191,6 +194,13 @@ def kernel_work(cls, info):
         return "mfma", lu["flops"] / n, "flop", "band column eliminated: 2 b^2 flops of the band LU"
     if cls == "band_panel":
         return "mfma", 1.0, "flop", "flop of the batch inverses of the diagonal blocks (64^3 products)"
+    if cls == "front_step":
+        return "mfma", 1.0, "flop", ("algorithmic flop of the fronts' partial LU: 2 (m - c - 1)^2 per eliminated "
+                                     "column c of a front of order m")
+    if cls == "front_assemble":
+        return "hbm", s, "B", "front entry written (the write alone: a lower bound of the kernel's bytes)"
+    if cls == "front_solve":
+        return "hbm", 1.0, "B", "factor byte read by the triangular solves"
     if cls == "band_sweep":
         return "hbm", FM.band_trsv(n, b, s)["bytes"] / n / 2, "B", "row solved: half of the triangular-solve bytes"
     if cls == "beam_assemble":

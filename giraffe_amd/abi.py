@@ -16,6 +16,7 @@ GF_E_CAPACITY = -7
 GF_PREC_FP64 = 0
 GF_PREC_MIXED = 1
 GF_GRAVITY_OFF, GF_GRAVITY_RAMP_UP, GF_GRAVITY_ON, GF_GRAVITY_RAMP_DOWN = 0, 1, 2, 3
+GF_SOLVER_AUTO, GF_SOLVER_BAND, GF_SOLVER_FRONTS = 0, 1, 2
 
 GF_LOAD_NODAL = 0
 GF_LOAD_NODAL_FOLLOWER = 1
@@ -117,6 +118,7 @@ class GfOptions(C.Structure):
         ("refine_max", C.c_int),
         ("refine_tol", C.c_double),
         ("refine_reuse", C.c_int),
+        ("solver", C.c_int),
     ]
 
 

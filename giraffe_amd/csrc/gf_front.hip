@@ -1,0 +1,445 @@
+// Multifrontal tangent solve on gfx950: the nested-dissection counterpart of
+// sparsesystem -> MKL PARDISO (src/SparseMatrix.cpp:74-166; PARDISO orders by
+// nested dissection, iparm[1] = 3, and factors supernode by supernode).
+//
+// Fronts (plan: gf_nd.cpp) are dense row-major matrices [mp][mp] over a
+// supernode's own DOFs (kp = 64 kb rows, the last kp - k of them identity
+// padding) followed by its update DOFs.  Per level of the supernode tree:
+//
+//   assemble  F(i, j) = sum over children c of U_c(a_c(i), a_c(j)) (the child's
+//             Schur complement rows that map to i and j, in child order), plus
+//             the tangent entry K(i, j) read from the band storage when row i or
+//             column j is one of the front's own DOFs (the front of the first of
+//             the two in elimination order holds it).  Every entry is written by
+//             one thread, in a fixed order: no atomics, bit-reproducible.
+//   factor    the block LU of the band solver on each front, restricted to its
+//             kb own block columns, over all fronts of the level at once:
+//             launch p updates A(i, j) -= A(i, p) (E_p A'(p, j)) for i, j > p
+//             (the update blocks included: after the last own step they hold the
+//             Schur complement the parent assembles), inverts the next diagonal
+//             block (E_{p+1}) and converts row p - 1 to U.  Pivots: as the band
+//             solver (perturbation below 1e-13 of the block's diagonal, flag
+//             bit 2; non-finite, bit 1).
+//
+// Solve, with A = L U (L(p, p) = D_p = E_p^-1, U block unit upper):
+//   forward   per level: a front gathers its own right-hand side and its
+//             children's update vectors, solves with its own L blocks (one
+//             workgroup, rows kept in LDS), then every update row tile forms
+//             its update vector  c = sum_children c_child - L(t, own) y.
+//   backward  per level from the root: x_own = y - U(own, update) x_update
+//             (row tiles in parallel, x_update read from the ancestors' rows of
+//             x), then the own-block back substitution and the scatter of x.
+#include "gf_tile.h"
+
+#include <algorithm>
+
+namespace {
+
+constexpr int KMAX = 1024;  // largest own block (rows in LDS of the solve); the plan splits larger supernodes
+
+struct FrontTab {
+    const gf_front_dev *fr;
+    const int *rows, *child, *inv, *items;
+    const long long *child_inv;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void front_assemble_kernel(FrontTab t, int first, const double *ab, int ld, int hb,
+                                                             const double *D, T *Fb)
+{
+    const int *it = t.items + 4 * (size_t)(first + blockIdx.x);
+    const int f = it[0];
+    const gf_front_dev F = t.fr[f];
+    const int r0 = NB * it[1], c0 = NB * it[2];
+    const int tid = threadIdx.x, j = tid & 63, i0 = tid >> 6;
+    const int cj = c0 + j;
+    const int *rows = t.rows + F.rows;
+    const int oj = rows[cj];
+    T v[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) v[r] = T(0);
+    for (int c = F.ch0; c < F.ch1; c++) {
+        const gf_front_dev C = t.fr[t.child[c]];
+        const int *im = t.inv + t.child_inv[c];
+        const int b = im[cj];
+        if (b < 0) continue;
+        const T *U = Fb + C.F + (size_t)(NB * C.kb) * C.mp + NB * C.kb + b;
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int a = im[r0 + i0 + 4 * r];
+            if (a >= 0) v[r] += U[(size_t)a * C.mp];
+        }
+    }
+    T *out = Fb + F.F + cj;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int ri = r0 + i0 + 4 * r;
+        const int oi = rows[ri];
+        const bool own = ri < F.k || cj < F.k;
+        if (ri == cj && ri >= F.k && ri < NB * F.kb) {
+            v[r] = T(1);  // padding of the own block: identity
+        } else if (own && oi >= 0 && oj >= 0 && abs(oi - oj) <= hb) {
+            double a = ab[(size_t)oi * (size_t)ld + (size_t)(oj + hb)];
+            if (D) a = D[oi] * a * D[oj];
+            v[r] += (T)a;
+        }
+        out[(size_t)ri * F.mp] = v[r];
+    }
+}
+
+// Launch p of a level's factorisation (the band step on every front of the
+// level; see the file header).  Item kinds: 0 diagonal, 1 update, 2 conversion.
+template <typename T>
+__global__ __launch_bounds__(256) void front_step_kernel(FrontTab t, int first, int p, T *Fb, T *Eb, int *flag)
+{
+    __shared__ StepSmem<T> sm;
+    const int *it = t.items + 4 * (size_t)(first + blockIdx.x);
+    const int f = it[0];
+    if (f < 0) return;
+    const int ti = it[1], tj = it[2], kind = it[3];
+    if (kind == 0) __builtin_amdgcn_s_setprio(3);
+    const gf_front_dev F = t.fr[f];
+    T *A = Fb + F.F;
+    const size_t ld = (size_t)F.mp;
+    T *Ef = Eb + F.E;
+    const int tid = threadIdx.x;
+    const int R0 = ti * NB, C0 = tj * NB;
+    Acc<T> acc;
+    acc.zero();
+    if (kind == 2) {
+        T e[16], a[16];
+        const T *E = Ef + (size_t)ti * NB * NB;
+        tile_load(e, [&](int i, int k) { return E[i * NB + k]; });
+        tile_load(a, [&](int k, int j) { return A[(R0 + k) * ld + C0 + j]; });
+        tile_put(sm.P, LDT, e);
+        tile_put(sm.Q, LDQ, a);
+        __syncthreads();
+        lds_mm<T>(acc, sm.P, LDT, sm.Q, LDQ);
+        acc_each<T>([&](int x, int y, int v, int i, int j) { A[(R0 + i) * ld + C0 + j] = acc.a[x][y][v]; });
+        return;
+    }
+    T old[2][2][4];
+    if (p >= 0) {
+        const int K0 = p * NB;
+        T e[16], a[16], l[16];
+        const T *E = Ef + (size_t)p * NB * NB;
+        tile_load(e, [&](int i, int k) { return E[i * NB + k]; });
+        tile_load(a, [&](int k, int j) { return A[(K0 + k) * ld + C0 + j]; });
+        tile_put(sm.P, LDT, e);
+        tile_put(sm.Q, LDQ, a);
+        __syncthreads();
+        tile_load(l, [&](int i, int k) { return A[(R0 + i) * ld + K0 + k]; });
+        acc_each<T>([&](int x, int y, int v, int i, int j) { old[x][y][v] = A[(R0 + i) * ld + C0 + j]; });
+        Acc<T> w;
+        w.zero();
+        lds_mm<T>(w, sm.P, LDT, sm.Q, LDQ);  // U(p, j)
+        __syncthreads();
+        acc_each<T>([&](int x, int y, int v, int i, int j) { sm.Q[i * LDQ + j] = w.a[x][y][v]; });
+        tile_put(sm.P, LDT, l);
+        __syncthreads();
+        lds_mm<T>(acc, sm.P, LDT, sm.Q, LDQ);  // L(i, p) U(p, j)
+    } else {
+        acc_each<T>([&](int x, int y, int v, int i, int j) { old[x][y][v] = A[(R0 + i) * ld + C0 + j]; });
+    }
+    if (kind == 1) {
+        acc_each<T>([&](int x, int y, int v, int i, int j) { A[(R0 + i) * ld + C0 + j] = old[x][y][v] - acc.a[x][y][v]; });
+        return;
+    }
+    __syncthreads();
+    acc_each<T>([&](int x, int y, int v, int i, int j) { sm.P[i * LDT + j] = old[x][y][v] - acc.a[x][y][v]; });
+    __syncthreads();
+    inverse64<T>(sm.P, sm.tmp, sm.aux, flag);  // E_{p+1}
+    T *E = Ef + (size_t)ti * NB * NB;
+    for (int q = tid; q < NB * NB; q += 256) E[q] = sm.P[(q / NB) * LDT + q % NB];
+}
+
+// ---- solve -----------------------------------------------------------------------
+// Row r of a 64-row tile belongs to wave r % 4 (rows wave + 4 s, s = 0..15); lane
+// l multiplies column l, so each load of a wave reads 64 consecutive entries of a
+// row, and the row sums are wavefront reductions.
+
+// Update vector entry of front position i gathered from the children.
+__device__ __forceinline__ double child_sum(const FrontTab &t, const gf_front_dev &F, int i, const double *Vb)
+{
+    double w = 0.0;
+    for (int c = F.ch0; c < F.ch1; c++) {
+        const int a = t.inv[t.child_inv[c] + i];
+        if (a >= 0) {
+            const gf_front_dev C = t.fr[t.child[c]];
+            w += Vb[C.V + NB * C.kb + a];
+        }
+    }
+    return w;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void front_fwd_own_kernel(FrontTab t, int f0, const T *Fb, const T *Eb,
+                                                            const double *rhs, double *Vb)
+{
+    __shared__ double y[KMAX];
+    __shared__ double tmp[NB];
+    const int f = f0 + blockIdx.x;
+    const gf_front_dev F = t.fr[f];
+    const int kp = NB * F.kb, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int *rows = t.rows + F.rows;
+    const T *A = Fb + F.F;
+    const size_t ld = (size_t)F.mp;
+    for (int i = tid; i < kp; i += 256) {
+        const int o = rows[i];
+        y[i] = (o >= 0 ? rhs[o] : 0.0) + child_sum(t, F, i, Vb);
+    }
+    __syncthreads();
+    for (int p = 0; p < F.kb; p++) {
+        double part[16];
+#pragma unroll
+        for (int s = 0; s < 16; s++) part[s] = 0.0;
+        for (int q = 0; q < p; q++) {
+            const double yq = y[NB * q + lane];
+            const T *L = A + (size_t)(NB * p) * ld + NB * q + lane;
+#pragma unroll
+            for (int s = 0; s < 16; s++) part[s] += (double)L[(size_t)(wave + 4 * s) * ld] * yq;
+        }
+#pragma unroll
+        for (int s = 0; s < 16; s++) {
+            const double r = sum64(part[s]);
+            if (lane == 0) tmp[wave + 4 * s] = y[NB * p + wave + 4 * s] - r;
+        }
+        __syncthreads();
+        const T *E = Eb + F.E + (size_t)p * NB * NB + lane;
+        const double tv = tmp[lane];
+#pragma unroll
+        for (int s = 0; s < 16; s++) part[s] = (double)E[(wave + 4 * s) * NB] * tv;
+#pragma unroll
+        for (int s = 0; s < 16; s++) {
+            const double r = sum64(part[s]);
+            if (lane == 0) y[NB * p + wave + 4 * s] = r;
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < kp; i += 256) Vb[F.V + i] = y[i];
+}
+
+// Update row tile t of a front: c = sum_children c_child - L(t, own) y_own.
+template <typename T>
+__global__ __launch_bounds__(256) void front_fwd_upd_kernel(FrontTab t, int first, const T *Fb, double *Vb)
+{
+    const int *it = t.items + 4 * (size_t)(first + blockIdx.x);
+    const gf_front_dev F = t.fr[it[0]];
+    const int tr = it[1], lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const T *A = Fb + F.F + (size_t)(NB * tr) * F.mp + lane;
+    const double *y = Vb + F.V + lane;
+    double part[16];
+#pragma unroll
+    for (int s = 0; s < 16; s++) part[s] = 0.0;
+    for (int q = 0; q < F.kb; q++) {
+        const double yq = y[NB * q];
+#pragma unroll
+        for (int s = 0; s < 16; s++) part[s] += (double)A[(size_t)(wave + 4 * s) * F.mp + NB * q] * yq;
+    }
+#pragma unroll
+    for (int s = 0; s < 16; s++) {
+        const double r = sum64(part[s]);
+        const int i = NB * tr + wave + 4 * s;
+        if (lane == 0) Vb[F.V + i] = child_sum(t, F, i, Vb) - r;
+    }
+}
+
+// Own row tile p of a front with an update part: z_p = y_p - U(p, update) x_update.
+template <typename T>
+__global__ __launch_bounds__(256) void front_bwd_upd_kernel(FrontTab t, int first, const T *Fb, const double *x,
+                                                            double *Vb)
+{
+    const int *it = t.items + 4 * (size_t)(first + blockIdx.x);
+    const gf_front_dev F = t.fr[it[0]];
+    const int p = it[1], lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int *rows = t.rows + F.rows;
+    const T *A = Fb + F.F + (size_t)(NB * p) * F.mp + lane;
+    double part[16];
+#pragma unroll
+    for (int s = 0; s < 16; s++) part[s] = 0.0;
+    for (int q = F.kb; q < F.mb; q++) {
+        const int o = rows[NB * q + lane];
+        const double xq = o >= 0 ? x[o] : 0.0;
+#pragma unroll
+        for (int s = 0; s < 16; s++) part[s] += (double)A[(size_t)(wave + 4 * s) * F.mp + NB * q] * xq;
+    }
+#pragma unroll
+    for (int s = 0; s < 16; s++) {
+        const double r = sum64(part[s]);
+        const int i = NB * p + wave + 4 * s;
+        if (lane == 0) Vb[F.V + i] -= r;
+    }
+}
+
+// Own-block back substitution x_p = z_p - sum_{p < j < kb} U(p, j) x_j, then the
+// scatter of the own rows of x.
+template <typename T>
+__global__ __launch_bounds__(256) void front_bwd_own_kernel(FrontTab t, int f0, const T *Fb, const double *Vb, double *x)
+{
+    __shared__ double y[KMAX];
+    const int f = f0 + blockIdx.x;
+    const gf_front_dev F = t.fr[f];
+    const int kp = NB * F.kb, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int *rows = t.rows + F.rows;
+    const T *A = Fb + F.F;
+    const size_t ld = (size_t)F.mp;
+    for (int i = tid; i < kp; i += 256) y[i] = Vb[F.V + i];
+    __syncthreads();
+    for (int p = F.kb - 2; p >= 0; p--) {
+        double part[16];
+#pragma unroll
+        for (int s = 0; s < 16; s++) part[s] = 0.0;
+        for (int j = p + 1; j < F.kb; j++) {
+            const double xj = y[NB * j + lane];
+            const T *U = A + (size_t)(NB * p) * ld + NB * j + lane;
+#pragma unroll
+            for (int s = 0; s < 16; s++) part[s] += (double)U[(size_t)(wave + 4 * s) * ld] * xj;
+        }
+#pragma unroll
+        for (int s = 0; s < 16; s++) {
+            const double r = sum64(part[s]);
+            if (lane == 0) y[NB * p + wave + 4 * s] -= r;
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < F.k; i += 256) x[rows[i]] = y[i];
+}
+
+FrontTab tab_of(const gf_fronts &fp)
+{
+    FrontTab t;
+    t.fr = fp.d_fr;
+    t.rows = fp.d_rows;
+    t.child = fp.d_child;
+    t.inv = fp.d_inv;
+    t.items = fp.d_items;
+    t.child_inv = fp.d_child_inv;
+    return t;
+}
+
+template <typename T>
+int front_factor_t(const gf_fronts &fp, const double *ab, int ld, int hb, const double *D, T *F, T *E, hipStream_t s,
+                   int *flag)
+{
+    const FrontTab t = tab_of(fp);
+    int launches = 0;
+    for (int l = 0; l < fp.n_level; l++) {
+        int pr = gf_probe_begin(s, GF_KC_FRONT_ASSEMBLE);
+        hipLaunchKernelGGL(front_assemble_kernel<T>, dim3(fp.asm_count[l]), dim3(256), 0, s, t, fp.asm_first[l], ab, ld,
+                           hb, D, F);
+        gf_probe_end(s, pr, (double)fp.asm_count[l] * NB * NB);
+        launches++;
+        for (int k = fp.lev_step[l]; k < fp.lev_step[l + 1]; k++) {
+            if (fp.step_count[k] == 0) continue;
+            const int p = k - fp.lev_step[l] - 1;
+            pr = gf_probe_begin(s, GF_KC_FRONT_STEP);
+            hipLaunchKernelGGL(front_step_kernel<T>, dim3(fp.step_count[k]), dim3(256), 0, s, t, fp.step_first[k], p,
+                               F, E, flag);
+            gf_probe_end(s, pr, fp.step_flops[k]);
+            launches++;
+        }
+    }
+    GF_HIP(hipGetLastError());
+    return launches;
+}
+
+template <typename T>
+int front_solve_t(const gf_fronts &fp, const T *F, const T *E, const double *rhs, double *x, double *V, hipStream_t s)
+{
+    const FrontTab t = tab_of(fp);
+    int launches = 0;
+    // probe units: factor bytes each launch reads (the own triangle with the
+    // diagonal inverses, or the update panel)
+    auto own_bytes = [&](int l, bool fwd) {
+        double b = 0.0;
+        for (int f = fp.lev_off[l]; f < fp.lev_off[l + 1]; f++) {
+            const double kp = NB * fp.fr[f].kb;
+            b += fwd ? kp * (kp + NB) / 2.0 : kp * (kp - NB) / 2.0;
+        }
+        return b * sizeof(T);
+    };
+    auto panel_bytes = [&](int l) {
+        double b = 0.0;
+        for (int f = fp.lev_off[l]; f < fp.lev_off[l + 1]; f++)
+            b += (double)NB * fp.fr[f].kb * (fp.fr[f].mp - NB * fp.fr[f].kb);
+        return b * sizeof(T);
+    };
+    for (int l = 0; l < fp.n_level; l++) {
+        const int nf = fp.lev_off[l + 1] - fp.lev_off[l];
+        int pr = gf_probe_begin(s, GF_KC_FRONT_SOLVE);
+        hipLaunchKernelGGL(front_fwd_own_kernel<T>, dim3(nf), dim3(256), 0, s, t, fp.lev_off[l], F, E, rhs, V);
+        gf_probe_end(s, pr, pr >= 0 ? own_bytes(l, true) : 0.0);
+        launches++;
+        if (fp.fupd_count[l] > 0) {
+            pr = gf_probe_begin(s, GF_KC_FRONT_SOLVE);
+            hipLaunchKernelGGL(front_fwd_upd_kernel<T>, dim3(fp.fupd_count[l]), dim3(256), 0, s, t, fp.fupd_first[l], F, V);
+            gf_probe_end(s, pr, pr >= 0 ? panel_bytes(l) : 0.0);
+            launches++;
+        }
+    }
+    for (int l = fp.n_level - 1; l >= 0; l--) {
+        const int nf = fp.lev_off[l + 1] - fp.lev_off[l];
+        int pr;
+        if (fp.bupd_count[l] > 0) {
+            pr = gf_probe_begin(s, GF_KC_FRONT_SOLVE);
+            hipLaunchKernelGGL(front_bwd_upd_kernel<T>, dim3(fp.bupd_count[l]), dim3(256), 0, s, t, fp.bupd_first[l], F,
+                               (const double *)x, V);
+            gf_probe_end(s, pr, pr >= 0 ? panel_bytes(l) : 0.0);
+            launches++;
+        }
+        pr = gf_probe_begin(s, GF_KC_FRONT_SOLVE);
+        hipLaunchKernelGGL(front_bwd_own_kernel<T>, dim3(nf), dim3(256), 0, s, t, fp.lev_off[l], F, (const double *)V, x);
+        gf_probe_end(s, pr, pr >= 0 ? own_bytes(l, false) : 0.0);
+        launches++;
+    }
+    GF_HIP(hipGetLastError());
+    return launches;
+}
+
+template <typename T> int put(T **dst, const std::vector<T> &src, hipStream_t s)
+{
+    const size_t n = std::max<size_t>(src.size(), 1);
+    GF_TRY(gf_pool_alloc((void **)dst, n * sizeof(T)));
+    if (!src.empty()) GF_HIP(hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    return GF_OK;
+}
+
+}  // namespace
+
+int gf_front_upload(gf_fronts &fp, hipStream_t s)
+{
+    if (fp.max_k > KMAX) {
+        gf_set_error("multifrontal plan: supernode larger than the solve supports");
+        return GF_E_STATE;
+    }
+    GF_TRY(put(&fp.d_fr, fp.fr, s));
+    GF_TRY(put(&fp.d_rows, fp.rows, s));
+    GF_TRY(put(&fp.d_child, fp.child, s));
+    GF_TRY(put(&fp.d_child_inv, fp.child_inv, s));
+    GF_TRY(put(&fp.d_inv, fp.inv, s));
+    GF_TRY(put(&fp.d_items, fp.items, s));
+    return GF_OK;
+}
+
+void gf_front_free(gf_fronts &fp)
+{
+    void *p[] = {fp.d_fr, fp.d_rows, fp.d_child, fp.d_child_inv, fp.d_inv, fp.d_items};
+    for (void *q : p) gf_pool_free(q);
+    fp.d_fr = nullptr;
+    fp.d_rows = fp.d_child = fp.d_inv = fp.d_items = nullptr;
+    fp.d_child_inv = nullptr;
+}
+
+int gf_front_factor(const gf_fronts &fp, int elem_size, const double *ab, int ld, int hb, const double *D, void *F,
+                    void *E, hipStream_t s, int *flag)
+{
+    if (elem_size == 8) return front_factor_t<double>(fp, ab, ld, hb, D, (double *)F, (double *)E, s, flag);
+    return front_factor_t<float>(fp, ab, ld, hb, D, (float *)F, (float *)E, s, flag);
+}
+
+int gf_front_solve(const gf_fronts &fp, int elem_size, const void *F, const void *E, const double *rhs, double *x,
+                   double *V, hipStream_t s)
+{
+    if (elem_size == 8) return front_solve_t<double>(fp, (const double *)F, (const double *)E, rhs, x, V, s);
+    return front_solve_t<float>(fp, (const float *)F, (const float *)E, rhs, x, V, s);
+}

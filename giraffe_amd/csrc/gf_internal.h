@@ -239,6 +239,62 @@ struct gf_btd {
 };
 int gf_btd_plan(gf_btd &b, int m, int nblk, int elem_size);  // level structure only, no device call
 
+// Nested-dissection multifrontal factorisation (gf_nd.cpp plans it on the host,
+// gf_front.hip runs it).  Supernodes are the parts and separators of a recursive
+// coordinate bisection of the node graph; the front of a supernode is a dense
+// matrix over its own DOFs (k, padded to kp = 64 kb) followed by its update DOFs
+// (u: the DOFs of ancestors it couples to, padded to up), mp = kp + up.  Fronts
+// are numbered level by level (leaves first, a parent's level above all of its
+// children's); fronts of one level are independent.
+struct gf_front_dev {
+    long long F, E, V;  // offsets (elements) of the front matrix [mp][mp], its kb diagonal inverses [64][64], its vector [mp]
+    int mp, kb, mb, k;  // padded order, own blocks, blocks, own DOFs
+    int rows;           // offset of the row map: mp entries, band row of each front row or -1 (padding)
+    int ch0, ch1;       // children: entries [ch0, ch1) of the child lists
+    int pad;
+};
+struct gf_fronts {
+    int n_front = 0, n_level = 0, max_k = 0;
+    std::vector<gf_front_dev> fr;
+    std::vector<int> lev_off;          // fronts of level l: [lev_off[l], lev_off[l+1])
+    std::vector<int> lev_kb;           // largest kb of level l
+    std::vector<int> rows;             // row maps
+    std::vector<int> child;            // child front of each child entry
+    std::vector<long long> child_inv;  // offset of each child entry's map in inv (parent position -> child update row or -1)
+    std::vector<int> inv;
+    std::vector<int> items;            // work items, 4 ints each: front, tile row, tile column, kind (front -1: none)
+    std::vector<int> step_first, step_count;  // factor launch s of level l: index lev_step[l] + s, s = 0 .. lev_kb[l] + 1
+    std::vector<int> lev_step;
+    std::vector<int> asm_first, asm_count;    // per level: assembly tiles
+    std::vector<int> fupd_first, fupd_count;  // per level: forward update row tiles (front, tile row)
+    std::vector<int> bupd_first, bupd_count;  // per level: backward own row tiles with an update part
+    std::vector<double> step_flops;    // algorithmic flops of each factor launch
+    long long f_elems = 0, e_elems = 0, v_elems = 0;
+    double flops = 0.0;                // algorithmic flops of one factorisation
+    long long n_sep_dofs = 0;          // DOFs owned by separators (non-leaf fronts)
+    // device copies
+    gf_front_dev *d_fr = nullptr;
+    int *d_rows = nullptr, *d_child = nullptr, *d_inv = nullptr, *d_items = nullptr;
+    long long *d_child_inv = nullptr;
+};
+// Plans the fronts of the free DOFs dof[6 node + comp] (band rows, -1 when not
+// free) over the node graph of the description's elements; host only.
+int gf_nd_plan(gf_fronts &fp, const gf_model_desc *d, const std::vector<int> &dof, int nn, int npad);
+// Launch count of one factorisation and of one solve of a plan.
+int gf_front_factor_launches(const gf_fronts &fp);
+int gf_front_solve_launches(const gf_fronts &fp);
+int gf_front_upload(gf_fronts &fp, hipStream_t s);  // device copies of the plan's tables
+void gf_front_free(gf_fronts &fp);
+// Factorisation of the band-stored tangent ab (row stride ld, half-bandwidth hb,
+// FP64) into fronts of precision elem_size (8, or 4 with the equilibration
+// scaling D applied on the way, D = null for none); returns launches.
+int gf_front_factor(const gf_fronts &fp, int elem_size, const double *ab, int ld, int hb, const double *D, void *F,
+                    void *E, hipStream_t s, int *flag);
+// Solve with a factor: rhs (band-row vector) in, x out (may alias rhs); V: the
+// plan's vector workspace (v_elems doubles).  Returns launches.
+int gf_front_solve(const gf_fronts &fp, int elem_size, const void *F, const void *E, const double *rhs, double *x,
+                   double *V, hipStream_t s);
+
 // ---- per-kernel timing --------------------------------------------------------
 // While a model's timing is on and its probe mask selects a kernel class, every
 // launch of that class is bracketed by a pair of HIP events on its stream; the
@@ -290,6 +346,7 @@ struct gf_model {
     int refine_max = 12;
     double refine_tol = 1e-11;
     bool refine_reuse = true;  // keep the FP32 factor across tangents while it converges
+    int solver = GF_SOLVER_AUTO;
 
     int nn = 0, nb = 0, ns = 0, nl = 0;
     int n_free = 0;
@@ -358,6 +415,12 @@ struct gf_model {
     // mixed precision: equilibrated FP32 factor + FP64 refinement buffers
     gf_btd btd32;
     float *d_band32 = nullptr;
+    // GM_BAND assembly with the multifrontal solve: the band holds the tangent
+    // only (no fill), the factor lives in the fronts
+    bool fronts_on = false;
+    gf_fronts fronts;
+    double *d_front = nullptr, *d_front_e = nullptr, *d_front_v = nullptr;  // FP64 factor, vector workspace
+    float *d_front32 = nullptr, *d_front32_e = nullptr;                      // FP32 factor (mixed)
     double *d_scale = nullptr, *d_res = nullptr, *d_w = nullptr;
     long long n_solves = 0, n_sweeps = 0, n_fallbacks = 0, n_direct = 0, n_f32_factor = 0;
     int fallback_streak = 0, fp64_direct_left = 0;  // mixed-precision back-off

@@ -141,6 +141,9 @@ int gf_mixed_alloc(gf_model *m)
     if (m->layout == GM_BTD) {
         int st = gf_btd_alloc(m->btd32, m->btd.m, m->btd.nblk, 4);
         if (st) return st;
+    } else if (m->fronts_on) {
+        GF_TRY(gf_pool_alloc((void **)&m->d_front32, (size_t)std::max(1LL, m->fronts.f_elems) * 4));
+        GF_TRY(gf_pool_alloc((void **)&m->d_front32_e, (size_t)std::max(1LL, m->fronts.e_elems) * 4));
     } else {
         GF_TRY(gf_pool_alloc((void **)&m->d_band32, m->band_elems * 4));
         GF_TRY(gf_pool_alloc((void **)&m->d_band32_inv, gf_band_inv_elems(m->npad) * 4));
@@ -150,7 +153,7 @@ int gf_mixed_alloc(gf_model *m)
 
 void gf_mixed_free(gf_model *m)
 {
-    void *p[] = {m->d_scale, m->d_res, m->d_w, m->d_band32, m->d_band32_inv};
+    void *p[] = {m->d_scale, m->d_res, m->d_w, m->d_band32, m->d_band32_inv, m->d_front32, m->d_front32_e};
     for (void *q : p) gf_pool_free(q);
     if (m->btd32.A) gf_btd_free(m->btd32);
 }
@@ -171,6 +174,13 @@ int gf_mixed_factor(gf_model *m)
         GF_HIP(hipGetLastError());
         int l = gf_btd_factor(m->btd32, m->stream, m->d_flag);
         return l < 0 ? l : l + 2;
+    }
+    if (m->fronts_on) {
+        // the fronts take the scaled entries straight from the FP64 band
+        GF_HIP(hipGetLastError());
+        const int l = gf_front_factor(m->fronts, 4, m->d_band, m->band_ld, m->half_band, m->d_scale, m->d_front32,
+                                      m->d_front32_e, m->stream, m->d_flag);
+        return l < 0 ? l : l + 1;
     }
     hipLaunchKernelGGL(convert_band_kernel, dim3(grid_for(m->band_elems, 256)), dim3(256), 0, m->stream, m->d_band,
                        m->d_band32, npad, m->half_band, m->band_ld, m->d_scale);
@@ -201,8 +211,10 @@ int gf_mixed_sweep(gf_model *m)
                        m->d_w);
     // BTD: solution in d_swork (d_w is reduced in place); band: in place in d_w
     const int l = m->layout == GM_BTD ? gf_btd_solve(m->btd32, m->stream, m->d_w, m->d_swork, npad, m->d_w)
-                                      : gf_band_solve(m->d_band32, 4, npad, m->half_band, m->stream, m->d_w,
-                                                      m->d_swork, m->d_band32_inv);
+                  : m->fronts_on    ? gf_front_solve(m->fronts, 4, m->d_front32, m->d_front32_e, m->d_w, m->d_w,
+                                                     m->d_front_v, m->stream)
+                                    : gf_band_solve(m->d_band32, 4, npad, m->half_band, m->stream, m->d_w,
+                                                    m->d_swork, m->d_band32_inv);
     if (l < 0) return l;
     const double *sol = m->layout == GM_BTD ? m->d_swork : m->d_w;
     GF_TRY(gf_launch_fill_bytes(m->stream, m->d_scal + GF_SCAL_REF, 2 * sizeof(double)));
@@ -224,6 +236,13 @@ int gf_mixed_fallback(gf_model *m)
     const int npad = m->npad;
     GF_TRY(gf_launch_fill_bytes(m->stream, m->d_flag, 4));
     if (m->layout == GM_BTD) return gf_btd_factor_solve(m->btd, m->stream, m->d_flag, m->d_r, m->d_x, npad, m->d_res);
+    if (m->fronts_on) {
+        const int l = gf_front_factor(m->fronts, 8, m->d_band, m->band_ld, m->half_band, nullptr, m->d_front,
+                                      m->d_front_e, m->stream, m->d_flag);
+        if (l < 0) return l;
+        const int l2 = gf_front_solve(m->fronts, 8, m->d_front, m->d_front_e, m->d_r, m->d_x, m->d_front_v, m->stream);
+        return l2 < 0 ? l2 : l + l2;
+    }
     const int l = gf_band_factor(m->d_band, 8, npad, m->half_band, m->stream, m->d_flag, m->d_band_inv);
     if (l < 0) return l;
     GF_HIP(hipMemcpyAsync(m->d_x, m->d_r, sizeof(double) * npad, hipMemcpyDeviceToDevice, m->stream));

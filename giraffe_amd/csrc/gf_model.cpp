@@ -37,6 +37,7 @@ extern "C" int gf_default_options(gf_options *o)
     o->refine_max = 12;
     o->refine_tol = 1e-11;
     o->refine_reuse = 1;
+    o->solver = GF_SOLVER_AUTO;
     return GF_OK;
 }
 
@@ -165,8 +166,10 @@ static void release(gf_model *m)
     void *bufs[] = {m->d_disp, m->d_copy, m->d_euler, m->d_dof, m->d_row_nc, m->d_bconn, m->d_bconst, m->d_bgp,
                     m->d_ln_node, m->d_ln_off, m->d_ln_ids, m->d_lrec, m->d_tab_t, m->d_tab_v, m->d_r, m->d_x,
                     m->d_scal, m->d_copy0, m->d_bgp0, m->d_sconn, m->d_sconst, m->d_spconst, m->d_sgp,
-                    m->d_sgp0, m->d_spload, m->d_spflag, m->d_band, m->d_swork, m->d_band_inv, m->d_bbuoy};
+                    m->d_sgp0, m->d_spload, m->d_spflag, m->d_band, m->d_swork, m->d_band_inv, m->d_bbuoy,
+                    m->d_front, m->d_front_e, m->d_front_v};
     for (void *b : bufs) gf_pool_free(b);
+    gf_front_free(m->fronts);
     if (m->h_scal) gf_pinned_free(m->h_scal);
     gf_btd_free(m->btd);
     gf_mixed_free(m);
@@ -285,8 +288,9 @@ static std::vector<int> rcm_order(const gf_model_desc *d, int nn)
 
 // Host-side planning shared by create() and gf_model_footprint: validation,
 // DOFsActive + SetGlobalDOFs, half-bandwidth, storage layout, colourings.
-static int plan(const gf_model_desc *d, gf_model *m, int &bm, int &nblk)
+static int plan(const gf_model_desc *d, gf_model *m, int &bm, int &nblk, int solver)
 {
+    if (solver < GF_SOLVER_AUTO || solver > GF_SOLVER_FRONTS) { gf_set_error("unknown solver"); return GF_E_ARG; }
     if (d->n_nodes <= 0 || !d->node_xyz) { gf_set_error("model has no nodes"); return GF_E_ARG; }
     m->nn = d->n_nodes;
     m->nb = d->n_beams;
@@ -385,6 +389,29 @@ static int plan(const gf_model_desc *d, gf_model *m, int &bm, int &nblk)
     } else {
         m->npad = gf_band_pad(nf);
     }
+    // Multifrontal solve over a nested-dissection order (PARDISO's strategy,
+    // src/SparseMatrix.cpp:115-117): when forced, or for a band too wide for the
+    // block-tridiagonal form when it takes less than half the band LU's flops
+    // (2 n hb^2) in fewer launches (the band LU's critical path is one launch
+    // per 64 columns).
+    m->fronts_on = false;
+    m->fronts = gf_fronts();
+    if (solver == GF_SOLVER_FRONTS || (solver == GF_SOLVER_AUTO && m->layout == GM_BAND)) {
+        const int npad = gf_band_pad(nf);
+        GF_TRY(gf_nd_plan(m->fronts, d, m->dof, nn, npad));
+        const double band_flops = 2.0 * (double)nf * (double)hb * (double)hb;
+        const bool take = solver == GF_SOLVER_FRONTS ||
+                          (m->fronts.flops < 0.5 * band_flops && gf_front_factor_launches(m->fronts) < npad / 64);
+        if (take) {
+            m->fronts_on = true;
+            m->layout = GM_BAND;
+            m->npad = npad;
+            bm = -1;
+            nblk = 0;
+        } else {
+            m->fronts = gf_fronts();
+        }
+    }
     GF_TRY(colour_elements(d->beam_conn, nb, 3, nn, m->beam_order, m->beam_color_off));
     GF_TRY(colour_elements(d->shell_conn, ns, 6, nn, m->shell_order, m->shell_color_off));
     return GF_OK;
@@ -407,6 +434,14 @@ static size_t footprint(const gf_model_desc *d, const gf_model *m, int precision
     if (m->layout == GM_BTD) {
         size_t blocks = 6 * (size_t)nblk + 2 * (size_t)nblk;                  // A B C Bi X Y + saved couplings
         tangent = blocks * (size_t)bm * bm;
+    } else if (m->fronts_on) {
+        const gf_fronts &fp = m->fronts;
+        const size_t factor = (size_t)(fp.f_elems + fp.e_elems);
+        b += (npad * (size_t)(2 * m->half_band + 1) + factor + (size_t)fp.v_elems) * 8;
+        b += (fp.rows.size() + fp.child.size() + fp.inv.size() + fp.items.size()) * 4 + fp.child_inv.size() * 8 +
+             fp.fr.size() * sizeof(gf_front_dev);
+        if (precision == GF_PREC_MIXED) b += factor * 4 + 3 * npad * 8;
+        return b;
     } else {
         const int hb = m->half_band;
         tangent = npad * (size_t)(hb + gf_band_ku(hb) + 1) + gf_band_inv_elems(m->npad);
@@ -419,7 +454,7 @@ static size_t footprint(const gf_model_desc *d, const gf_model *m, int precision
 static int create(const gf_model_desc *d, const gf_options *opt, gf_model *m)
 {
     int bm = -1, nblk = 0;
-    GF_TRY(plan(d, m, bm, nblk));
+    GF_TRY(plan(d, m, bm, nblk, m->solver));
     const int nn = m->nn, nb = m->nb, ns = m->ns, nf = m->n_free;
     const int hb = m->half_band;
     // device + stream
@@ -633,8 +668,18 @@ static int create(const gf_model_desc *d, const gf_options *opt, gf_model *m)
     GF_TRY(dalloc(&m->d_r, npad));
     GF_TRY(dalloc(&m->d_x, npad));
     GF_TRY(dalloc(&m->d_swork, npad));
+    GF_TRY(gf_launch_fill(m->stream, m->d_x, npad, 0.0));  // padding rows stay zero (the fronts solve writes free rows only)
     if (m->layout == GM_BTD) {
         GF_TRY(gf_btd_alloc(m->btd, bm, nblk, 8));
+    } else if (m->fronts_on) {
+        // the band holds the assembled tangent only; the factor lives in the fronts
+        m->band_ld = 2 * hb;
+        m->band_elems = npad * (size_t)(m->band_ld + 1);
+        GF_TRY(dalloc(&m->d_band, m->band_elems));
+        GF_TRY(dalloc(&m->d_front, (size_t)m->fronts.f_elems));
+        GF_TRY(dalloc(&m->d_front_e, (size_t)m->fronts.e_elems));
+        GF_TRY(dalloc(&m->d_front_v, (size_t)m->fronts.v_elems));
+        GF_TRY(gf_front_upload(m->fronts, m->stream));
     } else {
         m->band_ld = hb + gf_band_ku(hb);
         m->band_elems = npad * (size_t)(m->band_ld + 1);
@@ -723,11 +768,46 @@ extern "C" int gf_model_plan(const gf_model_desc *desc, long long *o)
     if (!desc || !o) return GF_E_ARG;
     gf_model tmp;
     int bm = -1, nblk = 0;
-    GF_TRY(plan(desc, &tmp, bm, nblk));
+    GF_TRY(plan(desc, &tmp, bm, nblk, GF_SOLVER_AUTO));
     o[0] = tmp.n_free;
     o[1] = tmp.half_band;
     o[2] = tmp.layout == GM_BTD ? bm : 0;
     o[3] = tmp.node_order_rcm ? 1 : 0;
+    return GF_OK;
+}
+
+extern "C" int gf_front_plan(const gf_model_desc *desc, long long *o, int *front_out, int *rows_out, int *dof_out)
+{
+    if (!desc || !o) return GF_E_ARG;
+    gf_model tmp;
+    int bm = -1, nblk = 0;
+    GF_TRY(plan(desc, &tmp, bm, nblk, GF_SOLVER_FRONTS));
+    const gf_fronts &fp = tmp.fronts;
+    o[0] = fp.n_front;
+    o[1] = fp.n_level;
+    o[2] = (long long)fp.rows.size();
+    o[3] = fp.f_elems;
+    o[4] = fp.e_elems;
+    o[5] = (long long)fp.flops;
+    o[6] = gf_front_factor_launches(fp);
+    o[7] = gf_front_solve_launches(fp);
+    if (front_out) {
+        for (int f = 0; f < fp.n_front; f++) {
+            int *q = front_out + 6 * (size_t)f;
+            int lev = 0;
+            while (fp.lev_off[lev + 1] <= f) lev++;
+            q[0] = fp.fr[f].k;
+            q[1] = 64 * fp.fr[f].kb;
+            q[2] = fp.fr[f].mp;
+            q[3] = lev;
+            q[4] = -1;
+            q[5] = fp.fr[f].rows;
+        }
+        for (int f = 0; f < fp.n_front; f++)
+            for (int c = fp.fr[f].ch0; c < fp.fr[f].ch1; c++) front_out[6 * (size_t)fp.child[c] + 4] = f;
+    }
+    if (rows_out) std::copy(fp.rows.begin(), fp.rows.end(), rows_out);
+    if (dof_out) std::copy(tmp.dof.begin(), tmp.dof.end(), dof_out);
     return GF_OK;
 }
 
@@ -736,7 +816,7 @@ extern "C" int gf_model_footprint(const gf_model_desc *desc, const gf_options *o
     if (!desc || !bytes) return GF_E_ARG;
     gf_model tmp;
     int bm = -1, nblk = 0;
-    GF_TRY(plan(desc, &tmp, bm, nblk));
+    GF_TRY(plan(desc, &tmp, bm, nblk, opt ? opt->solver : GF_SOLVER_AUTO));
     *bytes = footprint(desc, &tmp, opt ? opt->precision : GF_PREC_FP64, bm, nblk);
     return GF_OK;
 }
@@ -750,6 +830,7 @@ extern "C" int gf_model_create(const gf_model_desc *desc, const gf_options *opt,
         m->refine_max = opt->refine_max;
         m->refine_tol = opt->refine_tol;
         m->refine_reuse = opt->refine_reuse != 0;
+        m->solver = opt->solver;
     }
     int st = create(desc, opt, m);
     if (st != GF_OK) {
@@ -779,8 +860,8 @@ extern "C" int gf_model_info(const gf_model *m, long long *o)
     o[3] = m->layout == GM_BTD ? m->btd.nblk : m->npad;
     o[4] = (long long)std::max(m->beam_color_off.size(), m->shell_color_off.size()) - 1;
     o[5] = m->layout == GM_BTD ? (long long)(3 * (size_t)m->btd.nblk * m->btd.bsz * m->btd.elem_size)
-                               : (long long)(m->band_elems * 8);
-    o[6] = m->layout;
+                               : (long long)((m->band_elems + m->fronts.f_elems + m->fronts.e_elems) * 8);
+    o[6] = m->fronts_on ? 2 : m->layout;
     o[7] = m->precision;
     return GF_OK;
 }
@@ -1160,6 +1241,13 @@ static int solve_fp64_launches(gf_model *m, int &nl_fac, int &nl_sol)
 {
     const size_t npad = (size_t)m->npad;
     nl_sol = 0;
+    if (m->fronts_on) {
+        nl_fac = gf_front_factor(m->fronts, 8, m->d_band, m->band_ld, m->half_band, nullptr, m->d_front, m->d_front_e,
+                                 m->stream, m->d_flag);
+        if (nl_fac < 0) return nl_fac;
+        nl_sol = gf_front_solve(m->fronts, 8, m->d_front, m->d_front_e, m->d_r, m->d_x, m->d_front_v, m->stream);
+        return nl_sol < 0 ? nl_sol : GF_OK;
+    }
     if (m->layout == GM_BTD)
         nl_fac = gf_btd_factor_solve(m->btd, m->stream, m->d_flag, m->d_r, m->d_x, (int)npad, m->d_swork);
     else nl_fac = gf_band_factor(m->d_band, 8, m->npad, m->half_band, m->stream, m->d_flag, m->d_band_inv);
@@ -1204,7 +1292,15 @@ static int solve_fp64_enqueue(gf_model *m)
     }
     tick(m, GF_PH_FACTOR, true);
     int nl_fac = 0, nl_sol = 0;
-    if (m->layout == GM_BTD) {
+    if (m->fronts_on) {
+        nl_fac = gf_front_factor(m->fronts, 8, m->d_band, m->band_ld, m->half_band, nullptr, m->d_front, m->d_front_e,
+                                 m->stream, m->d_flag);
+        if (nl_fac < 0) return nl_fac;
+        tick(m, GF_PH_FACTOR, false);
+        tick(m, GF_PH_SOLVE, true);
+        nl_sol = gf_front_solve(m->fronts, 8, m->d_front, m->d_front_e, m->d_r, m->d_x, m->d_front_v, m->stream);
+        if (nl_sol < 0) return nl_sol;
+    } else if (m->layout == GM_BTD) {
         GF_TRY(solve_fp64_launches(m, nl_fac, nl_sol));
         tick(m, GF_PH_FACTOR, false);
         tick(m, GF_PH_SOLVE, true);

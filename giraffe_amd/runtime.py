@@ -76,6 +76,8 @@ def load():
     L.gf_kernel_probe_read.argtypes = [vp, dp, C.POINTER(C.c_longlong), dp, C.c_int]
     L.gf_model_footprint.argtypes = [C.POINTER(abi.GfModelDesc), C.POINTER(abi.GfOptions), C.POINTER(C.c_ulonglong)]
     L.gf_model_plan.argtypes = [C.POINTER(abi.GfModelDesc), C.POINTER(C.c_longlong)]
+    ip = C.POINTER(C.c_int)
+    L.gf_front_plan.argtypes = [C.POINTER(abi.GfModelDesc), C.POINTER(C.c_longlong), ip, ip, ip]
     L.gf_default_server_options.argtypes = [C.POINTER(abi.GfServerOptions)]
     L.gf_server_create.argtypes = [C.POINTER(abi.GfServerOptions), C.POINTER(vp)]
     L.gf_server_destroy.argtypes = [vp]
@@ -102,13 +104,15 @@ def check(status, what=""):
         raise GiraffeError(f"{what} failed with status {status}: {msg}")
 
 
-def options(precision=abi.GF_PREC_FP64, device=0, refine_max=None, refine_tol=None):
+def options(precision=abi.GF_PREC_FP64, device=0, refine_max=None, refine_tol=None, solver=None):
     """gf_options from the library's defaults (gf_default_options), with the
     given fields replaced."""
     o = abi.GfOptions()
     load().gf_default_options(C.byref(o))
     o.precision = precision
     o.device = device
+    if solver is not None:
+        o.solver = solver
     if refine_max is not None:
         o.refine_max = refine_max
     if refine_tol is not None:
@@ -119,11 +123,11 @@ def options(precision=abi.GF_PREC_FP64, device=0, refine_max=None, refine_tol=No
 class DeviceModel:
     """One model resident on the GPU; mirrors the gf_* entry points."""
 
-    def __init__(self, model, precision=abi.GF_PREC_FP64, device=0, refine_reuse=True):
+    def __init__(self, model, precision=abi.GF_PREC_FP64, device=0, refine_reuse=True, solver=abi.GF_SOLVER_AUTO):
         L = load()
         self.model = model
         self._desc = model.desc()
-        self._opt = options(precision, device)
+        self._opt = options(precision, device, solver=solver)
         self._opt.refine_reuse = 1 if refine_reuse else 0
         h = C.c_void_p()
         check(L.gf_model_create(C.byref(self._desc), C.byref(self._opt), C.byref(h)), "gf_model_create")
@@ -146,7 +150,7 @@ class DeviceModel:
     def info(self):
         o = (C.c_longlong * 8)()
         check(load().gf_model_info(self.h, o), "gf_model_info")
-        keys = ["n_free", "half_band", "block", "n_blocks", "colors", "factor_bytes", "levels", "precision"]
+        keys = ["n_free", "half_band", "block", "n_blocks", "colors", "factor_bytes", "layout", "precision"]
         return dict(zip(keys, list(o)))
 
     def stats(self):
@@ -235,7 +239,7 @@ class DeviceModel:
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(names)}
 
     KERNEL_CLASSES = ("btd_level", "btd_tail", "band_panel", "band_update", "band_sweep", "beam_assemble",
-                      "shell_assemble")
+                      "shell_assemble", "front_step", "front_assemble", "front_solve")
 
     def kernel_probe(self, classes):
         """Bracket every launch of the named kernel classes with HIP events
@@ -335,10 +339,29 @@ def btdsystem(A_blocks, B_blocks, C_blocks, b):
     return x, fail.value
 
 
-def footprint(model, precision=abi.GF_PREC_FP64):
+def front_plan(model):
+    """Host-side multifrontal plan of `model` (gf_front_plan): sizes, per-front
+    arrays (k, kp, mp, level, parent, row-map offset), row maps and the DOF map."""
+    desc = model.desc()
+    info = (C.c_longlong * 8)()
+    L = load()
+    check(L.gf_front_plan(C.byref(desc), info, None, None, None), "gf_front_plan")
+    nf, nrows = info[0], info[2]
+    fronts = np.zeros((nf, 6), np.int32)
+    rows = np.zeros(max(nrows, 1), np.int32)
+    dof = np.zeros((model.n_nodes, 6), np.int32)
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+    check(L.gf_front_plan(C.byref(desc), info, ip(fronts), ip(rows), ip(dof)), "gf_front_plan")
+    keys = ["fronts", "levels", "rows", "front_elems", "inv_elems", "flops", "factor_launches", "solve_launches"]
+    out = dict(zip(keys, list(info)))
+    out.update({"front": fronts, "row_map": rows[:nrows], "dof": dof})
+    return out
+
+
+def footprint(model, precision=abi.GF_PREC_FP64, solver=None):
     """Device bytes a DeviceModel of `model` would hold (gf_model_footprint)."""
     desc = model.desc()
-    o = options(precision)
+    o = options(precision, solver=solver)
     b = C.c_ulonglong()
     check(load().gf_model_footprint(C.byref(desc), C.byref(o), C.byref(b)), "gf_model_footprint")
     return int(b.value)

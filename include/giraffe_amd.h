@@ -163,7 +163,18 @@ typedef struct gf_options {
     int refine_reuse;            /* mixed: keep the FP32 factor of an earlier tangent while the
                                     refinement against the current FP64 tangent converges fast
                                     enough; it is refactored when it does not (default 1) */
+    int solver;                  /* GF_SOLVER_*: tangent solver of a model whose band is too
+                                    wide for the block-tridiagonal form (default AUTO) */
 } gf_options;
+
+/* Tangent solvers.  Narrow bands always take the block-tridiagonal cyclic
+ * reduction (except under FRONTS); wider ones the band LU or the multifrontal
+ * factorisation over a nested-dissection ordering (PARDISO's own strategy,
+ * src/SparseMatrix.cpp:115-117).  AUTO picks the multifrontal one when its
+ * factorisation takes less than half the band LU's flops and fewer launches. */
+#define GF_SOLVER_AUTO   0
+#define GF_SOLVER_BAND   1
+#define GF_SOLVER_FRONTS 2
 
 typedef struct gf_model gf_model;
 
@@ -176,7 +187,8 @@ int gf_default_options(gf_options *opt);
 int gf_model_create(const gf_model_desc *desc, const gf_options *opt, gf_model **out);
 int gf_model_destroy(gf_model *m);
 /* n_free, half-bandwidth, BTD block size (0 for the band layout), block count or
- * padded order, colours, bytes of the tangent storage, layout (0 BTD, 1 band), precision. */
+ * padded order, colours, bytes of the tangent storage, layout (0 BTD, 1 band, 2 band
+ * assembly with the multifrontal solve), precision. */
 int gf_model_info(const gf_model *m, long long *info_out /* [8] */);
 /* Tangent solves so far, FP64 refinement sweeps (mixed precision), solves that
  * fell back to the FP64 factorisation, solves factored in FP64 directly by the
@@ -193,6 +205,17 @@ int gf_model_footprint(const gf_model_desc *desc, const gf_options *opt, unsigne
  * the band layout), 1 when the DOFs are numbered in reverse Cuthill-McKee node
  * order (chosen when it narrows the band of the caller's node order). */
 int gf_model_plan(const gf_model_desc *desc, long long *info_out /* [4] */);
+/* Host-side planning only (no device call; no counterpart in the reference, whose
+ * supernodes are PARDISO's): the multifrontal plan the description gets under
+ * GF_SOLVER_FRONTS.  info_out: fronts, levels, row-map entries, front matrix
+ * elements, diagonal-inverse elements, algorithmic flops of one factorisation,
+ * factorisation launches, solve launches.  front_out (NULL or [fronts][6]): own
+ * DOFs k, padded own rows kp, padded order mp, level, parent front (-1 for a
+ * root), offset of the row map; rows_out (NULL or [row-map entries]): band row of
+ * each front row (-1 for padding); dof_out (NULL or [n_nodes][6]): band row of
+ * each node DOF (-1 when not free).  Fronts are numbered level by level. */
+int gf_front_plan(const gf_model_desc *desc, long long *info_out /* [8] */, int *front_out, int *rows_out,
+                  int *dof_out);
 
 /* ---- Newton step pieces --------------------------------------------------- */
 /* Time window of the current solution step; the gravity BoolTable ramp
@@ -253,7 +276,14 @@ int gf_timing_read(gf_model *m, double *ms_out, long long *launches_out, int res
                                    launches)                                        */
 #define GF_KC_BEAM_ASSEMBLE  5  /* beam_assemble_kernel; unit: element              */
 #define GF_KC_SHELL_ASSEMBLE 6  /* shell_assemble_kernel; unit: element             */
-#define GF_NKC               7
+#define GF_KC_FRONT_STEP     7  /* front_step_kernel; unit: algorithmic flop of the partial
+                                   factorisations, sum over eliminated columns c of
+                                   2 (m - c - 1)^2 for a front of order m            */
+#define GF_KC_FRONT_ASSEMBLE 8  /* front_assemble_kernel; unit: front entry written */
+#define GF_KC_FRONT_SOLVE    9  /* front_fwd_own/fwd_upd/bwd_upd/bwd_own_kernel; unit: factor
+                                   byte read (own triangle with the diagonal inverses,
+                                   or the update panel)                              */
+#define GF_NKC              10
 int gf_kernel_probe(gf_model *m, unsigned mask);
 /* ms_out, launches_out, units_out: GF_NKC entries each (any may be NULL). */
 int gf_kernel_probe_read(gf_model *m, double *ms_out, long long *launches_out, double *units_out, int reset);
