@@ -12,14 +12,18 @@
 //             column j is one of the front's own DOFs (the front of the first of
 //             the two in elimination order holds it).  Every entry is written by
 //             one thread, in a fixed order: no atomics, bit-reproducible.
+//             The workgroup of each front's first diagonal block also inverts it
+//             (E_0).
 //   factor    the block LU of the band solver on each front, restricted to its
 //             kb own block columns, over all fronts of the level at once:
-//             launch p updates A(i, j) -= A(i, p) (E_p A'(p, j)) for i, j > p
-//             (the update blocks included: after the last own step they hold the
-//             Schur complement the parent assembles), inverts the next diagonal
-//             block (E_{p+1}) and converts row p - 1 to U.  Pivots: as the band
-//             solver (perturbation below 1e-13 of the block's diagonal, flag
-//             bit 2; non-finite, bit 1).
+//             launch p = 0 .. kb - 1 updates A(i, j) -= A(i, p) (E_p A'(p, j))
+//             for i, j > p (the update blocks included: after the last own step
+//             they hold the Schur complement the parent assembles), inverts the
+//             next diagonal block (E_{p+1}) and converts row p - 1 to U.  The
+//             last own block row stays A'(kb - 1, .) = D U(kb - 1, .): the
+//             backward solve applies E_{kb-1} to it, which saves a launch per
+//             level.  Pivots: as the band solver (perturbation below 1e-13 of the
+//             block's diagonal, flag bit 2; non-finite, bit 1).
 //
 // Solve, with A = L U (L(p, p) = D_p = E_p^-1, U block unit upper):
 //   forward   per level: a front gathers its own right-hand side and its
@@ -43,10 +47,19 @@ struct FrontTab {
     const long long *child_inv;
 };
 
+// LDS of the assembly kernel: the first diagonal block of a front and the
+// inverse's scratch.
+template <typename T> struct AsmSmem {
+    T P[NB * LDT];
+    T tmp[3 * 16 * TMP_LD];
+    T aux[4 + NB];
+};
+
 template <typename T>
 __global__ __launch_bounds__(256) void front_assemble_kernel(FrontTab t, int first, const double *ab, int ld, int hb,
-                                                             const double *D, T *Fb)
+                                                             const double *D, T *Fb, T *Eb, int *flag)
 {
+    __shared__ AsmSmem<T> sm;
     const int *it = t.items + 4 * (size_t)(first + blockIdx.x);
     const int f = it[0];
     const gf_front_dev F = t.fr[f];
@@ -85,9 +98,17 @@ __global__ __launch_bounds__(256) void front_assemble_kernel(FrontTab t, int fir
         }
         out[(size_t)ri * F.mp] = v[r];
     }
+    if (it[3] == 0) return;
+    // the front's first diagonal block: E_0
+#pragma unroll
+    for (int r = 0; r < 16; r++) sm.P[(i0 + 4 * r) * LDT + j] = v[r];
+    __syncthreads();
+    inverse64<T>(sm.P, sm.tmp, sm.aux, flag);
+    T *E = Eb + F.E;
+    for (int q = threadIdx.x; q < NB * NB; q += 256) E[q] = sm.P[(q / NB) * LDT + q % NB];
 }
 
-// Launch p of a level's factorisation (the band step on every front of the
+// Launch p >= 0 of a level's factorisation (the band step on every front of the
 // level; see the file header).  Item kinds: 0 diagonal, 1 update, 2 conversion.
 template <typename T>
 __global__ __launch_bounds__(256) void front_step_kernel(FrontTab t, int first, int p, T *Fb, T *Eb, int *flag)
@@ -119,7 +140,7 @@ __global__ __launch_bounds__(256) void front_step_kernel(FrontTab t, int first, 
         return;
     }
     T old[2][2][4];
-    if (p >= 0) {
+    {
         const int K0 = p * NB;
         T e[16], a[16], l[16];
         const T *E = Ef + (size_t)p * NB * NB;
@@ -138,8 +159,6 @@ __global__ __launch_bounds__(256) void front_step_kernel(FrontTab t, int first, 
         tile_put(sm.P, LDT, l);
         __syncthreads();
         lds_mm<T>(acc, sm.P, LDT, sm.Q, LDQ);  // L(i, p) U(p, j)
-    } else {
-        acc_each<T>([&](int x, int y, int v, int i, int j) { old[x][y][v] = A[(R0 + i) * ld + C0 + j]; });
     }
     if (kind == 1) {
         acc_each<T>([&](int x, int y, int v, int i, int j) { A[(R0 + i) * ld + C0 + j] = old[x][y][v] - acc.a[x][y][v]; });
@@ -154,9 +173,49 @@ __global__ __launch_bounds__(256) void front_step_kernel(FrontTab t, int first, 
 }
 
 // ---- solve -----------------------------------------------------------------------
-// Row r of a 64-row tile belongs to wave r % 4 (rows wave + 4 s, s = 0..15); lane
-// l multiplies column l, so each load of a wave reads 64 consecutive entries of a
-// row, and the row sums are wavefront reductions.
+// Tile rows are summed by quads of lanes: thread t takes row t / 4 of a 64-row
+// tile and its 16 columns 16 (t % 4) .. 16 (t % 4) + 15 (one 128-byte run of an
+// FP64 row, read with 16-byte vector loads), so a row's partial sums sit in four
+// neighbouring lanes and two DPP moves reduce them.  Front rows are 512-byte
+// aligned (fronts and diagonal inverses start at multiples of 64 x 64 entries,
+// rows are multiples of 64 entries long).
+
+template <typename T> __device__ __forceinline__ void load16(const T *p, T (&v)[16]);
+template <> __device__ __forceinline__ void load16<double>(const double *p, double (&v)[16])
+{
+    const double2 *q = reinterpret_cast<const double2 *>(p);
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const double2 w = q[e];
+        v[2 * e] = w.x;
+        v[2 * e + 1] = w.y;
+    }
+}
+template <> __device__ __forceinline__ void load16<float>(const float *p, float (&v)[16])
+{
+    const float4 *q = reinterpret_cast<const float4 *>(p);
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const float4 w = q[e];
+        v[4 * e] = w.x;
+        v[4 * e + 1] = w.y;
+        v[4 * e + 2] = w.z;
+        v[4 * e + 3] = w.w;
+    }
+}
+template <typename T> __device__ __forceinline__ double dot16(const T (&v)[16], const double *y)
+{
+    double s = 0.0;
+#pragma unroll
+    for (int e = 0; e < 16; e++) s += (double)v[e] * y[e];
+    return s;
+}
+// Sum over the four lanes of a quad (DPP quad_perm [1,0,3,2] then [2,3,0,1]).
+__device__ __forceinline__ double quad_sum(double s)
+{
+    s += dpp_mov<0xB1>(s);
+    return s + dpp_mov<0x4E>(s);
+}
 
 // Update vector entry of front position i gathered from the children.
 __device__ __forceinline__ double child_sum(const FrontTab &t, const gf_front_dev &F, int i, const double *Vb)
@@ -172,6 +231,8 @@ __device__ __forceinline__ double child_sum(const FrontTab &t, const gf_front_de
     return w;
 }
 
+// Own rows of a front: w = b_own + children's update vectors, then
+// y_p = E_p (w_p - sum_{q<p} L(p, q) y_q) block by block (one workgroup, y in LDS).
 template <typename T>
 __global__ __launch_bounds__(256) void front_fwd_own_kernel(FrontTab t, int f0, const T *Fb, const T *Eb,
                                                             const double *rhs, double *Vb)
@@ -180,9 +241,8 @@ __global__ __launch_bounds__(256) void front_fwd_own_kernel(FrontTab t, int f0, 
     __shared__ double tmp[NB];
     const int f = f0 + blockIdx.x;
     const gf_front_dev F = t.fr[f];
-    const int kp = NB * F.kb, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int kp = NB * F.kb, tid = threadIdx.x, r = tid >> 2, sg = 16 * (tid & 3);
     const int *rows = t.rows + F.rows;
-    const T *A = Fb + F.F;
     const size_t ld = (size_t)F.mp;
     for (int i = tid; i < kp; i += 256) {
         const int o = rows[i];
@@ -190,30 +250,20 @@ __global__ __launch_bounds__(256) void front_fwd_own_kernel(FrontTab t, int f0, 
     }
     __syncthreads();
     for (int p = 0; p < F.kb; p++) {
-        double part[16];
-#pragma unroll
-        for (int s = 0; s < 16; s++) part[s] = 0.0;
+        T e[16];
+        load16(Eb + F.E + (size_t)p * NB * NB + r * NB + sg, e);
+        const T *L = Fb + F.F + (size_t)(NB * p + r) * ld + sg;
+        double part = 0.0;
         for (int q = 0; q < p; q++) {
-            const double yq = y[NB * q + lane];
-            const T *L = A + (size_t)(NB * p) * ld + NB * q + lane;
-#pragma unroll
-            for (int s = 0; s < 16; s++) part[s] += (double)L[(size_t)(wave + 4 * s) * ld] * yq;
+            T v[16];
+            load16(L + NB * q, v);
+            part += dot16(v, y + NB * q + sg);
         }
-#pragma unroll
-        for (int s = 0; s < 16; s++) {
-            const double r = sum64(part[s]);
-            if (lane == 0) tmp[wave + 4 * s] = y[NB * p + wave + 4 * s] - r;
-        }
+        part = quad_sum(part);
+        if ((tid & 3) == 0) tmp[r] = y[NB * p + r] - part;
         __syncthreads();
-        const T *E = Eb + F.E + (size_t)p * NB * NB + lane;
-        const double tv = tmp[lane];
-#pragma unroll
-        for (int s = 0; s < 16; s++) part[s] = (double)E[(wave + 4 * s) * NB] * tv;
-#pragma unroll
-        for (int s = 0; s < 16; s++) {
-            const double r = sum64(part[s]);
-            if (lane == 0) y[NB * p + wave + 4 * s] = r;
-        }
+        part = quad_sum(dot16(e, tmp + sg));
+        if ((tid & 3) == 0) y[NB * p + r] = part;
         __syncthreads();
     }
     for (int i = tid; i < kp; i += 256) Vb[F.V + i] = y[i];
@@ -223,52 +273,65 @@ __global__ __launch_bounds__(256) void front_fwd_own_kernel(FrontTab t, int f0, 
 template <typename T>
 __global__ __launch_bounds__(256) void front_fwd_upd_kernel(FrontTab t, int first, const T *Fb, double *Vb)
 {
+    __shared__ double y[KMAX];
     const int *it = t.items + 4 * (size_t)(first + blockIdx.x);
     const gf_front_dev F = t.fr[it[0]];
-    const int tr = it[1], lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const T *A = Fb + F.F + (size_t)(NB * tr) * F.mp + lane;
-    const double *y = Vb + F.V + lane;
-    double part[16];
-#pragma unroll
-    for (int s = 0; s < 16; s++) part[s] = 0.0;
+    const int tr = it[1], tid = threadIdx.x, r = tid >> 2, sg = 16 * (tid & 3);
+    const int kp = NB * F.kb, i = NB * tr + r;
+    for (int j = tid; j < kp; j += 256) y[j] = Vb[F.V + j];
+    const double c = (tid & 3) == 0 ? child_sum(t, F, i, Vb) : 0.0;
+    __syncthreads();
+    const T *L = Fb + F.F + (size_t)i * F.mp + sg;
+    double part = 0.0;
     for (int q = 0; q < F.kb; q++) {
-        const double yq = y[NB * q];
-#pragma unroll
-        for (int s = 0; s < 16; s++) part[s] += (double)A[(size_t)(wave + 4 * s) * F.mp + NB * q] * yq;
+        T v[16];
+        load16(L + NB * q, v);
+        part += dot16(v, y + NB * q + sg);
     }
-#pragma unroll
-    for (int s = 0; s < 16; s++) {
-        const double r = sum64(part[s]);
-        const int i = NB * tr + wave + 4 * s;
-        if (lane == 0) Vb[F.V + i] = child_sum(t, F, i, Vb) - r;
-    }
+    part = quad_sum(part);
+    if ((tid & 3) == 0) Vb[F.V + i] = c - part;
 }
 
-// Own row tile p of a front with an update part: z_p = y_p - U(p, update) x_update.
+// Own row tile p of a front with an update part: z_p = y_p - U(p, update) x_update
+// (x_update gathered from the ancestors' rows of x into LDS, 1024 at a time).
 template <typename T>
-__global__ __launch_bounds__(256) void front_bwd_upd_kernel(FrontTab t, int first, const T *Fb, const double *x,
-                                                            double *Vb)
+__global__ __launch_bounds__(256) void front_bwd_upd_kernel(FrontTab t, int first, const T *Fb, const T *Eb,
+                                                            const double *x, double *Vb)
 {
+    __shared__ double xu[KMAX];
     const int *it = t.items + 4 * (size_t)(first + blockIdx.x);
     const gf_front_dev F = t.fr[it[0]];
-    const int p = it[1], lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int p = it[1], tid = threadIdx.x, r = tid >> 2, sg = 16 * (tid & 3);
     const int *rows = t.rows + F.rows;
-    const T *A = Fb + F.F + (size_t)(NB * p) * F.mp + lane;
-    double part[16];
-#pragma unroll
-    for (int s = 0; s < 16; s++) part[s] = 0.0;
-    for (int q = F.kb; q < F.mb; q++) {
-        const int o = rows[NB * q + lane];
-        const double xq = o >= 0 ? x[o] : 0.0;
-#pragma unroll
-        for (int s = 0; s < 16; s++) part[s] += (double)A[(size_t)(wave + 4 * s) * F.mp + NB * q] * xq;
+    const T *U = Fb + F.F + (size_t)(NB * p + r) * F.mp + sg;
+    double part = 0.0;
+    for (int c0 = NB * F.kb; c0 < F.mp; c0 += KMAX) {
+        const int cn = min(KMAX, F.mp - c0);
+        __syncthreads();
+        for (int j = tid; j < cn; j += 256) {
+            const int o = rows[c0 + j];
+            xu[j] = o >= 0 ? x[o] : 0.0;
+        }
+        __syncthreads();
+        for (int q = 0; q < cn / NB; q++) {
+            T v[16];
+            load16(U + c0 + NB * q, v);
+            part += dot16(v, xu + NB * q + sg);
+        }
     }
-#pragma unroll
-    for (int s = 0; s < 16; s++) {
-        const double r = sum64(part[s]);
-        const int i = NB * p + wave + 4 * s;
-        if (lane == 0) Vb[F.V + i] -= r;
+    part = quad_sum(part);
+    if (p < F.kb - 1) {
+        if ((tid & 3) == 0) Vb[F.V + NB * p + r] -= part;
+        return;
     }
+    // the last own block row holds A'(p, .) = D_p U(p, .): subtract E_p (A' x)
+    T e[16];
+    load16(Eb + F.E + (size_t)p * NB * NB + r * NB + sg, e);
+    __syncthreads();
+    if ((tid & 3) == 0) xu[r] = part;
+    __syncthreads();
+    part = quad_sum(dot16(e, xu + sg));
+    if ((tid & 3) == 0) Vb[F.V + NB * p + r] -= part;
 }
 
 // Own-block back substitution x_p = z_p - sum_{p < j < kb} U(p, j) x_j, then the
@@ -279,27 +342,21 @@ __global__ __launch_bounds__(256) void front_bwd_own_kernel(FrontTab t, int f0, 
     __shared__ double y[KMAX];
     const int f = f0 + blockIdx.x;
     const gf_front_dev F = t.fr[f];
-    const int kp = NB * F.kb, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int kp = NB * F.kb, tid = threadIdx.x, r = tid >> 2, sg = 16 * (tid & 3);
     const int *rows = t.rows + F.rows;
-    const T *A = Fb + F.F;
     const size_t ld = (size_t)F.mp;
     for (int i = tid; i < kp; i += 256) y[i] = Vb[F.V + i];
     __syncthreads();
     for (int p = F.kb - 2; p >= 0; p--) {
-        double part[16];
-#pragma unroll
-        for (int s = 0; s < 16; s++) part[s] = 0.0;
+        const T *U = Fb + F.F + (size_t)(NB * p + r) * ld + sg;
+        double part = 0.0;
         for (int j = p + 1; j < F.kb; j++) {
-            const double xj = y[NB * j + lane];
-            const T *U = A + (size_t)(NB * p) * ld + NB * j + lane;
-#pragma unroll
-            for (int s = 0; s < 16; s++) part[s] += (double)U[(size_t)(wave + 4 * s) * ld] * xj;
+            T v[16];
+            load16(U + NB * j, v);
+            part += dot16(v, y + NB * j + sg);
         }
-#pragma unroll
-        for (int s = 0; s < 16; s++) {
-            const double r = sum64(part[s]);
-            if (lane == 0) y[NB * p + wave + 4 * s] -= r;
-        }
+        part = quad_sum(part);
+        if ((tid & 3) == 0) y[NB * p + r] -= part;
         __syncthreads();
     }
     for (int i = tid; i < F.k; i += 256) x[rows[i]] = y[i];
@@ -326,12 +383,12 @@ int front_factor_t(const gf_fronts &fp, const double *ab, int ld, int hb, const 
     for (int l = 0; l < fp.n_level; l++) {
         int pr = gf_probe_begin(s, GF_KC_FRONT_ASSEMBLE);
         hipLaunchKernelGGL(front_assemble_kernel<T>, dim3(fp.asm_count[l]), dim3(256), 0, s, t, fp.asm_first[l], ab, ld,
-                           hb, D, F);
+                           hb, D, F, E, flag);
         gf_probe_end(s, pr, (double)fp.asm_count[l] * NB * NB);
         launches++;
         for (int k = fp.lev_step[l]; k < fp.lev_step[l + 1]; k++) {
             if (fp.step_count[k] == 0) continue;
-            const int p = k - fp.lev_step[l] - 1;
+            const int p = k - fp.lev_step[l];
             pr = gf_probe_begin(s, GF_KC_FRONT_STEP);
             hipLaunchKernelGGL(front_step_kernel<T>, dim3(fp.step_count[k]), dim3(256), 0, s, t, fp.step_first[k], p,
                                F, E, flag);
@@ -383,7 +440,7 @@ int front_solve_t(const gf_fronts &fp, const T *F, const T *E, const double *rhs
         if (fp.bupd_count[l] > 0) {
             pr = gf_probe_begin(s, GF_KC_FRONT_SOLVE);
             hipLaunchKernelGGL(front_bwd_upd_kernel<T>, dim3(fp.bupd_count[l]), dim3(256), 0, s, t, fp.bupd_first[l], F,
-                               (const double *)x, V);
+                               E, (const double *)x, V);
             gf_probe_end(s, pr, pr >= 0 ? panel_bytes(l) : 0.0);
             launches++;
         }

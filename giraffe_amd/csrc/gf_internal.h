@@ -263,7 +263,7 @@ struct gf_fronts {
     std::vector<long long> child_inv;  // offset of each child entry's map in inv (parent position -> child update row or -1)
     std::vector<int> inv;
     std::vector<int> items;            // work items, 4 ints each: front, tile row, tile column, kind (front -1: none)
-    std::vector<int> step_first, step_count;  // factor launch s of level l: index lev_step[l] + s, s = 0 .. lev_kb[l] + 1
+    std::vector<int> step_first, step_count;  // factor launch p of level l: index lev_step[l] + p, p = 0 .. lev_kb[l] - 1
     std::vector<int> lev_step;
     std::vector<int> asm_first, asm_count;    // per level: assembly tiles
     std::vector<int> fupd_first, fupd_count;  // per level: forward update row tiles (front, tile row)
@@ -421,6 +421,10 @@ struct gf_model {
     gf_fronts fronts;
     double *d_front = nullptr, *d_front_e = nullptr, *d_front_v = nullptr;  // FP64 factor, vector workspace
     float *d_front32 = nullptr, *d_front32_e = nullptr;                      // FP32 factor (mixed)
+    // structure of the assembled tangent (the band entries of element node pairs,
+    // row by row): what the assembly clears and the mixed residual multiplies
+    std::vector<int> csr_ptr, csr_col;
+    int *d_csr_ptr = nullptr, *d_csr_col = nullptr;
     double *d_scale = nullptr, *d_res = nullptr, *d_w = nullptr;
     long long n_solves = 0, n_sweeps = 0, n_fallbacks = 0, n_direct = 0, n_f32_factor = 0;
     int fallback_streak = 0, fp64_direct_left = 0;  // mixed-precision back-off
@@ -467,6 +471,10 @@ int gf_launch_fill(hipStream_t s, double *p, size_t n, double v);
 int gf_launch_clear3(hipStream_t s, double *a, size_t na, double *b, size_t nb, double *c, size_t nc,
                      size_t dfirst = 0, size_t dstride = 0, size_t dcount = 0);
 int gf_launch_fill_bytes(hipStream_t s, void *p, size_t bytes);
+// Zeroes a[0..na), c[0..nc) and the band entries of the structure (ptr, cols)
+// (rows < nrows), stored as the band ab (row stride ld, half-bandwidth hb).
+int gf_launch_clear_csr(hipStream_t s, double *a, size_t na, double *c, size_t nc, double *ab, int ld, int hb,
+                        const int *ptr, const int *cols, int nrows);
 // p[0] = a, p[1] = b (a kernel, so it can sit in a captured graph)
 int gf_launch_set2(hipStream_t s, double *p, double a, double b);
 // Returns the number of kernel launches issued (>= 0) or a negative status.

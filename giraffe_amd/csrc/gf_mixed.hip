@@ -94,6 +94,27 @@ __global__ void residual_kernel(gf_gmat g, int npad, const double *r, const doub
     if (lane == 0) s[row] = r[row] - acc;
 }
 
+// s = r - K x over the tangent's structure (ptr, cols), one wavefront per row;
+// the padding rows carry the unit diagonal.
+__global__ void residual_csr_kernel(const double *ab, int ld, int hb, const int *ptr, const int *cols, int nrows,
+                                    int npad, const double *r, const double *x, double *s)
+{
+    const int lane = threadIdx.x & 63;
+    const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (row >= npad) return;
+    if (row >= nrows) {
+        if (lane == 0) s[row] = r[row] - x[row];
+        return;
+    }
+    const double *base = ab + (size_t)row * (size_t)ld + (size_t)hb;
+    double acc = 0.0;
+    for (int k = ptr[row] + lane; k < ptr[row + 1]; k += 64) {
+        const int c = cols[k];
+        acc += base[c] * x[c];
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane == 0) s[row] = r[row] - acc;
+}
 __global__ void scale_kernel(int n, const double *D, const double *v, double *w)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -220,8 +241,13 @@ int gf_mixed_sweep(gf_model *m)
     GF_TRY(gf_launch_fill_bytes(m->stream, m->d_scal + GF_SCAL_REF, 2 * sizeof(double)));
     hipLaunchKernelGGL(correct_kernel, dim3(grid_for(npad, 256, 512)), dim3(256), 0, m->stream, npad, m->d_scale, sol,
                        m->d_x, (unsigned long long *)(m->d_scal + GF_SCAL_REF));
-    hipLaunchKernelGGL(residual_kernel, dim3((npad * 64 + 255) / 256), dim3(256), 0, m->stream, g, npad, m->d_r,
-                       m->d_x, m->d_res);
+    if (m->fronts_on)
+        hipLaunchKernelGGL(residual_csr_kernel, dim3((npad * 64 + 255) / 256), dim3(256), 0, m->stream, m->d_band,
+                           m->band_ld, m->half_band, m->d_csr_ptr, m->d_csr_col, m->n_free, npad, m->d_r, m->d_x,
+                           m->d_res);
+    else
+        hipLaunchKernelGGL(residual_kernel, dim3((npad * 64 + 255) / 256), dim3(256), 0, m->stream, g, npad, m->d_r,
+                           m->d_x, m->d_res);
     GF_HIP(hipGetLastError());
     GF_HIP(hipMemcpyAsync(m->h_scal + GF_SCAL_FLAG, m->d_scal + GF_SCAL_FLAG, sizeof(double), hipMemcpyDeviceToHost,
                           m->stream));

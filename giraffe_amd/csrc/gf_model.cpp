@@ -167,7 +167,7 @@ static void release(gf_model *m)
                     m->d_ln_node, m->d_ln_off, m->d_ln_ids, m->d_lrec, m->d_tab_t, m->d_tab_v, m->d_r, m->d_x,
                     m->d_scal, m->d_copy0, m->d_bgp0, m->d_sconn, m->d_sconst, m->d_spconst, m->d_sgp,
                     m->d_sgp0, m->d_spload, m->d_spflag, m->d_band, m->d_swork, m->d_band_inv, m->d_bbuoy,
-                    m->d_front, m->d_front_e, m->d_front_v};
+                    m->d_front, m->d_front_e, m->d_front_v, m->d_csr_ptr, m->d_csr_col};
     for (void *b : bufs) gf_pool_free(b);
     gf_front_free(m->fronts);
     if (m->h_scal) gf_pinned_free(m->h_scal);
@@ -284,6 +284,39 @@ static std::vector<int> rcm_order(const gf_model_desc *d, int nn)
     for (int i = 0; i < nn; i++)
         if (!done[i]) order.push_back(i);
     return order;
+}
+
+// Band entries of the tangent's structure, row by row: for each free DOF row,
+// the free DOF columns of every node that shares an element with the row's node
+// (itself included), in increasing column order.
+static void tangent_structure(const gf_model_desc *d, const gf_model *m, std::vector<int> &ptr, std::vector<int> &col)
+{
+    const int nn = m->nn;
+    std::vector<std::vector<int>> adj(nn);
+    auto link = [&](const int *cn, int nper) {
+        for (int a = 0; a < nper; a++)
+            for (int b = 0; b < nper; b++) adj[cn[a]].push_back(cn[b]);
+    };
+    for (int e = 0; e < d->n_beams; e++) link(d->beam_conn + 3 * (size_t)e, 3);
+    for (int e = 0; e < d->n_shells; e++) link(d->shell_conn + 6 * (size_t)e, 6);
+    ptr.assign(1, 0);
+    col.clear();
+    std::vector<int> cols;
+    for (int r = 0; r < m->n_free; r++) {
+        const int a = m->row_nodecomp[r] / 6;
+        std::vector<int> &v = adj[a];
+        if (!v.empty() && (r == 0 || m->row_nodecomp[r - 1] / 6 != a)) {
+            std::sort(v.begin(), v.end());
+            v.erase(std::unique(v.begin(), v.end()), v.end());
+            cols.clear();
+            for (int b : v)
+                for (int c = 0; c < 6; c++)
+                    if (m->dof[6 * (size_t)b + c] >= 0) cols.push_back(m->dof[6 * (size_t)b + c]);
+            std::sort(cols.begin(), cols.end());
+        }
+        col.insert(col.end(), cols.begin(), cols.end());
+        ptr.push_back((int)col.size());
+    }
 }
 
 // Host-side planning shared by create() and gf_model_footprint: validation,
@@ -404,6 +437,7 @@ static int plan(const gf_model_desc *d, gf_model *m, int &bm, int &nblk, int sol
                           (m->fronts.flops < 0.5 * band_flops && gf_front_factor_launches(m->fronts) < npad / 64);
         if (take) {
             m->fronts_on = true;
+            tangent_structure(d, m, m->csr_ptr, m->csr_col);
             m->layout = GM_BAND;
             m->npad = npad;
             bm = -1;
@@ -439,7 +473,7 @@ static size_t footprint(const gf_model_desc *d, const gf_model *m, int precision
         const size_t factor = (size_t)(fp.f_elems + fp.e_elems);
         b += (npad * (size_t)(2 * m->half_band + 1) + factor + (size_t)fp.v_elems) * 8;
         b += (fp.rows.size() + fp.child.size() + fp.inv.size() + fp.items.size()) * 4 + fp.child_inv.size() * 8 +
-             fp.fr.size() * sizeof(gf_front_dev);
+             fp.fr.size() * sizeof(gf_front_dev) + (m->csr_ptr.size() + m->csr_col.size()) * 4;
         if (precision == GF_PREC_MIXED) b += factor * 4 + 3 * npad * 8;
         return b;
     } else {
@@ -680,6 +714,15 @@ static int create(const gf_model_desc *d, const gf_options *opt, gf_model *m)
         GF_TRY(dalloc(&m->d_front_e, (size_t)m->fronts.e_elems));
         GF_TRY(dalloc(&m->d_front_v, (size_t)m->fronts.v_elems));
         GF_TRY(gf_front_upload(m->fronts, m->stream));
+        GF_TRY(dalloc(&m->d_csr_ptr, m->csr_ptr.size()));
+        GF_TRY(dalloc(&m->d_csr_col, m->csr_col.size()));
+        GF_TRY(upload(m, m->d_csr_ptr, m->csr_ptr.data(), m->csr_ptr.size()));
+        GF_TRY(upload(m, m->d_csr_col, m->csr_col.data(), m->csr_col.size()));
+        // the assembly clears only the structure: everything else of the band is
+        // zero from here on, the padding rows' unit diagonal included
+        const size_t nfr = (size_t)m->n_free;
+        GF_TRY(gf_launch_clear3(m->stream, nullptr, 0, m->d_band, m->band_elems, nullptr, 0,
+                                nfr * (size_t)(m->band_ld + 1) + (size_t)hb, (size_t)(m->band_ld + 1), npad - nfr));
     } else {
         m->band_ld = hb + gf_band_ku(hb);
         m->band_elems = npad * (size_t)(m->band_ld + 1);
@@ -949,7 +992,11 @@ int gf_assemble_enqueue(gf_model *m, bool copy_norms)
         dfirst = nf * (size_t)(m->band_ld + 1) + (size_t)m->half_band;
         dstride = (size_t)(m->band_ld + 1);
     }
-    GF_TRY(gf_launch_clear3(m->stream, m->d_r, npad, tan, ntan, m->d_scal + GF_SCAL_RES, 8, dfirst, dstride, dcount));
+    if (m->fronts_on)
+        GF_TRY(gf_launch_clear_csr(m->stream, m->d_r, npad, m->d_scal + GF_SCAL_RES, 8, m->d_band, m->band_ld,
+                                   m->half_band, m->d_csr_ptr, m->d_csr_col, m->n_free));
+    else
+        GF_TRY(gf_launch_clear3(m->stream, m->d_r, npad, tan, ntan, m->d_scal + GF_SCAL_RES, 8, dfirst, dstride, dcount));
     // load time and gravity ramp factor from the scalar block (set by gf_begin_step)
     const double *tp = m->d_scal + GF_SCAL_TIME, *lfp = m->d_scal + GF_SCAL_LF;
     GF_TRY(gf_nodal_loads(m, tp));

@@ -291,7 +291,9 @@ int gf_nd_plan(gf_fronts &fp, const gf_model_desc *d, const std::vector<int> &do
         for (int i = 0; i < F.mp; i++)
             if (pr[i] >= 0) ppos[pr[i]] = -1;
     }
-    // factor launches: per level, s = 0 .. lev_kb + 1 is step p = s - 1; items go
+    // factor launches: per level, launch s = step p = 0 .. lev_kb - 1 (the first
+    // diagonal block is inverted by the assembly launch, and the last own block
+    // row is left unconverted: the backward solve applies its inverse); items go
     // to XCD (front + tile row) mod 8 (workgroup b runs on XCD b mod 8), the
     // diagonal tiles first
     auto emit = [&](std::vector<std::vector<int>> &q) {
@@ -308,16 +310,12 @@ int gf_nd_plan(gf_fronts &fp, const gf_model_desc *d, const std::vector<int> &do
     fp.lev_step.assign(nlev + 1, 0);
     for (int l = 0; l < nlev; l++) {
         fp.lev_step[l] = (int)fp.step_first.size();
-        for (int p = -1; p <= fp.lev_kb[l]; p++) {
+        for (int p = 0; p < fp.lev_kb[l]; p++) {
             std::vector<std::vector<int>> diag(8), rest(8);
             double fl = 0.0;
             for (int f = fp.lev_off[l]; f < fp.lev_off[l + 1]; f++) {
                 const gf_front_dev &F = fp.fr[f];
                 const int x0 = f;
-                if (p < 0) {
-                    diag[x0 & 7].insert(diag[x0 & 7].end(), {f, 0, 0, 0});
-                    continue;
-                }
                 if (p < F.kb) {
                     for (int i = p + 1; i < F.mb; i++)
                         for (int j = p + 1; j < F.mb; j++) {
@@ -328,7 +326,7 @@ int gf_nd_plan(gf_fronts &fp, const gf_model_desc *d, const std::vector<int> &do
                     const int m = kvec[f] + uvec[f];
                     for (int c = TB * p; c < std::min(TB * (p + 1), kvec[f]); c++) fl += 2.0 * (double)(m - c - 1) * (m - c - 1);
                 }
-                if (p >= 1 && p <= F.kb)
+                if (p >= 1 && p <= F.kb - 1)
                     for (int j = p; j < F.mb; j++) rest[(x0 + p - 1) & 7].insert(rest[(x0 + p - 1) & 7].end(), {f, p - 1, j, 2});
             }
             for (int x = 0; x < 8; x++) diag[x].insert(diag[x].end(), rest[x].begin(), rest[x].end());
@@ -342,10 +340,14 @@ int gf_nd_plan(gf_fronts &fp, const gf_model_desc *d, const std::vector<int> &do
     fp.lev_step[nlev] = (int)fp.step_first.size();
     // assembly tiles and solve row tiles per level
     for (int l = 0; l < nlev; l++) {
+        // assembly tiles, the first diagonal block of every front first (kind 1:
+        // assembled and inverted)
         const int a0 = (int)fp.items.size() / 4;
+        for (int f = fp.lev_off[l]; f < fp.lev_off[l + 1]; f++) fp.items.insert(fp.items.end(), {f, 0, 0, 1});
         for (int f = fp.lev_off[l]; f < fp.lev_off[l + 1]; f++)
             for (int i = 0; i < fp.fr[f].mb; i++)
-                for (int j = 0; j < fp.fr[f].mb; j++) fp.items.insert(fp.items.end(), {f, i, j, 0});
+                for (int j = 0; j < fp.fr[f].mb; j++)
+                    if (i || j) fp.items.insert(fp.items.end(), {f, i, j, 0});
         fp.asm_first.push_back(a0);
         fp.asm_count.push_back((int)fp.items.size() / 4 - a0);
         const int u0 = (int)fp.items.size() / 4;

@@ -8,6 +8,8 @@
 // Solution::UpdateDisps src/Solution.cpp:390-423,
 // Node::SaveConfiguration src/Node.cpp:325-384.
 #include "gf_internal.h"
+
+#include <algorithm>
 #include "gf_device.h"
 
 namespace {
@@ -34,6 +36,21 @@ __global__ __launch_bounds__(256) void clear3_kernel(double *a, size_t na, doubl
         }
         *p = v;
     }
+}
+
+// Zeroes a and c, and the band entries (row, cols[k]) for k in [ptr[row],
+// ptr[row + 1]) of rows < nrows (the positions the element scatter writes): one
+// wavefront per band row, the short ranges by every thread.
+__global__ __launch_bounds__(256) void clear_csr_kernel(double *a, size_t na, double *c, size_t nc, double *ab,
+                                                         int ld, int hb, const int *ptr, const int *cols, int nrows)
+{
+    const size_t gid = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (gid < na) a[gid] = 0.0;
+    if (gid < nc) c[gid] = 0.0;
+    const int row = (int)(gid >> 6), lane = threadIdx.x & 63;
+    if (row >= nrows) return;
+    double *base = ab + (size_t)row * (size_t)ld + (size_t)hb;
+    for (int k = ptr[row] + lane; k < ptr[row + 1]; k += 64) base[cols[k]] = 0.0;
 }
 
 __global__ void fill_u32_kernel(unsigned *p, size_t n)
@@ -224,6 +241,16 @@ int gf_launch_fill(hipStream_t s, double *p, size_t n, double v)
 {
     if (n == 0) return GF_OK;
     hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, v);
+    GF_HIP(hipGetLastError());
+    return GF_OK;
+}
+
+int gf_launch_clear_csr(hipStream_t s, double *a, size_t na, double *c, size_t nc, double *ab, int ld, int hb,
+                        const int *ptr, const int *cols, int nrows)
+{
+    const size_t n = std::max((size_t)nrows * 64, std::max(na, nc));
+    hipLaunchKernelGGL(clear_csr_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, na, c, nc, ab, ld, hb,
+                       ptr, cols, nrows);
     GF_HIP(hipGetLastError());
     return GF_OK;
 }
