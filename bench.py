@@ -89,14 +89,17 @@ def cpu_baseline(m, p, kind, gpu_iterations, budget_s=30.0):
             "iterations_timed": iters}
 
 
-def newton_iterations(m, p, precision, count):
+SOLVERS = {"auto": abi.GF_SOLVER_AUTO, "band": abi.GF_SOLVER_BAND, "fronts": abi.GF_SOLVER_FRONTS}
+
+
+def newton_iterations(m, p, precision, count, solver="auto"):
     """`count` Newton iterations of the first load step through the per-phase
     entry points (assembly, tangent solve, update), one line each: a short run
     of every kernel of the path for counter passes under rocprofv3."""
     from giraffe_amd import runtime
 
     prec = abi.GF_PREC_FP64 if precision == "fp64" else abi.GF_PREC_MIXED
-    dm = runtime.DeviceModel(m, precision=prec)
+    dm = runtime.DeviceModel(m, precision=prec, solver=SOLVERS[solver])
     dm.set_window(p.start_time, p.end_time)
     dm.begin_step(p.start_time, p.time_step)
     out = []
@@ -112,7 +115,7 @@ def newton_iterations(m, p, precision, count):
     return out
 
 
-def measure(m, p, precision, steps, warmup, probe_steps):
+def measure(m, p, precision, steps, warmup, probe_steps, solver="auto"):
     """Warmup + `steps` timed full solves of one resident model (the state is
     reset on the device before each solve), with no instrumentation in the timed
     region.  Then `probe_steps` further solves with per-phase HIP events and HIP
@@ -123,7 +126,7 @@ def measure(m, p, precision, steps, warmup, probe_steps):
     from giraffe_amd import runtime
 
     prec = abi.GF_PREC_FP64 if precision == "fp64" else abi.GF_PREC_MIXED
-    dm = runtime.DeviceModel(m, precision=prec)
+    dm = runtime.DeviceModel(m, precision=prec, solver=SOLVERS[solver])
     info = dm.info()
     log(f"[{precision}] model resident: {info}")
     for w in range(warmup):
@@ -400,6 +403,8 @@ def main():
                     help="serve: HIP hardware queues for the request streams (GPU_MAX_HW_QUEUES, at most 32; "
                          "0 keeps the environment's setting)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--solver", default="auto", choices=sorted(SOLVERS),
+                    help="tangent solver for wide bands (auto: the library's choice)")
     ap.add_argument("--compare-steps", type=int, default=2,
                     help="solves of the other precision, reported beside the main figure (0: skip)")
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "pardiso", "band", "none"])
@@ -441,11 +446,11 @@ def main():
         return serve(args)
     m, p, desc = workload(args.workload)
     if args.newton_iterations > 0:
-        its = newton_iterations(m, p, args.precision, args.newton_iterations)
+        its = newton_iterations(m, p, args.precision, args.newton_iterations, args.solver)
         print(json.dumps({"mode": "newton-iterations", "workload": desc, "precision": args.precision,
                           "iterations": its}))
         return 0
-    r = measure(m, p, args.precision, args.steps, args.warmup, 0 if args.no_timing else args.probe_steps)
+    r = measure(m, p, args.precision, args.steps, args.warmup, 0 if args.no_timing else args.probe_steps, args.solver)
     info, reps = r["info"], r["reps"]
     its, solves = reps[-1][0], reps[-1][1]
     out = {
@@ -462,7 +467,10 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f64 tangent/residual, f32 factor + f64 refinement",
         "data": "synthetic (seeded generator, giraffe_amd/model.py)",
         "config": {"workload": desc, "precision": args.precision, "free_dofs": info["n_free"],
-                   "half_bandwidth": info["half_band"], "block": info["block"], "newton_iterations": its,
+                   "half_bandwidth": info["half_band"], "block": info["block"],
+                   "solver": {0: "block-tridiagonal cyclic reduction", 1: "band LU",
+                              2: "multifrontal LU over nested dissection"}[info["layout"]],
+                   "newton_iterations": its,
                    "tangent_solves": solves, "step_iterations": [int(v) for v in reps[-1][4]]},
         "p50_ms": float(np.percentile(r["lat_ms"], 50)),
         "p99_ms": float(np.percentile(r["lat_ms"], 99)),
@@ -484,7 +492,7 @@ def main():
                                        f"timed launches would cost about a quarter of the throughput)")
     if args.compare_steps > 0:
         other = "mixed" if args.precision == "fp64" else "fp64"
-        c = measure(m, p, other, args.compare_steps, 1, 0)
+        c = measure(m, p, other, args.compare_steps, 1, 0, args.solver)
         out["compare"] = {"precision": other, "solves_per_s": args.compare_steps / c["wall"],
                           "p50_ms": float(np.percentile(c["lat_ms"], 50)),
                           "p99_ms": float(np.percentile(c["lat_ms"], 99)),
