@@ -421,6 +421,8 @@ struct gf_model {
     gf_fronts fronts;
     double *d_front = nullptr, *d_front_e = nullptr, *d_front_v = nullptr;  // FP64 factor, vector workspace
     float *d_front32 = nullptr, *d_front32_e = nullptr;                      // FP32 factor (mixed)
+    double *d_shell_rec = nullptr;   // per (Shell_1, Gauss point) phase-1 record [3 ns][276]
+    double *d_shell_prec = nullptr;  // per (Shell_1, u-node) follower-pressure terms [6 ns][57]
     // structure of the assembled tangent (the band entries of element node pairs,
     // row by row): what the assembly clears and the mixed residual multiplies
     std::vector<int> csr_ptr, csr_col;
@@ -480,8 +482,7 @@ int gf_launch_set2(hipStream_t s, double *p, double a, double b);
 // Returns the number of kernel launches issued (>= 0) or a negative status.
 int gf_beam_assemble(gf_model *m, const double *lfp);
 int gf_beam_commit(gf_model *m);
-int gf_shell_assemble(gf_model *m, const double *lfp);
-int gf_shell_pressure(gf_model *m, const double *tp);
+int gf_shell_assemble(gf_model *m, const double *lfp, const double *tp);  // elements + follower pressure
 int gf_shell_eval_device(int count, const double *x, const double *prop, const double *d, const double *gp,
                          double *K_out, double *f_out);
 int gf_shell_commit(gf_model *m);

@@ -15,7 +15,11 @@
 // block row: (R, R) and (R, 4)).
 //
 // Workgroup: 256 lanes, SEPB = 4 elements; LDS holds per Gauss point the
-// 16 blocks of D, Psi, G, M and Psi^T sigma (48 KB).
+// 16 blocks of D, Psi, G, M and Psi^T sigma (48 KB).  In the Newton loop phase 1
+// runs for all elements in one launch beforehand (shell_record_kernel) and the
+// follower pressure's terms likewise (shell_pressure_record_kernel); the colour
+// launches copy the records and run phases 2-4, the pressure terms added to the
+// element's own K_IJ and force.
 //   phase 1  lane per (element, gp): kinematics, 3-point thickness integration
 //   phase 2  lane per (element, gp, block RS): the 25 blocks of M
 //   phase 3  lane per (element, I, J): K_IJ, rotated to global, scattered
@@ -34,6 +38,7 @@ constexpr int L_G = 216;   // Gua1, Gua2, Vm1, Vm2, Gaa
 constexpr int L_S = 261;   // Psi^T sigma (15)
 constexpr int L_M = 276;   // M blocks (R,S), 25 x 9
 constexpr int L_GP = 501;
+constexpr int SPR = 57;    // pressure record of (element, u-node a): P_a (3), then K_p rows of a (3 x 18)
 
 __device__ __forceinline__ m3<double> ld3(const double *p)
 {
@@ -207,12 +212,117 @@ __device__ __forceinline__ void dof_block(const int *__restrict__ conn, size_t l
     else { node = conn[(3 + I - 6) * ld + e]; c0 = 3; }
 }
 
+// Phase 1 of Shell_1::Mount for one (element, Gauss point): kinematics, the
+// 3-point thickness integration, and the record o[0 .. L_M) (D, Psi, G blocks
+// and Psi^T sigma) that the later phases read.
+__device__ __forceinline__ void shell_gp_record(const double *__restrict__ cst, const double *__restrict__ gps,
+                                                size_t ld, int e, int gp, const double *__restrict__ disp,
+                                                const int *__restrict__ conn, double *o)
+{
+    ShellKin k;
+    shell_kin(cst, gps, ld, e, gp, disp, conn, k);
+    const double mu = cst[SC_MU * ld + e], la = cst[SC_LAMBDA * ld + e], th = cst[SC_THICK * ld + e];
+    const double drill = cst[SC_DRILL * ld + e];
+    for (int i = 0; i < 144; i++) o[L_D + i] = 0.0;
+    v3<double> n1 = mk3(0.0, 0.0, 0.0), n2 = n1, m1 = n1, m2 = n1;
+    const double xi[3] = {-0.77459666924148337703585307995648, 0.0, 0.77459666924148337703585307995648};
+    const double wt[3] = {0.55555555555555555555555555555556, 0.88888888888888888888888888888889,
+                          0.55555555555555555555555555555556};
+    const double J = th / 2.0;
+    for (int q = 0; q < 3; q++) {
+        double zeta = th * xi[q] / 2.0;
+        const ShellPt pt = shell_point(k, mu, la, zeta);
+        const double g11 = pt.g11, g12 = pt.g12, g21 = pt.g21, g22 = pt.g22, v = pt.v, dv = pt.dv;
+        const v3<double> tau1 = pt.tau1, tau2 = pt.tau2;
+        m3<double> C11, C22, C12, C21;
+#pragma unroll
+        for (int i = 0; i < 9; i++) C11.a[i] = C22.a[i] = C12.a[i] = 0.0;
+        C11.a[0] = mu * ((1.0 + g22) * (1.0 + g22) * dv + 1.0);
+        C11.a[1] = -mu * (1.0 + g22) * g21 * dv;
+        C11.a[3] = -mu * (1.0 + g22) * g21 * dv;
+        C11.a[4] = mu * (g21 * g21 * dv + 1.0);
+        C11.a[8] = mu;
+        C22.a[0] = mu * (g12 * g12 * dv + 1.0);
+        C22.a[1] = -mu * ((1.0 + g11) * g12 * dv);
+        C22.a[3] = -mu * ((1.0 + g11) * g12 * dv);
+        C22.a[4] = mu * ((1.0 + g11) * (1.0 + g11) * dv + 1.0);
+        C22.a[8] = mu;
+        C12.a[0] = -mu * ((1.0 + g22) * g12 * dv);
+        C12.a[1] = mu * (v - 1.0 + (1.0 + g11) * (1.0 + g22) * dv);
+        C12.a[3] = mu * (1.0 - v + g12 * g21 * dv);
+        C12.a[4] = -mu * ((1.0 + g11) * g21 * dv);
+        C21 = tr3(C12);
+        double w = wt[q] * J;
+        acc3(o + L_D + 9 * 0, w, C11);            // dn1/deta1
+        acc3(o + L_D + 9 * 1, -w * zeta, mE3(C11)); // dn1/dk1
+        acc3(o + L_D + 9 * 2, w, C12);            // dn1/deta2
+        acc3(o + L_D + 9 * 3, -w * zeta, mE3(C12)); // dn1/dk2
+        acc3(o + L_D + 9 * 4, w * zeta, E3m(C11));  // dm1/deta1
+        acc3(o + L_D + 9 * 5, -w * zeta * zeta, mE3(E3m(C11)));
+        acc3(o + L_D + 9 * 6, w * zeta, E3m(C12));
+        acc3(o + L_D + 9 * 7, -w * zeta * zeta, mE3(E3m(C12)));
+        acc3(o + L_D + 9 * 8, w, C21);            // dn2/deta1
+        acc3(o + L_D + 9 * 9, -w * zeta, mE3(C21));
+        acc3(o + L_D + 9 * 10, w, C22);
+        acc3(o + L_D + 9 * 11, -w * zeta, mE3(C22));
+        acc3(o + L_D + 9 * 12, w * zeta, E3m(C21)); // dm2/deta1
+        acc3(o + L_D + 9 * 13, -w * zeta * zeta, mE3(E3m(C21)));
+        acc3(o + L_D + 9 * 14, w * zeta, E3m(C22));
+        acc3(o + L_D + 9 * 15, -w * zeta * zeta, mE3(E3m(C22)));
+        n1 = add3(n1, scl3(w, tau1));
+        n2 = add3(n2, scl3(w, tau2));
+        // e3 x tau = (-tau.y, tau.x, 0)
+        m1 = add3(m1, scl3(w * zeta, mk3(-tau1.y, tau1.x, 0.0)));
+        m2 = add3(m2, scl3(w * zeta, mk3(-tau2.y, tau2.x, 0.0)));
+    }
+    // drilling (overwrites, as the reference)
+    o[L_D + 9 * 5 + 8] = drill;   // D(5,5): block (1,1) entry (2,2)
+    o[L_D + 9 * 15 + 8] = drill;  // D(11,11): block (3,3) entry (2,2)
+    m1.z = drill * k.k1.z;
+    m2.z = drill * k.k2.z;
+    m3<double> Qt = tr3(k.QQ);
+    m3<double> QtXi = mm3(Qt, k.Xi);
+    m3<double> Z1 = skew3(k.z1), Z2 = skew3(k.z2);
+    m3<double> W0 = mm3(mm3(Qt, Z1), k.Xi), W1 = mm3(Qt, k.Xi1);
+    m3<double> W2 = mm3(mm3(Qt, Z2), k.Xi), W3 = mm3(Qt, k.Xi2);
+    st3(o + L_P + 0, Qt);
+    st3(o + L_P + 9, QtXi);
+    st3(o + L_P + 18, Qt);
+    st3(o + L_P + 27, QtXi);
+    st3(o + L_W + 0, W0);
+    st3(o + L_W + 9, W1);
+    st3(o + L_W + 18, W2);
+    st3(o + L_W + 27, W3);
+    // Psi^T sigma
+    v3<double> s0 = mtv(Qt, n1), s1 = mtv(QtXi, m1), s2 = mtv(Qt, n2), s3 = mtv(QtXi, m2);
+    v3<double> s4 = add3(add3(mtv(W0, n1), mtv(W1, m1)), add3(mtv(W2, n2), mtv(W3, m2)));
+    double *S = o + L_S;
+    S[0] = s0.x; S[1] = s0.y; S[2] = s0.z; S[3] = s1.x; S[4] = s1.y; S[5] = s1.z;
+    S[6] = s2.x; S[7] = s2.y; S[8] = s2.z; S[9] = s3.x; S[10] = s3.y; S[11] = s3.z;
+    S[12] = s4.x; S[13] = s4.y; S[14] = s4.z;
+    // geometric blocks
+    v3<double> n_1 = mv(k.QQ, n1), n_2 = mv(k.QQ, n2), m_1 = mv(k.QQ, m1), m_2 = mv(k.QQ, m2);
+    m3<double> Gua1 = mscl(-1.0, mm3(skew3(n_1), k.Xi));
+    m3<double> Gua2 = mscl(-1.0, mm3(skew3(n_2), k.Xi));
+    m3<double> Vm1 = opV(k.a, m_1), Vm2 = opV(k.a, m_2);
+    m3<double> XiT = tr3(k.Xi);
+    m3<double> Ga1 = msub(mm3(XiT, mm3(mm3(Z1, skew3(n_1)), k.Xi)), opV(k.a, mv(Z1, n_1)));
+    Ga1 = madd(Ga1, msub(opdV(k.a, k.a1, m_1), mtm3(k.Xi1, mm3(skew3(m_1), k.Xi))));
+    m3<double> Ga2 = msub(mm3(XiT, mm3(mm3(Z2, skew3(n_2)), k.Xi)), opV(k.a, mv(Z2, n_2)));
+    Ga2 = madd(Ga2, msub(opdV(k.a, k.a2, m_2), mtm3(k.Xi2, mm3(skew3(m_2), k.Xi))));
+    st3(o + L_G + 0, Gua1);
+    st3(o + L_G + 9, Gua2);
+    st3(o + L_G + 18, Vm1);
+    st3(o + L_G + 27, Vm2);
+    st3(o + L_G + 36, madd(Ga1, Ga2));
+}
+
 template <int MODE>
 __global__ __launch_bounds__(SWG) void shell_assemble_kernel(
     int e0, int e1, size_t ld, const int *__restrict__ conn, const double *__restrict__ cst,
     const double *__restrict__ gps, const double *__restrict__ disp, const int *__restrict__ dof, gf_gmat gm,
     double *__restrict__ r, const double *__restrict__ lfp, double gx, double gy, double gz, double *__restrict__ Kout,
-    double *__restrict__ fout)
+    double *__restrict__ fout, const double *__restrict__ pre, const double *__restrict__ ppre)
 {
     __shared__ double sm[SEPB * 3 * L_GP];
     const int tid = threadIdx.x;
@@ -220,109 +330,15 @@ __global__ __launch_bounds__(SWG) void shell_assemble_kernel(
     // gravity ramp factor, device-resident so a captured Newton step reads the current one
     const double lf = lfp ? *lfp : 0.0;
 
-    // ---- phase 1 -------------------------------------------------------------
-    if (tid < 3 * SEPB) {
+    // ---- phase 1 (or its records from shell_prepare_kernel) ---------------------
+    if (pre) {
+        const int nv = min(SEPB, e1 - ebase);
+        const double *src = pre + (size_t)ebase * 3 * L_M;
+        for (int t = tid; t < nv * 3 * L_M; t += SWG) sm[(t / L_M) * L_GP + t % L_M] = src[t];
+    } else if (tid < 3 * SEPB) {
         int le = tid / 3, gp = tid - 3 * (tid / 3);
         int e = ebase + le;
-        double *o = sm + (le * 3 + gp) * L_GP;
-        if (e < e1) {
-            ShellKin k;
-            shell_kin(cst, gps, ld, e, gp, disp, conn, k);
-            const double mu = cst[SC_MU * ld + e], la = cst[SC_LAMBDA * ld + e], th = cst[SC_THICK * ld + e];
-            const double drill = cst[SC_DRILL * ld + e];
-            for (int i = 0; i < 144; i++) o[L_D + i] = 0.0;
-            v3<double> n1 = mk3(0.0, 0.0, 0.0), n2 = n1, m1 = n1, m2 = n1;
-            const double xi[3] = {-0.77459666924148337703585307995648, 0.0, 0.77459666924148337703585307995648};
-            const double wt[3] = {0.55555555555555555555555555555556, 0.88888888888888888888888888888889,
-                                  0.55555555555555555555555555555556};
-            const double J = th / 2.0;
-            for (int q = 0; q < 3; q++) {
-                double zeta = th * xi[q] / 2.0;
-                const ShellPt pt = shell_point(k, mu, la, zeta);
-                const double g11 = pt.g11, g12 = pt.g12, g21 = pt.g21, g22 = pt.g22, v = pt.v, dv = pt.dv;
-                const v3<double> tau1 = pt.tau1, tau2 = pt.tau2;
-                m3<double> C11, C22, C12, C21;
-#pragma unroll
-                for (int i = 0; i < 9; i++) C11.a[i] = C22.a[i] = C12.a[i] = 0.0;
-                C11.a[0] = mu * ((1.0 + g22) * (1.0 + g22) * dv + 1.0);
-                C11.a[1] = -mu * (1.0 + g22) * g21 * dv;
-                C11.a[3] = -mu * (1.0 + g22) * g21 * dv;
-                C11.a[4] = mu * (g21 * g21 * dv + 1.0);
-                C11.a[8] = mu;
-                C22.a[0] = mu * (g12 * g12 * dv + 1.0);
-                C22.a[1] = -mu * ((1.0 + g11) * g12 * dv);
-                C22.a[3] = -mu * ((1.0 + g11) * g12 * dv);
-                C22.a[4] = mu * ((1.0 + g11) * (1.0 + g11) * dv + 1.0);
-                C22.a[8] = mu;
-                C12.a[0] = -mu * ((1.0 + g22) * g12 * dv);
-                C12.a[1] = mu * (v - 1.0 + (1.0 + g11) * (1.0 + g22) * dv);
-                C12.a[3] = mu * (1.0 - v + g12 * g21 * dv);
-                C12.a[4] = -mu * ((1.0 + g11) * g21 * dv);
-                C21 = tr3(C12);
-                double w = wt[q] * J;
-                acc3(o + L_D + 9 * 0, w, C11);            // dn1/deta1
-                acc3(o + L_D + 9 * 1, -w * zeta, mE3(C11)); // dn1/dk1
-                acc3(o + L_D + 9 * 2, w, C12);            // dn1/deta2
-                acc3(o + L_D + 9 * 3, -w * zeta, mE3(C12)); // dn1/dk2
-                acc3(o + L_D + 9 * 4, w * zeta, E3m(C11));  // dm1/deta1
-                acc3(o + L_D + 9 * 5, -w * zeta * zeta, mE3(E3m(C11)));
-                acc3(o + L_D + 9 * 6, w * zeta, E3m(C12));
-                acc3(o + L_D + 9 * 7, -w * zeta * zeta, mE3(E3m(C12)));
-                acc3(o + L_D + 9 * 8, w, C21);            // dn2/deta1
-                acc3(o + L_D + 9 * 9, -w * zeta, mE3(C21));
-                acc3(o + L_D + 9 * 10, w, C22);
-                acc3(o + L_D + 9 * 11, -w * zeta, mE3(C22));
-                acc3(o + L_D + 9 * 12, w * zeta, E3m(C21)); // dm2/deta1
-                acc3(o + L_D + 9 * 13, -w * zeta * zeta, mE3(E3m(C21)));
-                acc3(o + L_D + 9 * 14, w * zeta, E3m(C22));
-                acc3(o + L_D + 9 * 15, -w * zeta * zeta, mE3(E3m(C22)));
-                n1 = add3(n1, scl3(w, tau1));
-                n2 = add3(n2, scl3(w, tau2));
-                // e3 x tau = (-tau.y, tau.x, 0)
-                m1 = add3(m1, scl3(w * zeta, mk3(-tau1.y, tau1.x, 0.0)));
-                m2 = add3(m2, scl3(w * zeta, mk3(-tau2.y, tau2.x, 0.0)));
-            }
-            // drilling (overwrites, as the reference)
-            o[L_D + 9 * 5 + 8] = drill;   // D(5,5): block (1,1) entry (2,2)
-            o[L_D + 9 * 15 + 8] = drill;  // D(11,11): block (3,3) entry (2,2)
-            m1.z = drill * k.k1.z;
-            m2.z = drill * k.k2.z;
-            m3<double> Qt = tr3(k.QQ);
-            m3<double> QtXi = mm3(Qt, k.Xi);
-            m3<double> Z1 = skew3(k.z1), Z2 = skew3(k.z2);
-            m3<double> W0 = mm3(mm3(Qt, Z1), k.Xi), W1 = mm3(Qt, k.Xi1);
-            m3<double> W2 = mm3(mm3(Qt, Z2), k.Xi), W3 = mm3(Qt, k.Xi2);
-            st3(o + L_P + 0, Qt);
-            st3(o + L_P + 9, QtXi);
-            st3(o + L_P + 18, Qt);
-            st3(o + L_P + 27, QtXi);
-            st3(o + L_W + 0, W0);
-            st3(o + L_W + 9, W1);
-            st3(o + L_W + 18, W2);
-            st3(o + L_W + 27, W3);
-            // Psi^T sigma
-            v3<double> s0 = mtv(Qt, n1), s1 = mtv(QtXi, m1), s2 = mtv(Qt, n2), s3 = mtv(QtXi, m2);
-            v3<double> s4 = add3(add3(mtv(W0, n1), mtv(W1, m1)), add3(mtv(W2, n2), mtv(W3, m2)));
-            double *S = o + L_S;
-            S[0] = s0.x; S[1] = s0.y; S[2] = s0.z; S[3] = s1.x; S[4] = s1.y; S[5] = s1.z;
-            S[6] = s2.x; S[7] = s2.y; S[8] = s2.z; S[9] = s3.x; S[10] = s3.y; S[11] = s3.z;
-            S[12] = s4.x; S[13] = s4.y; S[14] = s4.z;
-            // geometric blocks
-            v3<double> n_1 = mv(k.QQ, n1), n_2 = mv(k.QQ, n2), m_1 = mv(k.QQ, m1), m_2 = mv(k.QQ, m2);
-            m3<double> Gua1 = mscl(-1.0, mm3(skew3(n_1), k.Xi));
-            m3<double> Gua2 = mscl(-1.0, mm3(skew3(n_2), k.Xi));
-            m3<double> Vm1 = opV(k.a, m_1), Vm2 = opV(k.a, m_2);
-            m3<double> XiT = tr3(k.Xi);
-            m3<double> Ga1 = msub(mm3(XiT, mm3(mm3(Z1, skew3(n_1)), k.Xi)), opV(k.a, mv(Z1, n_1)));
-            Ga1 = madd(Ga1, msub(opdV(k.a, k.a1, m_1), mtm3(k.Xi1, mm3(skew3(m_1), k.Xi))));
-            m3<double> Ga2 = msub(mm3(XiT, mm3(mm3(Z2, skew3(n_2)), k.Xi)), opV(k.a, mv(Z2, n_2)));
-            Ga2 = madd(Ga2, msub(opdV(k.a, k.a2, m_2), mtm3(k.Xi2, mm3(skew3(m_2), k.Xi))));
-            st3(o + L_G + 0, Gua1);
-            st3(o + L_G + 9, Gua2);
-            st3(o + L_G + 18, Vm1);
-            st3(o + L_G + 27, Vm2);
-            st3(o + L_G + 36, madd(Ga1, Ga2));
-        }
+        if (e < e1) shell_gp_record(cst, gps, ld, e, gp, disp, conn, sm + (le * 3 + gp) * L_GP);
     }
     __syncthreads();
 
@@ -386,6 +402,14 @@ __global__ __launch_bounds__(SWG) void shell_assemble_kernel(
 #pragma unroll
         for (int q = 0; q < 9; q++) T3.a[q] = cst[(SC_T3 + q) * ld + e];
         K = mtm3(T3, mm3(K, T3));
+        if (ppre && I < 6 && J < 6) {
+            // follower pressure stiffness of node pair (I, J), global frame
+            const double *kp = ppre + ((size_t)e * 6 + I) * SPR + 3 + 3 * J;
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) K.a[3 * i + j] += kp[18 * i + j];
+        }
         if (MODE == 0) {
             int na, ca, nb_, cb;
             dof_block(conn, ld, e, I, na, ca);
@@ -446,6 +470,12 @@ __global__ __launch_bounds__(SWG) void shell_assemble_kernel(
             dof_block(conn, ld, e, I, node, c0);
             int R3[3];
             double V3[3] = {-f.x, -f.y, -f.z};
+            if (ppre && I < 6) {
+                // the follower pressure's unbalanced force of node I
+                const double *pa = ppre + ((size_t)e * 6 + I) * SPR;
+#pragma unroll
+                for (int i = 0; i < 3; i++) V3[i] -= pa[i];
+            }
 #pragma unroll
             for (int i = 0; i < 3; i++) R3[i] = dof[6 * node + c0 + i];
             vec_add_n<3>(r, R3, V3);
@@ -457,20 +487,28 @@ __global__ __launch_bounds__(SWG) void shell_assemble_kernel(
     }
 }
 
-// Follower pressure (MountShellSpecialLoads): lane per (element, u-node a).
-__global__ __launch_bounds__(256) void shell_pressure_kernel(
-    int e0, int e1, size_t ld, const int *__restrict__ conn, const double *__restrict__ cst,
+// Follower pressure (MountShellSpecialLoads), lane per (element, u-node a) over all
+// elements: the record out[(6 e + a) SPR ...] holds the unbalanced-force change P_a
+// (3) and the load stiffness rows of node a (3 x 18, global frame); zeros for an
+// element without pressure.  The colour launches of shell_assemble_kernel add them
+// to the element's own scatter.
+__global__ __launch_bounds__(256) void shell_pressure_record_kernel(
+    int ns, size_t ld, const int *__restrict__ conn, const double *__restrict__ cst,
     const double *__restrict__ pc, const double *__restrict__ disp, const double *__restrict__ copy,
     const double *__restrict__ copy0, const int *__restrict__ pload, const int *__restrict__ pflag,
     const double *__restrict__ lrec, const double *__restrict__ tab_t, const double *__restrict__ tab_v, const double *__restrict__ tp,
-    const int *__restrict__ dof, gf_gmat gm, double *__restrict__ r)
+    double *__restrict__ out)
 {
     const double time = *tp;  // load time, device-resident (a captured Newton step reads the current one)
     int t = blockIdx.x * blockDim.x + threadIdx.x;
-    int e = e0 + t / 6, a = t - 6 * (t / 6);
-    if (e >= e1) return;
+    int e = t / 6, a = t - 6 * (t / 6);
+    if (e >= ns) return;
+    double *o = out + (size_t)t * SPR;
     int L = pload[e];
-    if (L < 0) return;
+    if (L < 0) {
+        for (int q = 0; q < SPR; q++) o[q] = 0.0;
+        return;
+    }
     const double *rec = lrec + (size_t)L * NL_REC;
     double p = gf_table_value(tab_t, tab_v, (int)rec[NL_TAB0], (int)rec[NL_TABN], time, 0);
     int au = pflag[e];
@@ -523,27 +561,34 @@ __global__ __launch_bounds__(256) void shell_pressure_kernel(
                 for (int j = 0; j < 3; j++) Kr[i][3 * b + j] += s * B.a[3 * i + j];
         }
     }
-    int na = conn[a * ld + e];
-    int rr[3] = {dof[6 * na], dof[6 * na + 1], dof[6 * na + 2]};
-    double Pv[3] = {Pa.x, Pa.y, Pa.z};
-    // the element's unbalanced force grows by Pa; the residual is its negative sum
-    const double Pn[3] = {-Pv[0], -Pv[1], -Pv[2]};
-    vec_add_n<3>(r, rr, Pn);
-#pragma unroll
-    for (int b = 0; b < 6; b++) {
-        int nb_ = conn[b * ld + e];
-        int R9[9], C9[9];
-        double V9[9];
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++) {
-                R9[3 * i + j] = rr[i];
-                C9[3 * i + j] = dof[6 * nb_ + j];
-                V9[3 * i + j] = Kr[i][3 * b + j];
-            }
-        gm_add_n<9>(gm, R9, C9, V9);
+    // the element's unbalanced force grows by Pa (the residual takes its negative)
+    o[0] = Pa.x;
+    o[1] = Pa.y;
+    o[2] = Pa.z;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 18; j++) o[3 + 18 * i + j] = Kr[i][j];
+}
+
+// Phase 1 of every element's Gauss points in one launch: lane per (element, gp),
+// its record built in LDS and copied out with the workgroup's neighbours (64
+// consecutive records are one contiguous block).  The colour launches read them
+// instead of recomputing phase 1 on 12 lanes per workgroup.
+constexpr int REC_LANES = 64;
+__global__ __launch_bounds__(REC_LANES) void shell_record_kernel(int ns, size_t ld, const int *__restrict__ conn,
+                                                                  const double *__restrict__ cst,
+                                                                  const double *__restrict__ gps,
+                                                                  const double *__restrict__ disp, double *__restrict__ rec)
+{
+    __shared__ double srec[REC_LANES * L_M];
+    const int t0 = blockIdx.x * REC_LANES, t = t0 + threadIdx.x;
+    const int nt = min(REC_LANES, 3 * ns - t0);
+    if ((int)threadIdx.x < nt) {
+        const int e = t / 3, gp = t - 3 * (t / 3);
+        shell_gp_record(cst, gps, ld, e, gp, disp, conn, srec + threadIdx.x * L_M);
     }
+    __syncthreads();
+    double *dst = rec + (size_t)t0 * L_M;
+    for (int q = threadIdx.x; q < nt * L_M; q += REC_LANES) dst[q] = srec[q];
 }
 
 __global__ __launch_bounds__(256) void shell_commit_kernel(int ns, size_t ld, const int *__restrict__ conn,
@@ -723,11 +768,22 @@ void gf_shell_host_const(const double *x, const double *prop, double *rec, doubl
 }
 
 // ---- launchers --------------------------------------------------------------
-int gf_shell_assemble(gf_model *m, const double *lfp)
+int gf_shell_assemble(gf_model *m, const double *lfp, const double *tp)
 {
     const size_t ld = (size_t)m->ns;
     int ncol = (int)m->shell_color_off.size() - 1;
     int launches = 0;
+    if (m->ns == 0) return 0;
+    // phase 1 of every element and the pressure terms, then one scatter launch per colour
+    hipLaunchKernelGGL(shell_record_kernel, dim3((3 * m->ns + REC_LANES - 1) / REC_LANES), dim3(REC_LANES), 0,
+                       m->stream, m->ns, ld, m->d_sconn, m->d_sconst, m->d_sgp, m->d_disp, m->d_shell_rec);
+    launches++;
+    if (m->any_pressure) {
+        hipLaunchKernelGGL(shell_pressure_record_kernel, dim3((6 * m->ns + 255) / 256), dim3(256), 0, m->stream,
+                           m->ns, ld, m->d_sconn, m->d_sconst, m->d_spconst, m->d_disp, m->d_copy, m->d_copy0,
+                           m->d_spload, m->d_spflag, m->d_lrec, m->d_tab_t, m->d_tab_v, tp, m->d_shell_prec);
+        launches++;
+    }
     for (int c = 0; c < ncol; c++) {
         int e0 = m->shell_color_off[c], e1 = m->shell_color_off[c + 1];
         if (e1 <= e0) continue;
@@ -736,7 +792,8 @@ int gf_shell_assemble(gf_model *m, const double *lfp)
         const int pr = gf_probe_begin(m->stream, GF_KC_SHELL_ASSEMBLE);
         hipLaunchKernelGGL(shell_assemble_kernel<0>, dim3(grid), dim3(SWG), 0, m->stream, e0, e1, ld, m->d_sconn,
                            m->d_sconst, m->d_sgp, m->d_disp, m->d_dof, gf_model_gmat(m), m->d_r, lfp, m->gravity[0],
-                           m->gravity[1], m->gravity[2], (double *)nullptr, (double *)nullptr);
+                           m->gravity[1], m->gravity[2], (double *)nullptr, (double *)nullptr,
+                           (const double *)m->d_shell_rec, (const double *)(m->any_pressure ? m->d_shell_prec : nullptr));
         gf_probe_end(m->stream, pr, e1 - e0);
     }
     GF_HIP(hipGetLastError());
@@ -757,24 +814,6 @@ int gf_shell_results(gf_model *m, double *d_out)
     if (m->ns == 0) return GF_OK;
     hipLaunchKernelGGL(shell_results_kernel, dim3((m->ns + 255) / 256), dim3(256), 0, m->stream, m->ns, (size_t)m->ns,
                        m->d_sconst, m->d_sgp, d_out);
-    GF_HIP(hipGetLastError());
-    return GF_OK;
-}
-
-int gf_shell_pressure(gf_model *m, const double *tp)
-{
-    if (!m->any_pressure) return GF_OK;
-    const size_t ld = (size_t)m->ns;
-    int ncol = (int)m->shell_color_off.size() - 1;
-    for (int c = 0; c < ncol; c++) {
-        int e0 = m->shell_color_off[c], e1 = m->shell_color_off[c + 1];
-        if (e1 <= e0) continue;
-        int threads = 6 * (e1 - e0);
-        hipLaunchKernelGGL(shell_pressure_kernel, dim3((threads + 255) / 256), dim3(256), 0, m->stream, e0, e1, ld,
-                           m->d_sconn, m->d_sconst, m->d_spconst, m->d_disp, m->d_copy, m->d_copy0, m->d_spload,
-                           m->d_spflag, m->d_lrec, m->d_tab_t, m->d_tab_v, tp, m->d_dof, gf_model_gmat(m),
-                           m->d_r);
-    }
     GF_HIP(hipGetLastError());
     return GF_OK;
 }
@@ -811,7 +850,8 @@ int gf_shell_eval_device(int count, const double *x, const double *prop, const d
     gf_gmat none = {};
     hipLaunchKernelGGL(shell_assemble_kernel<1>, dim3((count + SEPB - 1) / SEPB), dim3(SWG), 0, s, 0, count, ld,
                        d_conn, d_cst, d_gps, d_disp, (const int *)nullptr, none, (double *)nullptr,
-                       (const double *)nullptr, 0.0, 0.0, 0.0, d_K, d_f);
+                       (const double *)nullptr, 0.0, 0.0, 0.0, d_K, d_f, (const double *)nullptr,
+                       (const double *)nullptr);
     GF_HIP(hipGetLastError());
     GF_HIP(hipMemcpyAsync(K_out, d_K, sizeof(double) * 729 * ld, hipMemcpyDeviceToHost, s));
     GF_HIP(hipMemcpyAsync(f_out, d_f, sizeof(double) * 27 * ld, hipMemcpyDeviceToHost, s));
