@@ -180,7 +180,7 @@ PROBES_FRONTS = ("front_step", "front_assemble", "front_solve", "beam_assemble",
 KERNEL_NAMES = {"btd_level": "btd_level_kernel", "btd_tail": "btd_tail_kernel",
                 "band_panel": "band_batch_inv_kernel", "band_update": "band_step_kernel",
                 "band_sweep": "band_batch_apply_kernel + band_couple_kernel",
-                "beam_assemble": "beam_assemble_kernel<0>", "shell_assemble": "shell_assemble_kernel<0>",
+                "beam_assemble": "beam_assemble_kernel<0>", "shell_assemble": "shell_element_kernel",
                 "front_step": "front_step_kernel", "front_assemble": "front_assemble_kernel",
                 "front_solve": "front_fwd_own/fwd_upd/bwd_upd/bwd_own_kernel"}
 

@@ -423,6 +423,7 @@ struct gf_model {
     float *d_front32 = nullptr, *d_front32_e = nullptr;                      // FP32 factor (mixed)
     double *d_shell_rec = nullptr;   // per (Shell_1, Gauss point) phase-1 record [3 ns][276]
     double *d_shell_prec = nullptr;  // per (Shell_1, u-node) follower-pressure terms [6 ns][57]
+    double *d_shell_K = nullptr, *d_shell_f = nullptr;  // per Shell_1 K [729] and f [27] of the current iteration
     // structure of the assembled tangent (the band entries of element node pairs,
     // row by row): what the assembly clears and the mixed residual multiplies
     std::vector<int> csr_ptr, csr_col;

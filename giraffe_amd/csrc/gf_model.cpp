@@ -168,7 +168,7 @@ static void release(gf_model *m)
                     m->d_scal, m->d_copy0, m->d_bgp0, m->d_sconn, m->d_sconst, m->d_spconst, m->d_sgp,
                     m->d_sgp0, m->d_spload, m->d_spflag, m->d_band, m->d_swork, m->d_band_inv, m->d_bbuoy,
                     m->d_front, m->d_front_e, m->d_front_v, m->d_csr_ptr, m->d_csr_col, m->d_shell_rec,
-                    m->d_shell_prec};
+                    m->d_shell_prec, m->d_shell_K, m->d_shell_f};
     for (void *b : bufs) gf_pool_free(b);
     gf_front_free(m->fronts);
     if (m->h_scal) gf_pinned_free(m->h_scal);
@@ -461,7 +461,7 @@ static size_t footprint(const gf_model_desc *d, const gf_model *m, int precision
     b += nn * (6 + 6 + 6 + 3) * 8 + nn * 6 * 4 + nf * 4;                       // node state, DOF map
     b += nb * (3 * 4 + BC_COUNT * 8 + 2 * BG_COUNT * 8 + 8);                  // Beam_1 / Pipe_1 records
     b += ns * (6 * 4 + SC_COUNT * 8 + SP_COUNT * 8 + 2 * SG_COUNT * 8 + 8);   // Shell_1 records
-    b += ns * (3 * 276 + 6 * 57) * 8;                                          // Shell_1 assembly records
+    b += ns * (3 * 276 + 6 * 57 + 729 + 27) * 8;                               // Shell_1 assembly records
     size_t items = d->n_loads ? (size_t)d->load_set_off[d->n_loads] : 0;
     size_t rows = d->n_loads ? (size_t)d->load_tab_off[d->n_loads] : 0;
     b += (size_t)d->n_loads * NL_REC * 8 + items * 12 + rows * 7 * 8;         // loads and tables
@@ -638,6 +638,8 @@ static int create(const gf_model_desc *d, const gf_options *opt, gf_model *m)
         // phase-1 records and pressure terms of the assembly (gf_shell_assemble)
         GF_TRY(dalloc(&m->d_shell_rec, 3 * (size_t)ns * 276));
         if (m->any_pressure) GF_TRY(dalloc(&m->d_shell_prec, 6 * (size_t)ns * 57));
+        GF_TRY(dalloc(&m->d_shell_K, 729 * (size_t)ns));
+        GF_TRY(dalloc(&m->d_shell_f, 27 * (size_t)ns));
     }
     // nodal loads grouped by node; multipliers as UpdateforSolutionStep
     {

@@ -16,14 +16,15 @@
 //
 // Workgroup: 256 lanes, SEPB = 4 elements; LDS holds per Gauss point the
 // 16 blocks of D, Psi, G, M and Psi^T sigma (48 KB).  In the Newton loop phase 1
-// runs for all elements in one launch beforehand (shell_record_kernel) and the
-// follower pressure's terms likewise (shell_pressure_record_kernel); the colour
-// launches copy the records and run phases 2-4, the pressure terms added to the
-// element's own K_IJ and force.
+// runs for all elements in one launch with every lane busy (shell_record_kernel),
+// the follower pressure's terms likewise (shell_pressure_record_kernel), then one
+// launch of phases 2-4 for all elements writes each element's K and f to HBM,
+// and a light scatter launch per colour (shell_scatter_kernel) adds them, with the
+// pressure terms and the self-weight, into the tangent and residual.
 //   phase 1  lane per (element, gp): kinematics, 3-point thickness integration
 //   phase 2  lane per (element, gp, block RS): the 25 blocks of M
-//   phase 3  lane per (element, I, J): K_IJ, rotated to global, scattered
-//   phase 4  lane per (element, I): internal force - self-weight into r
+//   phase 3  lane per (element, I, J): K_IJ, rotated to global
+//   phase 4  lane per (element, I): internal force
 #include "gf_internal.h"
 #include "gf_device.h"
 
@@ -317,20 +318,16 @@ __device__ __forceinline__ void shell_gp_record(const double *__restrict__ cst, 
     st3(o + L_G + 36, madd(Ga1, Ga2));
 }
 
-template <int MODE>
-__global__ __launch_bounds__(SWG) void shell_assemble_kernel(
+__global__ __launch_bounds__(SWG) void shell_element_kernel(
     int e0, int e1, size_t ld, const int *__restrict__ conn, const double *__restrict__ cst,
-    const double *__restrict__ gps, const double *__restrict__ disp, const int *__restrict__ dof, gf_gmat gm,
-    double *__restrict__ r, const double *__restrict__ lfp, double gx, double gy, double gz, double *__restrict__ Kout,
-    double *__restrict__ fout, const double *__restrict__ pre, const double *__restrict__ ppre)
+    const double *__restrict__ gps, const double *__restrict__ disp, double *__restrict__ Kout,
+    double *__restrict__ fout, const double *__restrict__ pre)
 {
     __shared__ double sm[SEPB * 3 * L_GP];
     const int tid = threadIdx.x;
     const int ebase = e0 + blockIdx.x * SEPB;
-    // gravity ramp factor, device-resident so a captured Newton step reads the current one
-    const double lf = lfp ? *lfp : 0.0;
 
-    // ---- phase 1 (or its records from shell_prepare_kernel) ---------------------
+    // ---- phase 1 (or its records from shell_record_kernel) ----------------------
     if (pre) {
         const int nv = min(SEPB, e1 - ebase);
         const double *src = pre + (size_t)ebase * 3 * L_M;
@@ -402,39 +399,14 @@ __global__ __launch_bounds__(SWG) void shell_assemble_kernel(
 #pragma unroll
         for (int q = 0; q < 9; q++) T3.a[q] = cst[(SC_T3 + q) * ld + e];
         K = mtm3(T3, mm3(K, T3));
-        if (ppre && I < 6 && J < 6) {
-            // follower pressure stiffness of node pair (I, J), global frame
-            const double *kp = ppre + ((size_t)e * 6 + I) * SPR + 3 + 3 * J;
+        double *Ke = Kout + (size_t)e * 729;
 #pragma unroll
-            for (int i = 0; i < 3; i++)
+        for (int i = 0; i < 3; i++)
 #pragma unroll
-                for (int j = 0; j < 3; j++) K.a[3 * i + j] += kp[18 * i + j];
-        }
-        if (MODE == 0) {
-            int na, ca, nb_, cb;
-            dof_block(conn, ld, e, I, na, ca);
-            dof_block(conn, ld, e, J, nb_, cb);
-            int R9[9], C9[9];
-            double V9[9];
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int j = 0; j < 3; j++) {
-                    R9[3 * i + j] = dof[6 * na + ca + i];
-                    C9[3 * i + j] = dof[6 * nb_ + cb + j];
-                    V9[3 * i + j] = K.a[3 * i + j];
-                }
-            gm_add_n<9>(gm, R9, C9, V9);
-        } else {
-            double *Ke = Kout + (size_t)e * 729;
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int j = 0; j < 3; j++) Ke[(3 * I + i) * 27 + 3 * J + j] = K.a[3 * i + j];
-        }
+            for (int j = 0; j < 3; j++) Ke[(3 * I + i) * 27 + 3 * J + j] = K.a[3 * i + j];
     }
 
-    // ---- phase 4: internal force - self-weight -------------------------------
+    // ---- phase 4: internal force ---------------------------------------------
     for (int t = tid; t < SEPB * 9; t += SWG) {
         int le = t / 9, I = t - 9 * le;
         int e = ebase + le;
@@ -455,42 +427,16 @@ __global__ __launch_bounds__(SWG) void shell_assemble_kernel(
 #pragma unroll
         for (int q = 0; q < 9; q++) T3.a[q] = cst[(SC_T3 + q) * ld + e];
         f = mtv(T3, f);
-        if (MODE == 0) {
-            if (I < 6 && lf != 0.0) {
-                // self-weight, accumulated twice as src/Shell_1.cpp:1340-1375 does
-                double mult = lf * cst[SC_RHO * ld + e] * cst[SC_THICK * ld + e];
-                double gw = cst[(SC_GW + I) * ld + e];
-                for (int rep = 0; rep < 2; rep++) {
-                    f.x -= gw * mult * gx;
-                    f.y -= gw * mult * gy;
-                    f.z -= gw * mult * gz;
-                }
-            }
-            int node, c0;
-            dof_block(conn, ld, e, I, node, c0);
-            int R3[3];
-            double V3[3] = {-f.x, -f.y, -f.z};
-            if (ppre && I < 6) {
-                // the follower pressure's unbalanced force of node I
-                const double *pa = ppre + ((size_t)e * 6 + I) * SPR;
-#pragma unroll
-                for (int i = 0; i < 3; i++) V3[i] -= pa[i];
-            }
-#pragma unroll
-            for (int i = 0; i < 3; i++) R3[i] = dof[6 * node + c0 + i];
-            vec_add_n<3>(r, R3, V3);
-        } else {
-            fout[(size_t)e * 27 + 3 * I + 0] = f.x;
-            fout[(size_t)e * 27 + 3 * I + 1] = f.y;
-            fout[(size_t)e * 27 + 3 * I + 2] = f.z;
-        }
+        fout[(size_t)e * 27 + 3 * I + 0] = f.x;
+        fout[(size_t)e * 27 + 3 * I + 1] = f.y;
+        fout[(size_t)e * 27 + 3 * I + 2] = f.z;
     }
 }
 
 // Follower pressure (MountShellSpecialLoads), lane per (element, u-node a) over all
 // elements: the record out[(6 e + a) SPR ...] holds the unbalanced-force change P_a
 // (3) and the load stiffness rows of node a (3 x 18, global frame); zeros for an
-// element without pressure.  The colour launches of shell_assemble_kernel add them
+// element without pressure.  The colour launches of shell_scatter_kernel add them
 // to the element's own scatter.
 __global__ __launch_bounds__(256) void shell_pressure_record_kernel(
     int ns, size_t ld, const int *__restrict__ conn, const double *__restrict__ cst,
@@ -767,6 +713,80 @@ void gf_shell_host_const(const double *x, const double *prop, double *rec, doubl
     }
 }
 
+// Scatter of one colour's element matrices and forces (written for all elements by
+// shell_element_kernel): lane per (element, I, J) adding K_IJ, plus the follower
+// pressure stiffness of u-node pair (I, J), into the tangent; lane per (element, I)
+// adding -(f_I - self-weight + P_I) into r.  Elements of a colour share no node, and
+// an element's lanes touch distinct entries, so no two lanes write the same place.
+__global__ __launch_bounds__(256) void shell_scatter_kernel(int e0, int e1, size_t ld, const int *__restrict__ conn,
+                                                            const double *__restrict__ cst, const int *__restrict__ dof,
+                                                            gf_gmat gm, double *__restrict__ r,
+                                                            const double *__restrict__ Ke, const double *__restrict__ fe,
+                                                            const double *__restrict__ ppre, const double *__restrict__ lfp,
+                                                            double gx, double gy, double gz)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ne = e1 - e0;
+    if (t < 81 * ne) {
+        const int e = e0 + t / 81, ij = t - 81 * (t / 81);
+        const int I = ij / 9, J = ij - 9 * (ij / 9);
+        const double *k = Ke + (size_t)e * 729 + (3 * I) * 27 + 3 * J;
+        double V9[9];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) V9[3 * i + j] = k[27 * i + j];
+        if (ppre && I < 6 && J < 6) {
+            const double *kp = ppre + ((size_t)e * 6 + I) * SPR + 3 + 3 * J;
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) V9[3 * i + j] += kp[18 * i + j];
+        }
+        int na, ca, nb_, cb;
+        dof_block(conn, ld, e, I, na, ca);
+        dof_block(conn, ld, e, J, nb_, cb);
+        int R9[9], C9[9];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                R9[3 * i + j] = dof[6 * na + ca + i];
+                C9[3 * i + j] = dof[6 * nb_ + cb + j];
+            }
+        gm_add_n<9>(gm, R9, C9, V9);
+        return;
+    }
+    const int u = t - 81 * ne;
+    if (u >= 9 * ne) return;
+    const int e = e0 + u / 9, I = u - 9 * (u / 9);
+    const double lf = lfp ? *lfp : 0.0;
+    const double *fp = fe + (size_t)e * 27 + 3 * I;
+    v3<double> f = mk3(fp[0], fp[1], fp[2]);
+    if (I < 6 && lf != 0.0) {
+        // self-weight, accumulated twice as src/Shell_1.cpp:1340-1375 does
+        double mult = lf * cst[SC_RHO * ld + e] * cst[SC_THICK * ld + e];
+        double gw = cst[(SC_GW + I) * ld + e];
+        for (int rep = 0; rep < 2; rep++) {
+            f.x -= gw * mult * gx;
+            f.y -= gw * mult * gy;
+            f.z -= gw * mult * gz;
+        }
+    }
+    int node, c0;
+    dof_block(conn, ld, e, I, node, c0);
+    int R3[3];
+    double V3[3] = {-f.x, -f.y, -f.z};
+    if (ppre && I < 6) {
+        const double *pa = ppre + ((size_t)e * 6 + I) * SPR;
+#pragma unroll
+        for (int i = 0; i < 3; i++) V3[i] -= pa[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) R3[i] = dof[6 * node + c0 + i];
+    vec_add_n<3>(r, R3, V3);
+}
+
 // ---- launchers --------------------------------------------------------------
 int gf_shell_assemble(gf_model *m, const double *lfp, const double *tp)
 {
@@ -784,17 +804,24 @@ int gf_shell_assemble(gf_model *m, const double *lfp, const double *tp)
                            m->d_spload, m->d_spflag, m->d_lrec, m->d_tab_t, m->d_tab_v, tp, m->d_shell_prec);
         launches++;
     }
+    // every element's K and f (phases 2-4 from the records), then one scatter launch per colour
+    {
+        const int pr = gf_probe_begin(m->stream, GF_KC_SHELL_ASSEMBLE);
+        hipLaunchKernelGGL(shell_element_kernel, dim3((m->ns + SEPB - 1) / SEPB), dim3(SWG), 0, m->stream, 0,
+                           m->ns, ld, m->d_sconn, m->d_sconst, m->d_sgp, m->d_disp, m->d_shell_K, m->d_shell_f,
+                           (const double *)m->d_shell_rec);
+        gf_probe_end(m->stream, pr, m->ns);
+        launches++;
+    }
+    const double *ppre = m->any_pressure ? m->d_shell_prec : nullptr;
     for (int c = 0; c < ncol; c++) {
         int e0 = m->shell_color_off[c], e1 = m->shell_color_off[c + 1];
         if (e1 <= e0) continue;
         launches++;
-        int grid = (e1 - e0 + SEPB - 1) / SEPB;
-        const int pr = gf_probe_begin(m->stream, GF_KC_SHELL_ASSEMBLE);
-        hipLaunchKernelGGL(shell_assemble_kernel<0>, dim3(grid), dim3(SWG), 0, m->stream, e0, e1, ld, m->d_sconn,
-                           m->d_sconst, m->d_sgp, m->d_disp, m->d_dof, gf_model_gmat(m), m->d_r, lfp, m->gravity[0],
-                           m->gravity[1], m->gravity[2], (double *)nullptr, (double *)nullptr,
-                           (const double *)m->d_shell_rec, (const double *)(m->any_pressure ? m->d_shell_prec : nullptr));
-        gf_probe_end(m->stream, pr, e1 - e0);
+        const int threads = 90 * (e1 - e0);
+        hipLaunchKernelGGL(shell_scatter_kernel, dim3((threads + 255) / 256), dim3(256), 0, m->stream, e0, e1, ld,
+                           m->d_sconn, m->d_sconst, m->d_dof, gf_model_gmat(m), m->d_r, (const double *)m->d_shell_K,
+                           (const double *)m->d_shell_f, ppre, lfp, m->gravity[0], m->gravity[1], m->gravity[2]);
     }
     GF_HIP(hipGetLastError());
     return launches;
@@ -847,11 +874,8 @@ int gf_shell_eval_device(int count, const double *x, const double *prop, const d
     GF_HIP(hipMemcpyAsync(d_cst, cst.data(), sizeof(double) * cst.size(), hipMemcpyHostToDevice, s));
     GF_HIP(hipMemcpyAsync(d_gps, gps.data(), sizeof(double) * gps.size(), hipMemcpyHostToDevice, s));
     GF_HIP(hipMemcpyAsync(d_disp, disp.data(), sizeof(double) * disp.size(), hipMemcpyHostToDevice, s));
-    gf_gmat none = {};
-    hipLaunchKernelGGL(shell_assemble_kernel<1>, dim3((count + SEPB - 1) / SEPB), dim3(SWG), 0, s, 0, count, ld,
-                       d_conn, d_cst, d_gps, d_disp, (const int *)nullptr, none, (double *)nullptr,
-                       (const double *)nullptr, 0.0, 0.0, 0.0, d_K, d_f, (const double *)nullptr,
-                       (const double *)nullptr);
+    hipLaunchKernelGGL(shell_element_kernel, dim3((count + SEPB - 1) / SEPB), dim3(SWG), 0, s, 0, count, ld,
+                       d_conn, d_cst, d_gps, d_disp, d_K, d_f, (const double *)nullptr);
     GF_HIP(hipGetLastError());
     GF_HIP(hipMemcpyAsync(K_out, d_K, sizeof(double) * 729 * ld, hipMemcpyDeviceToHost, s));
     GF_HIP(hipMemcpyAsync(f_out, d_f, sizeof(double) * 27 * ld, hipMemcpyDeviceToHost, s));

@@ -275,7 +275,7 @@ int gf_timing_read(gf_model *m, double *ms_out, long long *launches_out, int res
                                    solved (forward or backward; counted on the apply
                                    launches)                                        */
 #define GF_KC_BEAM_ASSEMBLE  5  /* beam_assemble_kernel; unit: element              */
-#define GF_KC_SHELL_ASSEMBLE 6  /* shell_assemble_kernel; unit: element             */
+#define GF_KC_SHELL_ASSEMBLE 6  /* shell_element_kernel; unit: element             */
 #define GF_KC_FRONT_STEP     7  /* front_step_kernel; unit: algorithmic flop of the partial
                                    factorisations, sum over eliminated columns c of
                                    2 (m - c - 1)^2 for a front of order m            */
