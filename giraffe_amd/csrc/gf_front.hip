@@ -55,9 +55,13 @@ template <typename T> struct AsmSmem {
     T aux[4 + NB];
 };
 
+// 512 threads (eight wavefronts) per tile: the inverse's 41 KB of LDS then still
+// leaves 24 wavefronts per CU to overlap the gathers.
+constexpr int ASM_T = 512, ASM_R = NB * NB / ASM_T;
+
 template <typename T>
-__global__ __launch_bounds__(256) void front_assemble_kernel(FrontTab t, int first, const double *ab, int ld, int hb,
-                                                             const double *D, T *Fb, T *Eb, int *flag)
+__global__ __launch_bounds__(ASM_T) void front_assemble_kernel(FrontTab t, int first, const double *ab, int ld, int hb,
+                                                               const double *D, T *Fb, T *Eb, int *flag)
 {
     __shared__ AsmSmem<T> sm;
     const int *it = t.items + 4 * (size_t)(first + blockIdx.x);
@@ -65,12 +69,13 @@ __global__ __launch_bounds__(256) void front_assemble_kernel(FrontTab t, int fir
     const gf_front_dev F = t.fr[f];
     const int r0 = NB * it[1], c0 = NB * it[2];
     const int tid = threadIdx.x, j = tid & 63, i0 = tid >> 6;
+    constexpr int RS = ASM_T / NB;  // row stride of a thread's rows
     const int cj = c0 + j;
     const int *rows = t.rows + F.rows;
     const int oj = rows[cj];
-    T v[16];
+    T v[ASM_R];
 #pragma unroll
-    for (int r = 0; r < 16; r++) v[r] = T(0);
+    for (int r = 0; r < ASM_R; r++) v[r] = T(0);
     for (int c = F.ch0; c < F.ch1; c++) {
         const gf_front_dev C = t.fr[t.child[c]];
         const int *im = t.inv + t.child_inv[c];
@@ -78,15 +83,15 @@ __global__ __launch_bounds__(256) void front_assemble_kernel(FrontTab t, int fir
         if (b < 0) continue;
         const T *U = Fb + C.F + (size_t)(NB * C.kb) * C.mp + NB * C.kb + b;
 #pragma unroll
-        for (int r = 0; r < 16; r++) {
-            const int a = im[r0 + i0 + 4 * r];
+        for (int r = 0; r < ASM_R; r++) {
+            const int a = im[r0 + i0 + RS * r];
             if (a >= 0) v[r] += U[(size_t)a * C.mp];
         }
     }
     T *out = Fb + F.F + cj;
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-        const int ri = r0 + i0 + 4 * r;
+    for (int r = 0; r < ASM_R; r++) {
+        const int ri = r0 + i0 + RS * r;
         const int oi = rows[ri];
         const bool own = ri < F.k || cj < F.k;
         if (ri == cj && ri >= F.k && ri < NB * F.kb) {
@@ -101,11 +106,11 @@ __global__ __launch_bounds__(256) void front_assemble_kernel(FrontTab t, int fir
     if (it[3] == 0) return;
     // the front's first diagonal block: E_0
 #pragma unroll
-    for (int r = 0; r < 16; r++) sm.P[(i0 + 4 * r) * LDT + j] = v[r];
+    for (int r = 0; r < ASM_R; r++) sm.P[(i0 + RS * r) * LDT + j] = v[r];
     __syncthreads();
     inverse64<T>(sm.P, sm.tmp, sm.aux, flag);
     T *E = Eb + F.E;
-    for (int q = threadIdx.x; q < NB * NB; q += 256) E[q] = sm.P[(q / NB) * LDT + q % NB];
+    for (int q = threadIdx.x; q < NB * NB; q += ASM_T) E[q] = sm.P[(q / NB) * LDT + q % NB];
 }
 
 // Launch p >= 0 of a level's factorisation (the band step on every front of the
@@ -382,7 +387,7 @@ int front_factor_t(const gf_fronts &fp, const double *ab, int ld, int hb, const 
     int launches = 0;
     for (int l = 0; l < fp.n_level; l++) {
         int pr = gf_probe_begin(s, GF_KC_FRONT_ASSEMBLE);
-        hipLaunchKernelGGL(front_assemble_kernel<T>, dim3(fp.asm_count[l]), dim3(256), 0, s, t, fp.asm_first[l], ab, ld,
+        hipLaunchKernelGGL(front_assemble_kernel<T>, dim3(fp.asm_count[l]), dim3(ASM_T), 0, s, t, fp.asm_first[l], ab, ld,
                            hb, D, F, E, flag);
         gf_probe_end(s, pr, (double)fp.asm_count[l] * NB * NB);
         launches++;

@@ -248,13 +248,15 @@ __device__ __forceinline__ typename mfma<T>::acc_t mm16(const T *P, int lp, cons
 // *flag); a non-finite pivot or an all-zero block sets bit 1.  (An FP32 factor
 // with perturbed pivots is still refined against the FP64 tangent, which
 // decides whether it was good enough.)  tmp: 3 x 16 x TMP_LD entries; aux:
-// 4 + 64 entries.
+// 4 + 64 entries.  Runs on the first four wavefronts; those of a larger
+// workgroup only take part in the barriers.
 template <typename T> __device__ __forceinline__ void inverse64(T *M, T *tmp, T *aux, int *flag)
 {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const bool act = tid < 256;
     const PivotGuard<T> pg;
     // largest magnitude in the block, and the original diagonal
-    {
+    if (act) {
         const int r = tid >> 2, c0 = 16 * (tid & 3);
         T mx = T(0);
 #pragma unroll
@@ -277,7 +279,7 @@ template <typename T> __device__ __forceinline__ void inverse64(T *M, T *tmp, T 
                 eps[k] = pg.rel * (dk > T(0) ? dk : mx);
             }
             inv16<T>(P, eps, pg.huge, bad, pert);
-        } else {
+        } else if (act) {
             const int ri = wave - 1, i = ri < kb ? ri : ri + 1;
             const T *src = M + 16 * i * LDT + 16 * kb;
             for (int t = lane; t < 256; t += 64) tmp[ri * 16 * TMP_LD + (t >> 4) * TMP_LD + (t & 15)] = src[(t >> 4) * LDT + (t & 15)];
@@ -291,7 +293,7 @@ template <typename T> __device__ __forceinline__ void inverse64(T *M, T *tmp, T 
         }
         __syncthreads();
 #pragma unroll
-        for (int s = 0; s < 3; s++) {
+        for (int s = 0; s < 3 && act; s++) {
             const int u = wave + 4 * s, ri = u >> 2, cj = u & 3;
             const int i = ri < kb ? ri : ri + 1;
             T *Mij = M + 16 * i * LDT + 16 * cj;
