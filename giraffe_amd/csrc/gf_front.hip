@@ -47,6 +47,20 @@ struct FrontTab {
     const long long *child_inv;
 };
 
+// Update vector entry of front position i gathered from the children.
+__device__ __forceinline__ double child_sum(const FrontTab &t, const gf_front_dev &F, int i, const double *Vb)
+{
+    double w = 0.0;
+    for (int c = F.ch0; c < F.ch1; c++) {
+        const int a = t.inv[t.child_inv[c] + i];
+        if (a >= 0) {
+            const gf_front_dev C = t.fr[t.child[c]];
+            w += Vb[C.V + NB * C.kb + a];
+        }
+    }
+    return w;
+}
+
 // LDS of the assembly kernel: the first diagonal block of a front and the
 // inverse's scratch.
 template <typename T> struct AsmSmem {
@@ -61,7 +75,8 @@ constexpr int ASM_T = 512, ASM_R = NB * NB / ASM_T;
 
 template <typename T>
 __global__ __launch_bounds__(ASM_T) void front_assemble_kernel(FrontTab t, int first, const double *ab, int ld, int hb,
-                                                               const double *D, T *Fb, T *Eb, int *flag)
+                                                               const double *D, T *Fb, T *Eb, int *flag,
+                                                               const double *rhs, double *Vb)
 {
     __shared__ AsmSmem<T> sm;
     const int *it = t.items + 4 * (size_t)(first + blockIdx.x);
@@ -103,6 +118,12 @@ __global__ __launch_bounds__(ASM_T) void front_assemble_kernel(FrontTab t, int f
         }
         out[(size_t)ri * F.mp] = v[r];
     }
+    if (rhs && it[2] == 0 && tid < NB) {
+        // the right-hand side rides along the factorisation: the front vector starts
+        // as the own rows of b plus the children's update vectors
+        const int i = r0 + tid, o = rows[i];
+        Vb[F.V + i] = (i < F.k && o >= 0 ? rhs[o] : 0.0) + child_sum(t, F, i, Vb);
+    }
     if (it[3] == 0) return;
     // the front's first diagonal block: E_0
 #pragma unroll
@@ -116,9 +137,11 @@ __global__ __launch_bounds__(ASM_T) void front_assemble_kernel(FrontTab t, int f
 // Launch p >= 0 of a level's factorisation (the band step on every front of the
 // level; see the file header).  Item kinds: 0 diagonal, 1 update, 2 conversion.
 template <typename T>
-__global__ __launch_bounds__(256) void front_step_kernel(FrontTab t, int first, int p, T *Fb, T *Eb, int *flag)
+__global__ __launch_bounds__(256) void front_step_kernel(FrontTab t, int first, int p, T *Fb, T *Eb, int *flag,
+                                                         double *Vb)
 {
     __shared__ StepSmem<T> sm;
+    __shared__ double rv[2][NB];  // the right-hand side block of row p (or p - 1) and E times it
     const int *it = t.items + 4 * (size_t)(first + blockIdx.x);
     const int f = it[0];
     if (f < 0) return;
@@ -132,14 +155,26 @@ __global__ __launch_bounds__(256) void front_step_kernel(FrontTab t, int first, 
     const int R0 = ti * NB, C0 = tj * NB;
     Acc<T> acc;
     acc.zero();
+    // with the right-hand side in the front vector (Vb): the first conversion tile of
+    // row p - 1 converts its block, y = E_{p-1} b_{p-1}; the first update tile of row
+    // i subtracts L(i, p) y_p, y_p = E_p b_p, from block i (each block written by one
+    // workgroup; b_p itself is final and unconverted during launch p)
+    const bool rhs_row = Vb && tj == (kind == 2 ? p : p + 1);
+    double *vrow = Vb ? Vb + F.V + (size_t)R0 : nullptr;
     if (kind == 2) {
         T e[16], a[16];
         const T *E = Ef + (size_t)ti * NB * NB;
         tile_load(e, [&](int i, int k) { return E[i * NB + k]; });
         tile_load(a, [&](int k, int j) { return A[(R0 + k) * ld + C0 + j]; });
+        if (rhs_row && tid < NB) rv[0][tid] = vrow[tid];
         tile_put(sm.P, LDT, e);
         tile_put(sm.Q, LDQ, a);
         __syncthreads();
+        if (rhs_row && tid < NB) {
+            double y = 0.0;
+            for (int c = 0; c < NB; c++) y += (double)sm.P[tid * LDT + c] * rv[0][c];
+            vrow[tid] = y;
+        }
         lds_mm<T>(acc, sm.P, LDT, sm.Q, LDQ);
         acc_each<T>([&](int x, int y, int v, int i, int j) { A[(R0 + i) * ld + C0 + j] = acc.a[x][y][v]; });
         return;
@@ -151,11 +186,17 @@ __global__ __launch_bounds__(256) void front_step_kernel(FrontTab t, int first, 
         const T *E = Ef + (size_t)p * NB * NB;
         tile_load(e, [&](int i, int k) { return E[i * NB + k]; });
         tile_load(a, [&](int k, int j) { return A[(K0 + k) * ld + C0 + j]; });
+        if (rhs_row && tid < NB) rv[0][tid] = Vb[F.V + (size_t)K0 + tid];
         tile_put(sm.P, LDT, e);
         tile_put(sm.Q, LDQ, a);
         __syncthreads();
         tile_load(l, [&](int i, int k) { return A[(R0 + i) * ld + K0 + k]; });
         acc_each<T>([&](int x, int y, int v, int i, int j) { old[x][y][v] = A[(R0 + i) * ld + C0 + j]; });
+        if (rhs_row && tid < NB) {
+            double y = 0.0;
+            for (int c = 0; c < NB; c++) y += (double)sm.P[tid * LDT + c] * rv[0][c];
+            rv[1][tid] = y;  // y_p
+        }
         Acc<T> w;
         w.zero();
         lds_mm<T>(w, sm.P, LDT, sm.Q, LDQ);  // U(p, j)
@@ -163,6 +204,11 @@ __global__ __launch_bounds__(256) void front_step_kernel(FrontTab t, int first, 
         acc_each<T>([&](int x, int y, int v, int i, int j) { sm.Q[i * LDQ + j] = w.a[x][y][v]; });
         tile_put(sm.P, LDT, l);
         __syncthreads();
+        if (rhs_row && tid < NB) {
+            double s = 0.0;
+            for (int c = 0; c < NB; c++) s += (double)sm.P[tid * LDT + c] * rv[1][c];
+            vrow[tid] -= s;  // b_i -= L(i, p) y_p
+        }
         lds_mm<T>(acc, sm.P, LDT, sm.Q, LDQ);  // L(i, p) U(p, j)
     }
     if (kind == 1) {
@@ -220,20 +266,6 @@ __device__ __forceinline__ double quad_sum(double s)
 {
     s += dpp_mov<0xB1>(s);
     return s + dpp_mov<0x4E>(s);
-}
-
-// Update vector entry of front position i gathered from the children.
-__device__ __forceinline__ double child_sum(const FrontTab &t, const gf_front_dev &F, int i, const double *Vb)
-{
-    double w = 0.0;
-    for (int c = F.ch0; c < F.ch1; c++) {
-        const int a = t.inv[t.child_inv[c] + i];
-        if (a >= 0) {
-            const gf_front_dev C = t.fr[t.child[c]];
-            w += Vb[C.V + NB * C.kb + a];
-        }
-    }
-    return w;
 }
 
 // Own rows of a front: w = b_own + children's update vectors, then
@@ -301,7 +333,7 @@ __global__ __launch_bounds__(256) void front_fwd_upd_kernel(FrontTab t, int firs
 // (x_update gathered from the ancestors' rows of x into LDS, 1024 at a time).
 template <typename T>
 __global__ __launch_bounds__(256) void front_bwd_upd_kernel(FrontTab t, int first, const T *Fb, const T *Eb,
-                                                            const double *x, double *Vb)
+                                                            const double *x, double *Vb, int raw_last)
 {
     __shared__ double xu[KMAX];
     const int *it = t.items + 4 * (size_t)(first + blockIdx.x);
@@ -329,20 +361,26 @@ __global__ __launch_bounds__(256) void front_bwd_upd_kernel(FrontTab t, int firs
         if ((tid & 3) == 0) Vb[F.V + NB * p + r] -= part;
         return;
     }
-    // the last own block row holds A'(p, .) = D_p U(p, .): subtract E_p (A' x)
+    // the last own block row holds A'(p, .) = D_p U(p, .): subtract E_p (A' x); when
+    // the factorisation carried the right-hand side, its block is unconverted too:
+    // z_p = E_p (b_p - A' x)
     T e[16];
     load16(Eb + F.E + (size_t)p * NB * NB + r * NB + sg, e);
     __syncthreads();
-    if ((tid & 3) == 0) xu[r] = part;
+    if ((tid & 3) == 0) xu[r] = raw_last ? Vb[F.V + NB * p + r] - part : part;
     __syncthreads();
     part = quad_sum(dot16(e, xu + sg));
-    if ((tid & 3) == 0) Vb[F.V + NB * p + r] -= part;
+    if ((tid & 3) == 0) {
+        if (raw_last) Vb[F.V + NB * p + r] = part;
+        else Vb[F.V + NB * p + r] -= part;
+    }
 }
 
 // Own-block back substitution x_p = z_p - sum_{p < j < kb} U(p, j) x_j, then the
 // scatter of the own rows of x.
 template <typename T>
-__global__ __launch_bounds__(256) void front_bwd_own_kernel(FrontTab t, int f0, const T *Fb, const double *Vb, double *x)
+__global__ __launch_bounds__(256) void front_bwd_own_kernel(FrontTab t, int f0, const T *Fb, const T *Eb,
+                                                            const double *Vb, double *x, int raw_last)
 {
     __shared__ double y[KMAX];
     const int f = f0 + blockIdx.x;
@@ -352,6 +390,16 @@ __global__ __launch_bounds__(256) void front_bwd_own_kernel(FrontTab t, int f0, 
     const size_t ld = (size_t)F.mp;
     for (int i = tid; i < kp; i += 256) y[i] = Vb[F.V + i];
     __syncthreads();
+    if (raw_last && F.mb == F.kb) {
+        // no update part, so no front_bwd_upd_kernel converted the last block: y = E b
+        const int p = F.kb - 1;
+        T e[16];
+        load16(Eb + F.E + (size_t)p * NB * NB + r * NB + sg, e);
+        const double v = quad_sum(dot16(e, y + NB * p + sg));
+        __syncthreads();
+        if ((tid & 3) == 0) y[NB * p + r] = v;
+        __syncthreads();
+    }
     for (int p = F.kb - 2; p >= 0; p--) {
         const T *U = Fb + F.F + (size_t)(NB * p + r) * ld + sg;
         double part = 0.0;
@@ -381,14 +429,14 @@ FrontTab tab_of(const gf_fronts &fp)
 
 template <typename T>
 int front_factor_t(const gf_fronts &fp, const double *ab, int ld, int hb, const double *D, T *F, T *E, hipStream_t s,
-                   int *flag)
+                   int *flag, const double *rhs, double *V)
 {
     const FrontTab t = tab_of(fp);
     int launches = 0;
     for (int l = 0; l < fp.n_level; l++) {
         int pr = gf_probe_begin(s, GF_KC_FRONT_ASSEMBLE);
         hipLaunchKernelGGL(front_assemble_kernel<T>, dim3(fp.asm_count[l]), dim3(ASM_T), 0, s, t, fp.asm_first[l], ab, ld,
-                           hb, D, F, E, flag);
+                           hb, D, F, E, flag, rhs, V);
         gf_probe_end(s, pr, (double)fp.asm_count[l] * NB * NB);
         launches++;
         for (int k = fp.lev_step[l]; k < fp.lev_step[l + 1]; k++) {
@@ -396,7 +444,7 @@ int front_factor_t(const gf_fronts &fp, const double *ab, int ld, int hb, const 
             const int p = k - fp.lev_step[l];
             pr = gf_probe_begin(s, GF_KC_FRONT_STEP);
             hipLaunchKernelGGL(front_step_kernel<T>, dim3(fp.step_count[k]), dim3(256), 0, s, t, fp.step_first[k], p,
-                               F, E, flag);
+                               F, E, flag, rhs ? V : nullptr);
             gf_probe_end(s, pr, fp.step_flops[k]);
             launches++;
         }
@@ -406,7 +454,8 @@ int front_factor_t(const gf_fronts &fp, const double *ab, int ld, int hb, const 
 }
 
 template <typename T>
-int front_solve_t(const gf_fronts &fp, const T *F, const T *E, const double *rhs, double *x, double *V, hipStream_t s)
+int front_solve_t(const gf_fronts &fp, const T *F, const T *E, const double *rhs, double *x, double *V, hipStream_t s,
+                  bool carried)
 {
     const FrontTab t = tab_of(fp);
     int launches = 0;
@@ -426,7 +475,7 @@ int front_solve_t(const gf_fronts &fp, const T *F, const T *E, const double *rhs
             b += (double)NB * fp.fr[f].kb * (fp.fr[f].mp - NB * fp.fr[f].kb);
         return b * sizeof(T);
     };
-    for (int l = 0; l < fp.n_level; l++) {
+    for (int l = 0; l < fp.n_level && !carried; l++) {
         const int nf = fp.lev_off[l + 1] - fp.lev_off[l];
         int pr = gf_probe_begin(s, GF_KC_FRONT_SOLVE);
         hipLaunchKernelGGL(front_fwd_own_kernel<T>, dim3(nf), dim3(256), 0, s, t, fp.lev_off[l], F, E, rhs, V);
@@ -445,12 +494,13 @@ int front_solve_t(const gf_fronts &fp, const T *F, const T *E, const double *rhs
         if (fp.bupd_count[l] > 0) {
             pr = gf_probe_begin(s, GF_KC_FRONT_SOLVE);
             hipLaunchKernelGGL(front_bwd_upd_kernel<T>, dim3(fp.bupd_count[l]), dim3(256), 0, s, t, fp.bupd_first[l], F,
-                               E, (const double *)x, V);
+                               E, (const double *)x, V, carried ? 1 : 0);
             gf_probe_end(s, pr, pr >= 0 ? panel_bytes(l) : 0.0);
             launches++;
         }
         pr = gf_probe_begin(s, GF_KC_FRONT_SOLVE);
-        hipLaunchKernelGGL(front_bwd_own_kernel<T>, dim3(nf), dim3(256), 0, s, t, fp.lev_off[l], F, (const double *)V, x);
+        hipLaunchKernelGGL(front_bwd_own_kernel<T>, dim3(nf), dim3(256), 0, s, t, fp.lev_off[l], F, E, (const double *)V,
+                           x, carried ? 1 : 0);
         gf_probe_end(s, pr, pr >= 0 ? own_bytes(l, false) : 0.0);
         launches++;
     }
@@ -493,15 +543,15 @@ void gf_front_free(gf_fronts &fp)
 }
 
 int gf_front_factor(const gf_fronts &fp, int elem_size, const double *ab, int ld, int hb, const double *D, void *F,
-                    void *E, hipStream_t s, int *flag)
+                    void *E, hipStream_t s, int *flag, const double *rhs, double *V)
 {
-    if (elem_size == 8) return front_factor_t<double>(fp, ab, ld, hb, D, (double *)F, (double *)E, s, flag);
-    return front_factor_t<float>(fp, ab, ld, hb, D, (float *)F, (float *)E, s, flag);
+    if (elem_size == 8) return front_factor_t<double>(fp, ab, ld, hb, D, (double *)F, (double *)E, s, flag, rhs, V);
+    return front_factor_t<float>(fp, ab, ld, hb, D, (float *)F, (float *)E, s, flag, rhs, V);
 }
 
 int gf_front_solve(const gf_fronts &fp, int elem_size, const void *F, const void *E, const double *rhs, double *x,
-                   double *V, hipStream_t s)
+                   double *V, hipStream_t s, bool carried)
 {
-    if (elem_size == 8) return front_solve_t<double>(fp, (const double *)F, (const double *)E, rhs, x, V, s);
-    return front_solve_t<float>(fp, (const float *)F, (const float *)E, rhs, x, V, s);
+    if (elem_size == 8) return front_solve_t<double>(fp, (const double *)F, (const double *)E, rhs, x, V, s, carried);
+    return front_solve_t<float>(fp, (const float *)F, (const float *)E, rhs, x, V, s, carried);
 }

@@ -287,13 +287,16 @@ int gf_front_upload(gf_fronts &fp, hipStream_t s);  // device copies of the plan
 void gf_front_free(gf_fronts &fp);
 // Factorisation of the band-stored tangent ab (row stride ld, half-bandwidth hb,
 // FP64) into fronts of precision elem_size (8, or 4 with the equilibration
-// scaling D applied on the way, D = null for none); returns launches.
+// scaling D applied on the way, D = null for none); returns launches.  rhs
+// non-null: the right-hand side rides along (forward substitution done in the
+// factorisation's launches, its state in the vector workspace V).
 int gf_front_factor(const gf_fronts &fp, int elem_size, const double *ab, int ld, int hb, const double *D, void *F,
-                    void *E, hipStream_t s, int *flag);
+                    void *E, hipStream_t s, int *flag, const double *rhs, double *V);
 // Solve with a factor: rhs (band-row vector) in, x out (may alias rhs); V: the
-// plan's vector workspace (v_elems doubles).  Returns launches.
+// plan's vector workspace (v_elems doubles); carried: the factorisation carried
+// this right-hand side (only the backward substitution is left).  Returns launches.
 int gf_front_solve(const gf_fronts &fp, int elem_size, const void *F, const void *E, const double *rhs, double *x,
-                   double *V, hipStream_t s);
+                   double *V, hipStream_t s, bool carried);
 
 // ---- per-kernel timing --------------------------------------------------------
 // While a model's timing is on and its probe mask selects a kernel class, every

@@ -200,7 +200,7 @@ int gf_mixed_factor(gf_model *m)
         // the fronts take the scaled entries straight from the FP64 band
         GF_HIP(hipGetLastError());
         const int l = gf_front_factor(m->fronts, 4, m->d_band, m->band_ld, m->half_band, m->d_scale, m->d_front32,
-                                      m->d_front32_e, m->stream, m->d_flag);
+                                      m->d_front32_e, m->stream, m->d_flag, nullptr, nullptr);
         return l < 0 ? l : l + 1;
     }
     hipLaunchKernelGGL(convert_band_kernel, dim3(grid_for(m->band_elems, 256)), dim3(256), 0, m->stream, m->d_band,
@@ -233,7 +233,7 @@ int gf_mixed_sweep(gf_model *m)
     // BTD: solution in d_swork (d_w is reduced in place); band: in place in d_w
     const int l = m->layout == GM_BTD ? gf_btd_solve(m->btd32, m->stream, m->d_w, m->d_swork, npad, m->d_w)
                   : m->fronts_on    ? gf_front_solve(m->fronts, 4, m->d_front32, m->d_front32_e, m->d_w, m->d_w,
-                                                     m->d_front_v, m->stream)
+                                                     m->d_front_v, m->stream, false)
                                     : gf_band_solve(m->d_band32, 4, npad, m->half_band, m->stream, m->d_w,
                                                     m->d_swork, m->d_band32_inv);
     if (l < 0) return l;
@@ -264,9 +264,10 @@ int gf_mixed_fallback(gf_model *m)
     if (m->layout == GM_BTD) return gf_btd_factor_solve(m->btd, m->stream, m->d_flag, m->d_r, m->d_x, npad, m->d_res);
     if (m->fronts_on) {
         const int l = gf_front_factor(m->fronts, 8, m->d_band, m->band_ld, m->half_band, nullptr, m->d_front,
-                                      m->d_front_e, m->stream, m->d_flag);
+                                      m->d_front_e, m->stream, m->d_flag, m->d_r, m->d_front_v);
         if (l < 0) return l;
-        const int l2 = gf_front_solve(m->fronts, 8, m->d_front, m->d_front_e, m->d_r, m->d_x, m->d_front_v, m->stream);
+        const int l2 =
+            gf_front_solve(m->fronts, 8, m->d_front, m->d_front_e, m->d_r, m->d_x, m->d_front_v, m->stream, true);
         return l2 < 0 ? l2 : l + l2;
     }
     const int l = gf_band_factor(m->d_band, 8, npad, m->half_band, m->stream, m->d_flag, m->d_band_inv);

@@ -1296,9 +1296,9 @@ static int solve_fp64_launches(gf_model *m, int &nl_fac, int &nl_sol)
     nl_sol = 0;
     if (m->fronts_on) {
         nl_fac = gf_front_factor(m->fronts, 8, m->d_band, m->band_ld, m->half_band, nullptr, m->d_front, m->d_front_e,
-                                 m->stream, m->d_flag);
+                                 m->stream, m->d_flag, m->d_r, m->d_front_v);
         if (nl_fac < 0) return nl_fac;
-        nl_sol = gf_front_solve(m->fronts, 8, m->d_front, m->d_front_e, m->d_r, m->d_x, m->d_front_v, m->stream);
+        nl_sol = gf_front_solve(m->fronts, 8, m->d_front, m->d_front_e, m->d_r, m->d_x, m->d_front_v, m->stream, true);
         return nl_sol < 0 ? nl_sol : GF_OK;
     }
     if (m->layout == GM_BTD)
@@ -1347,11 +1347,11 @@ static int solve_fp64_enqueue(gf_model *m)
     int nl_fac = 0, nl_sol = 0;
     if (m->fronts_on) {
         nl_fac = gf_front_factor(m->fronts, 8, m->d_band, m->band_ld, m->half_band, nullptr, m->d_front, m->d_front_e,
-                                 m->stream, m->d_flag);
+                                 m->stream, m->d_flag, m->d_r, m->d_front_v);
         if (nl_fac < 0) return nl_fac;
         tick(m, GF_PH_FACTOR, false);
         tick(m, GF_PH_SOLVE, true);
-        nl_sol = gf_front_solve(m->fronts, 8, m->d_front, m->d_front_e, m->d_r, m->d_x, m->d_front_v, m->stream);
+        nl_sol = gf_front_solve(m->fronts, 8, m->d_front, m->d_front_e, m->d_r, m->d_x, m->d_front_v, m->stream, true);
         if (nl_sol < 0) return nl_sol;
     } else if (m->layout == GM_BTD) {
         GF_TRY(solve_fp64_launches(m, nl_fac, nl_sol));
