@@ -45,6 +45,7 @@ struct FrontTab {
     const gf_front_dev *fr;
     const int *rows, *child, *inv, *items;
     const long long *child_inv;
+    const int *orig, *asm_orig;
 };
 
 // Update vector entry of front position i gathered from the children.
@@ -87,7 +88,6 @@ __global__ __launch_bounds__(ASM_T) void front_assemble_kernel(FrontTab t, int f
     constexpr int RS = ASM_T / NB;  // row stride of a thread's rows
     const int cj = c0 + j;
     const int *rows = t.rows + F.rows;
-    const int oj = rows[cj];
     T v[ASM_R];
 #pragma unroll
     for (int r = 0; r < ASM_R; r++) v[r] = T(0);
@@ -103,20 +103,29 @@ __global__ __launch_bounds__(ASM_T) void front_assemble_kernel(FrontTab t, int f
             if (a >= 0) v[r] += U[(size_t)a * C.mp];
         }
     }
-    T *out = Fb + F.F + cj;
+    // the children's sums into the tile in LDS, the tile's tangent entries (its
+    // precomputed list) added there, then the tile written out row by row
 #pragma unroll
     for (int r = 0; r < ASM_R; r++) {
         const int ri = r0 + i0 + RS * r;
-        const int oi = rows[ri];
-        const bool own = ri < F.k || cj < F.k;
-        if (ri == cj && ri >= F.k && ri < NB * F.kb) {
-            v[r] = T(1);  // padding of the own block: identity
-        } else if (own && oi >= 0 && oj >= 0 && abs(oi - oj) <= hb) {
-            double a = ab[(size_t)oi * (size_t)ld + (size_t)(oj + hb)];
-            if (D) a = D[oi] * a * D[oj];
-            v[r] += (T)a;
-        }
-        out[(size_t)ri * F.mp] = v[r];
+        if (ri == cj && ri >= F.k && ri < NB * F.kb) v[r] = T(1);  // padding of the own block: identity
+        sm.P[(i0 + RS * r) * LDT + j] = v[r];
+    }
+    __syncthreads();
+    const int *ol = t.orig + 3 * (size_t)t.asm_orig[2 * (size_t)(first + blockIdx.x)];
+    const int no = t.asm_orig[2 * (size_t)(first + blockIdx.x) + 1];
+    for (int q = tid; q < no; q += ASM_T) {
+        const int pos = ol[3 * q], oi = ol[3 * q + 1], oc = ol[3 * q + 2];
+        double a = ab[(size_t)oi * (size_t)ld + (size_t)(oc + hb)];
+        if (D) a = D[oi] * a * D[oc];
+        sm.P[(pos >> 6) * LDT + (pos & 63)] += (T)a;
+    }
+    __syncthreads();
+    T *out = Fb + F.F + cj;
+#pragma unroll
+    for (int r = 0; r < ASM_R; r++) {
+        v[r] = sm.P[(i0 + RS * r) * LDT + j];
+        out[(size_t)(r0 + i0 + RS * r) * F.mp] = v[r];
     }
     if (rhs && it[2] == 0 && tid < NB) {
         // the right-hand side rides along the factorisation: the front vector starts
@@ -125,10 +134,7 @@ __global__ __launch_bounds__(ASM_T) void front_assemble_kernel(FrontTab t, int f
         Vb[F.V + i] = (i < F.k && o >= 0 ? rhs[o] : 0.0) + child_sum(t, F, i, Vb);
     }
     if (it[3] == 0) return;
-    // the front's first diagonal block: E_0
-#pragma unroll
-    for (int r = 0; r < ASM_R; r++) sm.P[(i0 + RS * r) * LDT + j] = v[r];
-    __syncthreads();
+    // the front's first diagonal block (already in LDS): E_0
     inverse64<T>(sm.P, sm.tmp, sm.aux, flag);
     T *E = Eb + F.E;
     for (int q = threadIdx.x; q < NB * NB; q += ASM_T) E[q] = sm.P[(q / NB) * LDT + q % NB];
@@ -424,6 +430,8 @@ FrontTab tab_of(const gf_fronts &fp)
     t.inv = fp.d_inv;
     t.items = fp.d_items;
     t.child_inv = fp.d_child_inv;
+    t.orig = fp.d_orig;
+    t.asm_orig = fp.d_asm_orig;
     return t;
 }
 
@@ -530,15 +538,17 @@ int gf_front_upload(gf_fronts &fp, hipStream_t s)
     GF_TRY(put(&fp.d_child_inv, fp.child_inv, s));
     GF_TRY(put(&fp.d_inv, fp.inv, s));
     GF_TRY(put(&fp.d_items, fp.items, s));
+    GF_TRY(put(&fp.d_orig, fp.orig, s));
+    GF_TRY(put(&fp.d_asm_orig, fp.asm_orig, s));
     return GF_OK;
 }
 
 void gf_front_free(gf_fronts &fp)
 {
-    void *p[] = {fp.d_fr, fp.d_rows, fp.d_child, fp.d_child_inv, fp.d_inv, fp.d_items};
+    void *p[] = {fp.d_fr, fp.d_rows, fp.d_child, fp.d_child_inv, fp.d_inv, fp.d_items, fp.d_orig, fp.d_asm_orig};
     for (void *q : p) gf_pool_free(q);
     fp.d_fr = nullptr;
-    fp.d_rows = fp.d_child = fp.d_inv = fp.d_items = nullptr;
+    fp.d_rows = fp.d_child = fp.d_inv = fp.d_items = fp.d_orig = fp.d_asm_orig = nullptr;
     fp.d_child_inv = nullptr;
 }
 

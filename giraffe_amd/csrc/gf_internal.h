@@ -269,17 +269,23 @@ struct gf_fronts {
     std::vector<int> fupd_first, fupd_count;  // per level: forward update row tiles (front, tile row)
     std::vector<int> bupd_first, bupd_count;  // per level: backward own row tiles with an update part
     std::vector<double> step_flops;    // algorithmic flops of each factor launch
+    std::vector<int> orig;             // tangent entries of the assembly tiles: position in the tile, band row, band column
+    std::vector<int> asm_orig;         // per work item (2 ints): first entry of its tile in orig, count
     long long f_elems = 0, e_elems = 0, v_elems = 0;
     double flops = 0.0;                // algorithmic flops of one factorisation
     long long n_sep_dofs = 0;          // DOFs owned by separators (non-leaf fronts)
     // device copies
     gf_front_dev *d_fr = nullptr;
     int *d_rows = nullptr, *d_child = nullptr, *d_inv = nullptr, *d_items = nullptr;
+    int *d_orig = nullptr, *d_asm_orig = nullptr;
     long long *d_child_inv = nullptr;
 };
 // Plans the fronts of the free DOFs dof[6 node + comp] (band rows, -1 when not
 // free) over the node graph of the description's elements; host only.
 int gf_nd_plan(gf_fronts &fp, const gf_model_desc *d, const std::vector<int> &dof, int nn, int npad);
+// The assembly tiles' lists of tangent entries from the tangent's structure (CSR
+// over band rows); after gf_nd_plan.
+int gf_nd_orig_lists(gf_fronts &fp, const std::vector<int> &ptr, const std::vector<int> &col, int npad);
 // Launch count of one factorisation and of one solve of a plan.
 int gf_front_factor_launches(const gf_fronts &fp);
 int gf_front_solve_launches(const gf_fronts &fp);

@@ -439,6 +439,7 @@ static int plan(const gf_model_desc *d, gf_model *m, int &bm, int &nblk, int sol
         if (take) {
             m->fronts_on = true;
             tangent_structure(d, m, m->csr_ptr, m->csr_col);
+            GF_TRY(gf_nd_orig_lists(m->fronts, m->csr_ptr, m->csr_col, npad));
             m->layout = GM_BAND;
             m->npad = npad;
             bm = -1;
@@ -475,7 +476,8 @@ static size_t footprint(const gf_model_desc *d, const gf_model *m, int precision
         const size_t factor = (size_t)(fp.f_elems + fp.e_elems);
         b += (npad * (size_t)(2 * m->half_band + 1) + factor + (size_t)fp.v_elems) * 8;
         b += (fp.rows.size() + fp.child.size() + fp.inv.size() + fp.items.size()) * 4 + fp.child_inv.size() * 8 +
-             fp.fr.size() * sizeof(gf_front_dev) + (m->csr_ptr.size() + m->csr_col.size()) * 4;
+             fp.fr.size() * sizeof(gf_front_dev) + (m->csr_ptr.size() + m->csr_col.size()) * 4 +
+             (fp.orig.size() + fp.asm_orig.size()) * 4;
         if (precision == GF_PREC_MIXED) b += factor * 4 + 3 * npad * 8;
         return b;
     } else {

@@ -378,3 +378,65 @@ int gf_front_solve_launches(const gf_fronts &fp)
     for (int l = 0; l < fp.n_level; l++) n += 2 + (fp.fupd_count[l] > 0) + (fp.bupd_count[l] > 0);
     return n;
 }
+
+// Original tangent entries of every assembly tile: for each front's own DOF a and
+// each structural neighbour b of a (the tangent's structure, ptr / col, row by row)
+// the entry (a, b) and, when b is an update DOF of the front, (b, a); entries whose
+// other DOF is in no row of the front belong to a descendant's front.  Per
+// assembly item: the first entry of its tile and their count.
+int gf_nd_orig_lists(gf_fronts &fp, const std::vector<int> &ptr, const std::vector<int> &col, int npad)
+{
+    std::vector<int> ppos(npad, -1);
+    fp.orig.clear();
+    fp.asm_orig.assign(fp.items.size() / 2, 0);  // 2 ints per item
+    std::vector<long long> tile_off;
+    std::vector<int> cnt;
+    std::vector<long long> front_base(fp.n_front + 1, 0);
+    std::vector<std::vector<long long>> toff(fp.n_front);
+    for (int f = 0; f < fp.n_front; f++) {
+        const gf_front_dev &F = fp.fr[f];
+        const int *rows = fp.rows.data() + F.rows;
+        for (int i = 0; i < F.mp; i++)
+            if (rows[i] >= 0) ppos[rows[i]] = i;
+        const int nt = F.mb * F.mb;
+        cnt.assign(nt, 0);
+        auto each = [&](auto emit) {
+            for (int i = 0; i < F.k; i++) {
+                const int oa = rows[i];
+                for (int q = ptr[oa]; q < ptr[oa + 1]; q++) {
+                    const int ob = col[q], j = ppos[ob];
+                    if (j < 0) continue;
+                    emit(i, j, oa, ob);
+                    if (j >= F.k) emit(j, i, ob, oa);
+                }
+            }
+        };
+        each([&](int i, int j, int, int) { cnt[(i / TB) * F.mb + j / TB]++; });
+        std::vector<long long> &to = toff[f];
+        to.assign(nt + 1, 0);
+        for (int t = 0; t < nt; t++) to[t + 1] = to[t] + cnt[t];
+        const long long base = (long long)fp.orig.size() / 3;
+        fp.orig.resize(fp.orig.size() + 3 * (size_t)to[nt]);
+        std::vector<long long> fill(to.begin(), to.end() - 1);
+        each([&](int i, int j, int oa, int ob) {
+            const int t = (i / TB) * F.mb + j / TB;
+            const size_t e = 3 * (size_t)(base + fill[t]++);
+            fp.orig[e] = (i % TB) * TB + j % TB;
+            fp.orig[e + 1] = oa;
+            fp.orig[e + 2] = ob;
+        });
+        front_base[f] = base;
+        for (int i = 0; i < F.mp; i++)
+            if (rows[i] >= 0) ppos[rows[i]] = -1;
+    }
+    if (fp.orig.size() / 3 > (size_t)0x7fffffff) { gf_set_error("multifrontal plan: too many tangent entries"); return GF_E_UNSUPPORTED; }
+    for (int l = 0; l < fp.n_level; l++)
+        for (int it = fp.asm_first[l]; it < fp.asm_first[l] + fp.asm_count[l]; it++) {
+            const int f = fp.items[4 * (size_t)it], ti = fp.items[4 * (size_t)it + 1], tj = fp.items[4 * (size_t)it + 2];
+            const int t = ti * fp.fr[f].mb + tj;
+            fp.asm_orig[2 * (size_t)it] = (int)(front_base[f] + toff[f][t]);
+            fp.asm_orig[2 * (size_t)it + 1] = (int)(toff[f][t + 1] - toff[f][t]);
+        }
+    return GF_OK;
+}
+
