@@ -140,13 +140,104 @@ __global__ __launch_bounds__(ASM_T) void front_assemble_kernel(FrontTab t, int f
     for (int q = threadIdx.x; q < NB * NB; q += ASM_T) E[q] = sm.P[(q / NB) * LDT + q % NB];
 }
 
+// LDS of the step kernel, 52.6 KB in FP64 so that three workgroups share a CU: X
+// holds one full 64 x 64 operand (E_p, then U(p, j) transposed) and C a 32-deep
+// chunk of the other (A'(p, j) transposed, then L(i, p)), both at row strides
+// for which the MFMA operand reads (16 rows x 4 consecutive entries) hit distinct
+// banks; a B operand is kept transposed so that it is read the same way.  The
+// inverse's scratch reuses C.
+template <typename T> struct FStepSmem {
+    T X[NB * LDT];
+    union {
+        T C[NB * SA_LD];
+        T tmp[3 * 16 * TMP_LD];
+    };
+    T aux[4 + NB];
+};
+
+// acc += X(:, k0 .. k0+31) * CT^T, X [i][k] (stride LDT), CT [j][k - k0] (stride SA_LD)
+template <typename T> __device__ __forceinline__ void mm_xc(Acc<T> &acc, const T *X, const T *CT, int k0)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = acc_row(wave), wc = acc_col(wave);
+#pragma unroll
+    for (int kk = 0; kk < KT; kk += 4) {
+        T av[2], bv[2];
+#pragma unroll
+        for (int x = 0; x < 2; x++) av[x] = X[(wr + 16 * x + (lane & 15)) * LDT + k0 + kk + (lane >> 4)];
+#pragma unroll
+        for (int y = 0; y < 2; y++) bv[y] = CT[(wc + 16 * y + (lane & 15)) * SA_LD + kk + (lane >> 4)];
+#pragma unroll
+        for (int x = 0; x < 2; x++)
+#pragma unroll
+            for (int y = 0; y < 2; y++) acc.a[x][y] = mfma<T>::run(av[x], bv[y], acc.a[x][y]);
+    }
+}
+// acc += C * XT(:, k0 .. k0+31)^T, C [i][k - k0] (stride SA_LD), XT [j][k] (stride LDT)
+template <typename T> __device__ __forceinline__ void mm_cx(Acc<T> &acc, const T *C, const T *XT, int k0)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = acc_row(wave), wc = acc_col(wave);
+#pragma unroll
+    for (int kk = 0; kk < KT; kk += 4) {
+        T av[2], bv[2];
+#pragma unroll
+        for (int x = 0; x < 2; x++) av[x] = C[(wr + 16 * x + (lane & 15)) * SA_LD + kk + (lane >> 4)];
+#pragma unroll
+        for (int y = 0; y < 2; y++) bv[y] = XT[(wc + 16 * y + (lane & 15)) * LDT + k0 + kk + (lane >> 4)];
+#pragma unroll
+        for (int x = 0; x < 2; x++)
+#pragma unroll
+            for (int y = 0; y < 2; y++) acc.a[x][y] = mfma<T>::run(av[x], bv[y], acc.a[x][y]);
+    }
+}
+// A 32 x 64 row chunk (rows r0 .. r0+31 of a row-major tile at src, stride ld) into
+// registers (thread t holds t + 256 r: row (t + 256 r) / 64, column t % 64), and
+// into C transposed (C [column][row]).
+template <typename T> __device__ __forceinline__ void rowchunk_load(T (&v)[8], const T *src, size_t ld, int r0)
+{
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        const int t = threadIdx.x + 256 * r;
+        v[r] = src[(size_t)(r0 + (t >> 6)) * ld + (t & 63)];
+    }
+}
+template <typename T> __device__ __forceinline__ void rowchunk_put_t(T *C, const T (&v)[8])
+{
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        const int t = threadIdx.x + 256 * r;
+        C[(t & 63) * SA_LD + (t >> 6)] = v[r];
+    }
+}
+// A 64 x 32 column chunk (columns c0 .. c0+31) into registers (thread t holds
+// t + 256 r: row (t + 256 r) / 32, column t % 32), and into C as it is.
+template <typename T> __device__ __forceinline__ void colchunk_load(T (&v)[8], const T *src, size_t ld, int c0)
+{
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        const int t = threadIdx.x + 256 * r;
+        v[r] = src[(size_t)(t >> 5) * ld + c0 + (t & 31)];
+    }
+}
+template <typename T> __device__ __forceinline__ void colchunk_put(T *C, const T (&v)[8])
+{
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        const int t = threadIdx.x + 256 * r;
+        C[(t >> 5) * SA_LD + (t & 31)] = v[r];
+    }
+}
+
 // Launch p >= 0 of a level's factorisation (the band step on every front of the
 // level; see the file header).  Item kinds: 0 diagonal, 1 update, 2 conversion.
+// Each product runs in two 32-deep chunks; all of a workgroup's loads are issued
+// before the first product.
 template <typename T>
-__global__ __launch_bounds__(256) void front_step_kernel(FrontTab t, int first, int p, T *Fb, T *Eb, int *flag,
+__global__ __launch_bounds__(256, 3) void front_step_kernel(FrontTab t, int first, int p, T *Fb, T *Eb, int *flag,
                                                          double *Vb)
 {
-    __shared__ StepSmem<T> sm;
+    __shared__ FStepSmem<T> sm;
     __shared__ double rv[2][NB];  // the right-hand side block of row p (or p - 1) and E times it
     const int *it = t.items + 4 * (size_t)(first + blockIdx.x);
     const int f = it[0];
@@ -159,74 +250,78 @@ __global__ __launch_bounds__(256) void front_step_kernel(FrontTab t, int first, 
     T *Ef = Eb + F.E;
     const int tid = threadIdx.x;
     const int R0 = ti * NB, C0 = tj * NB;
-    Acc<T> acc;
-    acc.zero();
     // with the right-hand side in the front vector (Vb): the first conversion tile of
     // row p - 1 converts its block, y = E_{p-1} b_{p-1}; the first update tile of row
     // i subtracts L(i, p) y_p, y_p = E_p b_p, from block i (each block written by one
     // workgroup; b_p itself is final and unconverted during launch p)
     const bool rhs_row = Vb && tj == (kind == 2 ? p : p + 1);
     double *vrow = Vb ? Vb + F.V + (size_t)R0 : nullptr;
+    // product 1: W = E_q A'(q, j), q = p (update, diagonal) or p - 1 (conversion)
+    const int q = kind == 2 ? ti : p;
+    const T *E = Ef + (size_t)q * NB * NB;
+    const T *Aq = A + (size_t)(q * NB) * ld + C0;  // A'(q, j)
+    T e[16], a0[8], a1[8];
+    tile_load(e, [&](int i, int k) { return E[i * NB + k]; });
+    rowchunk_load(a0, Aq, ld, 0);
+    rowchunk_load(a1, Aq, ld, KT);
+    if (rhs_row && tid < NB) rv[0][tid] = kind == 2 ? vrow[tid] : Vb[F.V + (size_t)(p * NB) + tid];
+    tile_put(sm.X, LDT, e);
+    rowchunk_put_t(sm.C, a0);
+    __syncthreads();
+    T l0[8], l1[8], old[2][2][4];
+    const T *Lp = A + (size_t)R0 * ld + p * NB;  // L(i, p)
+    if (kind != 2) colchunk_load(l0, Lp, ld, 0);
+    if (rhs_row && tid < NB) {
+        double yv = 0.0;
+        for (int c = 0; c < NB; c++) yv += (double)sm.X[tid * LDT + c] * rv[0][c];
+        if (kind == 2) vrow[tid] = yv;
+        else rv[1][tid] = yv;  // y_p
+    }
+    Acc<T> w;
+    w.zero();
+    mm_xc<T>(w, sm.X, sm.C, 0);
+    __syncthreads();
+    rowchunk_put_t(sm.C, a1);
+    __syncthreads();
+    if (kind != 2) {
+        // behind the second half of product 1
+        colchunk_load(l1, Lp, ld, KT);
+        acc_each<T>([&](int x, int y, int v, int i, int j) { old[x][y][v] = A[(R0 + i) * ld + C0 + j]; });
+    }
+    mm_xc<T>(w, sm.X, sm.C, KT);
     if (kind == 2) {
-        T e[16], a[16];
-        const T *E = Ef + (size_t)ti * NB * NB;
-        tile_load(e, [&](int i, int k) { return E[i * NB + k]; });
-        tile_load(a, [&](int k, int j) { return A[(R0 + k) * ld + C0 + j]; });
-        if (rhs_row && tid < NB) rv[0][tid] = vrow[tid];
-        tile_put(sm.P, LDT, e);
-        tile_put(sm.Q, LDQ, a);
-        __syncthreads();
-        if (rhs_row && tid < NB) {
-            double y = 0.0;
-            for (int c = 0; c < NB; c++) y += (double)sm.P[tid * LDT + c] * rv[0][c];
-            vrow[tid] = y;
-        }
-        lds_mm<T>(acc, sm.P, LDT, sm.Q, LDQ);
-        acc_each<T>([&](int x, int y, int v, int i, int j) { A[(R0 + i) * ld + C0 + j] = acc.a[x][y][v]; });
+        acc_each<T>([&](int x, int y, int v, int i, int j) { A[(R0 + i) * ld + C0 + j] = w.a[x][y][v]; });
         return;
     }
-    T old[2][2][4];
-    {
-        const int K0 = p * NB;
-        T e[16], a[16], l[16];
-        const T *E = Ef + (size_t)p * NB * NB;
-        tile_load(e, [&](int i, int k) { return E[i * NB + k]; });
-        tile_load(a, [&](int k, int j) { return A[(K0 + k) * ld + C0 + j]; });
-        if (rhs_row && tid < NB) rv[0][tid] = Vb[F.V + (size_t)K0 + tid];
-        tile_put(sm.P, LDT, e);
-        tile_put(sm.Q, LDQ, a);
-        __syncthreads();
-        tile_load(l, [&](int i, int k) { return A[(R0 + i) * ld + K0 + k]; });
-        acc_each<T>([&](int x, int y, int v, int i, int j) { old[x][y][v] = A[(R0 + i) * ld + C0 + j]; });
-        if (rhs_row && tid < NB) {
-            double y = 0.0;
-            for (int c = 0; c < NB; c++) y += (double)sm.P[tid * LDT + c] * rv[0][c];
-            rv[1][tid] = y;  // y_p
-        }
-        Acc<T> w;
-        w.zero();
-        lds_mm<T>(w, sm.P, LDT, sm.Q, LDQ);  // U(p, j)
-        __syncthreads();
-        acc_each<T>([&](int x, int y, int v, int i, int j) { sm.Q[i * LDQ + j] = w.a[x][y][v]; });
-        tile_put(sm.P, LDT, l);
-        __syncthreads();
-        if (rhs_row && tid < NB) {
-            double s = 0.0;
-            for (int c = 0; c < NB; c++) s += (double)sm.P[tid * LDT + c] * rv[1][c];
-            vrow[tid] -= s;  // b_i -= L(i, p) y_p
-        }
-        lds_mm<T>(acc, sm.P, LDT, sm.Q, LDQ);  // L(i, p) U(p, j)
+    __syncthreads();
+    // product 2: old - L(i, p) W, W kept transposed in X
+    acc_each<T>([&](int x, int y, int v, int i, int j) { sm.X[j * LDT + i] = w.a[x][y][v]; });
+    colchunk_put(sm.C, l0);
+    __syncthreads();
+    double s = 0.0;
+    if (rhs_row && tid < NB)
+        for (int c = 0; c < KT; c++) s += (double)sm.C[tid * SA_LD + c] * rv[1][c];
+    Acc<T> acc;
+    acc.zero();
+    mm_cx<T>(acc, sm.C, sm.X, 0);
+    __syncthreads();
+    colchunk_put(sm.C, l1);
+    __syncthreads();
+    if (rhs_row && tid < NB) {
+        for (int c = 0; c < KT; c++) s += (double)sm.C[tid * SA_LD + c] * rv[1][KT + c];
+        vrow[tid] -= s;  // b_i -= L(i, p) y_p
     }
+    mm_cx<T>(acc, sm.C, sm.X, KT);
     if (kind == 1) {
         acc_each<T>([&](int x, int y, int v, int i, int j) { A[(R0 + i) * ld + C0 + j] = old[x][y][v] - acc.a[x][y][v]; });
         return;
     }
     __syncthreads();
-    acc_each<T>([&](int x, int y, int v, int i, int j) { sm.P[i * LDT + j] = old[x][y][v] - acc.a[x][y][v]; });
+    acc_each<T>([&](int x, int y, int v, int i, int j) { sm.X[i * LDT + j] = old[x][y][v] - acc.a[x][y][v]; });
     __syncthreads();
-    inverse64<T>(sm.P, sm.tmp, sm.aux, flag);  // E_{p+1}
-    T *E = Ef + (size_t)ti * NB * NB;
-    for (int q = tid; q < NB * NB; q += 256) E[q] = sm.P[(q / NB) * LDT + q % NB];
+    inverse64<T>(sm.X, sm.tmp, sm.aux, flag);  // E_{p+1}
+    T *En = Ef + (size_t)ti * NB * NB;
+    for (int q2 = tid; q2 < NB * NB; q2 += 256) En[q2] = sm.X[(q2 / NB) * LDT + q2 % NB];
 }
 
 // ---- solve -----------------------------------------------------------------------
