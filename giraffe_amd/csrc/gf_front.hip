@@ -160,7 +160,7 @@ template <typename T> __device__ __forceinline__ void mm_xc(Acc<T> &acc, const T
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wr = acc_row(wave), wc = acc_col(wave);
-#pragma unroll
+#pragma unroll 2
     for (int kk = 0; kk < KT; kk += 4) {
         T av[2], bv[2];
 #pragma unroll
@@ -178,7 +178,7 @@ template <typename T> __device__ __forceinline__ void mm_cx(Acc<T> &acc, const T
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wr = acc_row(wave), wc = acc_col(wave);
-#pragma unroll
+#pragma unroll 2
     for (int kk = 0; kk < KT; kk += 4) {
         T av[2], bv[2];
 #pragma unroll

@@ -178,11 +178,13 @@ template <int R> __device__ __forceinline__ float rowcast(float v)
 // pivot reaches every lane by two register moves (column K of the lane's row
 // from row block K / 4, then lane K of each DPP row), so the chain from one
 // pivot to the next is VALU only: no readlane, no scalar selects.  A pivot
-// below eps[K] takes the reciprocal of +-eps[K] (ieps, computed before the
-// first pivot), a non-finite one the reciprocal 1.
+// below eps_K = rel dg[K] (rel mx where the original diagonal dg[K] is zero)
+// takes the reciprocal of +-eps_K, a non-finite one the reciprocal 1; the
+// threshold is read from LDS per pivot, off the dependent chain, rather than
+// held in registers for all sixteen.
 template <typename T, int K>
-__device__ __forceinline__ void inv16_pivot(T (&a)[4], int i, int q, const T (&eps)[16], const T (&ieps)[16],
-                                            T huge, bool &bad, bool &pert)
+__device__ __forceinline__ void inv16_pivot(T (&a)[4], int i, int q, const T *dg, T mx, T rel, T huge, bool &bad,
+                                            bool &pert)
 {
     constexpr int KQ = K >> 2, KE = K & 3;
     T rk[4];
@@ -192,10 +194,12 @@ __device__ __forceinline__ void inv16_pivot(T (&a)[4], int i, int q, const T (&e
     const T p = dpp_mov<0x150 + K>(f);                               // the pivot
     const T ap = fabs(p);
     const bool nonfinite = !(ap < huge);
-    const bool small = !nonfinite && ap < eps[K];
+    const T dk = dg[K], epsk = rel * (dk > T(0) ? dk : mx);
+    const bool small = !nonfinite && ap < epsk;
     bad |= nonfinite;
     pert |= small;
-    const T ip = nonfinite ? T(1) : small ? (p >= T(0) ? ieps[K] : -ieps[K]) : rcp_of<T>(p);
+    // one reciprocal of the selected value (rcp(-x) = -rcp(x) exactly, rcp(1) = 1)
+    const T ip = rcp_of<T>(nonfinite ? T(1) : small ? (p >= T(0) ? epsk : -epsk) : p);
     const bool isk = i == K;
 #pragma unroll
     for (int e = 0; e < 4; e++) {
@@ -203,23 +207,22 @@ __device__ __forceinline__ void inv16_pivot(T (&a)[4], int i, int q, const T (&e
         const T upd = isk ? r : a[e] - f * r;
         a[e] = (4 * q + e == K) ? (isk ? ip : -f * ip) : upd;
     }
-    if constexpr (K + 1 < 16) inv16_pivot<T, K + 1>(a, i, q, eps, ieps, huge, bad, pert);
+    if constexpr (K + 1 < 16) inv16_pivot<T, K + 1>(a, i, q, dg, mx, rel, huge, bad, pert);
 }
 
 // One wavefront: in-place Gauss-Jordan inverse (no interchanges) of the 16 x 16
 // block at b (stride LDT); the 16 pivots run in registers with DPP and
-// permlane moves, no barrier and no LDS access between the load and the store.
-// eps[k]: the perturbation size of pivot k.
+// permlane moves, no barrier and no LDS access on the pivot chain.  dg: the
+// sixteen original diagonal magnitudes (LDS), mx: the block's largest
+// magnitude, rel: the relative perturbation size.
 template <typename T>
-__device__ __forceinline__ void inv16(T *b, const T (&eps)[16], T huge, bool &bad, bool &pert)
+__device__ __forceinline__ void inv16(T *b, const T *dg, T mx, T rel, T huge, bool &bad, bool &pert)
 {
     const int lane = threadIdx.x & 63, i = lane & 15, q = lane >> 4;
-    T a[4], ieps[16];
+    T a[4];
 #pragma unroll
     for (int e = 0; e < 4; e++) a[e] = b[i * LDT + 4 * q + e];
-#pragma unroll
-    for (int k = 0; k < 16; k++) ieps[k] = rcp_of<T>(eps[k]);
-    inv16_pivot<T, 0>(a, i, q, eps, ieps, huge, bad, pert);
+    inv16_pivot<T, 0>(a, i, q, dg, mx, rel, huge, bad, pert);
 #pragma unroll
     for (int e = 0; e < 4; e++) b[i * LDT + 4 * q + e] = a[e];
 }
@@ -272,13 +275,7 @@ template <typename T> __device__ __forceinline__ void inverse64(T *M, T *tmp, T 
     for (int kb = 0; kb < 4; kb++) {
         T *P = M + 16 * kb * LDT + 16 * kb;
         if (wave == 0) {
-            T eps[16];
-#pragma unroll
-            for (int k = 0; k < 16; k++) {
-                const T dk = aux[4 + 16 * kb + k];
-                eps[k] = pg.rel * (dk > T(0) ? dk : mx);
-            }
-            inv16<T>(P, eps, pg.huge, bad, pert);
+            inv16<T>(P, aux + 4 + 16 * kb, mx, pg.rel, pg.huge, bad, pert);
         } else if (act) {
             const int ri = wave - 1, i = ri < kb ? ri : ri + 1;
             const T *src = M + 16 * i * LDT + 16 * kb;
