@@ -1,8 +1,11 @@
 // Internal definitions of libgiraffe_amd (host side + kernel launch prototypes).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <string>
+#include <thread>
 #include <vector>
 #include "../../include/giraffe_amd.h"
 
@@ -31,6 +34,28 @@ void gf_set_error(const std::string &msg);
         int s_ = (x);               \
         if (s_ != GF_OK) return s_; \
     } while (0)
+
+// ---- host planning threads -------------------------------------------------
+// fn(worker, item) for item = 0 .. n - 1, items handed out in order to at most
+// 16 threads (worker = 0 .. workers - 1), or run inline when `work` (the
+// caller's cost estimate) is too small to pay for threads. The caller keeps the
+// result independent of which worker takes which item.
+int gf_plan_workers(long long work);
+template <class Fn> void gf_parallel_for(int n, int workers, Fn fn)
+{
+    if (workers <= 1 || n <= 1) {
+        for (int i = 0; i < n; i++) fn(0, i);
+        return;
+    }
+    std::atomic<int> next{0};
+    auto run = [&](int w) {
+        for (int i; (i = next.fetch_add(1)) < n;) fn(w, i);
+    };
+    std::vector<std::thread> th;
+    for (int w = 1; w < workers; w++) th.emplace_back(run, w);
+    run(0);
+    for (auto &t : th) t.join();
+}
 
 // ---- device memory cache (gf_pool.cpp) ---------------------------------------
 int gf_pool_alloc(void **out, size_t bytes);
@@ -286,6 +311,17 @@ int gf_nd_plan(gf_fronts &fp, const gf_model_desc *d, const std::vector<int> &do
 // The assembly tiles' lists of tangent entries from the tangent's structure (CSR
 // over band rows); after gf_nd_plan.
 int gf_nd_orig_lists(gf_fronts &fp, const std::vector<int> &ptr, const std::vector<int> &col, int npad);
+
+// gf_model_create in two halves, for callers that plan on one thread and touch
+// the device on another. gf_model_preplan: the host plan without the
+// multifrontal path's per-entry lists, and the device bytes the model will take
+// (what gf_model_footprint reports); no device call. gf_model_plan_lists: those
+// lists (host only). gf_model_create_planned: the device half (allocations,
+// uploads; the lists first if still missing); on failure the caller destroys
+// the model with gf_model_destroy, as it does a planned model it never creates.
+int gf_model_preplan(const gf_model_desc *d, const gf_options *opt, gf_model **out, unsigned long long *bytes);
+int gf_model_plan_lists(const gf_model_desc *d, gf_model *m);
+int gf_model_create_planned(const gf_model_desc *d, const gf_options *opt, gf_model *m);
 // Launch count of one factorisation and of one solve of a plan.
 int gf_front_factor_launches(const gf_fronts &fp);
 int gf_front_solve_launches(const gf_fronts &fp);
@@ -428,6 +464,10 @@ struct gf_model {
     // only (no fill), the factor lives in the fronts
     bool fronts_on = false;
     gf_fronts fronts;
+    // host plan made ahead of the device part (gf_model_preplan): the band
+    // layout's block size and count, and whether the per-entry lists exist
+    bool planned = false, lists_done = false;
+    int plan_bm = -1, plan_nblk = 0;
     double *d_front = nullptr, *d_front_e = nullptr, *d_front_v = nullptr;  // FP64 factor, vector workspace
     float *d_front32 = nullptr, *d_front32_e = nullptr;                      // FP32 factor (mixed)
     double *d_shell_rec = nullptr;   // per (Shell_1, Gauss point) phase-1 record [3 ns][276]

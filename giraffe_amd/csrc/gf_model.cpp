@@ -289,40 +289,70 @@ static std::vector<int> rcm_order(const gf_model_desc *d, int nn)
 
 // Band entries of the tangent's structure, row by row: for each free DOF row,
 // the free DOF columns of every node that shares an element with the row's node
-// (itself included), in increasing column order.
-static void tangent_structure(const gf_model_desc *d, const gf_model *m, std::vector<int> &ptr, std::vector<int> &col)
+// (itself included), in increasing column order. With count_only, ptr alone.
+static void tangent_structure(const gf_model_desc *d, const gf_model *m, std::vector<int> &ptr, std::vector<int> &col,
+                              bool count_only = false)
 {
-    const int nn = m->nn;
-    std::vector<std::vector<int>> adj(nn);
-    auto link = [&](const int *cn, int nper) {
-        for (int a = 0; a < nper; a++)
-            for (int b = 0; b < nper; b++) adj[cn[a]].push_back(cn[b]);
+    const int nn = m->nn, nb = m->nb, ne = m->nb + m->ns;
+    auto conn = [&](int e, int &nper) {
+        nper = e < nb ? 3 : 6;
+        return e < nb ? d->beam_conn + 3 * (size_t)e : d->shell_conn + 6 * (size_t)(e - nb);
     };
-    for (int e = 0; e < d->n_beams; e++) link(d->beam_conn + 3 * (size_t)e, 3);
-    for (int e = 0; e < d->n_shells; e++) link(d->shell_conn + 6 * (size_t)e, 6);
-    ptr.assign(1, 0);
-    col.clear();
-    std::vector<int> cols;
-    for (int r = 0; r < m->n_free; r++) {
-        const int a = m->row_nodecomp[r] / 6;
-        std::vector<int> &v = adj[a];
-        if (!v.empty() && (r == 0 || m->row_nodecomp[r - 1] / 6 != a)) {
-            std::sort(v.begin(), v.end());
-            v.erase(std::unique(v.begin(), v.end()), v.end());
-            cols.clear();
-            for (int b : v)
+    std::vector<int> eoff(nn + 1, 0);
+    for (int e = 0, np; e < ne; e++) {
+        const int *cn = conn(e, np);
+        for (int a = 0; a < np; a++) eoff[cn[a] + 1]++;
+    }
+    for (int v = 0; v < nn; v++) eoff[v + 1] += eoff[v];
+    std::vector<int> elist(eoff[nn]), at(eoff.begin(), eoff.end() - 1);
+    for (int e = 0, np; e < ne; e++) {
+        const int *cn = conn(e, np);
+        for (int a = 0; a < np; a++) elist[at[cn[a]]++] = e;
+    }
+    // columns of each node that holds a free DOF
+    const int W = gf_plan_workers(36LL * eoff[nn]);
+    std::vector<std::vector<int>> ncols(nn);
+    const int chunk = 256, nchunk = (nn + chunk - 1) / chunk;
+    gf_parallel_for(nchunk, W, [&](int, int x) {
+        std::vector<int> nbr;
+        for (int a = x * chunk; a < std::min(nn, (x + 1) * chunk); a++) {
+            bool has = false;
+            for (int c = 0; c < 6; c++) has |= m->dof[6 * (size_t)a + c] >= 0;
+            if (!has) continue;
+            nbr.clear();
+            for (int q = eoff[a], np; q < eoff[a + 1]; q++) {
+                const int *cn = conn(elist[q], np);
+                nbr.insert(nbr.end(), cn, cn + np);
+            }
+            std::sort(nbr.begin(), nbr.end());
+            nbr.erase(std::unique(nbr.begin(), nbr.end()), nbr.end());
+            std::vector<int> &cols = ncols[a];
+            for (int b : nbr)
                 for (int c = 0; c < 6; c++)
                     if (m->dof[6 * (size_t)b + c] >= 0) cols.push_back(m->dof[6 * (size_t)b + c]);
             std::sort(cols.begin(), cols.end());
         }
-        col.insert(col.end(), cols.begin(), cols.end());
-        ptr.push_back((int)col.size());
-    }
+    });
+    const int nf = m->n_free;
+    ptr.assign(nf + 1, 0);
+    for (int r = 0; r < nf; r++) ptr[r + 1] = ptr[r] + (int)ncols[m->row_nodecomp[r] / 6].size();
+    col.clear();
+    if (count_only) return;
+    col.resize(ptr[nf]);
+    const int rchunk = 1024;
+    gf_parallel_for((nf + rchunk - 1) / rchunk, W, [&](int, int x) {
+        for (int r = x * rchunk; r < std::min(nf, (x + 1) * rchunk); r++) {
+            const std::vector<int> &cols = ncols[m->row_nodecomp[r] / 6];
+            std::copy(cols.begin(), cols.end(), col.begin() + ptr[r]);
+        }
+    });
 }
 
 // Host-side planning shared by create() and gf_model_footprint: validation,
-// DOFsActive + SetGlobalDOFs, half-bandwidth, storage layout, colourings.
-static int plan(const gf_model_desc *d, gf_model *m, int &bm, int &nblk, int solver)
+// DOFsActive + SetGlobalDOFs, half-bandwidth, storage layout, colourings. Without
+// `lists` (sizes only), the multifrontal path counts the tangent's entries but
+// builds neither the column array nor the per-tile entry lists.
+static int plan(const gf_model_desc *d, gf_model *m, int &bm, int &nblk, int solver, bool lists = true)
 {
     if (solver < GF_SOLVER_AUTO || solver > GF_SOLVER_FRONTS) { gf_set_error("unknown solver"); return GF_E_ARG; }
     if (d->n_nodes <= 0 || !d->node_xyz) { gf_set_error("model has no nodes"); return GF_E_ARG; }
@@ -438,8 +468,9 @@ static int plan(const gf_model_desc *d, gf_model *m, int &bm, int &nblk, int sol
                           (m->fronts.flops < 0.5 * band_flops && gf_front_factor_launches(m->fronts) < npad / 64);
         if (take) {
             m->fronts_on = true;
-            tangent_structure(d, m, m->csr_ptr, m->csr_col);
-            GF_TRY(gf_nd_orig_lists(m->fronts, m->csr_ptr, m->csr_col, npad));
+            tangent_structure(d, m, m->csr_ptr, m->csr_col, !lists);
+            if (lists) GF_TRY(gf_nd_orig_lists(m->fronts, m->csr_ptr, m->csr_col, npad));
+            m->lists_done = lists;
             m->layout = GM_BAND;
             m->npad = npad;
             bm = -1;
@@ -476,8 +507,10 @@ static size_t footprint(const gf_model_desc *d, const gf_model *m, int precision
         const size_t factor = (size_t)(fp.f_elems + fp.e_elems);
         b += (npad * (size_t)(2 * m->half_band + 1) + factor + (size_t)fp.v_elems) * 8;
         b += (fp.rows.size() + fp.child.size() + fp.inv.size() + fp.items.size()) * 4 + fp.child_inv.size() * 8 +
-             fp.fr.size() * sizeof(gf_front_dev) + (m->csr_ptr.size() + m->csr_col.size()) * 4 +
-             (fp.orig.size() + fp.asm_orig.size()) * 4;
+             fp.fr.size() * sizeof(gf_front_dev);
+        // CSR structure and per-tile entry lists (3 ints per entry, 2 per assembly item)
+        const size_t nnz = m->csr_ptr.empty() ? 0 : (size_t)m->csr_ptr.back();
+        b += (m->csr_ptr.size() + nnz + 3 * nnz + fp.items.size() / 2) * 4;
         if (precision == GF_PREC_MIXED) b += factor * 4 + 3 * npad * 8;
         return b;
     } else {
@@ -492,7 +525,13 @@ static size_t footprint(const gf_model_desc *d, const gf_model *m, int precision
 static int create(const gf_model_desc *d, const gf_options *opt, gf_model *m)
 {
     int bm = -1, nblk = 0;
-    GF_TRY(plan(d, m, bm, nblk, m->solver));
+    if (m->planned) {
+        bm = m->plan_bm;
+        nblk = m->plan_nblk;
+        GF_TRY(gf_model_plan_lists(d, m));
+    } else {
+        GF_TRY(plan(d, m, bm, nblk, m->solver));
+    }
     const int nn = m->nn, nb = m->nb, ns = m->ns, nf = m->n_free;
     const int hb = m->half_band;
     // device + stream
@@ -820,7 +859,7 @@ extern "C" int gf_model_plan(const gf_model_desc *desc, long long *o)
     if (!desc || !o) return GF_E_ARG;
     gf_model tmp;
     int bm = -1, nblk = 0;
-    GF_TRY(plan(desc, &tmp, bm, nblk, GF_SOLVER_AUTO));
+    GF_TRY(plan(desc, &tmp, bm, nblk, GF_SOLVER_AUTO, false));
     o[0] = tmp.n_free;
     o[1] = tmp.half_band;
     o[2] = tmp.layout == GM_BTD ? bm : 0;
@@ -868,22 +907,60 @@ extern "C" int gf_model_footprint(const gf_model_desc *desc, const gf_options *o
     if (!desc || !bytes) return GF_E_ARG;
     gf_model tmp;
     int bm = -1, nblk = 0;
-    GF_TRY(plan(desc, &tmp, bm, nblk, opt ? opt->solver : GF_SOLVER_AUTO));
+    GF_TRY(plan(desc, &tmp, bm, nblk, opt ? opt->solver : GF_SOLVER_AUTO, false));
     *bytes = footprint(desc, &tmp, opt ? opt->precision : GF_PREC_FP64, bm, nblk);
     return GF_OK;
+}
+
+static gf_model *new_model(const gf_options *opt)
+{
+    gf_model *m = new gf_model();
+    if (opt) {
+        m->device = opt->device;
+        m->refine_max = opt->refine_max;
+        m->refine_tol = opt->refine_tol;
+        m->refine_reuse = opt->refine_reuse != 0;
+        m->solver = opt->solver;
+    }
+    return m;
+}
+
+int gf_model_preplan(const gf_model_desc *d, const gf_options *opt, gf_model **out, unsigned long long *bytes)
+{
+    if (!d || !out || !bytes) { gf_set_error("null argument"); return GF_E_ARG; }
+    *out = nullptr;
+    gf_model *m = new_model(opt);
+    int st = plan(d, m, m->plan_bm, m->plan_nblk, m->solver, false);
+    if (st != GF_OK) {
+        delete m;
+        return st;
+    }
+    m->planned = true;
+    *bytes = footprint(d, m, opt ? opt->precision : GF_PREC_FP64, m->plan_bm, m->plan_nblk);
+    *out = m;
+    return GF_OK;
+}
+
+int gf_model_plan_lists(const gf_model_desc *d, gf_model *m)
+{
+    if (!m->fronts_on || m->lists_done) return GF_OK;
+    tangent_structure(d, m, m->csr_ptr, m->csr_col);
+    GF_TRY(gf_nd_orig_lists(m->fronts, m->csr_ptr, m->csr_col, m->npad));
+    m->lists_done = true;
+    return GF_OK;
+}
+
+int gf_model_create_planned(const gf_model_desc *d, const gf_options *opt, gf_model *m)
+{
+    if (!d || !m || !m->planned) { gf_set_error("model not planned"); return GF_E_ARG; }
+    return create(d, opt, m);
 }
 
 extern "C" int gf_model_create(const gf_model_desc *desc, const gf_options *opt, gf_model **out)
 {
     if (!desc || !out) { gf_set_error("null argument"); return GF_E_ARG; }
     *out = nullptr;
-    gf_model *m = new gf_model();
-    if (opt) {
-        m->refine_max = opt->refine_max;
-        m->refine_tol = opt->refine_tol;
-        m->refine_reuse = opt->refine_reuse != 0;
-        m->solver = opt->solver;
-    }
+    gf_model *m = new_model(opt);
     int st = create(desc, opt, m);
     if (st != GF_OK) {
         release(m);

@@ -384,52 +384,76 @@ int gf_front_solve_launches(const gf_fronts &fp)
 // the entry (a, b) and, when b is an update DOF of the front, (b, a); entries whose
 // other DOF is in no row of the front belong to a descendant's front.  Per
 // assembly item: the first entry of its tile and their count.
+int gf_plan_workers(long long work)
+{
+    if (work < (1LL << 20)) return 1;
+    const unsigned hc = std::thread::hardware_concurrency();
+    return (int)std::max(1u, std::min(hc ? hc : 1u, 16u));
+}
+
 int gf_nd_orig_lists(gf_fronts &fp, const std::vector<int> &ptr, const std::vector<int> &col, int npad)
 {
-    std::vector<int> ppos(npad, -1);
+    const int S = fp.n_front;
     fp.orig.clear();
     fp.asm_orig.assign(fp.items.size() / 2, 0);  // 2 ints per item
-    std::vector<long long> tile_off;
-    std::vector<int> cnt;
-    std::vector<long long> front_base(fp.n_front + 1, 0);
-    std::vector<std::vector<long long>> toff(fp.n_front);
-    for (int f = 0; f < fp.n_front; f++) {
+    // largest fronts first (dynamic hand-out); every front's entries land at
+    // offsets fixed by the count pass, so the lists do not depend on the threads
+    std::vector<int> by_size(S);
+    for (int f = 0; f < S; f++) by_size[f] = f;
+    std::stable_sort(by_size.begin(), by_size.end(), [&](int a, int b) {
+        return (long long)fp.fr[a].k * fp.fr[a].mp > (long long)fp.fr[b].k * fp.fr[b].mp;
+    });
+    const int W = gf_plan_workers((long long)col.size());
+    std::vector<std::vector<int>> ppos(W);
+    std::vector<std::vector<long long>> toff(S);
+    // the (front position, front position, band row, band column) of every
+    // tangent entry whose earlier-eliminated index is one of the front's own rows
+    auto each = [&](int w, int f, auto emit) {
         const gf_front_dev &F = fp.fr[f];
         const int *rows = fp.rows.data() + F.rows;
+        std::vector<int> &pp = ppos[w];
+        if (pp.empty()) pp.assign(npad, -1);
         for (int i = 0; i < F.mp; i++)
-            if (rows[i] >= 0) ppos[rows[i]] = i;
-        const int nt = F.mb * F.mb;
-        cnt.assign(nt, 0);
-        auto each = [&](auto emit) {
-            for (int i = 0; i < F.k; i++) {
-                const int oa = rows[i];
-                for (int q = ptr[oa]; q < ptr[oa + 1]; q++) {
-                    const int ob = col[q], j = ppos[ob];
-                    if (j < 0) continue;
-                    emit(i, j, oa, ob);
-                    if (j >= F.k) emit(j, i, ob, oa);
-                }
+            if (rows[i] >= 0) pp[rows[i]] = i;
+        for (int i = 0; i < F.k; i++) {
+            const int oa = rows[i];
+            for (int q = ptr[oa]; q < ptr[oa + 1]; q++) {
+                const int ob = col[q], j = pp[ob];
+                if (j < 0) continue;
+                emit(i, j, oa, ob);
+                if (j >= F.k) emit(j, i, ob, oa);
             }
-        };
-        each([&](int i, int j, int, int) { cnt[(i / TB) * F.mb + j / TB]++; });
-        std::vector<long long> &to = toff[f];
-        to.assign(nt + 1, 0);
-        for (int t = 0; t < nt; t++) to[t + 1] = to[t] + cnt[t];
-        const long long base = (long long)fp.orig.size() / 3;
-        fp.orig.resize(fp.orig.size() + 3 * (size_t)to[nt]);
-        std::vector<long long> fill(to.begin(), to.end() - 1);
-        each([&](int i, int j, int oa, int ob) {
-            const int t = (i / TB) * F.mb + j / TB;
-            const size_t e = 3 * (size_t)(base + fill[t]++);
-            fp.orig[e] = (i % TB) * TB + j % TB;
-            fp.orig[e + 1] = oa;
-            fp.orig[e + 2] = ob;
-        });
-        front_base[f] = base;
+        }
         for (int i = 0; i < F.mp; i++)
-            if (rows[i] >= 0) ppos[rows[i]] = -1;
-    }
-    if (fp.orig.size() / 3 > (size_t)0x7fffffff) { gf_set_error("multifrontal plan: too many tangent entries"); return GF_E_UNSUPPORTED; }
+            if (rows[i] >= 0) pp[rows[i]] = -1;
+    };
+    gf_parallel_for(S, W, [&](int w, int x) {
+        const int f = by_size[x];
+        const int mb = fp.fr[f].mb;
+        std::vector<long long> &to = toff[f];
+        to.assign((size_t)mb * mb + 1, 0);
+        each(w, f, [&](int i, int j, int, int) { to[(i / TB) * mb + j / TB + 1]++; });
+        for (int t = 0; t < mb * mb; t++) to[t + 1] += to[t];
+    });
+    std::vector<long long> front_base(S + 1, 0);
+    for (int f = 0; f < S; f++) front_base[f + 1] = front_base[f] + toff[f].back();
+    // every structural entry is counted once, by the front that eliminates the
+    // earlier of its row and column
+    if (front_base[S] != (long long)col.size()) { gf_set_error("multifrontal plan: tangent entries outside the fronts"); return GF_E_STATE; }
+    if (front_base[S] > (long long)0x7fffffff) { gf_set_error("multifrontal plan: too many tangent entries"); return GF_E_UNSUPPORTED; }
+    fp.orig.resize(3 * (size_t)front_base[S]);
+    gf_parallel_for(S, W, [&](int w, int x) {
+        const int f = by_size[x];
+        const int mb = fp.fr[f].mb;
+        std::vector<long long> fill(toff[f].begin(), toff[f].end() - 1);
+        int *o = fp.orig.data() + 3 * (size_t)front_base[f];
+        each(w, f, [&](int i, int j, int oa, int ob) {
+            int *e = o + 3 * (size_t)fill[(i / TB) * mb + j / TB]++;
+            e[0] = (i % TB) * TB + j % TB;
+            e[1] = oa;
+            e[2] = ob;
+        });
+    });
     for (int l = 0; l < fp.n_level; l++)
         for (int it = fp.asm_first[l]; it < fp.asm_first[l] + fp.asm_count[l]; it++) {
             const int f = fp.items[4 * (size_t)it], ti = fp.items[4 * (size_t)it + 1], tj = fp.items[4 * (size_t)it + 2];

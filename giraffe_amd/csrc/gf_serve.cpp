@@ -13,14 +13,25 @@
 // granularity).  Finished requests release their memory and stream at once
 // and the next queued request is admitted in the same pump.
 //
-// Streams are created at the default priority.  One host thread drives a server.
+// Host planning stays off the thread that drives the device: submit makes the
+// plan's sizes (gf_model_preplan, which decides admission), and a planner
+// thread builds the multifrontal per-entry lists of the first PLAN_AHEAD queued
+// requests while the driving thread keeps the resident requests going; the
+// queue head is admitted once its plan is complete, and admission then only
+// allocates and uploads.
+//
+// Streams are created at the default priority.  One host thread drives a server
+// (submit, pump, take, counts); the planner thread is the server's own.
 #include "gf_internal.h"
 
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <thread>
 
 namespace {
@@ -91,12 +102,20 @@ struct DescCopy {
     }
 };
 
+// Lists of at most this many queued requests are built ahead of admission.
+constexpr int PLAN_AHEAD = 4;
+
+enum { PLAN_SIZES = 0, PLAN_BUSY = 1, PLAN_DONE = 2 };
+
 struct Request {
     long long id = 0;
     DescCopy desc;
     gf_static_params params{};
     unsigned long long bytes = 0;
-    gf_model *model = nullptr;
+    gf_model *model = nullptr;  // planned at submit, created at admission
+    std::atomic<int> plan_state{PLAN_SIZES};
+    int plan_status = GF_OK;    // of the planner's list build
+    std::string plan_error;
     int stream_slot = -1;
     gf_static_run run;
     std::vector<int> step_iters;
@@ -112,6 +131,10 @@ struct gf_server {
     gf_server_options opt{};
     std::vector<hipStream_t> streams;
     std::vector<int> free_slots;
+    std::mutex mu;               // guards queue (the planner reads it)
+    std::condition_variable cv;  // planner wake-up: a new request, a queue move, stop
+    bool stop = false;
+    std::thread planner;
     std::deque<std::unique_ptr<Request>> queue;
     std::vector<std::unique_ptr<Request>> inflight;
     std::map<long long, std::unique_ptr<Request>> done;
@@ -130,6 +153,35 @@ extern "C" int gf_default_server_options(gf_server_options *o)
     return GF_OK;
 }
 
+// Planner thread: builds the lists of the first queued requests, in queue order.
+static void planner_main(gf_server *s)
+{
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        Request *next = nullptr;
+        s->cv.wait(lk, [&] {
+            if (s->stop) return true;
+            const size_t n = std::min(s->queue.size(), (size_t)PLAN_AHEAD);
+            for (size_t i = 0; i < n; i++)
+                if (s->queue[i]->plan_state.load() == PLAN_SIZES) {
+                    next = s->queue[i].get();
+                    return true;
+                }
+            return false;
+        });
+        if (s->stop) return;
+        // the driving thread admits a request only once it is PLAN_DONE, so
+        // `next` stays in the queue (and alive) until then
+        next->plan_state.store(PLAN_BUSY);
+        lk.unlock();
+        const int st = gf_model_plan_lists(&next->desc.d, next->model);
+        if (st != GF_OK) next->plan_error = gf_last_error();
+        next->plan_status = st;
+        lk.lock();
+        next->plan_state.store(PLAN_DONE);
+    }
+}
+
 extern "C" int gf_server_create(const gf_server_options *opt, gf_server **out)
 {
     if (!opt || !out || opt->max_inflight < 1 || opt->max_inflight > 256) return GF_E_ARG;
@@ -141,6 +193,7 @@ extern "C" int gf_server_create(const gf_server_options *opt, gf_server **out)
         GF_HIP(hipStreamCreateWithFlags(&s->streams[i], hipStreamNonBlocking));
         s->free_slots.push_back(opt->max_inflight - 1 - i);
     }
+    s->planner = std::thread(planner_main, s.get());
     *out = s.release();
     return GF_OK;
 }
@@ -148,7 +201,14 @@ extern "C" int gf_server_create(const gf_server_options *opt, gf_server **out)
 extern "C" int gf_server_destroy(gf_server *s)
 {
     if (!s) return GF_E_ARG;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv.notify_all();
+    s->planner.join();
     (void)hipSetDevice(s->opt.device);
+    for (auto &r : s->queue) gf_model_destroy(r->model);
     for (auto &r : s->inflight) {
         if (r->model) {
             (void)hipStreamSynchronize(s->streams[r->stream_slot]);
@@ -171,8 +231,9 @@ extern "C" int gf_server_submit(gf_server *s, const gf_model_desc *d, const gf_s
     gf_default_options(&o);
     o.precision = s->opt.precision;
     o.device = s->opt.device;
-    GF_TRY(gf_model_footprint(&r->desc.d, &o, &r->bytes));
+    GF_TRY(gf_model_preplan(&r->desc.d, &o, &r->model, &r->bytes));
     if (r->bytes > s->opt.mem_cap) {
+        gf_model_destroy(r->model);
         gf_set_error("request needs more device memory than the server's cap");
         return GF_E_UNSUPPORTED;
     }
@@ -180,7 +241,11 @@ extern "C" int gf_server_submit(gf_server *s, const gf_model_desc *d, const gf_s
     r->step_iters.assign(4096, 0);
     r->t_submit = clk::now();
     *id_out = r->id;
-    s->queue.push_back(std::move(r));
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->queue.push_back(std::move(r));
+    }
+    s->cv.notify_one();
     return GF_OK;
 }
 
@@ -195,9 +260,13 @@ static int admit(gf_server *s, Request &r)
     s->free_slots.pop_back();
     o.stream = (void *)s->streams[r.stream_slot];
     r.t_admit = clk::now();
-    int st = gf_model_create(&r.desc.d, &o, &r.model);
+    s->mem_in_use += r.bytes;  // finish() gives it back, created or not
+    if (r.plan_status != GF_OK) {
+        gf_set_error(r.plan_error);
+        return r.plan_status;
+    }
+    int st = gf_model_create_planned(&r.desc.d, &o, r.model);
     if (st) return st;
-    s->mem_in_use += r.bytes;
     st = r.run.start(r.model, r.params, r.step_iters.data(), (int)r.step_iters.size());
     if (st) return st;
     st = r.run.advance();
@@ -231,11 +300,19 @@ extern "C" int gf_server_pump(gf_server *s, int wait_us)
     int finished = 0;
     auto deadline = clk::now() + std::chrono::microseconds(wait_us > 0 ? wait_us : 0);
     for (;;) {
-        // admission: FIFO head while slots and memory allow
-        while (!s->queue.empty() && (int)s->inflight.size() < s->opt.max_inflight &&
-               s->mem_in_use + s->queue.front()->bytes <= s->opt.mem_cap) {
-            std::unique_ptr<Request> r = std::move(s->queue.front());
-            s->queue.pop_front();
+        // admission: FIFO head while slots and memory allow and its plan is complete
+        for (;;) {
+            std::unique_ptr<Request> r;
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                if (s->queue.empty() || (int)s->inflight.size() >= s->opt.max_inflight ||
+                    s->mem_in_use + s->queue.front()->bytes > s->opt.mem_cap ||
+                    s->queue.front()->plan_state.load() != PLAN_DONE)
+                    break;
+                r = std::move(s->queue.front());
+                s->queue.pop_front();
+            }
+            s->cv.notify_one();
             int st = admit(s, *r);
             if (st) {
                 finish(s, r, st);
