@@ -66,7 +66,7 @@ __device__ __forceinline__ double child_sum(const FrontTab &t, const gf_front_de
 // inverse's scratch.
 template <typename T> struct AsmSmem {
     T P[NB * LDT];
-    T tmp[3 * 16 * TMP_LD];
+    T tmp[INV_TMP];
     T aux[4 + NB];
 };
 
@@ -150,7 +150,7 @@ template <typename T> struct FStepSmem {
     T X[NB * LDT];
     union {
         T C[NB * SA_LD];
-        T tmp[3 * 16 * TMP_LD];
+        T tmp[INV_TMP];
     };
     T aux[4 + NB];
 };

@@ -149,13 +149,13 @@ template <typename T> struct PivotGuard {
 // moves only: no LDS round trip on the pivot chain.
 template <int CTRL> __device__ __forceinline__ double dpp_mov(double v)
 {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
     return __hiloint2double(hi, lo);
 }
 template <int CTRL> __device__ __forceinline__ float dpp_mov(float v)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
 }
 template <int R> __device__ __forceinline__ unsigned rowcast_u32(unsigned v)
 {
@@ -174,57 +174,65 @@ template <int R> __device__ __forceinline__ float rowcast(float v)
     return __uint_as_float(rowcast_u32<R>(__float_as_uint(v)));
 }
 // Pivot K of the in-place Gauss-Jordan inverse of a 16 x 16 block held by one
-// wavefront: lane 16 q + i holds row i, columns 4 q .. 4 q + 3 (a[0..3]).  The
-// pivot reaches every lane by two register moves (column K of the lane's row
-// from row block K / 4, then lane K of each DPP row), so the chain from one
-// pivot to the next is VALU only: no readlane, no scalar selects.  A pivot
-// below eps_K = rel dg[K] (rel mx where the original diagonal dg[K] is zero)
-// takes the reciprocal of +-eps_K, a non-finite one the reciprocal 1; the
-// threshold is read from LDS per pivot, off the dependent chain, rather than
-// held in registers for all sixteen.
+// wavefront: lane 16 q + i holds row i, columns 4 q .. 4 q + 3 (a[0..3]).  Rows
+// are kept scaled: the true row i is s_i times the stored one, s_i = 1 until
+// row i has been the pivot row and the pivot's reciprocal after, so the pivot
+// row itself is left as it is (its diagonal entry set to 1) and every other
+// row takes one FMA per entry, a(i, j) -= (a(i, K) / p) a(K, j); the scales
+// are applied once at the end.  Row K is unscaled when it is the pivot row, so
+// its entries and the pivot are true values.  The pivot reaches every lane by
+// two register moves (column K of the lane's row from row block K / 4, then
+// lane K of each DPP row): no readlane, no LDS access on the pivot chain.  A
+// pivot below a threshold eps_K (|p| / eps_K < 1; ieps[K] = 1 / eps_K in LDS,
+// read per pivot off the dependent chain) takes the reciprocal of +-eps_K, a
+// non-finite one the reciprocal 1.
 template <typename T, int K>
-__device__ __forceinline__ void inv16_pivot(T (&a)[4], int i, int q, const T *dg, T mx, T rel, T huge, bool &bad,
+__device__ __forceinline__ void inv16_pivot(T (&a)[4], T &sc, int i, int q, const T *ieps, T huge, bool &bad,
                                             bool &pert)
 {
     constexpr int KQ = K >> 2, KE = K & 3;
+    const T ie = ieps[K];
     T rk[4];
 #pragma unroll
     for (int e = 0; e < 4; e++) rk[e] = dpp_mov<0x150 + K>(a[e]);  // row K, this lane's columns
     const T f = rowcast<KQ>(a[KE]);                                  // row i, column K
     const T p = dpp_mov<0x150 + K>(f);                               // the pivot
+    const T r0 = rcp_of<T>(p);  // replaced below when p is tiny or not finite
     const T ap = fabs(p);
     const bool nonfinite = !(ap < huge);
-    const T dk = dg[K], epsk = rel * (dk > T(0) ? dk : mx);
-    const bool small = !nonfinite && ap < epsk;
+    const bool small = !nonfinite && ap * ie < T(1);
     bad |= nonfinite;
     pert |= small;
-    // one reciprocal of the selected value (rcp(-x) = -rcp(x) exactly, rcp(1) = 1)
-    const T ip = rcp_of<T>(nonfinite ? T(1) : small ? (p >= T(0) ? epsk : -epsk) : p);
+    // rcp(-x) = -rcp(x) exactly, rcp(1) = 1
+    const T ip = nonfinite ? T(1) : small ? (p >= T(0) ? ie : -ie) : r0;
     const bool isk = i == K;
+    const T h = isk ? T(0) : f * ip;  // row K keeps its entries
 #pragma unroll
     for (int e = 0; e < 4; e++) {
-        const T r = rk[e] * ip;
-        const T upd = isk ? r : a[e] - f * r;
-        a[e] = (4 * q + e == K) ? (isk ? ip : -f * ip) : upd;
+        const T upd = fma(-h, rk[e], a[e]);
+        if (e == KE)
+            a[e] = q == KQ ? (isk ? T(1) : -h) : upd;  // column K: -a(i, K) / p; s_K / s_K on the diagonal
+        else
+            a[e] = upd;
     }
-    if constexpr (K + 1 < 16) inv16_pivot<T, K + 1>(a, i, q, dg, mx, rel, huge, bad, pert);
+    sc = isk ? ip : sc;
+    if constexpr (K + 1 < 16) inv16_pivot<T, K + 1>(a, sc, i, q, ieps, huge, bad, pert);
 }
 
 // One wavefront: in-place Gauss-Jordan inverse (no interchanges) of the 16 x 16
 // block at b (stride LDT); the 16 pivots run in registers with DPP and
-// permlane moves, no barrier and no LDS access on the pivot chain.  dg: the
-// sixteen original diagonal magnitudes (LDS), mx: the block's largest
-// magnitude, rel: the relative perturbation size.
+// permlane moves, no barrier and no LDS access on the pivot chain.  ieps: the
+// reciprocals of the sixteen pivot thresholds (LDS).
 template <typename T>
-__device__ __forceinline__ void inv16(T *b, const T *dg, T mx, T rel, T huge, bool &bad, bool &pert)
+__device__ __forceinline__ void inv16(T *b, const T *ieps, T huge, bool &bad, bool &pert)
 {
     const int lane = threadIdx.x & 63, i = lane & 15, q = lane >> 4;
-    T a[4];
+    T a[4], sc = T(1);
 #pragma unroll
     for (int e = 0; e < 4; e++) a[e] = b[i * LDT + 4 * q + e];
-    inv16_pivot<T, 0>(a, i, q, dg, mx, rel, huge, bad, pert);
+    inv16_pivot<T, 0>(a, sc, i, q, ieps, huge, bad, pert);
 #pragma unroll
-    for (int e = 0; e < 4; e++) b[i * LDT + 4 * q + e] = a[e];
+    for (int e = 0; e < 4; e++) b[i * LDT + 4 * q + e] = sc * a[e];
 }
 
 // acc = P Q for 16 x 16 blocks in LDS (strides lp, lq); one wavefront.
@@ -239,20 +247,25 @@ __device__ __forceinline__ typename mfma<T>::acc_t mm16(const T *P, int lp, cons
     return acc;
 }
 
+// Scratch of inverse64: two sets of three saved 16 x 16 column blocks.
+constexpr int INV_TMP = 2 * 3 * 16 * TMP_LD;
+
 // In-place inverse of the 64 x 64 matrix M ([64][LDT]) by 256 threads: block
-// Gauss-Jordan over 16 x 16 blocks.  For each diagonal block kb: wave 0 inverts
-// the pivot block P (inv16) while waves 1-3 save the column blocks C_i; waves
+// Gauss-Jordan over 16 x 16 blocks.  Step kb, with the pivot block P already
+// inverted in place and the column blocks C_i = M(i, kb) (i != kb) saved: waves
 // 0-2 form the row blocks R_j = P^-1 M(kb, j); then the four waves update
-// M(i, j) -= C_i R_j and set M(i, kb) = -C_i P^-1 (MFMA 16x16x4), three
-// barriers per block.  No interchanges: a pivot smaller than 1e-13 (FP32:
-// 1e-6) of its row's original diagonal entry (of the block's largest entry
-// where that is zero) -- a threshold that does not depend on how rows and
+// M(i, j) -= C_i R_j and set M(i, kb) = -C_i P^-1 (MFMA 16x16x4).  The next
+// pivot block is updated first, by wave 0, which then inverts it (inv16) while
+// waves 1-3 finish the step's other blocks and save the next step's column
+// blocks: two barriers per step.  No interchanges: a pivot smaller than 1e-13
+// (FP32: 1e-6) of its row's original diagonal entry (of the block's largest
+// entry where that is zero) -- a threshold that does not depend on how rows and
 // columns are scaled -- is replaced by that size with its sign (bit 2 of
 // *flag); a non-finite pivot or an all-zero block sets bit 1.  (An FP32 factor
 // with perturbed pivots is still refined against the FP64 tangent, which
-// decides whether it was good enough.)  tmp: 3 x 16 x TMP_LD entries; aux:
-// 4 + 64 entries.  Runs on the first four wavefronts; those of a larger
-// workgroup only take part in the barriers.
+// decides whether it was good enough.)  tmp: INV_TMP entries; aux: 4 + 64
+// entries.  Runs on the first four wavefronts; those of a larger workgroup only
+// take part in the barriers.
 template <typename T> __device__ __forceinline__ void inverse64(T *M, T *tmp, T *aux, int *flag)
 {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -272,16 +285,30 @@ template <typename T> __device__ __forceinline__ void inverse64(T *M, T *tmp, T 
     __syncthreads();
     const T mx = fmax(fmax(aux[0], aux[1]), fmax(aux[2], aux[3]));
     bool bad = !(mx > T(0)) || !(mx < pg.huge), pert = false;
-    for (int kb = 0; kb < 4; kb++) {
-        T *P = M + 16 * kb * LDT + 16 * kb;
-        if (wave == 0) {
-            inv16<T>(P, aux + 4 + 16 * kb, mx, pg.rel, pg.huge, bad, pert);
-        } else if (act) {
-            const int ri = wave - 1, i = ri < kb ? ri : ri + 1;
-            const T *src = M + 16 * i * LDT + 16 * kb;
-            for (int t = lane; t < 256; t += 64) tmp[ri * 16 * TMP_LD + (t >> 4) * TMP_LD + (t & 15)] = src[(t >> 4) * LDT + (t & 15)];
+    // wave 0: pivot thresholds of block kb (their reciprocals replace the
+    // original diagonal in aux; written and read by this wave only), then the
+    // inverse of the block in place
+    auto pivot_block = [&](int kb) {
+        T *ie = aux + 4 + 16 * kb;
+        if (lane < 16) {
+            const T dk = ie[lane];
+            ie[lane] = rcp_of<T>(pg.rel * (dk > T(0) ? dk : mx));
         }
-        __syncthreads();
+        inv16<T>(M + 16 * kb * LDT + 16 * kb, ie, pg.huge, bad, pert);
+    };
+    constexpr int CS = 16 * TMP_LD, CB = 3 * CS;  // one saved block, one set
+    // step 0's pivot block and column blocks
+    if (wave == 0) {
+        pivot_block(0);
+    } else if (act) {
+        const T *src = M + 16 * wave * LDT;
+        for (int t = lane; t < 256; t += 64) tmp[(wave - 1) * CS + (t >> 4) * TMP_LD + (t & 15)] = src[(t >> 4) * LDT + (t & 15)];
+    }
+    __syncthreads();
+    for (int kb = 0; kb < 4; kb++) {
+        const T *cur = tmp + (kb & 1) * CB;
+        T *nxt = tmp + ((kb + 1) & 1) * CB;
+        T *P = M + 16 * kb * LDT + 16 * kb;
         if (wave < 3) {
             T *R = M + 16 * kb * LDT + 16 * (wave < kb ? wave : wave + 1);
             const typename mfma<T>::acc_t acc = mm16<T>(P, LDT, R, LDT);
@@ -289,19 +316,42 @@ template <typename T> __device__ __forceinline__ void inverse64(T *M, T *tmp, T 
             for (int v = 0; v < 4; v++) R[mfma<T>::row(lane, v) * LDT + (lane & 15)] = acc[v];
         }
         __syncthreads();
-#pragma unroll
-        for (int s = 0; s < 3 && act; s++) {
-            const int u = wave + 4 * s, ri = u >> 2, cj = u & 3;
-            const int i = ri < kb ? ri : ri + 1;
-            T *Mij = M + 16 * i * LDT + 16 * cj;
-            const T *Ci = tmp + ri * 16 * TMP_LD;
-            const bool piv = cj == kb;
-            const typename mfma<T>::acc_t acc = mm16<T>(Ci, TMP_LD, piv ? P : M + 16 * kb * LDT + 16 * cj, LDT);
+        // the twelve block updates u = 4 ri + j (row block i = ri, skipping kb);
+        // with a next step, wave 0 takes the next pivot block (u0) and the other
+        // eleven go to waves 1-3 in turn, else three to each wave
+        const int nb = kb + 1, u0 = 5 * kb + 1;
+        auto update = [&](int u) {
+            const int ri = u >> 2, j = u & 3, i = ri < kb ? ri : ri + 1;
+            T *Mij = M + 16 * i * LDT + 16 * j;
+            const T *Ci = cur + ri * CS;
+            const bool piv = j == kb;
+            const typename mfma<T>::acc_t acc = mm16<T>(Ci, TMP_LD, piv ? P : M + 16 * kb * LDT + 16 * j, LDT);
+            // the next step's column block of row block i (i != nb)
+            T *sv = (nb < 4 && j == nb && i != nb) ? nxt + (i < nb ? i : i - 1) * CS : nullptr;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                T &e = Mij[mfma<T>::row(lane, v) * LDT + (lane & 15)];
-                e = piv ? -acc[v] : e - acc[v];
+                const int r = mfma<T>::row(lane, v), c = lane & 15;
+                T &e = Mij[r * LDT + c];
+                const T x = piv ? -acc[v] : e - acc[v];
+                e = x;
+                if (sv) sv[r * TMP_LD + c] = x;
             }
+        };
+        if (nb < 4) {
+            if (wave == 0) {
+                update(u0);
+                pivot_block(nb);
+            } else if (act) {
+                for (int k = wave - 1; k < 11; k += 3) update(k < u0 ? k : k + 1);
+                if (wave == 3) {
+                    // row block kb of the next column blocks: R_nb, final since the products above
+                    const T *src = M + 16 * kb * LDT + 16 * nb;
+                    T *dst = nxt + kb * CS;
+                    for (int t = lane; t < 256; t += 64) dst[(t >> 4) * TMP_LD + (t & 15)] = src[(t >> 4) * LDT + (t & 15)];
+                }
+            }
+        } else if (act) {
+            for (int u = wave; u < 12; u += 4) update(u);
         }
         __syncthreads();
     }
@@ -338,7 +388,7 @@ template <typename T> struct StepSmem {
     T P[NB * LDT];
     union {
         T Q[NB * LDQ];
-        T tmp[3 * 16 * TMP_LD];
+        T tmp[INV_TMP];
     };
     T aux[4 + NB];
 };
