@@ -318,7 +318,8 @@ __device__ __forceinline__ void shell_gp_record(const double *__restrict__ cst, 
     st3(o + L_G + 36, madd(Ga1, Ga2));
 }
 
-__global__ __launch_bounds__(SWG) void shell_element_kernel(
+template <bool PRE>
+__global__ __launch_bounds__(SWG, PRE ? 3 : 1) void shell_element_kernel(
     int e0, int e1, size_t ld, const int *__restrict__ conn, const double *__restrict__ cst,
     const double *__restrict__ gps, const double *__restrict__ disp, double *__restrict__ Kout,
     double *__restrict__ fout, const double *__restrict__ pre)
@@ -328,7 +329,7 @@ __global__ __launch_bounds__(SWG) void shell_element_kernel(
     const int ebase = e0 + blockIdx.x * SEPB;
 
     // ---- phase 1 (or its records from shell_record_kernel) ----------------------
-    if (pre) {
+    if constexpr (PRE) {
         const int nv = min(SEPB, e1 - ebase);
         const double *src = pre + (size_t)ebase * 3 * L_M;
         for (int t = tid; t < nv * 3 * L_M; t += SWG) sm[(t / L_M) * L_GP + t % L_M] = src[t];
@@ -807,7 +808,7 @@ int gf_shell_assemble(gf_model *m, const double *lfp, const double *tp)
     // every element's K and f (phases 2-4 from the records), then one scatter launch per colour
     {
         const int pr = gf_probe_begin(m->stream, GF_KC_SHELL_ASSEMBLE);
-        hipLaunchKernelGGL(shell_element_kernel, dim3((m->ns + SEPB - 1) / SEPB), dim3(SWG), 0, m->stream, 0,
+        hipLaunchKernelGGL(shell_element_kernel<true>, dim3((m->ns + SEPB - 1) / SEPB), dim3(SWG), 0, m->stream, 0,
                            m->ns, ld, m->d_sconn, m->d_sconst, m->d_sgp, m->d_disp, m->d_shell_K, m->d_shell_f,
                            (const double *)m->d_shell_rec);
         gf_probe_end(m->stream, pr, m->ns);
@@ -874,7 +875,7 @@ int gf_shell_eval_device(int count, const double *x, const double *prop, const d
     GF_HIP(hipMemcpyAsync(d_cst, cst.data(), sizeof(double) * cst.size(), hipMemcpyHostToDevice, s));
     GF_HIP(hipMemcpyAsync(d_gps, gps.data(), sizeof(double) * gps.size(), hipMemcpyHostToDevice, s));
     GF_HIP(hipMemcpyAsync(d_disp, disp.data(), sizeof(double) * disp.size(), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(shell_element_kernel, dim3((count + SEPB - 1) / SEPB), dim3(SWG), 0, s, 0, count, ld,
+    hipLaunchKernelGGL(shell_element_kernel<false>, dim3((count + SEPB - 1) / SEPB), dim3(SWG), 0, s, 0, count, ld,
                        d_conn, d_cst, d_gps, d_disp, d_K, d_f, (const double *)nullptr);
     GF_HIP(hipGetLastError());
     GF_HIP(hipMemcpyAsync(K_out, d_K, sizeof(double) * 729 * ld, hipMemcpyDeviceToHost, s));
