@@ -215,16 +215,18 @@ __device__ __forceinline__ void dof_block(const int *__restrict__ conn, size_t l
 
 // Phase 1 of Shell_1::Mount for one (element, Gauss point): kinematics, the
 // 3-point thickness integration, and the record o[0 .. L_M) (D, Psi, G blocks
-// and Psi^T sigma) that the later phases read.
+// and Psi^T sigma) that the later phases read.  The D blocks, accumulated over
+// the thickness points, go to oD[0 .. 144) (o + L_D, or scratch of their own
+// that the caller copies into the record); the rest is written once, into o.
 __device__ __forceinline__ void shell_gp_record(const double *__restrict__ cst, const double *__restrict__ gps,
                                                 size_t ld, int e, int gp, const double *__restrict__ disp,
-                                                const int *__restrict__ conn, double *o)
+                                                const int *__restrict__ conn, double *oD, double *o)
 {
     ShellKin k;
     shell_kin(cst, gps, ld, e, gp, disp, conn, k);
     const double mu = cst[SC_MU * ld + e], la = cst[SC_LAMBDA * ld + e], th = cst[SC_THICK * ld + e];
     const double drill = cst[SC_DRILL * ld + e];
-    for (int i = 0; i < 144; i++) o[L_D + i] = 0.0;
+    for (int i = 0; i < 144; i++) oD[i] = 0.0;
     v3<double> n1 = mk3(0.0, 0.0, 0.0), n2 = n1, m1 = n1, m2 = n1;
     const double xi[3] = {-0.77459666924148337703585307995648, 0.0, 0.77459666924148337703585307995648};
     const double wt[3] = {0.55555555555555555555555555555556, 0.88888888888888888888888888888889,
@@ -254,22 +256,22 @@ __device__ __forceinline__ void shell_gp_record(const double *__restrict__ cst, 
         C12.a[4] = -mu * ((1.0 + g11) * g21 * dv);
         C21 = tr3(C12);
         double w = wt[q] * J;
-        acc3(o + L_D + 9 * 0, w, C11);            // dn1/deta1
-        acc3(o + L_D + 9 * 1, -w * zeta, mE3(C11)); // dn1/dk1
-        acc3(o + L_D + 9 * 2, w, C12);            // dn1/deta2
-        acc3(o + L_D + 9 * 3, -w * zeta, mE3(C12)); // dn1/dk2
-        acc3(o + L_D + 9 * 4, w * zeta, E3m(C11));  // dm1/deta1
-        acc3(o + L_D + 9 * 5, -w * zeta * zeta, mE3(E3m(C11)));
-        acc3(o + L_D + 9 * 6, w * zeta, E3m(C12));
-        acc3(o + L_D + 9 * 7, -w * zeta * zeta, mE3(E3m(C12)));
-        acc3(o + L_D + 9 * 8, w, C21);            // dn2/deta1
-        acc3(o + L_D + 9 * 9, -w * zeta, mE3(C21));
-        acc3(o + L_D + 9 * 10, w, C22);
-        acc3(o + L_D + 9 * 11, -w * zeta, mE3(C22));
-        acc3(o + L_D + 9 * 12, w * zeta, E3m(C21)); // dm2/deta1
-        acc3(o + L_D + 9 * 13, -w * zeta * zeta, mE3(E3m(C21)));
-        acc3(o + L_D + 9 * 14, w * zeta, E3m(C22));
-        acc3(o + L_D + 9 * 15, -w * zeta * zeta, mE3(E3m(C22)));
+        acc3(oD + 9 * 0, w, C11);            // dn1/deta1
+        acc3(oD + 9 * 1, -w * zeta, mE3(C11)); // dn1/dk1
+        acc3(oD + 9 * 2, w, C12);            // dn1/deta2
+        acc3(oD + 9 * 3, -w * zeta, mE3(C12)); // dn1/dk2
+        acc3(oD + 9 * 4, w * zeta, E3m(C11));  // dm1/deta1
+        acc3(oD + 9 * 5, -w * zeta * zeta, mE3(E3m(C11)));
+        acc3(oD + 9 * 6, w * zeta, E3m(C12));
+        acc3(oD + 9 * 7, -w * zeta * zeta, mE3(E3m(C12)));
+        acc3(oD + 9 * 8, w, C21);            // dn2/deta1
+        acc3(oD + 9 * 9, -w * zeta, mE3(C21));
+        acc3(oD + 9 * 10, w, C22);
+        acc3(oD + 9 * 11, -w * zeta, mE3(C22));
+        acc3(oD + 9 * 12, w * zeta, E3m(C21)); // dm2/deta1
+        acc3(oD + 9 * 13, -w * zeta * zeta, mE3(E3m(C21)));
+        acc3(oD + 9 * 14, w * zeta, E3m(C22));
+        acc3(oD + 9 * 15, -w * zeta * zeta, mE3(E3m(C22)));
         n1 = add3(n1, scl3(w, tau1));
         n2 = add3(n2, scl3(w, tau2));
         // e3 x tau = (-tau.y, tau.x, 0)
@@ -277,8 +279,8 @@ __device__ __forceinline__ void shell_gp_record(const double *__restrict__ cst, 
         m2 = add3(m2, scl3(w * zeta, mk3(-tau2.y, tau2.x, 0.0)));
     }
     // drilling (overwrites, as the reference)
-    o[L_D + 9 * 5 + 8] = drill;   // D(5,5): block (1,1) entry (2,2)
-    o[L_D + 9 * 15 + 8] = drill;  // D(11,11): block (3,3) entry (2,2)
+    oD[9 * 5 + 8] = drill;   // D(5,5): block (1,1) entry (2,2)
+    oD[9 * 15 + 8] = drill;  // D(11,11): block (3,3) entry (2,2)
     m1.z = drill * k.k1.z;
     m2.z = drill * k.k2.z;
     m3<double> Qt = tr3(k.QQ);
@@ -336,7 +338,8 @@ __global__ __launch_bounds__(SWG, PRE ? 3 : 1) void shell_element_kernel(
     } else if (tid < 3 * SEPB) {
         int le = tid / 3, gp = tid - 3 * (tid / 3);
         int e = ebase + le;
-        if (e < e1) shell_gp_record(cst, gps, ld, e, gp, disp, conn, sm + (le * 3 + gp) * L_GP);
+        double *o = sm + (le * 3 + gp) * L_GP;
+        if (e < e1) shell_gp_record(cst, gps, ld, e, gp, disp, conn, o + L_D, o);
     }
     __syncthreads();
 
@@ -517,25 +520,29 @@ __global__ __launch_bounds__(256) void shell_pressure_record_kernel(
 }
 
 // Phase 1 of every element's Gauss points in one launch: lane per (element, gp),
-// its record built in LDS and copied out with the workgroup's neighbours (64
-// consecutive records are one contiguous block).  The colour launches read them
-// instead of recomputing phase 1 on 12 lanes per workgroup.
-constexpr int REC_LANES = 64;
+// its D blocks accumulated in LDS (74 KB per 64 lanes, two workgroups per CU;
+// a lane's stride of 145 entries keeps a wavefront's accesses off shared banks)
+// and copied out record by record, the write-once rest stored directly.  The
+// element kernel reads the records instead of recomputing phase 1 on 12 lanes
+// per workgroup.
+constexpr int REC_LANES = 64, REC_DS = 145;
 __global__ __launch_bounds__(REC_LANES) void shell_record_kernel(int ns, size_t ld, const int *__restrict__ conn,
                                                                   const double *__restrict__ cst,
                                                                   const double *__restrict__ gps,
                                                                   const double *__restrict__ disp, double *__restrict__ rec)
 {
-    __shared__ double srec[REC_LANES * L_M];
+    __shared__ double sD[REC_LANES * REC_DS];
     const int t0 = blockIdx.x * REC_LANES, t = t0 + threadIdx.x;
     const int nt = min(REC_LANES, 3 * ns - t0);
     if ((int)threadIdx.x < nt) {
         const int e = t / 3, gp = t - 3 * (t / 3);
-        shell_gp_record(cst, gps, ld, e, gp, disp, conn, srec + threadIdx.x * L_M);
+        shell_gp_record(cst, gps, ld, e, gp, disp, conn, sD + threadIdx.x * REC_DS, rec + (size_t)t * L_M);
     }
     __syncthreads();
-    double *dst = rec + (size_t)t0 * L_M;
-    for (int q = threadIdx.x; q < nt * L_M; q += REC_LANES) dst[q] = srec[q];
+    for (int q = threadIdx.x; q < nt * 144; q += REC_LANES) {
+        const int r = q / 144, i = q - 144 * r;
+        rec[(size_t)(t0 + r) * L_M + L_D + i] = sD[r * REC_DS + i];
+    }
 }
 
 __global__ __launch_bounds__(256) void shell_commit_kernel(int ns, size_t ld, const int *__restrict__ conn,
