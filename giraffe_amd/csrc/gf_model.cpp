@@ -1135,7 +1135,7 @@ static int update_tail_enqueue(gf_model *m, bool then_assemble)
     return GF_OK;
 }
 
-static void note_fallback(gf_model *m, bool fell_back, bool divergent = false);
+static void note_fallback(gf_model *m, bool fell_back, bool divergent = false, bool slow = false);
 
 // Outside timing, a launch sequence that depends on the model only is
 // captured into *exec on first use (kernel and copy nodes) and replayed: a
@@ -1176,6 +1176,7 @@ static int mixed_factor_enqueue(gf_model *m)
     tick(m, GF_PH_FACTOR, false);
     m->pending_launches[GF_PH_FACTOR] += nl_f32;
     m->n_f32_factor++;
+    m->solve_f32++;
     m->f32_have = true;
     m->f32_reused = false;
     return GF_OK;
@@ -1190,6 +1191,8 @@ static int mixed_factor_enqueue(gf_model *m)
 // refinement that stagnates on the kept factor refactors and continues.
 static int mixed_start_enqueue(gf_model *m)
 {
+    m->solve_sweeps = 1;
+    m->solve_f32 = 0;
     if (m->refine_reuse && m->f32_have) m->f32_reused = true;
     else GF_TRY(mixed_factor_enqueue(m));
     GF_TRY(gf_mixed_refine_begin(m));
@@ -1255,6 +1258,7 @@ static int refine_decide(gf_model *m, int *next)
             m->refine_it++;
         }
         m->n_sweeps++;
+        m->solve_sweeps++;
         tick(m, GF_PH_SOLVE, true);
         GF_TRYN(l, mixed_sweep_enqueue(m));
         tick(m, GF_PH_SOLVE, false);
@@ -1264,7 +1268,10 @@ static int refine_decide(gf_model *m, int *next)
     }
     m->refining = false;
     m->factored = true;
-    note_fallback(m, fall, divergent);
+    // a converged FP32-refined solve that cost more than the FP64 one would have
+    const bool slow = ok && !fall && m->fronts_on &&
+                      GF_MIXED_F32_SWEEPS * m->solve_f32 + m->solve_sweeps > GF_MIXED_F64_SWEEPS;
+    note_fallback(m, fall, divergent, slow);
     *next = ok ? 0 : 2;
     if (fall) {
         // a fresh FP32 factor that could not drive the refinement is not kept
@@ -1460,9 +1467,20 @@ static int solve_fp64_enqueue(gf_model *m)
 // A refinement whose second correction is no smaller than its first (the FP32
 // factor of this tangent does not contract the error at all, as for slender
 // beam chains whose equilibrated tangent is beyond FP32's range) starts the
-// longest back-off at once.
-static void note_fallback(gf_model *m, bool fell_back, bool divergent)
+// longest back-off at once.  On the multifrontal path a solve that converged
+// but whose sweeps and FP32 factorisations cost more than an FP64 factorisation
+// and solve (GF_MIXED_*_SWEEPS) starts a back-off too, without discarding the
+// counts of fallbacks; the FP32 factor is then dropped, since the next attempt
+// comes a back-off's tangents later.
+static void note_fallback(gf_model *m, bool fell_back, bool divergent, bool slow)
 {
+    if (!fell_back && slow) {
+        m->fp64_direct_left = m->backoff_len;
+        m->backoff_len = std::min(2 * m->backoff_len, 64 * GF_MIXED_BACKOFF);
+        m->fallback_streak = 0;
+        m->f32_have = false;
+        return;
+    }
     if (!fell_back) {
         m->fallback_streak = 0;
         m->backoff_len = GF_MIXED_BACKOFF;
