@@ -502,6 +502,12 @@ def main():
                           "fp64_fallbacks": c["fallbacks"], "fp64_direct": c["fp64_direct"],
                           "fp32_factorizations": c["fp32_factorizations"], "tangent_solves": c["solves"]}
         out["compare"]["same_iterations"] = c["reps"][-1][4] == reps[-1][4]
+        if other == "mixed":
+            nd, ns = c["fp64_direct"], max(1, c["solves"])
+            out["compare"]["paths"] = (
+                f"{ns - nd} of {ns} tangent solves FP32-factored and refined in FP64, {nd} factored in FP64 directly "
+                f"(the mixed mode's back-off: after FP32 refinement fell back, or cost more than an FP64 "
+                f"factorisation and solve)")
     if args.cpu_baseline != "none":
         out["cpu_baseline"] = cpu_baseline(m, p, args.cpu_baseline, its, args.cpu_budget)
     print(json.dumps(out))

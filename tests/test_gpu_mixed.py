@@ -126,3 +126,29 @@ def test_mixed_singular_tangent_reports_info_fail():
     dm.begin_step(0.0, 0.05)
     dm.assemble()
     assert dm.solve() == 1
+
+
+def test_mixed_fronts_backs_off_when_refinement_costs_more():
+    """On the multifrontal path a converged refined solve whose sweeps and FP32
+    factorisations cost more than an FP64 factorisation and solve starts the
+    back-off a fallback starts: later tangents are factored in FP64 directly.
+    The Newton history is the FP64 one and the nodes agree to the refinement's
+    accuracy; no solve falls back."""
+    m, p, _ = M.shell_plate(nq=32)  # 15 load steps, each converging
+    print("mixed back-off plate32", flush=True)
+    d64 = runtime.DeviceModel(m, precision=abi.GF_PREC_FP64)
+    r64, i64 = d64.static_solve(p)
+    dmx = runtime.DeviceModel(m, precision=abi.GF_PREC_MIXED)
+    assert dmx.info()["layout"] == 2
+    rmx, imx = dmx.static_solve(p)
+    assert rmx.aborted == r64.aborted == 0
+    assert list(imx) == list(i64)
+    st = dmx.stats()
+    print("stats", st, flush=True)
+    assert st["fallbacks"] == 0
+    assert st["fp64_direct"] > 0
+    # every solve is FP64-direct or refined; refined ones need at least one sweep
+    assert st["refine_sweeps"] >= st["solves"] - st["fp64_direct"]
+    n64, nmx = d64.nodes(), dmx.nodes()
+    x0 = m.freeze()["xyz"]
+    assert np.abs(nmx[:, :3] - n64[:, :3]).max() <= 1e-8 * np.abs(n64[:, :3] - x0).max()
