@@ -13,12 +13,15 @@
 // granularity).  Finished requests release their memory and stream at once
 // and the next queued request is admitted in the same pump.
 //
-// Host planning stays off the thread that drives the device: submit makes the
-// plan's sizes (gf_model_preplan, which decides admission), and a planner
-// thread builds the multifrontal per-entry lists of the first PLAN_AHEAD queued
-// requests while the driving thread keeps the resident requests going; the
-// queue head is admitted once its plan is complete, and admission then only
-// allocates and uploads.
+// Host planning stays off the thread that drives the device: submit plans the
+// request's sizes (gf_model_preplan: the device footprint that decides
+// admission) and keeps that plan while fewer than KEEP_PLANS requests are
+// queued (beyond that it is dropped, so a long queue holds only the copied
+// descriptions), and a planner thread completes the plans of the first
+// PLAN_AHEAD queued requests (the sizes again where they were dropped, then the
+// multifrontal per-entry lists) while the driving thread keeps the resident
+// requests going; the queue head is admitted once its plan is complete, and
+// admission then only allocates and uploads.
 //
 // Streams are created at the default priority.  One host thread drives a server
 // (submit, pump, take, counts); the planner thread is the server's own.
@@ -29,6 +32,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <exception>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -102,8 +106,9 @@ struct DescCopy {
     }
 };
 
-// Lists of at most this many queued requests are built ahead of admission.
-constexpr int PLAN_AHEAD = 4;
+// Lists of at most this many queued requests are built ahead of admission; the
+// size plans of at most KEEP_PLANS queued requests are kept from submit.
+constexpr int PLAN_AHEAD = 4, KEEP_PLANS = 32;
 
 enum { PLAN_SIZES = 0, PLAN_BUSY = 1, PLAN_DONE = 2 };
 
@@ -112,7 +117,7 @@ struct Request {
     DescCopy desc;
     gf_static_params params{};
     unsigned long long bytes = 0;
-    gf_model *model = nullptr;  // planned at submit, created at admission
+    gf_model *model = nullptr;  // planned (submit, planner thread), created at admission
     std::atomic<int> plan_state{PLAN_SIZES};
     int plan_status = GF_OK;    // of the planner's list build
     std::string plan_error;
@@ -132,6 +137,7 @@ struct gf_server {
     std::vector<hipStream_t> streams;
     std::vector<int> free_slots;
     std::mutex mu;               // guards queue (the planner reads it)
+    gf_options plan_opt{};       // what the planner plans requests with
     std::condition_variable cv;  // planner wake-up: a new request, a queue move, stop
     bool stop = false;
     std::thread planner;
@@ -174,8 +180,16 @@ static void planner_main(gf_server *s)
         // `next` stays in the queue (and alive) until then
         next->plan_state.store(PLAN_BUSY);
         lk.unlock();
-        const int st = gf_model_plan_lists(&next->desc.d, next->model);
-        if (st != GF_OK) next->plan_error = gf_last_error();
+        int st;
+        try {
+            unsigned long long bytes = 0;
+            st = next->model ? GF_OK : gf_model_preplan(&next->desc.d, &s->plan_opt, &next->model, &bytes);
+            if (st == GF_OK) st = gf_model_plan_lists(&next->desc.d, next->model);
+            if (st != GF_OK) next->plan_error = gf_last_error();
+        } catch (const std::exception &e) {
+            st = GF_E_UNSUPPORTED;
+            next->plan_error = std::string("request planning: ") + e.what();
+        }
         next->plan_status = st;
         lk.lock();
         next->plan_state.store(PLAN_DONE);
@@ -193,6 +207,9 @@ extern "C" int gf_server_create(const gf_server_options *opt, gf_server **out)
         GF_HIP(hipStreamCreateWithFlags(&s->streams[i], hipStreamNonBlocking));
         s->free_slots.push_back(opt->max_inflight - 1 - i);
     }
+    gf_default_options(&s->plan_opt);
+    s->plan_opt.precision = opt->precision;
+    s->plan_opt.device = opt->device;
     s->planner = std::thread(planner_main, s.get());
     *out = s.release();
     return GF_OK;
@@ -232,8 +249,11 @@ extern "C" int gf_server_submit(gf_server *s, const gf_model_desc *d, const gf_s
     o.precision = s->opt.precision;
     o.device = s->opt.device;
     GF_TRY(gf_model_preplan(&r->desc.d, &o, &r->model, &r->bytes));
-    if (r->bytes > s->opt.mem_cap) {
+    if (r->bytes > s->opt.mem_cap || (int)s->queue.size() >= KEEP_PLANS) {
         gf_model_destroy(r->model);
+        r->model = nullptr;
+    }
+    if (r->bytes > s->opt.mem_cap) {
         gf_set_error("request needs more device memory than the server's cap");
         return GF_E_UNSUPPORTED;
     }
@@ -260,11 +280,13 @@ static int admit(gf_server *s, Request &r)
     s->free_slots.pop_back();
     o.stream = (void *)s->streams[r.stream_slot];
     r.t_admit = clk::now();
-    s->mem_in_use += r.bytes;  // finish() gives it back, created or not
     if (r.plan_status != GF_OK) {
+        gf_model_destroy(r.model);  // a partial plan, if any
+        r.model = nullptr;
         gf_set_error(r.plan_error);
         return r.plan_status;
     }
+    s->mem_in_use += r.bytes;  // finish() gives it back, created or not
     int st = gf_model_create_planned(&r.desc.d, &o, r.model);
     if (st) return st;
     st = r.run.start(r.model, r.params, r.step_iters.data(), (int)r.step_iters.size());
