@@ -149,9 +149,8 @@ template <typename T> struct PivotGuard {
 // moves only: no LDS round trip on the pivot chain.
 template <int CTRL> __device__ __forceinline__ double dpp_mov(double v)
 {
-    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
-    return __hiloint2double(hi, lo);
+    // one 64-bit move where the control allows it (row_newbcast), else two 32-bit ones
+    return __builtin_amdgcn_update_dpp(0.0, v, CTRL, 0xF, 0xF, false);
 }
 template <int CTRL> __device__ __forceinline__ float dpp_mov(float v)
 {
