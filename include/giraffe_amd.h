@@ -195,7 +195,9 @@ int gf_model_info(const gf_model *m, long long *info_out /* [8] */);
  * mixed path's back-off (after two fallbacks in a row the next 16 solves; a retry
  * that falls back again doubles the next back-off, up to 1,024; a converged FP32
  * solve resets it; a refinement that does not contract at all starts the 1,024
- * at once), FP32 factorisations. */
+ * at once; with the multifrontal solver, a converged FP32 solve that cost more
+ * than an FP64 factorisation and solve starts a back-off as two fallbacks do),
+ * FP32 factorisations. */
 int gf_model_stats(const gf_model *m, long long *stats_out /* [5] */);
 /* Device bytes gf_model_create would allocate for this description (host-side
  * planning only; no device call).  Used for admission by the request server. */
