@@ -70,7 +70,7 @@ template <typename T> struct AsmSmem {
     T aux[4 + NB];
 };
 
-// 512 threads (eight wavefronts) per tile: the inverse's 41 KB of LDS then still
+// 512 threads (eight wavefronts) per tile: the inverse's 48 KB of LDS then still
 // leaves 24 wavefronts per CU to overlap the gathers.
 constexpr int ASM_T = 512, ASM_R = NB * NB / ASM_T;
 
